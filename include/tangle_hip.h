@@ -1,0 +1,181 @@
+/*
+ * tangle_hip.h — C ABI of the MI355X relaxation backend for TANGLE.
+ *
+ * One world holds one fiber assembly resident on the GPU and relaxes it with
+ * position-based contact, stretch/bend and admissible-curvature projections.
+ * The entry points mirror TANGLE's device-world operator API (Rust):
+ *
+ *   tgh_world_create            DeviceWorld::new / DeviceFiberWorld::upload
+ *                               crates/tangle_relax/src/device/plugin.rs:422-424,
+ *                               crates/tangle_relax/src/device/world.rs:308-690
+ *                               (packing: crates/tangle_relax/src/device/packed.rs:150-320)
+ *   tgh_world_run_batch         DeviceWorld::run_batch + read_status
+ *                               plugin.rs:407-418, world.rs:1020-1505, 2337-2369
+ *   tgh_world_download_positions DeviceWorld::download_positions   world.rs:2384-2390
+ *   tgh_world_upload_positions  (no direct counterpart: the reference re-uploads
+ *                               through DeviceFiberWorld::restore, world.rs:692-823;
+ *                               here it replaces positions in place and requests a
+ *                               neighbour-list rebuild, request_neighbor_list_rebuild
+ *                               crates/tangle_contact/src/device/neighbor.rs:434-441)
+ *   tgh_world_info              cell_size()/cell_count()/neighbor_list_rebuilds()
+ *                               plugin.rs:324-893, world/neighbor_list.rs:166-176
+ *   tgh_world_cell_list,        read-back of cell_counts/cell_offsets/cell_segments
+ *   tgh_world_neighbor_lists    and neighbor_counts/neighbor_segments, as the
+ *                               reference's own tests read them
+ *                               (crates/tangle_relax/src/device/world_tests.rs:1342-1362)
+ *   tgh_generate_population     generate_biased_fiber_population
+ *                               crates/tangle_generate/src/population.rs:246-357
+ *
+ * Conventions. Every function returns 0 on success and a negative code on
+ * failure; tgh_last_error() then names the problem (thread-local string). The
+ * caller owns every input and output buffer; the world owns device memory.
+ * A world is not thread-safe: use one world from one thread at a time.
+ * All lengths are SI metres, positions are interleaved xyz.
+ */
+#ifndef TANGLE_HIP_H
+#define TANGLE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TGH_OK 0
+#define TGH_ERR_INVALID (-1)   /* invalid argument or configuration (Python: ValueError) */
+#define TGH_ERR_DEVICE (-2)    /* HIP runtime failure (Python: RuntimeError)             */
+#define TGH_ERR_SIZE (-3)      /* caller buffer too small                                 */
+
+/* A solver-neutral assembly: the FiberAssembly fields the relaxation reads
+ * (crates/tangle_core/src/assembly.rs; packed by packed.rs:150-320).
+ * Fibers are stored one after another: fiber f owns vertices
+ * [sum(counts[0..f]), sum(counts[0..f+1])). Round sections only. */
+typedef struct tgh_assembly {
+    uint32_t fiber_count;
+    uint32_t vertex_count;
+    const uint32_t *fiber_vertex_counts;  /* [fiber_count], each >= 2            */
+    const double *placed;                 /* [3*vertex_count] current positions  */
+    const double *intrinsic;              /* [3*vertex_count] rest shape          */
+    const double *fiber_radius;           /* [fiber_count] > 0                    */
+    const double *fiber_min_bend_radius;  /* [fiber_count]; <= 0 or NaN: no limit, NULL: none */
+    const uint8_t *vertex_pinned;         /* [vertex_count] or NULL               */
+    double cell_origin[3];
+    double cell_lengths[3];
+    uint32_t cell_periodic[3];
+    uint32_t pin_fiber_ends;              /* pin first/last vertex of every fiber */
+} tgh_assembly;
+
+/* RelaxationConfig (plugin.rs:77-175) restricted to the flexible, round,
+ * non-adaptive relaxation loop; CellListConfig (tangle_contact/src/lib.rs:14-40). */
+typedef struct tgh_config {
+    float penetration_tolerance;
+    float correction_fraction;
+    float stretch_stiffness;
+    float bend_stiffness;
+    float curvature_limit_stiffness;
+    float curvature_limit_safety_margin;
+    float curvature_ratio_tolerance;
+    float max_step;
+    float cell_size_scale;
+    float neighbor_skin_scale;
+    uint32_t contact_aggregation;        /* 0 average, 1 penetration-weighted, 2 deepest */
+    uint32_t constraint_iterations;
+    uint32_t curvature_cleanup_sweeps;
+    uint32_t force_full_iterations;
+    uint32_t neighbor_capacity;
+    uint32_t maximum_neighbor_list_bytes; /* 0: the reference's 120 MiB (world.rs:36) */
+} tgh_config;
+
+/* BatchStatus (world.rs:89-114). */
+typedef struct tgh_batch_status {
+    uint64_t total_iterations;
+    uint64_t batch_iterations;
+    float max_penetration;
+    float max_displacement;
+    float max_curvature_ratio;
+    uint32_t converged;
+    uint32_t cell_list_overflow;
+} tgh_batch_status;
+
+typedef struct tgh_world_info {
+    uint32_t fiber_count;
+    uint32_t segment_count;
+    uint32_t vertex_count;
+    uint32_t cells[3];
+    uint32_t cell_count;
+    float cell_size;
+    float neighbor_skin;
+    uint32_t neighbor_block;             /* list slots per segment actually used */
+    uint32_t neighbor_list_rebuilds;     /* neighbor_state[1] */
+    uint64_t device_bytes;               /* device memory held by the world */
+} tgh_world_info;
+
+typedef struct tgh_world tgh_world;
+
+/* Fills `config` with RelaxationConfig::default() (plugin.rs:145-175). */
+void tgh_config_default(tgh_config *config);
+
+int tgh_world_create(const tgh_assembly *assembly, const tgh_config *config, tgh_world **out);
+int tgh_world_run_batch(tgh_world *world, const tgh_config *config, uint32_t iterations,
+                        tgh_batch_status *status);
+int tgh_world_download_positions(const tgh_world *world, float *out, size_t count);
+int tgh_world_upload_positions(tgh_world *world, const float *positions, size_t count);
+int tgh_world_info_get(const tgh_world *world, tgh_world_info *info);
+/* Rebuilds the cell list and neighbour lists from the current positions now
+ * (request_neighbor_list_rebuild followed by rebuild_neighbor_lists_if_requested,
+ * crates/tangle_contact/src/device/neighbor.rs:434-441,
+ * crates/tangle_relax/src/device/world/neighbor_list.rs:19-62); used to compare
+ * lists built from identical positions. */
+int tgh_world_rebuild_lists(tgh_world *world);
+/* counts/offsets: [cell_count]; segments: [segment_count] in cell order. */
+int tgh_world_cell_list(const tgh_world *world, uint32_t *counts, uint32_t *offsets,
+                        uint32_t *segments, size_t cell_capacity, size_t segment_capacity);
+/* counts: [segment_count] true neighbour counts (may exceed the block);
+ * lists: [segment_count * neighbor_block], row s holds segment s's list in
+ * ascending order when counts[s] <= neighbor_block (unspecified otherwise). */
+int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t *lists,
+                             size_t list_capacity);
+void tgh_world_destroy(tgh_world *world);
+const char *tgh_last_error(void);
+
+/* ---- seeded population generator (population.rs:246-357, 554-813) ---- */
+
+#define TGH_ORIENT_ISOTROPIC 0   /* OrientationDistribution::Isotropic3d           */
+#define TGH_ORIENT_PLANAR 1      /* ::Planar { normal, maximum_tilt }              */
+#define TGH_ORIENT_ALIGNED 2     /* ::Aligned { axis, maximum_angle }              */
+#define TGH_ORIENT_SCHLADITZ 3   /* Schladitz density about `axis`, parameter beta
+                                    (no sampler in the reference; its density and
+                                    quantile: tangle_characterize/src/shape.rs:26-30,
+                                    552-556)                                       */
+
+typedef struct tgh_population_spec {
+    uint32_t count;
+    uint32_t segments_per_fiber;
+    uint64_t seed;
+    double length_min, length_max;        /* equal: ScalarDistribution::Constant */
+    double radius_min, radius_max;
+    double amplitude_min, amplitude_max;
+    uint32_t orientation;                 /* TGH_ORIENT_* */
+    double orientation_vector[3];         /* planar normal / aligned or Schladitz axis */
+    double orientation_angle;             /* planar max tilt / aligned max angle */
+    double orientation_beta;              /* Schladitz beta (shape.rs convention) */
+    double min_bend_radius;               /* <= 0: none */
+    uint32_t max_attempts_per_fiber;
+    double cell_origin[3];
+    double cell_lengths[3];
+    uint32_t cell_periodic[3];
+} tgh_population_spec;
+
+/* Writes spec->count fibers of (segments_per_fiber+1) vertices each:
+ * intrinsic/placed [3 * count * (segments_per_fiber+1)], radius [count].
+ * Returns TGH_ERR_INVALID with "could not place fiber i" when the placement
+ * search is exhausted (PopulationGenerationError::PlacementExhausted). */
+int tgh_generate_population(const tgh_population_spec *spec, double *intrinsic, double *placed,
+                            double *radius, size_t vertex_capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TANGLE_HIP_H */

@@ -1,0 +1,577 @@
+"""TANGLE relaxation on the MI355X: the reference's recipe API over a HIP backend.
+
+This package mirrors the part of TANGLE's Python API that drives the
+relaxation hot path (crates/tangle_python/python/tangle/_tangle.pyi):
+``Cell``, ``Material``, ``FiberPopulation`` with its orientation classes,
+``generate_fiber_population``, ``FiberCollection``, ``Recipe`` with
+``insert`` / ``relax_until_converged`` / ``relax_for`` / ``run``,
+``RelaxationSettings`` and ``RunResult``. ``RelaxationSettings(backend="hip")``
+selects the hand-written CDNA4 kernels of ``libtangle_hip.so``; it is the only
+backend this package builds, and there is no silent fallback: other backend
+names raise ``ValueError`` and a missing library raises ``RuntimeError``.
+
+Lengths are SI metres, as in the reference.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field, replace as _dc_replace
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _native
+from .units import um
+
+__all__ = [
+    "Cell",
+    "Material",
+    "FiberCollection",
+    "FiberPopulation",
+    "IsotropicOrientation",
+    "PlanarOrientation",
+    "AlignedOrientation",
+    "SchladitzOrientation",
+    "UniformPosition",
+    "generate_fiber_population",
+    "RelaxationSettings",
+    "Recipe",
+    "RecipeError",
+    "RunResult",
+    "um",
+]
+
+_AXES = {"x": 0, "y": 1, "z": 2, 0: 0, 1: 1, 2: 2}
+
+
+def _axis_set(value) -> list[bool]:
+    if isinstance(value, str):
+        flags = [False, False, False]
+        for letter in value.lower():
+            if letter not in "xyz":
+                raise ValueError(f"unknown axis {letter!r} in {value!r}")
+            flags[_AXES[letter]] = True
+        return flags
+    if isinstance(value, int):
+        flags = [False, False, False]
+        flags[_AXES[value]] = True
+        return flags
+    flags = [bool(v) for v in value]
+    if len(flags) != 3:
+        raise ValueError("an axis set needs three flags")
+    return flags
+
+
+def _direction(value) -> list[float]:
+    if isinstance(value, (str, int)):
+        d = [0.0, 0.0, 0.0]
+        d[_AXES[value]] = 1.0
+        return d
+    d = [float(v) for v in value]
+    if len(d) != 3 or not all(math.isfinite(v) for v in d) or not any(d):
+        raise ValueError("orientation direction must be finite and nonzero")
+    return d
+
+
+class Cell:
+    """Orthorhombic simulation box (``lengths`` in metres, ``periodic`` axes)."""
+
+    def __init__(self, lengths: Sequence[float], periodic="", origin: Sequence[float] = (0.0, 0.0, 0.0), *,
+                 stack_axis=None):
+        self._lengths = [float(v) for v in lengths]
+        if len(self._lengths) != 3 or not all(v > 0 and math.isfinite(v) for v in self._lengths):
+            raise ValueError("cell lengths must be three positive finite numbers")
+        self._periodic = _axis_set(periodic)
+        self._origin = [float(v) for v in origin]
+        self._stack_axis = 2 if stack_axis is None else _AXES[stack_axis]
+
+    lengths = property(lambda self: list(self._lengths))
+    periodic = property(lambda self: list(self._periodic))
+    origin = property(lambda self: list(self._origin))
+    stack_axis = property(lambda self: self._stack_axis)
+
+    def __repr__(self) -> str:
+        axes = "".join("xyz"[k] for k in range(3) if self._periodic[k])
+        return f"Cell(lengths={self._lengths}, periodic={axes!r})"
+
+
+class Material:
+    """A fiber type: ``diameter`` in metres and an optional ``min_bend_radius``."""
+
+    def __init__(self, name: str, diameter: float, min_bend_radius: float | None = None, *,
+                 thickness: float | None = None):
+        if not (diameter > 0 and math.isfinite(diameter)):
+            raise ValueError("diameter must be positive and finite")
+        if min_bend_radius is not None and not (min_bend_radius > 0 and math.isfinite(min_bend_radius)):
+            raise ValueError("min_bend_radius must be positive and finite")
+        self._name = str(name)
+        self._diameter = float(diameter)
+        self._min_bend_radius = None if min_bend_radius is None else float(min_bend_radius)
+        self._thickness = None if thickness is None else float(thickness)
+
+    name = property(lambda self: self._name)
+    diameter = property(lambda self: self._diameter)
+    radius = property(lambda self: 0.5 * self._diameter)
+    min_bend_radius = property(lambda self: self._min_bend_radius)
+    thickness = property(lambda self: self._thickness)
+    is_oval = property(lambda self: self._thickness is not None and self._thickness < self._diameter)
+
+    def __repr__(self) -> str:
+        return f'Material(name="{self._name}", diameter={self._diameter!r})'
+
+
+@dataclass(frozen=True)
+class IsotropicOrientation:
+    """Uniform directions on the sphere (OrientationDistribution::Isotropic3d)."""
+
+
+@dataclass(frozen=True)
+class PlanarOrientation:
+    """Directions within ``max_tilt`` radians of the plane normal to ``normal``."""
+
+    normal: object = "z"
+    max_tilt: float = 0.0
+
+
+@dataclass(frozen=True)
+class AlignedOrientation:
+    """Directions in a cone of half-angle ``max_angle`` around ``axis``."""
+
+    axis: object = "z"
+    max_angle: float = 0.0
+
+
+@dataclass(frozen=True)
+class SchladitzOrientation:
+    """Schladitz orientation density β/(4π(1+(β²−1)cos²θ)^{3/2}) about ``axis``.
+
+    ``beta`` follows the reference's fit convention
+    (crates/tangle_characterize/src/shape.rs:26-30): β = 1 is isotropic,
+    β → 0 aligns fibers with ``axis`` and β > 1 lays them in the plane
+    normal to it. The reference only fits β; this sampler is an addition.
+    """
+
+    beta: float = 1.0
+    axis: object = "z"
+
+
+@dataclass(frozen=True)
+class UniformPosition:
+    """Uniform centers over each fiber's feasible region."""
+
+
+_DEFAULT_MATERIAL = Material("fiber", 0.025, 0.08)  # FiberPopulationSpec::default (population.rs:146-175)
+
+
+def _range(value, name):
+    if value is None:
+        return None
+    if isinstance(value, (int, float)):
+        return (float(value), float(value))
+    lo, hi = (float(v) for v in value)
+    if lo > hi:
+        raise ValueError(f"{name} range must be ordered as (min, max)")
+    return (lo, hi)
+
+
+@dataclass
+class FiberPopulation:
+    """Random fibers for ``generate_fiber_population`` (generators.rs:384-600).
+
+    Unset, ``length`` is 16-24 mean diameters and ``curvature_amplitude``
+    0-0.8 mean diameters; ``diameter`` defaults to the material's.
+    """
+
+    material: Material = field(default_factory=lambda: _DEFAULT_MATERIAL)
+    count: int = 32
+    segments_per_fiber: int = 8
+    seed: int = 1
+    length: object = None
+    diameter: object = None
+    curvature_amplitude: object = None
+    nominal_parent_length: float | None = None
+    orientation: object = field(default_factory=IsotropicOrientation)
+    position: object = field(default_factory=UniformPosition)
+    max_attempts_per_fiber: int = 128
+
+    def copy(self) -> "FiberPopulation":
+        return _dc_replace(self)
+
+    def replace(self, **changes) -> "FiberPopulation":
+        return _dc_replace(self, **changes)
+
+    def _spec(self, cell: Cell) -> _native.PopulationSpec:
+        spec = _native.PopulationSpec()
+        diameter = _range(self.diameter, "diameter") or (self.material.diameter, self.material.diameter)
+        mean_d = diameter[0] if diameter[0] == diameter[1] else 0.5 * (diameter[0] + diameter[1])
+        length = _range(self.length, "length") or (16.0 * mean_d, 24.0 * mean_d)
+        amplitude = _range(self.curvature_amplitude, "curvature_amplitude") or (0.0, 0.8 * mean_d)
+        if self.material.is_oval:
+            raise ValueError("oval sections are not supported by the hip backend")
+        if not isinstance(self.position, UniformPosition):
+            raise ValueError("only UniformPosition is supported")
+        spec.count = int(self.count)
+        spec.segments_per_fiber = int(self.segments_per_fiber)
+        spec.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
+        spec.length_min, spec.length_max = length
+        spec.radius_min, spec.radius_max = 0.5 * diameter[0], 0.5 * diameter[1]
+        spec.amplitude_min, spec.amplitude_max = amplitude
+        o = self.orientation
+        if isinstance(o, IsotropicOrientation):
+            spec.orientation = _native.ORIENT_ISOTROPIC
+        elif isinstance(o, PlanarOrientation):
+            spec.orientation = _native.ORIENT_PLANAR
+            spec.orientation_vector[:] = _direction(o.normal)
+            spec.orientation_angle = float(o.max_tilt)
+        elif isinstance(o, AlignedOrientation):
+            spec.orientation = _native.ORIENT_ALIGNED
+            spec.orientation_vector[:] = _direction(o.axis)
+            spec.orientation_angle = float(o.max_angle)
+        elif isinstance(o, SchladitzOrientation):
+            spec.orientation = _native.ORIENT_SCHLADITZ
+            spec.orientation_vector[:] = _direction(o.axis)
+            spec.orientation_beta = float(o.beta)
+        else:
+            raise TypeError(f"unsupported orientation {o!r}")
+        spec.min_bend_radius = self.material.min_bend_radius or 0.0
+        spec.max_attempts_per_fiber = int(self.max_attempts_per_fiber)
+        spec.cell_origin[:] = cell.origin
+        spec.cell_lengths[:] = cell.lengths
+        spec.cell_periodic[:] = [1 if p else 0 for p in cell.periodic]
+        return spec
+
+
+@dataclass
+class _Fiber:
+    material: Material
+    radius: float
+    placed: np.ndarray      # (n, 3) float64
+    intrinsic: np.ndarray   # (n, 3) float64
+
+
+class FiberCollection:
+    """Fibers with their rest (intrinsic) and placed centrelines."""
+
+    def __init__(self, name: str = "fibers"):
+        self.name = name
+        self._fibers: list[_Fiber] = []
+
+    def add_fiber(self, points, material: Material, rest_points=None) -> int:
+        placed = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+        intrinsic = placed.copy() if rest_points is None else np.asarray(rest_points, dtype=np.float64).reshape(-1, 3)
+        if placed.shape[0] < 2 or intrinsic.shape != placed.shape:
+            raise ValueError("a fiber needs at least two points and matching rest points")
+        self._fibers.append(_Fiber(material, material.radius, placed, intrinsic))
+        return len(self._fibers) - 1
+
+    def centerlines(self):
+        return [f.placed.tolist() for f in self._fibers]
+
+    def rest_centerlines(self):
+        return [f.intrinsic.tolist() for f in self._fibers]
+
+    def extend(self, other: "FiberCollection") -> None:
+        self._fibers.extend(other._fibers)
+
+    def __add__(self, other: "FiberCollection") -> "FiberCollection":
+        out = FiberCollection(self.name)
+        out._fibers = self._fibers + other._fibers
+        return out
+
+    def __len__(self) -> int:
+        return len(self._fibers)
+
+
+def generate_fiber_population(cell: Cell, population: FiberPopulation, *, name: str = "population") -> FiberCollection:
+    """Places ``population``'s seeded random fibers in ``cell`` (population.rs:246-357)."""
+    spec = population._spec(cell)
+    intrinsic, placed, radius = _native.generate_population(spec)
+    nv = population.segments_per_fiber + 1
+    out = FiberCollection(name)
+    for f in range(population.count):
+        sl = slice(f * nv, (f + 1) * nv)
+        out._fibers.append(_Fiber(population.material, float(radius[f]), placed[sl], intrinsic[sl]))
+    return out
+
+
+_BACKENDS = ("hip",)
+_AGGREGATIONS = {"uniform_average": 0, "penetration_weighted": 1, "deepest_only": 2}
+
+
+@dataclass
+class RelaxationSettings:
+    """Run-wide solver settings (settings.rs:250-609). Unset,
+    ``penetration_tolerance`` is 1 % and ``max_step`` 10 % of the smallest
+    fiber diameter in the recipe."""
+
+    backend: str = "hip"
+    motion_model: str = "flexible"
+    pin_fiber_ends: bool = False
+    penetration_tolerance: float | None = None
+    force_full_iterations: bool = False
+    correction_fraction: float = 0.8
+    contact_aggregation: str = "uniform_average"
+    stretch_stiffness: float = 0.35
+    bend_stiffness: float = 0.03
+    curvature_limit_stiffness: float = 1.0
+    curvature_limit_safety_margin: float = 1.0e-3
+    curvature_ratio_tolerance: float = 1.0e-5
+    constraint_iterations: int = 2
+    curvature_cleanup_sweeps: int = 4
+    max_step: float | None = None
+    max_iterations: int = 2000
+    iterations_per_batch: int = 128
+    cell_size_scale: float = 1.0
+    neighbor_skin_scale: float = 2.0
+    neighbor_capacity: int = 48
+
+    def __post_init__(self):
+        self._validate()
+
+    def copy(self) -> "RelaxationSettings":
+        return _dc_replace(self)
+
+    def replace(self, **changes) -> "RelaxationSettings":
+        return _dc_replace(self, **changes)
+
+    def to_dict(self) -> dict:
+        return {k: getattr(self, k) for k in self.__dataclass_fields__}
+
+    def _validate(self) -> None:
+        if self.backend not in _BACKENDS:
+            raise ValueError(
+                f"backend must be one of {list(_BACKENDS)}, got {self.backend!r} "
+                "(this package builds only the HIP backend)"
+            )
+        if self.motion_model != "flexible":
+            raise ValueError("the hip backend implements motion_model='flexible'")
+        if self.contact_aggregation not in _AGGREGATIONS:
+            raise ValueError(f"contact_aggregation must be one of {list(_AGGREGATIONS)}")
+        for name in ("correction_fraction", "stretch_stiffness", "bend_stiffness", "curvature_limit_stiffness"):
+            value = getattr(self, name)
+            if not math.isfinite(value) or not (0.0 <= value <= 1.0) or (value == 0.0 and name == "correction_fraction"):
+                raise ValueError(f"{name} must be finite and within its [0, 1] range")
+        if not (math.isfinite(self.curvature_limit_safety_margin) and 0.0 <= self.curvature_limit_safety_margin < 0.1):
+            raise ValueError("curvature_limit_safety_margin must be in [0, 0.1)")
+        if not (math.isfinite(self.curvature_ratio_tolerance) and self.curvature_ratio_tolerance >= 0.0):
+            raise ValueError("curvature_ratio_tolerance must be nonnegative and finite")
+        if min(self.constraint_iterations, self.curvature_cleanup_sweeps, self.max_iterations,
+               self.iterations_per_batch) <= 0:
+            raise ValueError("iteration and sweep counts must be positive")
+        if not (math.isfinite(self.cell_size_scale) and self.cell_size_scale >= 1.0):
+            raise ValueError("cell_size_scale must be finite and at least 1")
+        if not (math.isfinite(self.neighbor_skin_scale) and self.neighbor_skin_scale >= 0.0):
+            raise ValueError("neighbor_skin_scale must be finite and non-negative")
+        if self.neighbor_capacity < 1:
+            raise ValueError("neighbor_capacity must be at least 1")
+        if self.penetration_tolerance is not None and not (
+            math.isfinite(self.penetration_tolerance) and self.penetration_tolerance >= 0.0
+        ):
+            raise ValueError("penetration_tolerance must be nonnegative and finite")
+        if self.max_step is not None and not (math.isfinite(self.max_step) and self.max_step > 0.0):
+            raise ValueError("max_step must be positive and finite")
+
+    def to_config(self, smallest_diameter: float | None) -> _native.Config:
+        """RelaxationConfig for a recipe whose thinnest fiber is ``smallest_diameter``
+        (PyRelaxationSettings::to_rust_for, settings.rs:485-609)."""
+        self._validate()
+        c = _native.Config()
+        tol = self.penetration_tolerance
+        if tol is None:
+            tol = 1.0e-4 if smallest_diameter is None else 0.01 * smallest_diameter
+        step = self.max_step
+        if step is None:
+            step = 0.006 if smallest_diameter is None else 0.1 * smallest_diameter
+        c.penetration_tolerance = tol
+        c.correction_fraction = self.correction_fraction
+        c.stretch_stiffness = self.stretch_stiffness
+        c.bend_stiffness = self.bend_stiffness
+        c.curvature_limit_stiffness = self.curvature_limit_stiffness
+        c.curvature_limit_safety_margin = self.curvature_limit_safety_margin
+        c.curvature_ratio_tolerance = self.curvature_ratio_tolerance
+        c.max_step = step
+        c.cell_size_scale = self.cell_size_scale
+        c.neighbor_skin_scale = self.neighbor_skin_scale
+        c.contact_aggregation = _AGGREGATIONS[self.contact_aggregation]
+        c.constraint_iterations = self.constraint_iterations
+        c.curvature_cleanup_sweeps = self.curvature_cleanup_sweeps
+        c.force_full_iterations = 1 if self.force_full_iterations else 0
+        c.neighbor_capacity = self.neighbor_capacity
+        c.maximum_neighbor_list_bytes = 0
+        return c
+
+
+class RecipeError(RuntimeError):
+    """A recipe operation failed (tangle_python/src/recipe.rs:705-722)."""
+
+    def __init__(self, operation_index: int, operation: str, iteration: int, reason: str):
+        super().__init__(f"operation {operation_index} ({operation}) failed at iteration {iteration}: {reason}")
+        self.operation_index = operation_index
+        self.operation = operation
+        self.iteration = iteration
+        self.reason = reason
+
+
+@dataclass
+class RunResult:
+    """The relaxed fibers and run statistics of ``Recipe.run``."""
+
+    fiber_count: int
+    iterations: int
+    converged: bool
+    max_penetration: float
+    max_curvature_ratio: float
+    max_displacement: float
+    active_segments: int
+    active_vertices: int
+    cell_count: int
+    uploaded_bytes: int
+    downloaded_bytes: int
+    events: list
+    warnings: list
+    neighbor_list_rebuilds: int
+    wall_seconds: float
+    positions: np.ndarray
+    vertex_counts: np.ndarray
+
+    def centerlines(self):
+        out, start = [], 0
+        xyz = self.positions.reshape(-1, 3).astype(np.float64)
+        for n in self.vertex_counts:
+            out.append(xyz[start:start + n].tolist())
+            start += n
+        return out
+
+    def __repr__(self) -> str:
+        return (f"RunResult(fibers={self.fiber_count}, iterations={self.iterations}, converged={self.converged}, "
+                f"max_penetration={self.max_penetration:.3e}, max_curvature_ratio={self.max_curvature_ratio:.6f})")
+
+
+class Recipe:
+    """An ordered list of steps on one cell; nothing runs until ``run``."""
+
+    def __init__(self, cell: Cell, *, stack_axis=None):
+        if not isinstance(cell, Cell):
+            raise TypeError("Recipe needs a Cell")
+        self._cell = cell
+        self._fibers: list[_Fiber] = []
+        self._operations: list[tuple] = []
+        self._descriptions: list[str] = []
+        self._stack_axis = cell.stack_axis if stack_axis is None else _AXES[stack_axis]
+
+    @property
+    def stack_axis(self) -> int:
+        return self._stack_axis
+
+    def insert(self, collection, *, name: str | None = None):
+        if isinstance(collection, FiberPopulation):
+            collection = generate_fiber_population(self._cell, collection)
+        if not isinstance(collection, FiberCollection):
+            raise TypeError("insert takes a FiberCollection or a FiberPopulation")
+        self._fibers.extend(collection._fibers)
+        self._operations.append(("insert", len(collection)))
+        self._descriptions.append(f"insert {name or collection.name} ({len(collection)} fibers)")
+        return len(collection)
+
+    def relax_until_converged(self, *, max_iterations: int = 2000) -> None:
+        if max_iterations <= 0:
+            raise ValueError("max_iterations must be positive")
+        self._operations.append(("relax_until_converged", int(max_iterations)))
+        self._descriptions.append(f"relax until converged (at most {max_iterations} iterations)")
+
+    def relax_for(self, iterations: int) -> None:
+        if iterations <= 0:
+            raise ValueError("iterations must be positive")
+        self._operations.append(("relax_for", int(iterations)))
+        self._descriptions.append(f"relax for {iterations} iterations")
+
+    def operations(self) -> list[str]:
+        return list(self._descriptions)
+
+    def centerlines(self):
+        return [f.placed.tolist() for f in self._fibers]
+
+    def _packed(self, pin_fiber_ends: bool) -> _native.PackedAssembly:
+        counts = np.array([f.placed.shape[0] for f in self._fibers], dtype=np.uint32)
+        placed = np.concatenate([f.placed for f in self._fibers])
+        intrinsic = np.concatenate([f.intrinsic for f in self._fibers])
+        radius = np.array([f.radius for f in self._fibers])
+        bend = np.array([f.material.min_bend_radius or 0.0 for f in self._fibers])
+        if any(f.material.is_oval for f in self._fibers):
+            raise ValueError("oval sections are not supported by the hip backend")
+        return _native.PackedAssembly(counts, placed, intrinsic, radius, bend, self._cell.origin,
+                                      self._cell.lengths, self._cell.periodic, pin_fiber_ends=pin_fiber_ends)
+
+    def run(self, settings: RelaxationSettings | None = None) -> RunResult:
+        """Simulates every recorded step (recipe.rs:618-723 and the batch
+        controller plugin.rs:1041-1181, tangle_generate/src/recipe.rs:1311-1397)."""
+        if not self._fibers:
+            raise ValueError("recipe contains no fibers")
+        if not self._operations:
+            raise ValueError("recipe contains no operations")
+        settings = settings or RelaxationSettings()
+        started = time.perf_counter()
+        d_min = min(2.0 * f.radius for f in self._fibers)
+        config = settings.to_config(d_min)
+        operations = list(self._operations)
+        descriptions = list(self._descriptions)
+        if all(op[0] == "insert" for op in operations):
+            operations.append(("relax_until_converged", settings.max_iterations))
+            descriptions.append("relax until converged")
+        packed = self._packed(settings.pin_fiber_ends)
+        events: list[str] = []
+        with _native.World(packed, config) as world:
+            info = world.info()
+            uploaded = int(info.device_bytes)
+            downloaded = 0
+            total = 0
+            status = None
+            converged = False
+            for index, (kind, amount) in enumerate(operations):
+                if kind == "insert":
+                    continue
+                begin = total
+                force_full = kind == "relax_for"
+                batch_config = _native.Config.from_buffer_copy(config)
+                batch_config.force_full_iterations = 1 if (force_full or settings.force_full_iterations) else 0
+                converged = False
+                while True:
+                    completed = total - begin
+                    if kind == "relax_until_converged" and converged:
+                        events.append(f"relax to convergence in {completed} iterations (penetration "
+                                      f"{status.max_penetration:.3e}, curvature ratio {status.max_curvature_ratio:.6f})")
+                        break
+                    if completed >= amount:
+                        if kind == "relax_for":
+                            events.append(f"relax for {amount} iterations")
+                            break
+                        raise RecipeError(index, descriptions[index], total,
+                                          f"failed to converge in {amount} iterations: penetration "
+                                          f"{status.max_penetration:.3e}, curvature ratio "
+                                          f"{status.max_curvature_ratio:.6f}")
+                    if total >= settings.max_iterations:
+                        raise RecipeError(index, descriptions[index], total,
+                                          f"iteration budget {settings.max_iterations} exhausted")
+                    batch = min(settings.max_iterations - total, settings.iterations_per_batch, amount - completed)
+                    status = world.run_batch(batch_config, batch)
+                    downloaded += 11 * 4 + 3 * 4
+                    total = int(status.total_iterations)
+                    converged = bool(status.converged)
+                    if status.cell_list_overflow:
+                        raise RecipeError(index, descriptions[index], total, "cell list overflow")
+            positions = world.download_positions()
+            downloaded += positions.nbytes
+            rebuilds = int(world.info().neighbor_list_rebuilds)
+        wall = time.perf_counter() - started
+        return RunResult(
+            fiber_count=len(self._fibers), iterations=total, converged=converged,
+            max_penetration=float(status.max_penetration) if status else float("inf"),
+            max_curvature_ratio=float(status.max_curvature_ratio) if status else float("inf"),
+            max_displacement=float(status.max_displacement) if status else 0.0,
+            active_segments=int(info.segment_count), active_vertices=int(info.vertex_count),
+            cell_count=int(info.cell_count), uploaded_bytes=uploaded, downloaded_bytes=downloaded,
+            events=events, warnings=[], neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
+            positions=positions, vertex_counts=packed.counts.copy(),
+        )

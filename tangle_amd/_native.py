@@ -1,0 +1,317 @@
+"""ctypes binding of libtangle_hip.so (the C ABI in include/tangle_hip.h).
+
+The library is required: importing a function here that needs it raises
+RuntimeError when it has not been built (``python -m tangle_amd.build``).
+There is no fallback implementation.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "_lib" / "libtangle_hip.so"
+
+TGH_OK = 0
+TGH_ERR_INVALID = -1
+TGH_ERR_DEVICE = -2
+TGH_ERR_SIZE = -3
+
+ORIENT_ISOTROPIC = 0
+ORIENT_PLANAR = 1
+ORIENT_ALIGNED = 2
+ORIENT_SCHLADITZ = 3
+
+_u32p = C.POINTER(C.c_uint32)
+_f64p = C.POINTER(C.c_double)
+_f32p = C.POINTER(C.c_float)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class Assembly(C.Structure):
+    _fields_ = [
+        ("fiber_count", C.c_uint32),
+        ("vertex_count", C.c_uint32),
+        ("fiber_vertex_counts", _u32p),
+        ("placed", _f64p),
+        ("intrinsic", _f64p),
+        ("fiber_radius", _f64p),
+        ("fiber_min_bend_radius", _f64p),
+        ("vertex_pinned", _u8p),
+        ("cell_origin", C.c_double * 3),
+        ("cell_lengths", C.c_double * 3),
+        ("cell_periodic", C.c_uint32 * 3),
+        ("pin_fiber_ends", C.c_uint32),
+    ]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("penetration_tolerance", C.c_float),
+        ("correction_fraction", C.c_float),
+        ("stretch_stiffness", C.c_float),
+        ("bend_stiffness", C.c_float),
+        ("curvature_limit_stiffness", C.c_float),
+        ("curvature_limit_safety_margin", C.c_float),
+        ("curvature_ratio_tolerance", C.c_float),
+        ("max_step", C.c_float),
+        ("cell_size_scale", C.c_float),
+        ("neighbor_skin_scale", C.c_float),
+        ("contact_aggregation", C.c_uint32),
+        ("constraint_iterations", C.c_uint32),
+        ("curvature_cleanup_sweeps", C.c_uint32),
+        ("force_full_iterations", C.c_uint32),
+        ("neighbor_capacity", C.c_uint32),
+        ("maximum_neighbor_list_bytes", C.c_uint32),
+    ]
+
+
+class BatchStatus(C.Structure):
+    _fields_ = [
+        ("total_iterations", C.c_uint64),
+        ("batch_iterations", C.c_uint64),
+        ("max_penetration", C.c_float),
+        ("max_displacement", C.c_float),
+        ("max_curvature_ratio", C.c_float),
+        ("converged", C.c_uint32),
+        ("cell_list_overflow", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class WorldInfo(C.Structure):
+    _fields_ = [
+        ("fiber_count", C.c_uint32),
+        ("segment_count", C.c_uint32),
+        ("vertex_count", C.c_uint32),
+        ("cells", C.c_uint32 * 3),
+        ("cell_count", C.c_uint32),
+        ("cell_size", C.c_float),
+        ("neighbor_skin", C.c_float),
+        ("neighbor_block", C.c_uint32),
+        ("neighbor_list_rebuilds", C.c_uint32),
+        ("device_bytes", C.c_uint64),
+    ]
+
+
+class PopulationSpec(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint32),
+        ("segments_per_fiber", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("length_min", C.c_double),
+        ("length_max", C.c_double),
+        ("radius_min", C.c_double),
+        ("radius_max", C.c_double),
+        ("amplitude_min", C.c_double),
+        ("amplitude_max", C.c_double),
+        ("orientation", C.c_uint32),
+        ("orientation_vector", C.c_double * 3),
+        ("orientation_angle", C.c_double),
+        ("orientation_beta", C.c_double),
+        ("min_bend_radius", C.c_double),
+        ("max_attempts_per_fiber", C.c_uint32),
+        ("cell_origin", C.c_double * 3),
+        ("cell_lengths", C.c_double * 3),
+        ("cell_periodic", C.c_uint32 * 3),
+    ]
+
+
+# Every symbol the header declares, with (restype, argtypes).
+SIGNATURES = {
+    "tgh_config_default": (None, [C.POINTER(Config)]),
+    "tgh_world_create": (C.c_int, [C.POINTER(Assembly), C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    "tgh_world_run_batch": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_uint32, C.POINTER(BatchStatus)]),
+    "tgh_world_download_positions": (C.c_int, [C.c_void_p, _f32p, C.c_size_t]),
+    "tgh_world_upload_positions": (C.c_int, [C.c_void_p, _f32p, C.c_size_t]),
+    "tgh_world_info_get": (C.c_int, [C.c_void_p, C.POINTER(WorldInfo)]),
+    "tgh_world_rebuild_lists": (C.c_int, [C.c_void_p]),
+    "tgh_world_cell_list": (C.c_int, [C.c_void_p, _u32p, _u32p, _u32p, C.c_size_t, C.c_size_t]),
+    "tgh_world_neighbor_lists": (C.c_int, [C.c_void_p, _u32p, _u32p, C.c_size_t]),
+    "tgh_world_destroy": (None, [C.c_void_p]),
+    "tgh_last_error": (C.c_char_p, []),
+    "tgh_generate_population": (C.c_int, [C.POINTER(PopulationSpec), _f64p, _f64p, _f64p, C.c_size_t]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Loads libtangle_hip.so once; raises RuntimeError when it is missing."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIBRARY_PATH.exists():
+                raise RuntimeError(
+                    f"{LIBRARY_PATH} is not built; run `python -m tangle_amd.build` "
+                    "(the HIP backend has no fallback)"
+                )
+            lib = C.CDLL(str(LIBRARY_PATH))
+            for name, (restype, argtypes) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = restype
+                fn.argtypes = argtypes
+            _lib = lib
+    return _lib
+
+
+class NativeError(Exception):
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+def check(code: int) -> None:
+    if code != TGH_OK:
+        message = library().tgh_last_error().decode("utf-8", "replace")
+        if code in (TGH_ERR_INVALID, TGH_ERR_SIZE):
+            raise ValueError(message)
+        raise RuntimeError(message)
+
+
+def ptr(array: np.ndarray, ctype):
+    return array.ctypes.data_as(C.POINTER(ctype))
+
+
+def default_config() -> Config:
+    config = Config()
+    library().tgh_config_default(C.byref(config))
+    return config
+
+
+class PackedAssembly:
+    """Holds contiguous arrays alive for a tgh_assembly view."""
+
+    def __init__(self, counts, placed, intrinsic, radius, min_bend_radius, origin, lengths, periodic,
+                 pin_fiber_ends=False, vertex_pinned=None):
+        self.counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        self.placed = np.ascontiguousarray(placed, dtype=np.float64).reshape(-1)
+        self.intrinsic = np.ascontiguousarray(intrinsic, dtype=np.float64).reshape(-1)
+        self.radius = np.ascontiguousarray(radius, dtype=np.float64)
+        self.min_bend_radius = (
+            None if min_bend_radius is None else np.ascontiguousarray(min_bend_radius, dtype=np.float64)
+        )
+        self.vertex_pinned = None if vertex_pinned is None else np.ascontiguousarray(vertex_pinned, dtype=np.uint8)
+        V = int(self.counts.sum())
+        if self.placed.size != 3 * V or self.intrinsic.size != 3 * V:
+            raise ValueError("placed/intrinsic must hold 3 coordinates per vertex")
+        if self.radius.size != self.counts.size:
+            raise ValueError("one radius per fiber is required")
+        if self.min_bend_radius is not None and self.min_bend_radius.size != self.counts.size:
+            raise ValueError("one min_bend_radius per fiber is required")
+        if self.vertex_pinned is not None and self.vertex_pinned.size != V:
+            raise ValueError("one pinned flag per vertex is required")
+        self.struct = Assembly()
+        self.struct.fiber_count = int(self.counts.size)
+        self.struct.vertex_count = V
+        self.struct.fiber_vertex_counts = ptr(self.counts, C.c_uint32)
+        self.struct.placed = ptr(self.placed, C.c_double)
+        self.struct.intrinsic = ptr(self.intrinsic, C.c_double)
+        self.struct.fiber_radius = ptr(self.radius, C.c_double)
+        self.struct.fiber_min_bend_radius = (
+            ptr(self.min_bend_radius, C.c_double) if self.min_bend_radius is not None else None
+        )
+        self.struct.vertex_pinned = ptr(self.vertex_pinned, C.c_uint8) if self.vertex_pinned is not None else None
+        for k in range(3):
+            self.struct.cell_origin[k] = float(origin[k])
+            self.struct.cell_lengths[k] = float(lengths[k])
+            self.struct.cell_periodic[k] = 1 if periodic[k] else 0
+        self.struct.pin_fiber_ends = 1 if pin_fiber_ends else 0
+
+    @property
+    def vertex_count(self) -> int:
+        return int(self.struct.vertex_count)
+
+
+class World:
+    """One resident device world (DeviceFiberWorld on the HIP backend)."""
+
+    def __init__(self, packed: PackedAssembly, config: Config):
+        lib = library()
+        self._packed = packed
+        handle = C.c_void_p()
+        check(lib.tgh_world_create(C.byref(packed.struct), C.byref(config), C.byref(handle)))
+        self._handle = handle
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            library().tgh_world_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run_batch(self, config: Config, iterations: int) -> BatchStatus:
+        status = BatchStatus()
+        check(library().tgh_world_run_batch(self._handle, C.byref(config), int(iterations), C.byref(status)))
+        return status
+
+    def info(self) -> WorldInfo:
+        info = WorldInfo()
+        check(library().tgh_world_info_get(self._handle, C.byref(info)))
+        return info
+
+    def download_positions(self) -> np.ndarray:
+        n = 3 * self._packed.vertex_count
+        out = np.empty(n, dtype=np.float32)
+        check(library().tgh_world_download_positions(self._handle, ptr(out, C.c_float), n))
+        return out
+
+    def upload_positions(self, positions: np.ndarray) -> None:
+        arr = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        check(library().tgh_world_upload_positions(self._handle, ptr(arr, C.c_float), arr.size))
+
+    def rebuild_lists(self) -> None:
+        check(library().tgh_world_rebuild_lists(self._handle))
+
+    def cell_list(self):
+        info = self.info()
+        counts = np.empty(info.cell_count, dtype=np.uint32)
+        offsets = np.empty(info.cell_count, dtype=np.uint32)
+        segments = np.empty(info.segment_count, dtype=np.uint32)
+        check(
+            library().tgh_world_cell_list(
+                self._handle, ptr(counts, C.c_uint32), ptr(offsets, C.c_uint32), ptr(segments, C.c_uint32),
+                counts.size, segments.size,
+            )
+        )
+        return counts, offsets, segments
+
+    def neighbor_lists(self):
+        info = self.info()
+        counts = np.empty(info.segment_count, dtype=np.uint32)
+        lists = np.zeros(max(1, info.segment_count * info.neighbor_block), dtype=np.uint32)
+        check(
+            library().tgh_world_neighbor_lists(self._handle, ptr(counts, C.c_uint32), ptr(lists, C.c_uint32), lists.size)
+        )
+        return counts, lists[: info.segment_count * info.neighbor_block].reshape(info.segment_count, -1)
+
+
+def generate_population(spec: PopulationSpec):
+    """Runs tgh_generate_population; returns (intrinsic, placed, radius)."""
+    nv = int(spec.segments_per_fiber) + 1
+    n = int(spec.count)
+    intrinsic = np.empty((n * nv, 3), dtype=np.float64)
+    placed = np.empty((n * nv, 3), dtype=np.float64)
+    radius = np.empty(n, dtype=np.float64)
+    check(
+        library().tgh_generate_population(
+            C.byref(spec), ptr(intrinsic, C.c_double), ptr(placed, C.c_double), ptr(radius, C.c_double), n * nv
+        )
+    )
+    return intrinsic, placed, radius

@@ -1,0 +1,88 @@
+// Device-side data layout shared by the relaxation kernels and the host world.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tgh {
+
+// Vertex flag bits (vB.z): neighbours along the fiber and pinning.
+constexpr uint32_t kHasPrev = 1u;   // previous segment exists
+constexpr uint32_t kHasNext = 2u;   // next segment exists
+constexpr uint32_t kHasPrev2 = 4u;  // second previous vertex exists (rest-chord bend)
+constexpr uint32_t kHasNext2 = 8u;  // second next vertex exists
+constexpr uint32_t kPinned = 16u;
+
+constexpr uint32_t kBuildThreads = 128;
+constexpr uint32_t kLdsListCapacity = 64;
+
+// Uniform-grid parameters: widths are (upper - lower) / cells computed once
+// in f32, exactly as the reference's proxy_cell computes them per call.
+struct GridParams {
+    float lower[3];
+    float upper[3];
+    float width[3];
+    float length[3];
+    uint32_t cells[3];
+    uint32_t periodic[3];
+};
+
+// Device-resident control block. `control` and `nstate` follow the
+// reference's buffers (world.rs:466-467, neighbor.rs:1-9): control =
+// {running, lifetime iterations, batch iterations, terminal failure},
+// nstate = {rebuild pending, rebuild count}. red_* hold u32 bit patterns of
+// nonnegative floats under atomicMax.
+struct DevState {
+    uint32_t control[4];
+    float metrics[3];
+    uint32_t nstate[2];
+    uint32_t red_pen;
+    uint32_t red_step;
+    uint32_t red_curv;
+    uint32_t done_blocks;
+    uint32_t pad[3];
+};
+
+// Device pointers and sizes of one world, passed to the launchers.
+struct WorldDev {
+    uint32_t F, V, S, C;
+    GridParams g;
+    float skin;
+    uint32_t block;  // neighbour-list slots per segment
+    uint32_t track_walls;
+    DevState *st;
+    float4 *pos, *alt, *refpos, *wall, *vA, *vB;
+    uint4 *sstat;
+    uint2 *fspan;
+    uint32_t *cell_counts, *cell_offsets, *cell_cursor, *cell_segs, *seg_cell, *scattered;
+    float4 *slotA, *slotB;
+    uint4 *slotC;
+    uint32_t *ncount, *nbr;
+    float4 *corr0, *corr1;
+};
+
+// Per-batch scalar settings of the relaxation loop (RelaxationConfig).
+struct IterParams {
+    float penetration_tolerance;
+    float curvature_ratio_tolerance;
+    float correction_fraction;
+    float stretch;
+    float bend;
+    float max_step;
+    float curvature_stiffness;
+    float curvature_margin;
+    uint32_t aggregation;
+    uint32_t constraint_iterations;
+    uint32_t sweeps;
+    uint32_t force_full;
+    uint32_t iterations;  // batch size, compared with control[2]
+};
+
+// Enqueue work on `stream`; every launcher checks its grid against the
+// world's sizes and returns a HIP error code (hipSuccess when enqueued).
+hipError_t launch_begin_batch(const WorldDev &w, hipStream_t stream);
+hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream);
+hipError_t launch_iteration(const WorldDev &w, const IterParams &p, hipStream_t stream);
+hipError_t launch_rebuild_now(const WorldDev &w, hipStream_t stream);
+
+}  // namespace tgh

@@ -1,0 +1,636 @@
+// Host side of the HIP relaxation world: packs a solver-neutral assembly into
+// the device layout, sizes the uniform grid and neighbour lists, owns the
+// device buffers and drives batches of relaxation iterations as captured
+// hipGraphs. Exposes the C ABI declared in include/tangle_hip.h.
+//
+// Reference behaviour followed here:
+//   packing        crates/tangle_relax/src/device/packed.rs:150-320, 527-580
+//   sizing         crates/tangle_relax/src/device/world.rs:322-417, 496-509, 868-917
+//   batch/status   crates/tangle_relax/src/device/world.rs:1020-1505, 2337-2369
+//   defaults       crates/tangle_relax/src/device/plugin.rs:145-175,
+//                  crates/tangle_contact/src/lib.rs:32-40
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tangle_hip.h"
+#include "errors.h"
+#include "relax_device.h"
+
+using tgh::DevState;
+using tgh::GridParams;
+using tgh::IterParams;
+using tgh::WorldDev;
+
+namespace {
+
+thread_local std::string g_error;
+
+}  // namespace
+
+int tgh::fail(int code, const std::string &message) {
+    g_error = message;
+    return code;
+}
+
+namespace {
+
+inline int set_error(int code, const std::string &message) { return tgh::fail(code, message); }
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            return set_error(TGH_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));  \
+    } while (0)
+
+constexpr size_t kReferenceMaxListBytes = size_t(120) << 20;  // world.rs:36
+
+inline float fclamp_rust(float x, float lo, float hi) {
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
+inline uint32_t to_u32_saturating(float x) {
+    if (std::isnan(x) || x <= 0.0f) return 0;
+    if (x >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)x;
+}
+
+// Host copy of the packed assembly in the reference's f32 representation.
+struct Packed {
+    uint32_t F = 0, V = 0, S = 0;
+    std::vector<float> pos;        // 3V
+    std::vector<float> intr;       // 3V
+    std::vector<uint32_t> fstart;  // F
+    std::vector<uint32_t> fcount;  // F
+    std::vector<float> radius;     // F
+    std::vector<float> rest;       // S
+    std::vector<float> kmax;       // V
+    std::vector<uint8_t> pinned;   // V
+    std::vector<float> arc;        // V
+    float lower[3], upper[3];
+    uint32_t periodic[3];
+};
+
+int pack_assembly(const tgh_assembly *a, Packed &p) {
+    if (!a) return set_error(TGH_ERR_INVALID, "assembly is null");
+    if (a->fiber_count == 0 || a->vertex_count == 0)
+        return set_error(TGH_ERR_INVALID, "assembly holds no fibers");
+    if (!a->fiber_vertex_counts || !a->placed || !a->intrinsic || !a->fiber_radius)
+        return set_error(TGH_ERR_INVALID, "assembly is missing an array");
+    for (int k = 0; k < 3; ++k)
+        if (!(a->cell_lengths[k] > 0.0) || !std::isfinite(a->cell_lengths[k]))
+            return set_error(TGH_ERR_INVALID, "cell lengths must be positive and finite");
+    p.F = a->fiber_count;
+    p.V = a->vertex_count;
+    uint64_t sum = 0;
+    p.fstart.resize(p.F);
+    p.fcount.resize(p.F);
+    for (uint32_t f = 0; f < p.F; ++f) {
+        uint32_t n = a->fiber_vertex_counts[f];
+        if (n < 2) return set_error(TGH_ERR_INVALID, "fiber " + std::to_string(f) + " has fewer than two vertices");
+        p.fstart[f] = (uint32_t)sum;
+        p.fcount[f] = n;
+        sum += n;
+    }
+    if (sum != p.V) return set_error(TGH_ERR_INVALID, "fiber vertex counts do not add up to vertex_count");
+    p.S = p.V - p.F;
+    p.pos.resize(3 * (size_t)p.V);
+    p.intr.resize(3 * (size_t)p.V);
+    for (size_t i = 0; i < 3 * (size_t)p.V; ++i) {
+        if (!std::isfinite(a->placed[i]) || !std::isfinite(a->intrinsic[i]))
+            return set_error(TGH_ERR_INVALID, "assembly positions must be finite");
+        p.pos[i] = (float)a->placed[i];
+        p.intr[i] = (float)a->intrinsic[i];
+    }
+    p.radius.resize(p.F);
+    p.rest.resize(p.S);
+    p.kmax.assign(p.V, 0.0f);
+    p.pinned.assign(p.V, 0);
+    for (uint32_t f = 0; f < p.F; ++f) {
+        double r = a->fiber_radius[f];
+        if (!(r > 0.0) || !std::isfinite(r))
+            return set_error(TGH_ERR_INVALID, "fiber " + std::to_string(f) + " radius must be positive");
+        p.radius[f] = (float)r;
+        const uint32_t st = p.fstart[f], n = p.fcount[f];
+        for (uint32_t l = 0; l + 1 < n; ++l) {
+            const double *x0 = &a->intrinsic[3 * (size_t)(st + l)];
+            const double *x1 = &a->intrinsic[3 * (size_t)(st + l + 1)];
+            double dx = x1[0] - x0[0], dy = x1[1] - x0[1], dz = x1[2] - x0[2];
+            p.rest[st - f + l] = (float)std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        double limit = a->fiber_min_bend_radius ? a->fiber_min_bend_radius[f] : 0.0;
+        if (limit > 0.0 && std::isfinite(limit)) {
+            float k = (float)(1.0 / limit);
+            for (uint32_t l = 1; l + 1 < n; ++l) p.kmax[st + l] = k;
+        }
+        if (a->pin_fiber_ends) {
+            p.pinned[st] = 1;
+            p.pinned[st + n - 1] = 1;
+        }
+    }
+    if (a->vertex_pinned)
+        for (uint32_t v = 0; v < p.V; ++v) p.pinned[v] |= a->vertex_pinned[v] ? 1 : 0;
+    // rest arc length along the intrinsic centreline, f64 sums of f32 data
+    p.arc.assign(p.V, 0.0f);
+    for (uint32_t f = 0; f < p.F; ++f) {
+        double total = 0.0;
+        for (uint32_t v = p.fstart[f] + 1; v < p.fstart[f] + p.fcount[f]; ++v) {
+            double sq = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                double d = (double)p.intr[3 * (size_t)v + k] - (double)p.intr[3 * (size_t)(v - 1) + k];
+                sq += d * d;
+            }
+            total += std::sqrt(sq);
+            p.arc[v] = (float)total;
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        p.lower[k] = (float)a->cell_origin[k];
+        p.upper[k] = (float)(a->cell_origin[k] + a->cell_lengths[k]);
+        p.periodic[k] = a->cell_periodic[k] ? 1u : 0u;
+    }
+    return TGH_OK;
+}
+
+struct Sizing {
+    float skin = 0.f, cell_size = 0.f;
+    uint32_t cells[3] = {1, 1, 1};
+    uint32_t C = 1;
+    size_t slots = 1;
+    uint32_t block = 48;
+};
+
+int validate_config(const tgh_config *c) {
+    if (!c) return set_error(TGH_ERR_INVALID, "config is null");
+    if (!(c->cell_size_scale >= 1.0f) || !std::isfinite(c->cell_size_scale))
+        return set_error(TGH_ERR_INVALID, "cell_size_scale must be finite and at least 1");
+    if (!std::isfinite(c->neighbor_skin_scale) || c->neighbor_skin_scale < 0.0f)
+        return set_error(TGH_ERR_INVALID, "neighbor_skin_scale must be finite and non-negative");
+    if (c->neighbor_capacity == 0) return set_error(TGH_ERR_INVALID, "neighbor_capacity must be at least 1");
+    if (!(c->max_step > 0.0f) || !std::isfinite(c->max_step))
+        return set_error(TGH_ERR_INVALID, "max_step must be positive and finite");
+    if (!std::isfinite(c->penetration_tolerance) || c->penetration_tolerance < 0.0f)
+        return set_error(TGH_ERR_INVALID, "penetration_tolerance must be nonnegative and finite");
+    if (c->contact_aggregation > 2) return set_error(TGH_ERR_INVALID, "contact_aggregation must be 0, 1 or 2");
+    return TGH_OK;
+}
+
+float segment_length_now(const Packed &p, uint32_t f, uint32_t l) {
+    const float *a = &p.pos[3 * (size_t)(p.fstart[f] + l)];
+    const float *b = a + 3;
+    float dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+    return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+int size_grid(const Packed &p, const tgh_config *c, Sizing &z) {
+    float max_rest = 0.f, max_init = 0.f, max_r = 0.f, max_leaf = 0.f;
+    for (uint32_t f = 0; f < p.F; ++f) {
+        max_r = std::fmax(max_r, p.radius[f]);
+        for (uint32_t l = 0; l + 1 < p.fcount[f]; ++l) {
+            float rest = p.rest[p.fstart[f] - f + l];
+            float now = segment_length_now(p, f, l);
+            max_rest = std::fmax(max_rest, rest);
+            max_init = std::fmax(max_init, now);
+            max_leaf = std::fmax(max_leaf, std::fmax(rest, now));
+        }
+    }
+    const float max_len = std::fmax(max_rest, max_init);
+    z.skin = c->neighbor_skin_scale * max_r;
+    const float margin = 2.0f * max_r + std::fmax(2.0f * c->max_step, z.skin);
+    const float block_len = std::fmin(max_len, std::fmax(max_leaf, 2.0f * margin));
+    // A segment longer than one list block would be binned as several proxy
+    // pieces; with a uniform (non-adaptive) layout every segment is at most
+    // the longest one, so this never triggers, but guard it all the same.
+    for (uint32_t f = 0; f < p.F; ++f)
+        for (uint32_t l = 0; l + 1 < p.fcount[f]; ++l) {
+            float len = std::fmax(p.rest[p.fstart[f] - f + l], segment_length_now(p, f, l));
+            if (to_u32_saturating(std::ceil(len / block_len)) > 1)
+                return set_error(TGH_ERR_INVALID, "segments split into proxy pieces are not supported");
+        }
+    const float piece = block_len < max_len ? 1.05f * block_len : block_len;
+    z.cell_size = (piece + margin) * c->cell_size_scale;
+    uint64_t C = 1;
+    for (int k = 0; k < 3; ++k) {
+        float extent = p.upper[k] - p.lower[k];
+        z.cells[k] = to_u32_saturating(std::fmax(std::floor(extent / z.cell_size), 1.0f));
+        C *= z.cells[k];
+    }
+    if (C > (uint64_t(1) << 28)) return set_error(TGH_ERR_INVALID, "cell grid exceeds 2^28 cells");
+    z.C = (uint32_t)C;
+    const size_t max_bytes = c->maximum_neighbor_list_bytes ? c->maximum_neighbor_list_bytes : kReferenceMaxListBytes;
+    z.slots = std::max<size_t>(1, std::min<size_t>((size_t)c->neighbor_capacity * p.S, max_bytes / 4));
+    z.block = (uint32_t)std::min<size_t>(c->neighbor_capacity, z.slots / std::max<uint32_t>(1, p.S));
+    return TGH_OK;
+}
+
+IterParams iter_params(const tgh_config *c, uint32_t iterations) {
+    IterParams q;
+    q.penetration_tolerance = c->penetration_tolerance;
+    q.curvature_ratio_tolerance = c->curvature_ratio_tolerance;
+    q.correction_fraction = c->correction_fraction;
+    q.stretch = c->stretch_stiffness;
+    q.bend = c->bend_stiffness;
+    q.max_step = c->max_step;
+    q.curvature_stiffness = c->curvature_limit_stiffness;
+    q.curvature_margin = c->curvature_limit_safety_margin;
+    q.aggregation = c->contact_aggregation;
+    q.constraint_iterations = c->constraint_iterations;
+    q.sweeps = c->curvature_cleanup_sweeps;
+    q.force_full = c->force_full_iterations ? 1u : 0u;
+    q.iterations = iterations;
+    return q;
+}
+
+}  // namespace
+
+struct tgh_world {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Packed host;
+    Sizing size;
+    WorldDev d{};
+    DevState *host_state = nullptr;  // pinned
+    uint64_t device_bytes = 0;
+    std::vector<void *> allocations;
+    std::map<std::string, hipGraphExec_t> graphs;
+    std::vector<hipGraph_t> graph_templates;
+    bool use_graphs = true;
+
+    template <typename T>
+    hipError_t alloc(T **out, size_t count) {
+        size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+        hipError_t e = hipMalloc((void **)out, bytes);
+        if (e == hipSuccess) {
+            allocations.push_back(*out);
+            device_bytes += bytes;
+        }
+        return e;
+    }
+
+    ~tgh_world() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (hipGraph_t g : graph_templates) (void)hipGraphDestroy(g);
+        for (void *p : allocations) (void)hipFree(p);
+        if (host_state) (void)hipHostFree(host_state);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int upload_static(tgh_world *w) {
+    const Packed &p = w->host;
+    WorldDev &d = w->d;
+    const uint32_t V = p.V, S = p.S, F = p.F;
+    std::vector<float4> pos4(V), vA(V), vB(V);
+    std::vector<uint4> sstat(S);
+    std::vector<uint2> fspan(F);
+    for (uint32_t f = 0; f < F; ++f) {
+        const uint32_t st = p.fstart[f], n = p.fcount[f];
+        fspan[f] = make_uint2(st, n);
+        for (uint32_t l = 0; l < n; ++l) {
+            const uint32_t v = st + l;
+            pos4[v] = make_float4(p.pos[3 * (size_t)v], p.pos[3 * (size_t)v + 1], p.pos[3 * (size_t)v + 2], p.arc[v]);
+            uint32_t flags = 0;
+            float rest_prev = 0.f, rest_next = 0.f, chord_prev = 0.f, chord_next = 0.f;
+            if (l > 0) {
+                flags |= tgh::kHasPrev;
+                rest_prev = p.rest[v - f - 1];
+            }
+            if (l + 1 < n) {
+                flags |= tgh::kHasNext;
+                rest_next = p.rest[v - f];
+            }
+            // rest chords to the second neighbours, f32 as in kernels.rs:1098-1101
+            auto chord = [&](uint32_t other) {
+                const float *I = p.intr.data();
+                float rx = I[3 * (size_t)other] - I[3 * (size_t)v];
+                float ry = I[3 * (size_t)other + 1] - I[3 * (size_t)v + 1];
+                float rz = I[3 * (size_t)other + 2] - I[3 * (size_t)v + 2];
+                return std::sqrt(rx * rx + ry * ry + rz * rz);
+            };
+            if (l >= 2) {
+                flags |= tgh::kHasPrev2;
+                chord_prev = chord(v - 2);
+            }
+            if (l + 2 < n) {
+                flags |= tgh::kHasNext2;
+                chord_next = chord(v + 2);
+            }
+            if (p.pinned[v]) flags |= tgh::kPinned;
+            float flag_bits, fiber_bits;
+            std::memcpy(&flag_bits, &flags, 4);
+            std::memcpy(&fiber_bits, &f, 4);
+            vA[v] = make_float4(p.kmax[v], rest_prev, rest_next, p.radius[f]);
+            vB[v] = make_float4(chord_prev, chord_next, flag_bits, fiber_bits);
+        }
+        for (uint32_t l = 0; l + 1 < n; ++l) {
+            uint32_t s = st - f + l;
+            uint32_t rbits;
+            std::memcpy(&rbits, &p.radius[f], 4);
+            sstat[s] = make_uint4(st + l, f, rbits, 0u);
+        }
+    }
+    HIP_TRY(hipMemcpy(d.pos, pos4.data(), V * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.refpos, pos4.data(), V * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.alt, pos4.data(), V * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.vA, vA.data(), V * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.vB, vB.data(), V * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.sstat, sstat.data(), S * sizeof(uint4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.fspan, fspan.data(), F * sizeof(uint2), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d.wall, 0, V * sizeof(float4)));
+    HIP_TRY(hipMemset(d.corr0, 0, S * sizeof(float4)));
+    HIP_TRY(hipMemset(d.corr1, 0, S * sizeof(float4)));
+    HIP_TRY(hipMemset(d.ncount, 0, S * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(d.cell_counts, 0, d.C * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(d.cell_offsets, 0, d.C * sizeof(uint32_t)));
+    DevState init;
+    std::memset(&init, 0, sizeof(init));
+    init.metrics[0] = std::numeric_limits<float>::infinity();
+    init.metrics[2] = std::numeric_limits<float>::infinity();
+    init.nstate[0] = 1;  // lists start stale so the first contact pass builds them (world.rs:510-511)
+    HIP_TRY(hipMemcpy(d.st, &init, sizeof(init), hipMemcpyHostToDevice));
+    return TGH_OK;
+}
+
+int create_world(const tgh_assembly *a, const tgh_config *c, tgh_world **out) {
+    int rc = validate_config(c);
+    if (rc) return rc;
+    tgh_world *w = new tgh_world();
+    rc = pack_assembly(a, w->host);
+    if (!rc) rc = size_grid(w->host, c, w->size);
+    if (rc) {
+        delete w;
+        return rc;
+    }
+    const char *env = std::getenv("TANGLE_HIP_GRAPHS");
+    if (env && env[0] == '0') w->use_graphs = false;
+    auto fail = [&](int code) {
+        delete w;
+        return code;
+    };
+    hipError_t e = hipGetDevice(&w->device);
+    if (e != hipSuccess) return fail(set_error(TGH_ERR_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e)));
+    e = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(set_error(TGH_ERR_DEVICE, hipGetErrorString(e)));
+    e = hipHostMalloc((void **)&w->host_state, sizeof(DevState), hipHostMallocDefault);
+    if (e != hipSuccess) return fail(set_error(TGH_ERR_DEVICE, hipGetErrorString(e)));
+    const Packed &p = w->host;
+    WorldDev &d = w->d;
+    d.F = p.F;
+    d.V = p.V;
+    d.S = p.S;
+    d.C = w->size.C;
+    d.skin = w->size.skin;
+    d.block = w->size.block;
+    d.track_walls = (p.periodic[0] && p.periodic[1] && p.periodic[2]) ? 0u : 1u;
+    for (int k = 0; k < 3; ++k) {
+        d.g.lower[k] = p.lower[k];
+        d.g.upper[k] = p.upper[k];
+        d.g.cells[k] = w->size.cells[k];
+        d.g.periodic[k] = p.periodic[k];
+        d.g.width[k] = (p.upper[k] - p.lower[k]) / (float)w->size.cells[k];
+        d.g.length[k] = p.upper[k] - p.lower[k];
+    }
+    const size_t V = p.V, S = p.S, C = w->size.C;
+    e = hipSuccess;
+    if (e == hipSuccess) e = w->alloc(&d.st, 1);
+    if (e == hipSuccess) e = w->alloc(&d.pos, V);
+    if (e == hipSuccess) e = w->alloc(&d.alt, V);
+    if (e == hipSuccess) e = w->alloc(&d.refpos, V);
+    if (e == hipSuccess) e = w->alloc(&d.wall, V);
+    if (e == hipSuccess) e = w->alloc(&d.vA, V);
+    if (e == hipSuccess) e = w->alloc(&d.vB, V);
+    if (e == hipSuccess) e = w->alloc(&d.sstat, S);
+    if (e == hipSuccess) e = w->alloc(&d.fspan, p.F);
+    if (e == hipSuccess) e = w->alloc(&d.cell_counts, C);
+    if (e == hipSuccess) e = w->alloc(&d.cell_offsets, C);
+    if (e == hipSuccess) e = w->alloc(&d.cell_cursor, C);
+    if (e == hipSuccess) e = w->alloc(&d.cell_segs, S);
+    if (e == hipSuccess) e = w->alloc(&d.seg_cell, S);
+    if (e == hipSuccess) e = w->alloc(&d.scattered, S);
+    if (e == hipSuccess) e = w->alloc(&d.slotA, S);
+    if (e == hipSuccess) e = w->alloc(&d.slotB, S);
+    if (e == hipSuccess) e = w->alloc(&d.slotC, S);
+    if (e == hipSuccess) e = w->alloc(&d.ncount, S);
+    if (e == hipSuccess) e = w->alloc(&d.nbr, std::max<size_t>(1, (size_t)d.block * S));
+    if (e == hipSuccess) e = w->alloc(&d.corr0, S);
+    if (e == hipSuccess) e = w->alloc(&d.corr1, S);
+    if (e != hipSuccess) return fail(set_error(TGH_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e)));
+    rc = upload_static(w);
+    if (rc) return fail(rc);
+    *out = w;
+    return TGH_OK;
+}
+
+std::string graph_key(const IterParams &q) {
+    std::string k(sizeof(q), '\0');
+    std::memcpy(&k[0], &q, sizeof(q));
+    return k;
+}
+
+int enqueue_batch(tgh_world *w, const IterParams &q) {
+    hipError_t e = tgh::launch_begin_batch(w->d, w->stream);
+    // begin_relaxation_batch is followed by the displacement check
+    // (world.rs:1033-1035); between batches nothing moves the positions
+    // except tgh_world_upload_positions, which requests a rebuild itself.
+    for (uint32_t step = 0; e == hipSuccess && step <= q.iterations; ++step)
+        e = tgh::launch_iteration(w->d, q, w->stream);
+    if (e != hipSuccess) return set_error(TGH_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    return TGH_OK;
+}
+
+int run_batch(tgh_world *w, const tgh_config *c, uint32_t iterations, tgh_batch_status *st) {
+    int rc = validate_config(c);
+    if (rc) return rc;
+    if (iterations == 0) return set_error(TGH_ERR_INVALID, "iterations must be positive");
+    if (c->neighbor_capacity != 0 && c->neighbor_skin_scale != c->neighbor_skin_scale)
+        return set_error(TGH_ERR_INVALID, "neighbor_skin_scale is NaN");
+    const IterParams q = iter_params(c, iterations);
+    if (w->use_graphs) {
+        const std::string key = graph_key(q);
+        auto it = w->graphs.find(key);
+        if (it == w->graphs.end()) {
+            HIP_TRY(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+            int erc = enqueue_batch(w, q);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(w->stream, &graph);
+            if (erc) return erc;
+            if (e != hipSuccess) return set_error(TGH_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+            hipGraphExec_t exec = nullptr;
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                return set_error(TGH_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+            }
+            w->graph_templates.push_back(graph);
+            it = w->graphs.emplace(key, exec).first;
+        }
+        HIP_TRY(hipGraphLaunch(it->second, w->stream));
+    } else {
+        rc = enqueue_batch(w, q);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(w->host_state, w->d.st, sizeof(DevState), hipMemcpyDeviceToHost, w->stream));
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    const DevState &s = *w->host_state;
+    st->total_iterations = s.control[1];
+    st->batch_iterations = s.control[2];
+    st->max_penetration = s.metrics[0];
+    st->max_displacement = s.metrics[1];
+    st->max_curvature_ratio = s.metrics[2];
+    const bool overflow = s.control[3] != 0;
+    st->cell_list_overflow = overflow ? 1u : 0u;
+    st->converged = (!c->force_full_iterations && s.metrics[0] <= c->penetration_tolerance &&
+                     s.metrics[2] <= 1.0f + c->curvature_ratio_tolerance && !overflow)
+                        ? 1u
+                        : 0u;
+    return TGH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void tgh_config_default(tgh_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->penetration_tolerance = 1.0e-4f;
+    c->correction_fraction = 0.8f;
+    c->stretch_stiffness = 0.35f;
+    c->bend_stiffness = 0.03f;
+    c->curvature_limit_stiffness = 1.0f;
+    c->curvature_limit_safety_margin = 1.0e-3f;
+    c->curvature_ratio_tolerance = 1.0e-5f;
+    c->max_step = 0.006f;
+    c->cell_size_scale = 1.0f;
+    c->neighbor_skin_scale = 2.0f;
+    c->contact_aggregation = 0;
+    c->constraint_iterations = 2;
+    c->curvature_cleanup_sweeps = 4;
+    c->force_full_iterations = 0;
+    c->neighbor_capacity = 48;
+    c->maximum_neighbor_list_bytes = 0;
+}
+
+int tgh_world_create(const tgh_assembly *assembly, const tgh_config *config, tgh_world **out) {
+    if (!out) return set_error(TGH_ERR_INVALID, "output pointer is null");
+    *out = nullptr;
+    return create_world(assembly, config, out);
+}
+
+int tgh_world_run_batch(tgh_world *world, const tgh_config *config, uint32_t iterations, tgh_batch_status *status) {
+    if (!world || !status) return set_error(TGH_ERR_INVALID, "world or status is null");
+    return run_batch(world, config, iterations, status);
+}
+
+int tgh_world_download_positions(const tgh_world *world, float *out, size_t count) {
+    if (!world || !out) return set_error(TGH_ERR_INVALID, "null argument");
+    const size_t V = world->d.V;
+    if (count < 3 * V) return set_error(TGH_ERR_SIZE, "position buffer holds fewer than 3*vertex_count floats");
+    std::vector<float4> tmp(V);
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    HIP_TRY(hipMemcpy(tmp.data(), world->d.pos, V * sizeof(float4), hipMemcpyDeviceToHost));
+    for (size_t v = 0; v < V; ++v) {
+        out[3 * v] = tmp[v].x;
+        out[3 * v + 1] = tmp[v].y;
+        out[3 * v + 2] = tmp[v].z;
+    }
+    return TGH_OK;
+}
+
+int tgh_world_upload_positions(tgh_world *world, const float *positions, size_t count) {
+    if (!world || !positions) return set_error(TGH_ERR_INVALID, "null argument");
+    const size_t V = world->d.V;
+    if (count != 3 * V) return set_error(TGH_ERR_SIZE, "expected 3*vertex_count floats");
+    std::vector<float4> tmp(V);
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    HIP_TRY(hipMemcpy(tmp.data(), world->d.pos, V * sizeof(float4), hipMemcpyDeviceToHost));
+    for (size_t v = 0; v < V; ++v) {
+        tmp[v].x = positions[3 * v];
+        tmp[v].y = positions[3 * v + 1];
+        tmp[v].z = positions[3 * v + 2];
+    }
+    HIP_TRY(hipMemcpy(world->d.pos, tmp.data(), V * sizeof(float4), hipMemcpyHostToDevice));
+    DevState s;
+    HIP_TRY(hipMemcpy(&s, world->d.st, sizeof(s), hipMemcpyDeviceToHost));
+    s.nstate[0] = 1;
+    HIP_TRY(hipMemcpy(world->d.st, &s, sizeof(s), hipMemcpyHostToDevice));
+    return TGH_OK;
+}
+
+int tgh_world_info_get(const tgh_world *world, tgh_world_info *info) {
+    if (!world || !info) return set_error(TGH_ERR_INVALID, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->fiber_count = world->d.F;
+    info->segment_count = world->d.S;
+    info->vertex_count = world->d.V;
+    for (int k = 0; k < 3; ++k) info->cells[k] = world->d.g.cells[k];
+    info->cell_count = world->d.C;
+    info->cell_size = world->size.cell_size;
+    info->neighbor_skin = world->d.skin;
+    info->neighbor_block = world->d.block;
+    DevState s;
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    HIP_TRY(hipMemcpy(&s, world->d.st, sizeof(s), hipMemcpyDeviceToHost));
+    info->neighbor_list_rebuilds = s.nstate[1];
+    info->device_bytes = world->device_bytes;
+    return TGH_OK;
+}
+
+int tgh_world_cell_list(const tgh_world *world, uint32_t *counts, uint32_t *offsets, uint32_t *segments,
+                        size_t cell_capacity, size_t segment_capacity) {
+    if (!world || !counts || !offsets || !segments) return set_error(TGH_ERR_INVALID, "null argument");
+    if (cell_capacity < world->d.C || segment_capacity < world->d.S)
+        return set_error(TGH_ERR_SIZE, "cell-list buffers too small");
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    HIP_TRY(hipMemcpy(counts, world->d.cell_counts, world->d.C * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(offsets, world->d.cell_offsets, world->d.C * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(segments, world->d.cell_segs, (size_t)world->d.S * 4, hipMemcpyDeviceToHost));
+    return TGH_OK;
+}
+
+int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t *lists, size_t list_capacity) {
+    if (!world || !counts || !lists) return set_error(TGH_ERR_INVALID, "null argument");
+    const size_t S = world->d.S, B = world->d.block;
+    if (list_capacity < S * B) return set_error(TGH_ERR_SIZE, "list buffer smaller than segment_count*neighbor_block");
+    std::vector<uint32_t> slot_seg(S), slot_count(S), cols(std::max<size_t>(1, S * B));
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    HIP_TRY(hipMemcpy(slot_seg.data(), world->d.cell_segs, S * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(slot_count.data(), world->d.ncount, S * 4, hipMemcpyDeviceToHost));
+    if (S * B) HIP_TRY(hipMemcpy(cols.data(), world->d.nbr, S * B * 4, hipMemcpyDeviceToHost));
+    std::memset(lists, 0, S * B * 4);
+    for (size_t slot = 0; slot < S; ++slot) {
+        const uint32_t seg = slot_seg[slot];
+        counts[seg] = slot_count[slot];
+        const size_t keep = std::min<size_t>(slot_count[slot], B);
+        for (size_t k = 0; k < keep; ++k) lists[(size_t)seg * B + k] = cols[k * S + slot];
+    }
+    return TGH_OK;
+}
+
+int tgh_world_rebuild_lists(tgh_world *world) {
+    if (!world) return set_error(TGH_ERR_INVALID, "world is null");
+    hipError_t e = tgh::launch_rebuild_now(world->d, world->stream);
+    if (e != hipSuccess) return set_error(TGH_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    return TGH_OK;
+}
+
+void tgh_world_destroy(tgh_world *world) { delete world; }
+
+const char *tgh_last_error(void) { return g_error.c_str(); }
+
+}  // extern "C"

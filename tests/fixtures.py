@@ -1,0 +1,153 @@
+"""Assemblies and settings shared by the tests and the benchmark.
+
+The hand-built assemblies restate the fixtures of the reference's device-world
+tests (crates/tangle_relax/src/device/world_tests.rs); each builder cites the
+test it follows. `population_case` builds the benchmark workloads from the
+seeded generator.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+import tangle_amd as tg
+from tangle_amd import _native
+
+
+def config(**changes) -> _native.Config:
+    """RelaxationConfig::default() (plugin.rs:145-175) with `changes` applied."""
+    c = _native.default_config()
+    for key, value in changes.items():
+        setattr(c, key, value)
+    return c
+
+
+def packed(fibers, radius, lengths, periodic, *, rest=None, pin_fiber_ends=False, bend_radius=None,
+           vertex_pinned=None, origin=(0.0, 0.0, 0.0)):
+    """fibers: list of (n_i, 3) point lists. `rest` defaults to the placed points."""
+    counts = [len(f) for f in fibers]
+    placed = np.concatenate([np.asarray(f, dtype=np.float64) for f in fibers])
+    intrinsic = placed.copy() if rest is None else np.concatenate([np.asarray(f, dtype=np.float64) for f in rest])
+    radii = np.full(len(fibers), radius) if np.isscalar(radius) else np.asarray(radius)
+    bend = None
+    if bend_radius is not None:
+        bend = np.full(len(fibers), bend_radius) if np.isscalar(bend_radius) else np.asarray(bend_radius)
+    return _native.PackedAssembly(counts, placed, intrinsic, radii, bend, origin, lengths, periodic,
+                                  pin_fiber_ends=pin_fiber_ends, vertex_pinned=vertex_pinned)
+
+
+# world_tests.rs:8-21
+def crossed(pin_fiber_ends=False):
+    return packed([[[1.0, 2.0, 2.0], [3.0, 2.0, 2.0]], [[2.0, 1.0, 2.0], [2.0, 3.0, 2.0]]], 0.1, [4.0] * 3,
+                  [False] * 3, pin_fiber_ends=pin_fiber_ends)
+
+
+# world_tests.rs:283-300: twenty 0.05-long segments of a 0.2-diameter fiber
+def short_segments():
+    return packed([[[1.0 + 0.05 * i, 2.0, 2.0] for i in range(21)]], 0.1, [4.0] * 3, [False] * 3)
+
+
+# world_tests.rs:303-332: a hairpin, legs 0.1 apart
+def hairpin():
+    pts = [[1.0 + 0.1 * i, 2.0, 2.0] for i in range(16)] + [[1.0 + 0.1 * i, 2.1, 2.0] for i in reversed(range(16))]
+    return packed([pts], 0.1, [4.0] * 3, [False] * 3)
+
+
+# world_tests.rs:430-458: three segments in one high-index cell of a long box
+def dense_cell():
+    pts = [[39.0, 0.1, 0.1], [39.01, 0.1, 0.1]]
+    return packed([pts, pts, pts], 0.02, [40.0, 0.2, 0.2], [False] * 3)
+
+
+# world_tests.rs:461-509 (pin_fiber_ends) and :512-554 (every vertex pinned)
+def bent_triplet(pin_all=False):
+    rest = [[[-0.15, 0.0, 0.0], [0.0, 0.0, 0.0], [0.15, 0.0, 0.0]]]
+    placed = [[[0.8, 1.0, 1.0], [1.0, 1.15, 1.0], [1.2, 1.0, 1.0]]]
+    return packed(placed, 0.01, [2.0] * 3, [False] * 3, rest=rest, pin_fiber_ends=not pin_all, bend_radius=0.25,
+                  vertex_pinned=np.ones(3, dtype=np.uint8) if pin_all else None)
+
+
+# world_tests.rs:934-969: minimum-image contact across a periodic x face
+def periodic_pair():
+    fibers = [[[x, 1.5, 2.0], [x, 2.5, 2.0]] for x in (0.05, 3.95)]
+    return packed(fibers, 0.1, [4.0] * 3, [True, False, False])
+
+
+# world_tests.rs:972-996: SI-scale interior crossing
+def si_crossing():
+    fibers = [[[0.0006, 0.001, 0.001], [0.0014, 0.001, 0.001]], [[0.001, 0.0006, 0.001], [0.001, 0.0014, 0.001]]]
+    return packed(fibers, 10.0e-6, [0.002] * 3, [False] * 3)
+
+
+# world_tests.rs:999-1026: a fiber several box images away
+def unwrapped_images():
+    fibers = [[[0.0005, 0.0006, 0.001], [0.0005, 0.0014, 0.001]], [[0.0051, 0.001, 0.001], [0.0059, 0.001, 0.001]]]
+    return packed(fibers, 10.0e-6, [0.0025, 0.0025, 0.002], [True, True, False])
+
+
+# world_tests.rs:1029-1064: self-contact through the periodic image
+def self_image():
+    pts = [[0.05, 1.0, 2.0], [0.05, 1.5, 2.0], [2.0, 1.5, 2.0], [3.95, 1.5, 2.0], [3.95, 1.0, 2.0]]
+    return packed([pts], 0.1, [4.0] * 3, [True, False, False])
+
+
+# world_tests.rs:1126-1157: pinned 81-vertex zigzag with a bend limit
+def zigzag():
+    rest = [[[0.02 * i, 0.0, 0.0] for i in range(81)]]
+    placed = [[[0.2 + 0.02 * i, 1.0 + (0.0 if i % 2 == 0 else 0.01), 1.0] for i in range(81)]]
+    return packed(placed, 0.005, [2.0] * 3, [False] * 3, rest=rest, pin_fiber_ends=True, bend_radius=0.3)
+
+
+# world_tests.rs:1262-1287: 24 crossing fibers in a periodic-xy mat
+def dense_mat():
+    fibers = []
+    for index in range(12):
+        offset = 0.05 + 0.18 * index
+        wobble = 0.03 * (index % 3)
+        fibers.append([[0.2 + 0.25 * p, offset, 0.45 + wobble] for p in range(9)])
+        fibers.append([[offset + 0.07, 0.2 + 0.25 * p, 0.55 - wobble] for p in range(9)])
+    return packed(fibers, 0.1, [2.4, 2.4, 1.0], [True, True, False])
+
+
+# ---- benchmark workloads ---------------------------------------------------
+
+def config1_recipe(count=40, seed=1):
+    """40 straight fibers, d = 10 um, L = 300 um, 30 segments, 300 um periodic cube."""
+    um = tg.um
+    cell = tg.Cell([300 * um] * 3, periodic="xyz")
+    fiber = tg.Material("fiber", diameter=10 * um)
+    pop = tg.FiberPopulation(material=fiber, count=count, length=300 * um, segments_per_fiber=30,
+                             curvature_amplitude=0.0, seed=seed)
+    recipe = tg.Recipe(cell)
+    recipe.insert(tg.generate_fiber_population(cell, pop))
+    return recipe
+
+
+# The bimodal felt specimen: 1,763 fibers of 7 um and 237 of 19 um (equal
+# volume), 1 mm long in a 1 mm periodic cube, segment length ~2 d, planar
+# bias about z. Schladitz beta = 1/0.3 in the reference's fit convention
+# (beta > 1 lays fibers in the plane). Bend limits and seeds follow the
+# reference's felt example (examples/felt_20ply_control_vs_needled/src/recipe.rs:397-420).
+BIMODAL_FINE = dict(diameter=7e-6, count=1763, segments=71, bend=200e-6, seed=20_260_921)
+BIMODAL_COARSE = dict(diameter=19e-6, count=237, segments=26, bend=60e-6, seed=20_260_922)
+
+
+def bimodal_recipe(scale=1.0, box=1e-3, length=1e-3, beta=1.0 / 0.3):
+    """scale < 1 keeps the volume fraction and shrinks the box and fiber counts."""
+    side = box * scale ** (1.0 / 3.0) if scale != 1.0 else box
+    cell = tg.Cell([side] * 3, periodic="xyz")
+    recipe = tg.Recipe(cell)
+    for spec in (BIMODAL_FINE, BIMODAL_COARSE):
+        material = tg.Material("fine" if spec is BIMODAL_FINE else "coarse", spec["diameter"], spec["bend"])
+        count = max(1, int(round(spec["count"] * scale)))
+        flen = min(length, side)
+        segments = max(2, int(round(spec["segments"] * flen / length)))
+        pop = tg.FiberPopulation(material=material, count=count, length=flen, segments_per_fiber=segments,
+                                 seed=spec["seed"], orientation=tg.SchladitzOrientation(beta=beta, axis="z"))
+        recipe.insert(tg.generate_fiber_population(cell, pop))
+    return recipe
+
+
+def hard_gate_settings(d_min, **changes):
+    """Hard admissibility: penetration <= 1e-3 d_min, curvature ratio <= 1 + 1e-5."""
+    return tg.RelaxationSettings(penetration_tolerance=1e-3 * d_min, curvature_ratio_tolerance=1e-5, **changes)
