@@ -1,0 +1,228 @@
+"""Benchmark: relaxation throughput and time to hard admissibility on one MI355X.
+
+Workload: the bimodal felt specimen — 2,000 fibers 1 mm long in a 1 mm
+periodic cube, 1,763 of 7 um and 237 of 19 um diameter (equal volume,
+13.5 % fill), segment length ~2 d (131,335 segments), planar-biased
+(Schladitz beta = 1/0.3 about z in the convention where beta > 1 lays fibers
+flat), bend limits 200 um / 60 um, fp32 positions, seeded synthetic input.
+
+A step is one relaxation batch of 128 iterations (iterations_per_batch)
+starting from the generated (unrelaxed) specimen, run with
+force_full_iterations so every iteration of the step executes. The timed
+region is K steps; positions are reset to the generated specimen before it,
+outside the timing. value = segment-iterations per second over the timed
+region (segments x iterations / seconds).
+
+Also reported: the wall time for Recipe.run to reach the hard admissibility
+gates (max penetration <= 1e-3 d_min, max curvature ratio <= 1 + 1e-5), the
+HBM roofline of the dominant kernel from HIP events captured in the timed
+batches, and the CPU baseline (the C++ restatement of the reference loop,
+OpenMP on the host cores) on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "segment-iters/s + time-to-hard-admissibility, learned vs plain, 1 MI355X"
+PEAK_HBM_GBS = 8000.0
+GROUPS = ["rebuild", "contact", "measure_assess", "vertex_passes", "curvature_gs"]
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8, help="timed relaxation batches of 128 iterations")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=128, help="iterations per step")
+    ap.add_argument("--tta-max-iterations", type=int, default=20000)
+    ap.add_argument("--skip-tta", action="store_true", help="skip the time-to-admissibility run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-iterations", type=int, default=40, help="iterations in the CPU baseline sample")
+    ap.add_argument("--scale", type=float, default=1.0, help="specimen scale (1.0 = the 2,000-fiber workload)")
+    ap.add_argument("--dump", type=str, default="", help="write final tta positions (.npy) here")
+    return ap.parse_args()
+
+
+def contact_bytes_per_launch(S: int, V: int, mean_list: float) -> float:
+    """Algorithmic bytes of one contact launch: each vertex position read once
+    (16 B float4 record), each segment's static record (16 B: first vertex,
+    fiber, radius), list count (4 B) and list entries (4 B each), and the two
+    corrections written (32 B)."""
+    return 16.0 * V + S * (16.0 + 4.0 + 4.0 * mean_list + 32.0)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of this "
+                                   "build (see DESIGN.md for how the path would shard)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import fixtures as fx
+    import tangle_amd as tg
+    from tangle_amd import _native
+
+    d_min = 7e-6
+    recipe = fx.bimodal_recipe(scale=args.scale)
+    packed = recipe._packed(False)
+    settings = fx.hard_gate_settings(d_min, force_full_iterations=True)
+    cfg = settings.to_config(d_min)
+    S = int(packed.vertex_count - packed.counts.size)
+    V = int(packed.vertex_count)
+    initial = packed.placed.astype(np.float32)
+
+    lib = _native.library()
+    import ctypes as C
+
+    world = _native.World(packed, cfg)
+    info = world.info()
+    # warmup: builds and caches the batch graph
+    for _ in range(args.warmup):
+        world.run_batch(cfg, args.batch)
+    world.upload_positions(initial)
+    rebuilds_before = world.info().neighbor_list_rebuilds
+
+    t0 = time.perf_counter()
+    total_iterations = 0
+    for _ in range(args.steps):
+        status = world.run_batch(cfg, args.batch)
+        total_iterations += int(status.batch_iterations)
+    t1 = time.perf_counter()
+    seconds = t1 - t0
+    rebuilds = world.info().neighbor_list_rebuilds - rebuilds_before
+    final_timed = world.download_positions()
+    counts, _ = world.neighbor_lists()
+    block = info.neighbor_block
+    mean_list = float(np.mean(np.minimum(counts, block)))
+
+    # Kernel timing: the same K steps from the same start, replayed with every
+    # kernel group bracketed by HIP events on the world's stream (captured
+    # graphs do not expose event times, so these batches launch directly).
+    world.upload_positions(initial)
+    _native.check(lib.tgh_world_set_timing(world._handle, 1))
+    ms = (C.c_double * 5)()
+    launches = (C.c_uint64 * 5)()
+    _native.check(lib.tgh_world_timing(world._handle, ms, launches, 5, 1))
+    for _ in range(args.steps):
+        world.run_batch(cfg, args.batch)
+    _native.check(lib.tgh_world_timing(world._handle, ms, launches, 5, 1))
+    replay_matches = bool(np.array_equal(world.download_positions().view(np.uint32), final_timed.view(np.uint32)))
+    group_ms = {GROUPS[g]: ms[g] for g in range(5)}
+    group_launches = {GROUPS[g]: launches[g] for g in range(5)}
+    world.close()
+
+    seg_iters = S * total_iterations
+    value = seg_iters / seconds
+    ms_per_step = 1000.0 * seconds / args.steps
+
+    # dominant kernel group by event time
+    dominant = max(GROUPS, key=lambda g: group_ms[g])
+    contact_avg_ms = group_ms["contact"] / max(1, group_launches["contact"])
+    contact_bytes = contact_bytes_per_launch(S, V, mean_list)
+    achieved = contact_bytes / (contact_avg_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": "tgh::k_contact",
+        "achieved": round(achieved, 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 4),
+        "traffic": None,
+        "avg_launch_us": round(1000.0 * contact_avg_ms, 2),
+        "bytes_per_launch": int(contact_bytes),
+        "dominant_group": dominant,
+        "group_ms_per_iteration": {g: round(group_ms[g] / max(1, total_iterations + args.steps), 4) for g in GROUPS},
+        "timing_replay_bitwise_equal": replay_matches,
+    }
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "segment-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {
+            "workload": "bimodal felt: 2,000 fibers (1,763 x 7 um + 237 x 19 um), 1 mm in a 1 mm periodic cube, "
+                        "planar-biased, plain HIP PBD",
+            "segments": S,
+            "vertices": V,
+            "iterations_per_step": args.batch,
+            "scale": args.scale,
+            "cells": list(info.cells),
+            "neighbor_block": block,
+            "mean_list_length": round(mean_list, 2),
+            "rebuilds_in_timed_region": int(rebuilds),
+        },
+        "roofline": roofline,
+    }
+
+    if not args.skip_tta:
+        r = fx.bimodal_recipe(scale=args.scale)
+        r.relax_until_converged(max_iterations=args.tta_max_iterations)
+        tta_settings = fx.hard_gate_settings(d_min, max_iterations=args.tta_max_iterations)
+        tta = {"plain": None, "learned": None}
+        try:
+            t0 = time.perf_counter()
+            res = r.run(tta_settings)
+            wall = time.perf_counter() - t0
+            tta["plain"] = {"seconds": round(wall, 3), "iterations": res.iterations, "converged": res.converged,
+                            "max_penetration": res.max_penetration, "max_curvature_ratio": res.max_curvature_ratio}
+            positions = res.positions
+        except tg.RecipeError as err:
+            wall = time.perf_counter() - t0
+            tta["plain"] = {"seconds": None, "iterations": err.iteration, "converged": False,
+                            "error": str(err), "elapsed_s": round(wall, 3)}
+            positions = None
+        result["time_to_hard_admissibility"] = tta
+        if args.dump and positions is not None:
+            np.save(args.dump, positions)
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(packed, cfg, S, args.cpu_iterations)
+
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(packed, cfg, S, iterations):
+    """The oracle (C++ restatement of the reference loop, OpenMP) on the first
+    `iterations` iterations of the same specimen, on the host cores."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    t0 = time.perf_counter()
+    with oracle.OracleWorld(packed, cfg) as w:
+        t1 = time.perf_counter()
+        status = w.run_batch(cfg, iterations)
+        t2 = time.perf_counter()
+    done = int(status.batch_iterations)
+    return {
+        "value": round(S * done / (t2 - t1), 1),
+        "unit": "segment-iters/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {done} iterations of the same specimen from the generated state "
+                  f"({t2 - t1:.1f} s; upload and sizing {t1 - t0:.2f} s excluded)",
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -136,6 +136,16 @@ int tgh_world_cell_list(const tgh_world *world, uint32_t *counts, uint32_t *offs
  * ascending order when counts[s] <= neighbor_block (unspecified otherwise). */
 int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t *lists,
                              size_t list_capacity);
+/* Kernel-group timing with HIP events on the world's stream (no counterpart
+ * in the reference, which has no kernel profiler). When enabled, later
+ * batches are launched directly (not as a graph) and every group of every
+ * iteration is bracketed by events: 0 list rebuild, 1 contact, 2 curvature measure and
+ * convergence assessment, 3 stretch/bend/contact-apply passes, 4 curvature
+ * Gauss-Seidel. tgh_world_timing returns the accumulated milliseconds and
+ * launch counts per group (groups >= 5) and clears them when `reset`. */
+int tgh_world_set_timing(tgh_world *world, uint32_t enable);
+int tgh_world_timing(tgh_world *world, double *group_ms, uint64_t *group_launches, size_t groups,
+                     uint32_t reset);
 void tgh_world_destroy(tgh_world *world);
 const char *tgh_last_error(void);
 

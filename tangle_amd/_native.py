@@ -133,6 +133,8 @@ SIGNATURES = {
     "tgh_world_rebuild_lists": (C.c_int, [C.c_void_p]),
     "tgh_world_cell_list": (C.c_int, [C.c_void_p, _u32p, _u32p, _u32p, C.c_size_t, C.c_size_t]),
     "tgh_world_neighbor_lists": (C.c_int, [C.c_void_p, _u32p, _u32p, C.c_size_t]),
+    "tgh_world_set_timing": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "tgh_world_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32]),
     "tgh_world_destroy": (None, [C.c_void_p]),
     "tgh_last_error": (C.c_char_p, []),
     "tgh_generate_population": (C.c_int, [C.POINTER(PopulationSpec), _f64p, _f64p, _f64p, C.c_size_t]),

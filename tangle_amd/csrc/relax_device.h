@@ -82,7 +82,14 @@ struct IterParams {
 // world's sizes and returns a HIP error code (hipSuccess when enqueued).
 hipError_t launch_begin_batch(const WorldDev &w, hipStream_t stream);
 hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream);
-hipError_t launch_iteration(const WorldDev &w, const IterParams &p, hipStream_t stream);
 hipError_t launch_rebuild_now(const WorldDev &w, hipStream_t stream);
+// One relaxation iteration is these five groups in order (world.rs:1038-1501).
+hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t stream);
+hipError_t launch_measure(const WorldDev &w, const IterParams &p, hipStream_t stream);
+hipError_t launch_vertex_passes(const WorldDev &w, const IterParams &p, hipStream_t stream);
+hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t stream);
+
+// Kernel groups of one iteration, in launch order; used for event timing.
+enum KernelGroup { kGroupRebuild = 0, kGroupContact, kGroupMeasure, kGroupVertex, kGroupCurvature, kGroupCount };
 
 }  // namespace tgh

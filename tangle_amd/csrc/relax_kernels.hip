@@ -887,17 +887,26 @@ hipError_t launch_rebuild_now(const WorldDev &w, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_iteration(const WorldDev &w, const IterParams &p, hipStream_t stream) {
-    hipError_t e = launch_rebuild(w, stream);
-    if (e != hipSuccess) return e;
+hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t stream) {
     hipLaunchKernelGGL(k_contact, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.g, w.pos, w.sstat,
                        w.cell_segs, w.seg_cell, w.cell_counts, w.cell_offsets, w.ncount, w.nbr, w.corr0, w.corr1,
                        p.correction_fraction, p.aggregation, w.block, w.S);
+    return hipGetLastError();
+}
+
+hipError_t launch_measure(const WorldDev &w, const IterParams &p, hipStream_t stream) {
     hipLaunchKernelGGL(k_measure_assess, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.st, w.pos, w.vA, w.vB,
                        w.V, p.penetration_tolerance, p.curvature_ratio_tolerance, p.iterations, p.force_full);
-    // Jacobi passes alternate between the two position buffers; the last one
-    // also applies contact, and the curvature kernel writes back to `pos`.
-    const uint32_t passes = p.constraint_iterations > 0 ? p.constraint_iterations : 1u;
+    return hipGetLastError();
+}
+
+// Jacobi passes alternate between the two position buffers; the last one
+// also applies contact. The curvature kernel reads the buffer the last pass
+// wrote and writes back to `pos`, so every iteration ends in `pos`.
+static uint32_t vertex_passes(const IterParams &p) { return p.constraint_iterations > 0 ? p.constraint_iterations : 1u; }
+
+hipError_t launch_vertex_passes(const WorldDev &w, const IterParams &p, hipStream_t stream) {
+    const uint32_t passes = vertex_passes(p);
     float4 *src = w.pos, *dst = w.alt;
     for (uint32_t i = 0; i < passes; ++i) {
         hipLaunchKernelGGL(k_vertex_pass, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.st, w.g, src, dst,
@@ -908,6 +917,11 @@ hipError_t launch_iteration(const WorldDev &w, const IterParams &p, hipStream_t 
         src = dst;
         dst = t;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t stream) {
+    const float4 *src = (vertex_passes(p) % 2 == 1) ? w.alt : w.pos;
     hipLaunchKernelGGL(k_curvature_gs, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.st, w.g, src, w.pos,
                        w.refpos, w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness,
                        p.curvature_margin, 0.5f * w.skin, w.track_walls);

@@ -266,6 +266,12 @@ struct tgh_world {
     std::map<std::string, hipGraphExec_t> graphs;
     std::vector<hipGraph_t> graph_templates;
     bool use_graphs = true;
+    // Kernel-group timing: events bracket every group of every iteration of
+    // a batch, launched directly on the stream.
+    bool timing = false;
+    std::vector<hipEvent_t> direct_events;
+    double group_ms[tgh::kGroupCount] = {0, 0, 0, 0, 0};
+    uint64_t group_launches[tgh::kGroupCount] = {0, 0, 0, 0, 0};
 
     template <typename T>
     hipError_t alloc(T **out, size_t count) {
@@ -282,6 +288,7 @@ struct tgh_world {
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
         for (hipGraph_t g : graph_templates) (void)hipGraphDestroy(g);
+        for (hipEvent_t e : direct_events) (void)hipEventDestroy(e);
         for (void *p : allocations) (void)hipFree(p);
         if (host_state) (void)hipHostFree(host_state);
         if (stream) (void)hipStreamDestroy(stream);
@@ -435,20 +442,64 @@ int create_world(const tgh_assembly *a, const tgh_config *c, tgh_world **out) {
     return TGH_OK;
 }
 
-std::string graph_key(const IterParams &q) {
-    std::string k(sizeof(q), '\0');
+std::string graph_key(const IterParams &q, bool timing) {
+    std::string k(sizeof(q) + 1, '\0');
     std::memcpy(&k[0], &q, sizeof(q));
+    k[sizeof(q)] = timing ? 1 : 0;
     return k;
 }
 
-int enqueue_batch(tgh_world *w, const IterParams &q) {
-    hipError_t e = tgh::launch_begin_batch(w->d, w->stream);
+hipError_t ensure_events(std::vector<hipEvent_t> &events, size_t count) {
+    while (events.size() < count) {
+        hipEvent_t e = nullptr;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        events.push_back(e);
+    }
+    return hipSuccess;
+}
+
+// Enqueues begin_relaxation_batch and iterations+1 passes (world.rs:1020-1505).
+// With `events`, event i*G+g marks the start of group g of pass i.
+int enqueue_batch(tgh_world *w, const IterParams &q, std::vector<hipEvent_t> *events) {
+    constexpr int G = tgh::kGroupCount;
+    const WorldDev &d = w->d;
+    hipStream_t s = w->stream;
+    hipError_t e = tgh::launch_begin_batch(d, s);
     // begin_relaxation_batch is followed by the displacement check
     // (world.rs:1033-1035); between batches nothing moves the positions
     // except tgh_world_upload_positions, which requests a rebuild itself.
-    for (uint32_t step = 0; e == hipSuccess && step <= q.iterations; ++step)
-        e = tgh::launch_iteration(w->d, q, w->stream);
+    if (events && e == hipSuccess) e = ensure_events(*events, (size_t)(q.iterations + 1) * G + 1);
+    auto mark = [&](size_t i) {
+        if (events && e == hipSuccess) e = hipEventRecord((*events)[i], s);
+    };
+    for (uint32_t step = 0; e == hipSuccess && step <= q.iterations; ++step) {
+        const size_t base = (size_t)step * G;
+        mark(base + tgh::kGroupRebuild);
+        if (e == hipSuccess) e = tgh::launch_rebuild(d, s);
+        mark(base + tgh::kGroupContact);
+        if (e == hipSuccess) e = tgh::launch_contact(d, q, s);
+        mark(base + tgh::kGroupMeasure);
+        if (e == hipSuccess) e = tgh::launch_measure(d, q, s);
+        mark(base + tgh::kGroupVertex);
+        if (e == hipSuccess) e = tgh::launch_vertex_passes(d, q, s);
+        mark(base + tgh::kGroupCurvature);
+        if (e == hipSuccess) e = tgh::launch_curvature(d, q, s);
+    }
+    mark((size_t)(q.iterations + 1) * G);
     if (e != hipSuccess) return set_error(TGH_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    return TGH_OK;
+}
+
+int accumulate_timing(tgh_world *w, const IterParams &q, const std::vector<hipEvent_t> &events) {
+    constexpr int G = tgh::kGroupCount;
+    const size_t marks = (size_t)(q.iterations + 1) * G + 1;
+    for (size_t i = 0; i + 1 < marks; ++i) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, events[i], events[i + 1]));
+        w->group_ms[i % G] += ms;
+        w->group_launches[i % G] += 1;
+    }
     return TGH_OK;
 }
 
@@ -459,12 +510,15 @@ int run_batch(tgh_world *w, const tgh_config *c, uint32_t iterations, tgh_batch_
     if (c->neighbor_capacity != 0 && c->neighbor_skin_scale != c->neighbor_skin_scale)
         return set_error(TGH_ERR_INVALID, "neighbor_skin_scale is NaN");
     const IterParams q = iter_params(c, iterations);
-    if (w->use_graphs) {
-        const std::string key = graph_key(q);
+    std::vector<hipEvent_t> *events = nullptr;
+    // Event records captured into a graph cannot be queried for elapsed time
+    // on this runtime, so a timed batch is launched directly.
+    if (w->use_graphs && !w->timing) {
+        const std::string key = graph_key(q, false);
         auto it = w->graphs.find(key);
         if (it == w->graphs.end()) {
             HIP_TRY(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
-            int erc = enqueue_batch(w, q);
+            int erc = enqueue_batch(w, q, nullptr);
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(w->stream, &graph);
             if (erc) return erc;
@@ -480,11 +534,16 @@ int run_batch(tgh_world *w, const tgh_config *c, uint32_t iterations, tgh_batch_
         }
         HIP_TRY(hipGraphLaunch(it->second, w->stream));
     } else {
-        rc = enqueue_batch(w, q);
+        if (w->timing) events = &w->direct_events;
+        rc = enqueue_batch(w, q, events);
         if (rc) return rc;
     }
     HIP_TRY(hipMemcpyAsync(w->host_state, w->d.st, sizeof(DevState), hipMemcpyDeviceToHost, w->stream));
     HIP_TRY(hipStreamSynchronize(w->stream));
+    if (events) {
+        rc = accumulate_timing(w, q, *events);
+        if (rc) return rc;
+    }
     const DevState &s = *w->host_state;
     st->total_iterations = s.control[1];
     st->batch_iterations = s.control[2];
@@ -617,6 +676,26 @@ int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t 
         counts[seg] = slot_count[slot];
         const size_t keep = std::min<size_t>(slot_count[slot], B);
         for (size_t k = 0; k < keep; ++k) lists[(size_t)seg * B + k] = cols[k * S + slot];
+    }
+    return TGH_OK;
+}
+
+int tgh_world_set_timing(tgh_world *world, uint32_t enable) {
+    if (!world) return set_error(TGH_ERR_INVALID, "world is null");
+    world->timing = enable != 0;
+    return TGH_OK;
+}
+
+int tgh_world_timing(tgh_world *world, double *group_ms, uint64_t *group_launches, size_t groups, uint32_t reset) {
+    if (!world || !group_ms || !group_launches) return set_error(TGH_ERR_INVALID, "null argument");
+    if (groups < (size_t)tgh::kGroupCount) return set_error(TGH_ERR_SIZE, "timing buffers hold fewer than 5 groups");
+    for (int g = 0; g < tgh::kGroupCount; ++g) {
+        group_ms[g] = world->group_ms[g];
+        group_launches[g] = world->group_launches[g];
+        if (reset) {
+            world->group_ms[g] = 0.0;
+            world->group_launches[g] = 0;
+        }
     }
     return TGH_OK;
 }

@@ -28,33 +28,7 @@ def max_abs(a, b):
     return float(np.max(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))))
 
 
-# (name, builder, config changes, iterations, expectations on the GPU status, d_min)
-CASES = [
-    ("crossed", fx.crossed, dict(penetration_tolerance=1e-5, correction_fraction=1.0, max_step=0.2), 20,
-     lambda s: s.converged and s.total_iterations > 0 and s.max_penetration <= 1e-5, 0.2),
-    ("pinned_overlap", lambda: fx.crossed(True), dict(penetration_tolerance=1e-5, correction_fraction=1.0,
-                                                      max_step=0.2), 5,
-     lambda s: (not s.converged) and s.max_penetration > 1e-5, 0.2),
-    ("short_segments", fx.short_segments, dict(penetration_tolerance=1e-5, max_step=0.02), 50,
-     lambda s: s.converged and s.max_penetration <= 1e-5, 0.2),
-    ("hairpin", fx.hairpin, dict(penetration_tolerance=1e-4, max_step=0.02), 4000,
-     lambda s: s.max_penetration < 0.01, 0.2),
-    ("dense_cell", fx.dense_cell, dict(max_step=0.01), 5,
-     lambda s: (not s.cell_list_overflow) and np.isfinite(s.max_penetration), 0.04),
-    ("curvature_limit", fx.bent_triplet, dict(penetration_tolerance=1e-5, stretch_stiffness=0.0, bend_stiffness=0.0,
-                                              constraint_iterations=1, max_step=0.02), 40,
-     lambda s: s.converged and s.max_curvature_ratio <= 1.0 + 1e-5, 0.02),
-    ("pinned_bend", lambda: fx.bent_triplet(pin_all=True), dict(stretch_stiffness=0.0, bend_stiffness=0.0,
-                                                               constraint_iterations=1), 5,
-     lambda s: (not s.converged) and s.max_curvature_ratio > 1.0 + 1e-5, 0.02),
-    ("periodic_pair", fx.periodic_pair, dict(penetration_tolerance=1e-5, max_step=0.05), 40,
-     lambda s: s.converged and s.max_penetration <= 1e-5, 0.2),
-    ("zigzag", fx.zigzag, dict(penetration_tolerance=1e-5, stretch_stiffness=0.0, bend_stiffness=0.0,
-                               constraint_iterations=1, curvature_cleanup_sweeps=1, max_step=0.02), 30,
-     lambda s: s.converged and s.max_curvature_ratio <= 1.0 + 1e-5, 0.01),
-    ("dense_mat", fx.dense_mat, dict(penetration_tolerance=0.0, force_full_iterations=1, max_step=0.01), 40,
-     lambda s: np.isfinite(s.max_penetration), 0.2),
-]
+CASES = fx.REFERENCE_CASES
 
 
 @pytest.mark.parametrize("name,builder,changes,iterations,expect,d_min", CASES, ids=[c[0] for c in CASES])
