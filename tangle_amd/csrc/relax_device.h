@@ -15,6 +15,13 @@ constexpr uint32_t kPinned = 16u;
 
 constexpr uint32_t kBuildThreads = 128;
 constexpr uint32_t kLdsListCapacity = 64;
+// Entries kept per overflowed list (candidates in the reference's scan order).
+constexpr uint32_t kOverflowListCapacity = 256;
+// Longest fiber the per-wave curvature kernel stages in LDS (9 floats each).
+constexpr uint32_t kMaxLdsFiberVertices = 1536;
+// Distributed atomic-max slots per metric (penetration, step, curvature).
+constexpr uint32_t kRedSlots = 256;
+constexpr uint32_t kRedPen = 0, kRedStep = kRedSlots, kRedCurv = 2 * kRedSlots;
 
 // Uniform-grid parameters: widths are (upper - lower) / cells computed once
 // in f32, exactly as the reference's proxy_cell computes them per call.
@@ -30,17 +37,13 @@ struct GridParams {
 // Device-resident control block. `control` and `nstate` follow the
 // reference's buffers (world.rs:466-467, neighbor.rs:1-9): control =
 // {running, lifetime iterations, batch iterations, terminal failure},
-// nstate = {rebuild pending, rebuild count}. red_* hold u32 bit patterns of
-// nonnegative floats under atomicMax.
+// nstate = {rebuild pending, rebuild count}.
 struct DevState {
     uint32_t control[4];
     float metrics[3];
     uint32_t nstate[2];
-    uint32_t red_pen;
-    uint32_t red_step;
-    uint32_t red_curv;
     uint32_t done_blocks;
-    uint32_t pad[3];
+    uint32_t pad[6];
 };
 
 // Device pointers and sizes of one world, passed to the launchers.
@@ -49,15 +52,18 @@ struct WorldDev {
     GridParams g;
     float skin;
     uint32_t block;  // neighbour-list slots per segment
+    uint32_t ovf_cap;
     uint32_t track_walls;
+    uint32_t max_fiber_vertices;
     DevState *st;
+    uint32_t *red;  // 3 * kRedSlots u32 bit patterns of nonnegative floats
     float4 *pos, *alt, *refpos, *wall, *vA, *vB;
     uint4 *sstat;
     uint2 *fspan;
     uint32_t *cell_counts, *cell_offsets, *cell_cursor, *cell_segs, *seg_cell, *scattered;
     float4 *slotA, *slotB;
     uint4 *slotC;
-    uint32_t *ncount, *nbr;
+    uint32_t *ncount, *nbr, *ovf;
     float4 *corr0, *corr1;
 };
 

@@ -245,17 +245,22 @@ __global__ __launch_bounds__(256) void k_rank_cells(const DevState *st, const ui
 }
 
 // One lane per cell-list slot builds that slot's segment list from the 27
-// stencil cells (build_segment_neighbor_lists, neighbor.rs:155-364), keeping
-// it sorted by insertion. `Lds` keeps the lane's list in LDS (block size up
-// to kLdsListCapacity); otherwise it is sorted in place in global memory.
+// stencil cells (build_segment_neighbor_lists, neighbor.rs:155-364). Lists are
+// rows of `nbr` (row `slot`, `block` entries) kept in ascending segment order,
+// which replaces the separate sort (neighbor.rs:366-408). A segment with more
+// than `block` neighbours keeps its candidates in stencil-then-cell order in
+// a row of `ovf` (`ovf_cap` entries): exactly the order in which the
+// reference's overflow scan visits them (contact.rs:348-428), restricted to
+// the pairs within the skin, the only ones that can touch before the next
+// rebuild. The first kLdsListCapacity entries are staged in LDS.
 template <bool Lds>
 __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *st, GridParams g,
                                                                const float4 *slotA, const float4 *slotB,
                                                                const uint4 *slotC, const uint32_t *seg_cell,
                                                                const uint32_t *cell_counts,
                                                                const uint32_t *cell_offsets, uint32_t *ncount,
-                                                               uint32_t *nbr, float skin, uint32_t block,
-                                                               uint32_t S) {
+                                                               uint32_t *nbr, uint32_t *ovf, float skin,
+                                                               uint32_t block, uint32_t ovf_cap, uint32_t S) {
     if (st->nstate[0] == 0) return;
     __shared__ uint32_t lds[Lds ? kLdsListCapacity * kBuildThreads : 1];
     uint32_t slot = blockIdx.x * kBuildThreads + threadIdx.x;
@@ -270,6 +275,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
     const float d1x = q1x - p1x, d1y = q1y - p1y, d1z = q1z - p1z;
     const float half_length = 0.5f * sqrtf(d1x * d1x + d1y * d1y + d1z * d1z);
     const uint32_t home = seg_cell[seg];
+    uint32_t *row = ovf + (size_t)slot * ovf_cap;  // encounter order
     uint32_t count = 0;
     for (uint32_t n = 0; n < 27; ++n) {
         uint32_t cell;
@@ -307,224 +313,337 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
             float reach = half_length + 0.5f * sqrtf(d2x * d2x + d2y * d2y + d2z * d2z) + inter;
             if (!(mdx * mdx + mdy * mdy + mdz * mdz <= reach * reach)) continue;
             if (!(axis_distance(p1x, p1y, p1z, d1x, d1y, d1z, p2x, p2y, p2z, d2x, d2y, d2z) <= inter)) continue;
-            if (count < block) {
-                uint32_t at = count;
-                if (Lds) {
-                    while (at > 0 && lds[(at - 1) * kBuildThreads + tid] > other) {
-                        lds[at * kBuildThreads + tid] = lds[(at - 1) * kBuildThreads + tid];
-                        --at;
-                    }
-                    lds[at * kBuildThreads + tid] = other;
-                } else {
-                    while (at > 0 && nbr[(size_t)(at - 1) * S + slot] > other) {
-                        nbr[(size_t)at * S + slot] = nbr[(size_t)(at - 1) * S + slot];
-                        --at;
-                    }
-                    nbr[(size_t)at * S + slot] = other;
-                }
+            if (Lds && count < kLdsListCapacity) {
+                lds[count * kBuildThreads + tid] = other;
+            } else if (count < ovf_cap) {
+                row[count] = other;
             }
             ++count;
         }
     }
     ncount[slot] = count;
-    if (Lds) {
-        uint32_t keep = count < block ? count : block;
-        for (uint32_t k = 0; k < keep; ++k) nbr[(size_t)k * S + slot] = lds[k * kBuildThreads + tid];
+    const uint32_t staged = Lds ? (count < kLdsListCapacity ? count : kLdsListCapacity) : 0u;
+    if (count <= block) {
+        // listed: insertion sort into ascending order
+        uint32_t *out = nbr + (size_t)slot * block;
+        if (Lds) {
+            for (uint32_t i = 1; i < count; ++i) {
+                uint32_t key = lds[i * kBuildThreads + tid];
+                uint32_t at = i;
+                while (at > 0 && lds[(at - 1) * kBuildThreads + tid] > key) {
+                    lds[at * kBuildThreads + tid] = lds[(at - 1) * kBuildThreads + tid];
+                    --at;
+                }
+                lds[at * kBuildThreads + tid] = key;
+            }
+            for (uint32_t i = 0; i < count; ++i) out[i] = lds[i * kBuildThreads + tid];
+        } else {
+            for (uint32_t i = 0; i < count; ++i) out[i] = row[i];
+            for (uint32_t i = 1; i < count; ++i) {
+                uint32_t key = out[i];
+                uint32_t at = i;
+                while (at > 0 && out[at - 1] > key) {
+                    out[at] = out[at - 1];
+                    --at;
+                }
+                out[at] = key;
+            }
+        }
+    } else if (count <= ovf_cap) {
+        for (uint32_t i = 0; i < staged; ++i) row[i] = lds[i * kBuildThreads + tid];
     }
 }
 
 // ---------------------------------------------------------------------------
-// contact projection (find_segment_corrections, contact.rs:232-902, round
-// sections), one lane per slot. Thread 0 of the grid first closes a pending
-// rebuild (finish_neighbor_list_rebuild, neighbor.rs:424-432).
+// Contact projection (find_segment_corrections, contact.rs:232-902, round
+// sections). Terms of one candidate: the correction each endpoint would take
+// and its aggregation weight, exactly as the sequential loop computes them.
 
-__global__ __launch_bounds__(256) void k_contact(DevState *st, GridParams g, const float4 *pos,
-                                                 const uint4 *sstat, const uint32_t *cell_segs,
-                                                 const uint32_t *seg_cell, const uint32_t *cell_counts,
-                                                 const uint32_t *cell_offsets, const uint32_t *ncount,
-                                                 const uint32_t *nbr, float4 *corr0, float4 *corr1,
-                                                 float correction_fraction, uint32_t aggregation, uint32_t block,
+struct SelfSeg {
+    float4 p, q;  // endpoints (.w: rest arc length)
+    float d1x, d1y, d1z;
+    float radius, half_length;
+    uint32_t seg, owner;
+};
+
+struct Term {
+    float pen;
+    float t0x, t0y, t0z, t1x, t1y, t1z;
+    float aw;
+};
+
+__device__ __forceinline__ bool contact_term(const GridParams &g, const SelfSeg &me, uint32_t other,
+                                             const float4 *pos, const uint4 *sstat, float cf, uint32_t aggregation,
+                                             Term &t) {
+    const float eps_len = 1.0e-20f, eps_par = 1.0e-6f;
+    uint4 ot = sstat[other];
+    float4 P2 = pos[ot.x], Q2 = pos[ot.x + 1];
+    float other_radius = __uint_as_float(ot.z);
+    float gap = fmaxf(P2.w - me.q.w, me.p.w - Q2.w);
+    bool near = ot.y == me.owner && gap < me.radius + other_radius;
+    if (other == me.seg || near) return false;
+    const float p1x = me.p.x, p1y = me.p.y, p1z = me.p.z, q1x = me.q.x, q1y = me.q.y, q1z = me.q.z;
+    const float d1x = me.d1x, d1y = me.d1y, d1z = me.d1z;
+    float p2x = P2.x, p2y = P2.y, p2z = P2.z, q2x = Q2.x, q2y = Q2.y, q2z = Q2.z;
+    float idx = 0.5f * (p2x + q2x - p1x - q1x);
+    float idy = 0.5f * (p2y + q2y - p1y - q1y);
+    float idz = 0.5f * (p2z + q2z - p1z - q1z);
+    if (g.periodic[0]) {
+        float sh = roundf(idx / g.length[0]) * g.length[0];
+        p2x -= sh;
+        q2x -= sh;
+    }
+    if (g.periodic[1]) {
+        float sh = roundf(idy / g.length[1]) * g.length[1];
+        p2y -= sh;
+        q2y -= sh;
+    }
+    if (g.periodic[2]) {
+        float sh = roundf(idz / g.length[2]) * g.length[2];
+        p2z -= sh;
+        q2z -= sh;
+    }
+    float d2x = q2x - p2x, d2y = q2y - p2y, d2z = q2z - p2z;
+    float mdx = 0.5f * (p2x + q2x - p1x - q1x);
+    float mdy = 0.5f * (p2y + q2y - p1y - q1y);
+    float mdz = 0.5f * (p2z + q2z - p1z - q1z);
+    float reach = (me.half_length + 0.5f * sqrtf(d2x * d2x + d2y * d2y + d2z * d2z) + me.radius + other_radius) *
+                  (1.0f + 1.0e-4f);
+    if (!(mdx * mdx + mdy * mdy + mdz * mdz <= reach * reach)) return false;
+    float rx = p1x - p2x, ry = p1y - p2y, rz = p1z - p2z;
+    float a = d1x * d1x + d1y * d1y + d1z * d1z;
+    float e = d2x * d2x + d2y * d2y + d2z * d2z;
+    float f = d2x * rx + d2y * ry + d2z * rz;
+    float s = 0.f, tt = 0.f;
+    if (a <= eps_len && e > eps_len) {
+        tt = rclamp(f / e, 0.f, 1.f);
+    } else if (a > eps_len) {
+        float cc = d1x * rx + d1y * ry + d1z * rz;
+        if (e <= eps_len) {
+            s = rclamp(-cc / a, 0.f, 1.f);
+        } else {
+            float b = d1x * d2x + d1y * d2y + d1z * d2z;
+            float den = a * e - b * b;
+            bool parallel = fabsf(den) <= eps_par * a * e;
+            if (!parallel) s = rclamp((b * f - cc * e) / den, 0.f, 1.f);
+            float proj = (b * s + f) / e;
+            if (proj < 0.f) {
+                s = rclamp(-cc / a, 0.f, 1.f);
+            } else if (proj > 1.f) {
+                tt = 1.f;
+                s = rclamp((b - cc) / a, 0.f, 1.f);
+            } else {
+                tt = proj;
+            }
+            if (parallel) {  // contact.rs:695-726
+                float es = 1.f, et = 0.f;
+                float eproj = (b + f) / e;
+                if (eproj < 0.f) {
+                    es = rclamp(-cc / a, 0.f, 1.f);
+                } else if (eproj > 1.f) {
+                    et = 1.f;
+                    es = rclamp((b - cc) / a, 0.f, 1.f);
+                } else {
+                    et = eproj;
+                }
+                float gx = rx + d1x * s - d2x * tt, gy = ry + d1y * s - d2y * tt, gz = rz + d1z * s - d2z * tt;
+                float ex = rx + d1x * es - d2x * et, ey = ry + d1y * es - d2y * et, ez = rz + d1z * es - d2z * et;
+                if (ex * ex + ey * ey + ez * ez < gx * gx + gy * gy + gz * gz) {
+                    s = es;
+                    tt = et;
+                }
+            }
+        }
+    }
+    float dx = (p2x + d2x * tt) - (p1x + d1x * s);
+    float dy = (p2y + d2y * tt) - (p1y + d1y * s);
+    float dz = (p2z + d2z * tt) - (p1z + d1z * s);
+    float distance = sqrtf(dx * dx + dy * dy + dz * dz);
+    float pen = me.radius + other_radius - distance;
+    if (!(pen > 0.f)) return false;
+    float nx, ny, nz;
+    if (distance > 1.0e-7f) {
+        float inv = 1.0f / distance;
+        nx = dx * inv;
+        ny = dy * inv;
+        nz = dz * inv;
+    } else {  // contact.rs:768-812
+        nx = d1y * d2z - d1z * d2y;
+        ny = d1z * d2x - d1x * d2z;
+        nz = d1x * d2y - d1y * d2x;
+        float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+        if (nl <= 1.0e-7f) {
+            if (fabsf(d1x) <= fabsf(d1y) && fabsf(d1x) <= fabsf(d1z)) {
+                nx = 0.f; ny = d1z; nz = -d1y;
+            } else if (fabsf(d1y) <= fabsf(d1z)) {
+                nx = -d1z; ny = 0.f; nz = d1x;
+            } else {
+                nx = d1y; ny = -d1x; nz = 0.f;
+            }
+            nl = sqrtf(nx * nx + ny * ny + nz * nz);
+        }
+        if (nl > 1.0e-7f) {
+            nx /= nl;
+            ny /= nl;
+            nz /= nl;
+        } else {
+            nx = 1.0f;
+        }
+        float dom = (fabsf(nx) >= fabsf(ny) && fabsf(nx) >= fabsf(nz)) ? nx : (fabsf(ny) >= fabsf(nz) ? ny : nz);
+        if (dom < 0.f) { nx = -nx; ny = -ny; nz = -nz; }
+        if (me.seg > other) { nx = -nx; ny = -ny; nz = -nz; }
+    }
+    float w0 = 1.0f - s, w1 = s, ow0 = 1.0f - tt;
+    // round sections: the director terms of the denominator are exactly zero
+    float den = w0 * w0 + w1 * w1 + ow0 * ow0 + tt * tt + 0.0f;
+    float mag = -cf * pen / den;
+    float aw = aggregation == 1u ? pen : 1.0f;
+    t.pen = pen;
+    t.aw = aw;
+    t.t0x = nx * mag * w0 * aw;
+    t.t0y = ny * mag * w0 * aw;
+    t.t0z = nz * mag * w0 * aw;
+    t.t1x = nx * mag * w1 * aw;
+    t.t1y = ny * mag * w1 * aw;
+    t.t1z = nz * mag * w1 * aw;
+    return true;
+}
+
+// Running sums of one segment, added in candidate order (contact.rs:830-891).
+struct Accum {
+    float c0x = 0.f, c0y = 0.f, c0z = 0.f, c1x = 0.f, c1y = 0.f, c1z = 0.f, weight = 0.f, max_pen = 0.f;
+
+    __device__ __forceinline__ void add(const Term &t, uint32_t aggregation) {
+        bool deepest = t.pen > max_pen;
+        if (deepest) max_pen = t.pen;
+        if (aggregation == 2u) {
+            if (deepest) {
+                c0x = t.t0x; c0y = t.t0y; c0z = t.t0z;
+                c1x = t.t1x; c1y = t.t1y; c1z = t.t1z;
+                weight = 1.0f;
+            }
+        } else {
+            c0x += t.t0x; c0y += t.t0y; c0z += t.t0z;
+            c1x += t.t1x; c1y += t.t1y; c1z += t.t1z;
+            weight += t.aw;
+        }
+    }
+};
+
+// Logical block for an XCD-aware dispatch: the hardware deals workgroups to
+// the eight XCDs round-robin, so physical block b runs on XCD b % 8. Giving
+// each XCD a contiguous range of logical blocks keeps spatially adjacent
+// slots, which gather the same candidates, on one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb / 8u, r = nb % 8u, xcd = b % 8u, idx = b / 8u;
+    return xcd < r ? xcd * (q + 1u) + idx : r * (q + 1u) + (xcd - r) * q + idx;
+}
+
+// A group of G lanes serves one cell-list slot: lane l evaluates candidates
+// l, l+G, ... of the segment's list in parallel, then every lane of the
+// group adds the contacting candidates' terms in list order (a ballot picks
+// them, shuffles broadcast their terms), so the sums are the sequential
+// loop's, bit for bit, with no float atomics. Overflowed lists read their
+// stored stencil-order row; a list longer than that row scans the 27 cells
+// sequentially in lane 0 of its group. Thread 0 of the grid first closes a
+// pending rebuild (finish_neighbor_list_rebuild, neighbor.rs:424-432).
+template <int G>
+__global__ __launch_bounds__(256) void k_contact(DevState *st, GridParams g, const float4 *pos, const uint4 *sstat,
+                                                 const uint32_t *cell_segs, const uint32_t *seg_cell,
+                                                 const uint32_t *cell_counts, const uint32_t *cell_offsets,
+                                                 const uint32_t *ncount, const uint32_t *nbr, const uint32_t *ovf,
+                                                 float4 *corr0, float4 *corr1, uint32_t *red, float cf,
+                                                 uint32_t aggregation, uint32_t block, uint32_t ovf_cap,
                                                  uint32_t S) {
-    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot == 0 && st->nstate[0] != 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && st->nstate[0] != 0) {
         st->nstate[0] = 0;
         st->nstate[1] += 1;
     }
     if (st->control[0] == 0) return;
-    float max_pen = 0.f;
-    if (slot < S) {
-        const uint32_t seg = cell_segs[slot];
-        const uint4 me = sstat[seg];
-        const float4 P1 = pos[me.x], Q1 = pos[me.x + 1];
-        const float p1x = P1.x, p1y = P1.y, p1z = P1.z, q1x = Q1.x, q1y = Q1.y, q1z = Q1.z;
-        const float d1x = q1x - p1x, d1y = q1y - p1y, d1z = q1z - p1z;
-        const uint32_t owner = me.y;
-        const float radius = __uint_as_float(me.z);
-        const float half_length = 0.5f * sqrtf(d1x * d1x + d1y * d1y + d1z * d1z);
-        const float eps_len = 1.0e-20f, eps_par = 1.0e-6f;
-        float c0x = 0.f, c0y = 0.f, c0z = 0.f, c1x = 0.f, c1y = 0.f, c1z = 0.f, cweight = 0.f;
-        const uint32_t listed_count = ncount[slot];
-        const bool listed = listed_count <= block;
-        const uint32_t ranges = listed ? 1u : 27u;
-        const uint32_t home = seg_cell[seg];
-        for (uint32_t range = 0; range < ranges; ++range) {
-            uint32_t count = 0, start = 0;
-            if (listed) {
-                count = listed_count;
-            } else {
-                uint32_t cell;
-                if (stencil_cell(g, home, range, cell)) {
-                    count = cell_counts[cell];
-                    start = cell_offsets[cell];
-                }
-            }
-            for (uint32_t k = 0; k < count; ++k) {
-                uint32_t other = listed ? nbr[(size_t)k * S + slot] : cell_segs[start + k];
-                uint4 ot = sstat[other];
-                float4 P2 = pos[ot.x], Q2 = pos[ot.x + 1];
-                float other_radius = __uint_as_float(ot.z);
-                float gap = fmaxf(P2.w - Q1.w, P1.w - Q2.w);
-                bool near = ot.y == owner && gap < radius + other_radius;
-                if (other == seg || near) continue;
-                float p2x = P2.x, p2y = P2.y, p2z = P2.z, q2x = Q2.x, q2y = Q2.y, q2z = Q2.z;
-                float idx = 0.5f * (p2x + q2x - p1x - q1x);
-                float idy = 0.5f * (p2y + q2y - p1y - q1y);
-                float idz = 0.5f * (p2z + q2z - p1z - q1z);
-                if (g.periodic[0]) {
-                    float sh = roundf(idx / g.length[0]) * g.length[0];
-                    p2x -= sh;
-                    q2x -= sh;
-                }
-                if (g.periodic[1]) {
-                    float sh = roundf(idy / g.length[1]) * g.length[1];
-                    p2y -= sh;
-                    q2y -= sh;
-                }
-                if (g.periodic[2]) {
-                    float sh = roundf(idz / g.length[2]) * g.length[2];
-                    p2z -= sh;
-                    q2z -= sh;
-                }
-                float d2x = q2x - p2x, d2y = q2y - p2y, d2z = q2z - p2z;
-                float mdx = 0.5f * (p2x + q2x - p1x - q1x);
-                float mdy = 0.5f * (p2y + q2y - p1y - q1y);
-                float mdz = 0.5f * (p2z + q2z - p1z - q1z);
-                float reach = (half_length + 0.5f * sqrtf(d2x * d2x + d2y * d2y + d2z * d2z) + radius + other_radius) *
-                              (1.0f + 1.0e-4f);
-                if (!(mdx * mdx + mdy * mdy + mdz * mdz <= reach * reach)) continue;
-                float rx = p1x - p2x, ry = p1y - p2y, rz = p1z - p2z;
-                float a = d1x * d1x + d1y * d1y + d1z * d1z;
-                float e = d2x * d2x + d2y * d2y + d2z * d2z;
-                float f = d2x * rx + d2y * ry + d2z * rz;
-                float s = 0.f, t = 0.f;
-                if (a <= eps_len && e > eps_len) {
-                    t = rclamp(f / e, 0.f, 1.f);
-                } else if (a > eps_len) {
-                    float cc = d1x * rx + d1y * ry + d1z * rz;
-                    if (e <= eps_len) {
-                        s = rclamp(-cc / a, 0.f, 1.f);
-                    } else {
-                        float b = d1x * d2x + d1y * d2y + d1z * d2z;
-                        float den = a * e - b * b;
-                        bool parallel = fabsf(den) <= eps_par * a * e;
-                        if (!parallel) s = rclamp((b * f - cc * e) / den, 0.f, 1.f);
-                        float proj = (b * s + f) / e;
-                        if (proj < 0.f) {
-                            s = rclamp(-cc / a, 0.f, 1.f);
-                        } else if (proj > 1.f) {
-                            t = 1.f;
-                            s = rclamp((b - cc) / a, 0.f, 1.f);
-                        } else {
-                            t = proj;
-                        }
-                        if (parallel) {  // contact.rs:695-726
-                            float es = 1.f, et = 0.f;
-                            float eproj = (b + f) / e;
-                            if (eproj < 0.f) {
-                                es = rclamp(-cc / a, 0.f, 1.f);
-                            } else if (eproj > 1.f) {
-                                et = 1.f;
-                                es = rclamp((b - cc) / a, 0.f, 1.f);
-                            } else {
-                                et = eproj;
-                            }
-                            float gx = rx + d1x * s - d2x * t, gy = ry + d1y * s - d2y * t, gz = rz + d1z * s - d2z * t;
-                            float ex = rx + d1x * es - d2x * et, ey = ry + d1y * es - d2y * et,
-                                  ez = rz + d1z * es - d2z * et;
-                            if (ex * ex + ey * ey + ez * ez < gx * gx + gy * gy + gz * gz) {
-                                s = es;
-                                t = et;
-                            }
-                        }
-                    }
-                }
-                float dx = (p2x + d2x * t) - (p1x + d1x * s);
-                float dy = (p2y + d2y * t) - (p1y + d1y * s);
-                float dz = (p2z + d2z * t) - (p1z + d1z * s);
-                float distance = sqrtf(dx * dx + dy * dy + dz * dz);
-                float pen = radius + other_radius - distance;
-                if (!(pen > 0.f)) continue;
-                bool deepest = pen > max_pen;
-                if (deepest) max_pen = pen;
-                float nx, ny, nz;
-                if (distance > 1.0e-7f) {
-                    float inv = 1.0f / distance;
-                    nx = dx * inv;
-                    ny = dy * inv;
-                    nz = dz * inv;
-                } else {  // contact.rs:768-812
-                    nx = d1y * d2z - d1z * d2y;
-                    ny = d1z * d2x - d1x * d2z;
-                    nz = d1x * d2y - d1y * d2x;
-                    float nl = sqrtf(nx * nx + ny * ny + nz * nz);
-                    if (nl <= 1.0e-7f) {
-                        if (fabsf(d1x) <= fabsf(d1y) && fabsf(d1x) <= fabsf(d1z)) {
-                            nx = 0.f; ny = d1z; nz = -d1y;
-                        } else if (fabsf(d1y) <= fabsf(d1z)) {
-                            nx = -d1z; ny = 0.f; nz = d1x;
-                        } else {
-                            nx = d1y; ny = -d1x; nz = 0.f;
-                        }
-                        nl = sqrtf(nx * nx + ny * ny + nz * nz);
-                    }
-                    if (nl > 1.0e-7f) {
-                        nx /= nl;
-                        ny /= nl;
-                        nz /= nl;
-                    } else {
-                        nx = 1.0f;
-                    }
-                    float dom = (fabsf(nx) >= fabsf(ny) && fabsf(nx) >= fabsf(nz)) ? nx
-                                : (fabsf(ny) >= fabsf(nz) ? ny : nz);
-                    if (dom < 0.f) { nx = -nx; ny = -ny; nz = -nz; }
-                    if (seg > other) { nx = -nx; ny = -ny; nz = -nz; }
-                }
-                float w0 = 1.0f - s, w1 = s, ow0 = 1.0f - t;
-                float den = w0 * w0 + w1 * w1 + ow0 * ow0 + t * t + 0.0f;
-                float mag = -correction_fraction * pen / den;
-                float aw = aggregation == 1u ? pen : 1.0f;
-                if (aggregation == 2u) {
-                    if (deepest) {
-                        c0x = nx * mag * w0; c0y = ny * mag * w0; c0z = nz * mag * w0;
-                        c1x = nx * mag * w1; c1y = ny * mag * w1; c1z = nz * mag * w1;
-                        cweight = 1.0f;
-                    }
-                } else {
-                    c0x += nx * mag * w0 * aw; c0y += ny * mag * w0 * aw; c0z += nz * mag * w0 * aw;
-                    c1x += nx * mag * w1 * aw; c1y += ny * mag * w1 * aw; c1z += nz * mag * w1 * aw;
-                    cweight += aw;
-                }
-            }
+    constexpr uint32_t groups = 256 / G;
+    const uint32_t lane = threadIdx.x % G, group = threadIdx.x / G;
+    const uint32_t wave_lane = threadIdx.x & 63u, group_base = wave_lane - lane;
+    const uint32_t slot = xcd_block(blockIdx.x, gridDim.x) * groups + group;
+    const bool active = slot < S;
+    SelfSeg me;
+    uint32_t count = 0;
+    const uint32_t *list = nullptr;
+    bool slow = false;
+    if (active) {
+        me.seg = cell_segs[slot];
+        uint4 ss = sstat[me.seg];
+        me.p = pos[ss.x];
+        me.q = pos[ss.x + 1];
+        me.owner = ss.y;
+        me.radius = __uint_as_float(ss.z);
+        me.d1x = me.q.x - me.p.x;
+        me.d1y = me.q.y - me.p.y;
+        me.d1z = me.q.z - me.p.z;
+        me.half_length = 0.5f * sqrtf(me.d1x * me.d1x + me.d1y * me.d1y + me.d1z * me.d1z);
+        count = ncount[slot];
+        if (count <= block) {
+            list = nbr + (size_t)slot * block;
+        } else if (count <= ovf_cap) {
+            list = ovf + (size_t)slot * ovf_cap;
+        } else {
+            slow = true;
         }
-        if (aggregation != 2u && cweight > 1.0e-20f) {
-            c0x /= cweight; c0y /= cweight; c0z /= cweight;
-            c1x /= cweight; c1y /= cweight; c1z /= cweight;
-        }
-        corr0[seg] = make_float4(c0x, c0y, c0z, max_pen);
-        corr1[seg] = make_float4(c1x, c1y, c1z, 0.f);
     }
-    uint32_t m = wave_max_u32(__float_as_uint(fmaxf(max_pen, 0.f)));
-    if ((threadIdx.x & 63) == 0 && m != 0u) atomicMax(&st->red_pen, m);
+    Accum acc;
+    if (active && !slow) {
+        for (uint32_t base = 0; base < count; base += G) {
+            const uint32_t k = base + lane;
+            Term t;
+            bool hit = k < count && contact_term(g, me, list[k], pos, sstat, cf, aggregation, t);
+            uint64_t mask = (uint64_t)__ballot(hit);
+            uint32_t mine = (uint32_t)((mask >> group_base) & (G == 64 ? ~0ull : ((1ull << G) - 1ull)));
+            while (mine) {
+                const int j = __builtin_ctz(mine);
+                mine &= mine - 1u;
+                const int src = (int)group_base + j;
+                Term u;
+                u.pen = __shfl(t.pen, src, 64);
+                u.t0x = __shfl(t.t0x, src, 64);
+                u.t0y = __shfl(t.t0y, src, 64);
+                u.t0z = __shfl(t.t0z, src, 64);
+                u.t1x = __shfl(t.t1x, src, 64);
+                u.t1y = __shfl(t.t1y, src, 64);
+                u.t1z = __shfl(t.t1z, src, 64);
+                u.aw = __shfl(t.aw, src, 64);
+                acc.add(u, aggregation);
+            }
+        }
+    } else if (active && slow && lane == 0) {
+        // the reference's overflow scan: 27 stencil cells of the home cell at
+        // the list-build positions, each cell in ascending segment order
+        const uint32_t home = seg_cell[me.seg];
+        for (uint32_t n = 0; n < 27; ++n) {
+            uint32_t cell;
+            if (!stencil_cell(g, home, n, cell)) continue;
+            const uint32_t cs = cell_offsets[cell], cn = cell_counts[cell];
+            for (uint32_t k = 0; k < cn; ++k) {
+                Term t;
+                if (contact_term(g, me, cell_segs[cs + k], pos, sstat, cf, aggregation, t)) acc.add(t, aggregation);
+            }
+        }
+    }
+    if (active && lane == 0) {
+        if (aggregation != 2u && acc.weight > 1.0e-20f) {
+            acc.c0x /= acc.weight; acc.c0y /= acc.weight; acc.c0z /= acc.weight;
+            acc.c1x /= acc.weight; acc.c1y /= acc.weight; acc.c1z /= acc.weight;
+        }
+        corr0[me.seg] = make_float4(acc.c0x, acc.c0y, acc.c0z, acc.max_pen);
+        corr1[me.seg] = make_float4(acc.c1x, acc.c1y, acc.c1z, 0.f);
+    }
+    // segment max penetration, max(0) as bits, reduced per workgroup
+    __shared__ uint32_t wave_max[4];
+    uint32_t m = wave_max_u32(active && lane == 0 ? __float_as_uint(fmaxf(acc.max_pen, 0.f)) : 0u);
+    if (wave_lane == 0) wave_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = max(max(wave_max[0], wave_max[1]), max(wave_max[2], wave_max[3]));
+        if (b) atomicMax(&red[kRedPen + blockIdx.x % kRedSlots], b);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -532,11 +651,22 @@ __global__ __launch_bounds__(256) void k_contact(DevState *st, GridParams g, con
 // metric reductions (kernels.rs:95-127) and, in the last workgroup to finish,
 // the convergence assessment (assess_reduced_metrics, kernels.rs:129-162).
 
+__device__ __forceinline__ uint32_t block_max_256(uint32_t v, uint32_t *scratch4) {
+    uint32_t m = wave_max_u32(v);
+    if ((threadIdx.x & 63u) == 0) scratch4[threadIdx.x >> 6] = m;
+    __syncthreads();
+    uint32_t b = max(max(scratch4[0], scratch4[1]), max(scratch4[2], scratch4[3]));
+    __syncthreads();
+    return b;
+}
+
 __global__ __launch_bounds__(256) void k_measure_assess(DevState *st, const float4 *pos, const float4 *vA,
-                                                        const float4 *vB, uint32_t V, float penetration_tolerance,
-                                                        float curvature_ratio_tolerance, uint32_t iterations,
-                                                        uint32_t force_full) {
+                                                        const float4 *vB, uint32_t *red, uint32_t V,
+                                                        float penetration_tolerance, float curvature_ratio_tolerance,
+                                                        uint32_t iterations, uint32_t force_full) {
     if (st->control[0] == 0) return;
+    __shared__ uint32_t scratch[4];
+    __shared__ uint32_t last;
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     float ratio = 0.f;
     if (v < V) {
@@ -556,36 +686,43 @@ __global__ __launch_bounds__(256) void k_measure_assess(DevState *st, const floa
             }
         }
     }
-    uint32_t m = wave_max_u32(__float_as_uint(fmaxf(ratio, 0.f)));
-    if ((threadIdx.x & 63) == 0 && m != 0u) atomicMax(&st->red_curv, m);
-    __shared__ uint32_t last;
-    __syncthreads();
+    uint32_t b = block_max_256(__float_as_uint(fmaxf(ratio, 0.f)), scratch);
     if (threadIdx.x == 0) {
+        if (b) atomicMax(&red[kRedCurv + blockIdx.x % kRedSlots], b);
         __threadfence();
         last = atomicAdd(&st->done_blocks, 1u) == gridDim.x - 1 ? 1u : 0u;
     }
     __syncthreads();
-    if (last == 0u || threadIdx.x != 0) return;
+    if (last == 0u) return;
     __threadfence();
-    uint32_t pen_bits = atomicAdd(&st->red_pen, 0u);
-    uint32_t step_bits = atomicAdd(&st->red_step, 0u);
-    uint32_t curv_bits = atomicAdd(&st->red_curv, 0u);
-    float pen = __uint_as_float(pen_bits), curv = __uint_as_float(curv_bits);
-    st->metrics[0] = pen;
-    st->metrics[1] = __uint_as_float(step_bits);
-    st->metrics[2] = curv;
-    bool stop = (force_full == 0u && pen <= penetration_tolerance && curv <= 1.0f + curvature_ratio_tolerance) ||
-                st->control[2] >= iterations;
-    if (stop) {
-        st->control[0] = 0;
-    } else {
-        st->control[1] += 1;
-        st->control[2] += 1;
-        st->red_step = 0u;  // recomputed by this iteration's contact application
+    // the last workgroup: reduce the distributed slots and assess
+    const uint32_t t = threadIdx.x;  // blockDim == kRedSlots
+    uint32_t pen_bits = block_max_256(__hip_atomic_load(&red[kRedPen + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), scratch);
+    uint32_t step_bits = block_max_256(__hip_atomic_load(&red[kRedStep + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), scratch);
+    uint32_t curv_bits = block_max_256(__hip_atomic_load(&red[kRedCurv + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), scratch);
+    __shared__ uint32_t keep_going;
+    if (t == 0) {
+        float pen = __uint_as_float(pen_bits), curv = __uint_as_float(curv_bits);
+        st->metrics[0] = pen;
+        st->metrics[1] = __uint_as_float(step_bits);
+        st->metrics[2] = curv;
+        bool stop = (force_full == 0u && pen <= penetration_tolerance && curv <= 1.0f + curvature_ratio_tolerance) ||
+                    st->control[2] >= iterations;
+        if (stop) {
+            st->control[0] = 0;
+        } else {
+            st->control[1] += 1;
+            st->control[2] += 1;
+        }
+        keep_going = stop ? 0u : 1u;
+        st->done_blocks = 0u;
     }
-    st->red_pen = 0u;
-    st->red_curv = 0u;
-    st->done_blocks = 0u;
+    __syncthreads();
+    red[kRedPen + t] = 0u;
+    red[kRedCurv + t] = 0u;
+    // the step maxima are recomputed by this iteration's contact application;
+    // a stopping pass leaves them for the next batch's first assessment
+    if (keep_going) red[kRedStep + t] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -612,10 +749,11 @@ __device__ __forceinline__ float wall_move(float moved, uint32_t periodic, float
 
 __global__ __launch_bounds__(256) void k_vertex_pass(DevState *st, GridParams g, const float4 *in, float4 *out,
                                                      const float4 *vA, const float4 *vB, const float4 *corr0,
-                                                     const float4 *corr1, float4 *wall, uint32_t V,
+                                                     const float4 *corr1, float4 *wall, uint32_t *red, uint32_t V,
                                                      float stretch, float bend, float max_step, uint32_t do_internal,
                                                      uint32_t do_contact, uint32_t first_pass, uint32_t track_walls) {
     if (st->control[0] == 0) return;
+    __shared__ uint32_t scratch[4];
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     float step = 0.f;
     if (v < V) {
@@ -681,24 +819,25 @@ __global__ __launch_bounds__(256) void k_vertex_pass(DevState *st, GridParams g,
         if (track_walls) wall[v] = w;
     }
     if (do_contact) {
-        uint32_t m = wave_max_u32(__float_as_uint(fmaxf(step, 0.f)));
-        if ((threadIdx.x & 63) == 0 && m != 0u) atomicMax(&st->red_step, m);
+        uint32_t b = block_max_256(__float_as_uint(fmaxf(step, 0.f)), scratch);
+        if (threadIdx.x == 0 && b) atomicMax(&red[kRedStep + blockIdx.x % kRedSlots], b);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Per-fiber curvature Gauss–Seidel (project_fiber_curvature_in_place and
-// project_curvature_triplet_in_place, kernels.rs:1165-1415), one lane per
-// fiber, then the neighbour-list displacement trigger for the fiber's
-// vertices (flag_neighbor_list_displacement, neighbor.rs:443-465).
+// project_curvature_triplet_in_place, kernels.rs:1165-1415), then the
+// neighbour-list displacement trigger (flag_neighbor_list_displacement,
+// neighbor.rs:443-465) for the fiber's vertices.
 
-// Projects one triplet in material order (previous, center, next) held in
-// registers, accumulating wall reactions into wp/wc/wn in the reference's
-// order. Returns true when the projection branch ran.
-__device__ __forceinline__ bool project_triplet(float3 &pp, float3 &pc, float3 &pn, float kmax_center,
-                                                bool pin_p, bool pin_c, bool pin_n, float ext_p, float ext_c,
-                                                float ext_n, const GridParams &g, float stiffness, float margin,
-                                                float3 &wp, float3 &wc, float3 &wn) {
+// Projection of the triplet (previous, center, next) in material order.
+// Returns true when the projection branch runs; the moved positions and the
+// wall reactions of each vertex are returned in mp/mc/mn and rp/rc/rn (zero
+// for a pinned vertex, which keeps its position).
+__device__ __forceinline__ bool triplet_projection(float3 pp, float3 pc, float3 pn, float kmax_center, bool pin_p,
+                                                   bool pin_c, bool pin_n, float ext_p, float ext_c, float ext_n,
+                                                   const GridParams &g, float stiffness, float margin, float3 &mp,
+                                                   float3 &mc, float3 &mn, float3 &rp, float3 &rc, float3 &rn) {
     float kmax = kmax_center * (1.0f - margin);
     if (!(kmax > 0.f)) return false;
     float ux = pc.x - pp.x, uy = pc.y - pp.y, uz = pc.z - pp.z;
@@ -726,104 +865,194 @@ __device__ __forceinline__ bool project_triplet(float3 &pp, float3 &pc, float3 &
     if (!(den > 1.0e-12f)) return false;
     float scale = stiffness * violation / den;
     const float gp[3] = {gux, guy, guz}, gc[3] = {gcx, gcy, gcz}, gn[3] = {gvx, gvy, gvz};
-    float *P = &pp.x, *Cc = &pc.x, *N = &pn.x, *WP = &wp.x, *WC = &wc.x, *WN = &wn.x;
+    const float P[3] = {pp.x, pp.y, pp.z}, Cc[3] = {pc.x, pc.y, pc.z}, N[3] = {pn.x, pn.y, pn.z};
+    float oP[3], oC[3], oN[3], wP[3], wC[3], wN[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float mp = P[k] - fwp * scale * gp[k];
-        float mc = Cc[k] + fwc * scale * gc[k];
-        float mn = N[k] + fwn * scale * gn[k];
-        float vp = mp, vc = mc, vn = mn;
+        float m_p = P[k] - fwp * scale * gp[k];
+        float m_c = Cc[k] + fwc * scale * gc[k];
+        float m_n = N[k] + fwn * scale * gn[k];
+        float v_p = m_p, v_c = m_c, v_n = m_n;
+        wP[k] = wC[k] = wN[k] = 0.f;
         if (!g.periodic[k]) {
-            vp = rclamp(mp, g.lower[k] + ext_p, g.upper[k] - ext_p);
-            vc = rclamp(mc, g.lower[k] + ext_c, g.upper[k] - ext_c);
-            vn = rclamp(mn, g.lower[k] + ext_n, g.upper[k] - ext_n);
-            if (fwp > 0.f) WP[k] += mp - vp;
-            if (fwc > 0.f) WC[k] += mc - vc;
-            if (fwn > 0.f) WN[k] += mn - vn;
+            v_p = rclamp(m_p, g.lower[k] + ext_p, g.upper[k] - ext_p);
+            v_c = rclamp(m_c, g.lower[k] + ext_c, g.upper[k] - ext_c);
+            v_n = rclamp(m_n, g.lower[k] + ext_n, g.upper[k] - ext_n);
+            wP[k] = m_p - v_p;
+            wC[k] = m_c - v_c;
+            wN[k] = m_n - v_n;
         }
-        if (fwp > 0.f) P[k] = vp;
-        if (fwc > 0.f) Cc[k] = vc;
-        if (fwn > 0.f) N[k] = vn;
+        oP[k] = fwp > 0.f ? v_p : P[k];
+        oC[k] = fwc > 0.f ? v_c : Cc[k];
+        oN[k] = fwn > 0.f ? v_n : N[k];
     }
+    mp = make_float3(oP[0], oP[1], oP[2]);
+    mc = make_float3(oC[0], oC[1], oC[2]);
+    mn = make_float3(oN[0], oN[1], oN[2]);
+    rp = make_float3(wP[0], wP[1], wP[2]);
+    rc = make_float3(wC[0], wC[1], wC[2]);
+    rn = make_float3(wN[0], wN[1], wN[2]);
     return true;
 }
 
-__device__ __forceinline__ float3 load3(const float4 *p, int32_t i) {
-    float4 v = p[i];
-    return make_float3(v.x, v.y, v.z);
+// One wave per fiber with the fiber staged in LDS. In sweep order the lanes
+// evaluate the next 64 triplets at once from the current positions; the
+// first one whose projection runs (a ballot) is applied by its lane, and the
+// scan resumes just after it. Triplets before it in the sweep would not move
+// anything, and each evaluation reads the same values the sequential sweep
+// would, so the result is the Gauss–Seidel sweep's, bit for bit. A forward
+// sweep that projects nothing ends the cleanup: later sweeps would revisit
+// the same unchanged triplets.
+__global__ __launch_bounds__(64) void k_curvature_gs_wave(DevState *st, GridParams g, const float4 *in, float4 *pos,
+                                                          const float4 *refpos, const float4 *vA, const float4 *vB,
+                                                          const uint2 *fspan, float4 *wall, uint32_t sweeps,
+                                                          float stiffness, float margin, float half_skin,
+                                                          uint32_t track_walls, uint32_t max_vertices) {
+    if (st->control[0] == 0) return;
+    extern __shared__ float lds[];
+    float *X = lds, *Y = X + max_vertices, *Z = Y + max_vertices, *K = Z + max_vertices, *E = K + max_vertices;
+    uint32_t *F = (uint32_t *)(E + max_vertices);
+    float *WX = (float *)(F + max_vertices), *WY = WX + max_vertices, *WZ = WY + max_vertices;
+    const uint32_t lane = threadIdx.x;
+    const uint2 span = fspan[blockIdx.x];
+    const uint32_t start = span.x, n = span.y;
+    for (uint32_t i = lane; i < n; i += 64) {
+        float4 p = in[start + i];
+        float4 a = vA[start + i];
+        X[i] = p.x;
+        Y[i] = p.y;
+        Z[i] = p.z;
+        K[i] = a.x;
+        E[i] = a.w;
+        F[i] = __float_as_uint(vB[start + i].z);
+        if (track_walls) {
+            float4 w = wall[start + i];
+            WX[i] = w.x;
+            WY[i] = w.y;
+            WZ[i] = w.z;
+        }
+    }
+    __syncthreads();
+    const uint32_t trip = n >= 3 ? n - 2 : 0;  // triplet t has center t + 1
+    bool done = trip == 0;
+    for (uint32_t sweep = 0; sweep < sweeps && !done; ++sweep) {
+        bool any = false;
+        for (int dir = 0; dir < 2 && !done; ++dir) {
+            uint32_t cursor = 0;
+            while (cursor < trip) {
+                const uint32_t order = cursor + lane;
+                bool hit = false;
+                float3 mp, mc, mn, rp, rc, rn;
+                uint32_t t = 0;
+                if (order < trip) {
+                    t = dir == 0 ? order : trip - 1 - order;
+                    hit = triplet_projection(make_float3(X[t], Y[t], Z[t]), make_float3(X[t + 1], Y[t + 1], Z[t + 1]),
+                                             make_float3(X[t + 2], Y[t + 2], Z[t + 2]), K[t + 1], F[t] & kPinned,
+                                             F[t + 1] & kPinned, F[t + 2] & kPinned, E[t], E[t + 1], E[t + 2], g,
+                                             stiffness, margin, mp, mc, mn, rp, rc, rn);
+                }
+                const uint64_t mask = (uint64_t)__ballot(hit);
+                if (mask == 0) {
+                    cursor += 64;
+                    continue;
+                }
+                const uint32_t j = (uint32_t)__builtin_ctzll(mask);
+                if (lane == j) {
+                    const bool pp = F[t] & kPinned, pc = F[t + 1] & kPinned, pn = F[t + 2] & kPinned;
+                    X[t] = mp.x; Y[t] = mp.y; Z[t] = mp.z;
+                    X[t + 1] = mc.x; Y[t + 1] = mc.y; Z[t + 1] = mc.z;
+                    X[t + 2] = mn.x; Y[t + 2] = mn.y; Z[t + 2] = mn.z;
+                    if (track_walls) {
+                        if (!pp) { WX[t] += rp.x; WY[t] += rp.y; WZ[t] += rp.z; }
+                        if (!pc) { WX[t + 1] += rc.x; WY[t + 1] += rc.y; WZ[t + 1] += rc.z; }
+                        if (!pn) { WX[t + 2] += rn.x; WY[t + 2] += rn.y; WZ[t + 2] += rn.z; }
+                    }
+                }
+                any = true;
+                cursor += j + 1;
+                __syncthreads();
+            }
+            if (dir == 0 && !any) done = true;
+        }
+    }
+    __syncthreads();
+    const float limit = half_skin * half_skin;
+    bool flag = false;
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t v = start + i;
+        const float4 old = in[v];
+        pos[v] = make_float4(X[i], Y[i], Z[i], old.w);
+        if (track_walls) wall[v] = make_float4(WX[i], WY[i], WZ[i], 0.f);
+        const float4 r = refpos[v];
+        float dx = X[i] - r.x, dy = Y[i] - r.y, dz = Z[i] - r.z;
+        if (dx * dx + dy * dy + dz * dz > limit) flag = true;
+    }
+    if (__ballot(flag) != 0 && lane == 0) st->nstate[0] = 1u;
 }
 
-__global__ __launch_bounds__(64) void k_curvature_gs(DevState *st, GridParams g, const float4 *in, float4 *pos,
-                                                     const float4 *refpos, const float4 *vA, const float4 *vB,
-                                                     const uint2 *fspan, float4 *wall, uint32_t F, uint32_t sweeps,
-                                                     float stiffness, float margin, float half_skin,
-                                                     uint32_t track_walls) {
+// Long fibers that do not fit in LDS: one lane per fiber walks the chain in
+// global memory with the same evaluate-then-apply projection.
+__global__ __launch_bounds__(64) void k_curvature_gs_lane(DevState *st, GridParams g, const float4 *in, float4 *pos,
+                                                          const float4 *refpos, const float4 *vA, const float4 *vB,
+                                                          const uint2 *fspan, float4 *wall, uint32_t F,
+                                                          uint32_t sweeps, float stiffness, float margin,
+                                                          float half_skin, uint32_t track_walls) {
     if (st->control[0] == 0) return;
-    uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     const uint2 span = fspan[f];
-    const uint32_t start = span.x, count = span.y;
-    if (in != pos) {
-        for (uint32_t v = start; v < start + count; ++v) pos[v] = in[v];
-    }
-    const float3 zero3 = make_float3(0.f, 0.f, 0.f);
-    if (count >= 3) {
-        for (uint32_t sweep = 0; sweep < sweeps; ++sweep) {
-            bool any = false;
-            for (int dir = 0; dir < 2; ++dir) {
-                // Window over three consecutive vertices in visiting order:
-                // a (oldest), b, c (just loaded). Forward, (a, b, c) is
-                // (previous, center, next); backward it is (next, center,
-                // previous) in material order.
-                const int32_t stepv = dir == 0 ? 1 : -1;
-                const int32_t first = dir == 0 ? (int32_t)start : (int32_t)(start + count - 1);
-                int32_t ia = first, ib = first + stepv;
-                float3 a = load3(pos, ia), b = load3(pos, ib);
-                float3 wa = track_walls ? load3(wall, ia) : zero3, wb = track_walls ? load3(wall, ib) : zero3;
-                bool moved_a = false, moved_b = false;
-                for (uint32_t l = 2; l < count; ++l) {
-                    const int32_t ic = first + stepv * (int32_t)l;
-                    float3 c = load3(pos, ic);
-                    float3 wc = track_walls ? load3(wall, ic) : zero3;
-                    const int32_t ip = dir == 0 ? ia : ic, inx = dir == 0 ? ic : ia;
-                    const float4 sp = vA[ip], sc = vA[ib], sn = vA[inx];
-                    const uint32_t fp = __float_as_uint(vB[ip].z), fc = __float_as_uint(vB[ib].z),
-                                   fn = __float_as_uint(vB[inx].z);
-                    bool ran = dir == 0
-                                   ? project_triplet(a, b, c, sc.x, fp & kPinned, fc & kPinned, fn & kPinned, sp.w,
-                                                     sc.w, sn.w, g, stiffness, margin, wa, wb, wc)
-                                   : project_triplet(c, b, a, sc.x, fp & kPinned, fc & kPinned, fn & kPinned, sp.w,
-                                                     sc.w, sn.w, g, stiffness, margin, wc, wb, wa);
-                    if (ran) any = true;
-                    moved_a = moved_a || ran;
-                    if (moved_a) {  // vertex a leaves the window
-                        pos[ia] = make_float4(a.x, a.y, a.z, pos[ia].w);
-                        if (track_walls) wall[ia] = make_float4(wa.x, wa.y, wa.z, 0.f);
-                    }
-                    a = b; wa = wb; moved_a = moved_b || ran; ia = ib;
-                    b = c; wb = wc; moved_b = ran; ib = ic;
+    const uint32_t start = span.x, n = span.y;
+    if (in != pos)
+        for (uint32_t v = start; v < start + n; ++v) pos[v] = in[v];
+    const uint32_t trip = n >= 3 ? n - 2 : 0;
+    for (uint32_t sweep = 0; sweep < sweeps && trip > 0; ++sweep) {
+        bool any = false;
+        for (int dir = 0; dir < 2; ++dir) {
+            for (uint32_t order = 0; order < trip; ++order) {
+                const uint32_t t = start + (dir == 0 ? order : trip - 1 - order);
+                const float4 A = pos[t], B = pos[t + 1], C = pos[t + 2];
+                const float4 sa = vA[t], sb = vA[t + 1], sc = vA[t + 2];
+                const uint32_t fa = __float_as_uint(vB[t].z), fb = __float_as_uint(vB[t + 1].z),
+                               fc = __float_as_uint(vB[t + 2].z);
+                float3 mp, mc, mn, rp, rc, rn;
+                if (!triplet_projection(make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
+                                        make_float3(C.x, C.y, C.z), sb.x, fa & kPinned, fb & kPinned, fc & kPinned,
+                                        sa.w, sb.w, sc.w, g, stiffness, margin, mp, mc, mn, rp, rc, rn))
+                    continue;
+                any = true;
+                pos[t] = make_float4(mp.x, mp.y, mp.z, A.w);
+                pos[t + 1] = make_float4(mc.x, mc.y, mc.z, B.w);
+                pos[t + 2] = make_float4(mn.x, mn.y, mn.z, C.w);
+                if (track_walls) {
+                    if (!(fa & kPinned)) { float4 w = wall[t]; wall[t] = make_float4(w.x + rp.x, w.y + rp.y, w.z + rp.z, 0.f); }
+                    if (!(fb & kPinned)) { float4 w = wall[t + 1]; wall[t + 1] = make_float4(w.x + rc.x, w.y + rc.y, w.z + rc.z, 0.f); }
+                    if (!(fc & kPinned)) { float4 w = wall[t + 2]; wall[t + 2] = make_float4(w.x + rn.x, w.y + rn.y, w.z + rn.z, 0.f); }
                 }
-                if (moved_a) {
-                    pos[ia] = make_float4(a.x, a.y, a.z, pos[ia].w);
-                    if (track_walls) wall[ia] = make_float4(wa.x, wa.y, wa.z, 0.f);
-                }
-                if (moved_b) {
-                    pos[ib] = make_float4(b.x, b.y, b.z, pos[ib].w);
-                    if (track_walls) wall[ib] = make_float4(wb.x, wb.y, wb.z, 0.f);
-                }
-                if (dir == 0 && !any) break;
             }
-            if (!any) break;
+            if (dir == 0 && !any) break;
         }
+        if (!any) break;
     }
     const float limit = half_skin * half_skin;
     bool flag = false;
-    for (uint32_t v = start; v < start + count; ++v) {
+    for (uint32_t v = start; v < start + n; ++v) {
         float4 p = pos[v], r = refpos[v];
         float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
         if (dx * dx + dy * dy + dz * dz > limit) flag = true;
     }
     if (flag) st->nstate[0] = 1u;
+}
+
+// Rebuild on request: request_neighbor_list_rebuild (neighbor.rs:434-441),
+// the rebuild, and finish_neighbor_list_rebuild.
+__global__ void k_request_rebuild(DevState *st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->nstate[0] = 1u;
+}
+
+__global__ void k_finish_rebuild(DevState *st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || st->nstate[0] == 0) return;
+    st->nstate[0] = 0u;
+    st->nstate[1] += 1u;
 }
 
 }  // namespace tgh
@@ -835,7 +1064,10 @@ namespace tgh {
 
 namespace {
 inline uint32_t blocks_for(uint64_t n, uint32_t threads) { return (uint32_t)((n + threads - 1) / threads); }
+constexpr int kContactGroup = 16;
 }  // namespace
+
+size_t curvature_lds_bytes(uint32_t max_vertices) { return (size_t)max_vertices * 9u * sizeof(float); }
 
 hipError_t launch_begin_batch(const WorldDev &w, hipStream_t stream) {
     hipLaunchKernelGGL(k_begin_batch, dim3(1), dim3(64), 0, stream, w.st);
@@ -857,26 +1089,13 @@ hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
     if (w.block <= kLdsListCapacity) {
         hipLaunchKernelGGL(k_build_lists<true>, dim3(blocks_for(w.S, kBuildThreads)), dim3(kBuildThreads), 0, stream,
                            w.st, w.g, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets, w.ncount,
-                           w.nbr, w.skin, w.block, w.S);
+                           w.nbr, w.ovf, w.skin, w.block, w.ovf_cap, w.S);
     } else {
         hipLaunchKernelGGL(k_build_lists<false>, dim3(blocks_for(w.S, kBuildThreads)), dim3(kBuildThreads), 0,
                            stream, w.st, w.g, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets,
-                           w.ncount, w.nbr, w.skin, w.block, w.S);
+                           w.ncount, w.nbr, w.ovf, w.skin, w.block, w.ovf_cap, w.S);
     }
     return hipGetLastError();
-}
-
-// Rebuilds the cell list and neighbour lists from the current positions now,
-// whatever the displacement trigger says: request_neighbor_list_rebuild
-// (neighbor.rs:434-441), the rebuild, and finish_neighbor_list_rebuild.
-__global__ void k_request_rebuild(DevState *st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) st->nstate[0] = 1u;
-}
-
-__global__ void k_finish_rebuild(DevState *st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || st->nstate[0] == 0) return;
-    st->nstate[0] = 0u;
-    st->nstate[1] += 1u;
 }
 
 hipError_t launch_rebuild_now(const WorldDev &w, hipStream_t stream) {
@@ -888,15 +1107,16 @@ hipError_t launch_rebuild_now(const WorldDev &w, hipStream_t stream) {
 }
 
 hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t stream) {
-    hipLaunchKernelGGL(k_contact, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.g, w.pos, w.sstat,
-                       w.cell_segs, w.seg_cell, w.cell_counts, w.cell_offsets, w.ncount, w.nbr, w.corr0, w.corr1,
-                       p.correction_fraction, p.aggregation, w.block, w.S);
+    constexpr uint32_t per_block = 256 / kContactGroup;
+    hipLaunchKernelGGL(k_contact<kContactGroup>, dim3(blocks_for(w.S, per_block)), dim3(256), 0, stream, w.st, w.g,
+                       w.pos, w.sstat, w.cell_segs, w.seg_cell, w.cell_counts, w.cell_offsets, w.ncount, w.nbr, w.ovf,
+                       w.corr0, w.corr1, w.red, p.correction_fraction, p.aggregation, w.block, w.ovf_cap, w.S);
     return hipGetLastError();
 }
 
 hipError_t launch_measure(const WorldDev &w, const IterParams &p, hipStream_t stream) {
     hipLaunchKernelGGL(k_measure_assess, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.st, w.pos, w.vA, w.vB,
-                       w.V, p.penetration_tolerance, p.curvature_ratio_tolerance, p.iterations, p.force_full);
+                       w.red, w.V, p.penetration_tolerance, p.curvature_ratio_tolerance, p.iterations, p.force_full);
     return hipGetLastError();
 }
 
@@ -910,7 +1130,7 @@ hipError_t launch_vertex_passes(const WorldDev &w, const IterParams &p, hipStrea
     float4 *src = w.pos, *dst = w.alt;
     for (uint32_t i = 0; i < passes; ++i) {
         hipLaunchKernelGGL(k_vertex_pass, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.st, w.g, src, dst,
-                           w.vA, w.vB, w.corr0, w.corr1, w.wall, w.V, p.stretch, p.bend, p.max_step,
+                           w.vA, w.vB, w.corr0, w.corr1, w.wall, w.red, w.V, p.stretch, p.bend, p.max_step,
                            p.constraint_iterations > 0 ? 1u : 0u, i + 1 == passes ? 1u : 0u, i == 0 ? 1u : 0u,
                            w.track_walls);
         float4 *t = src;
@@ -922,9 +1142,16 @@ hipError_t launch_vertex_passes(const WorldDev &w, const IterParams &p, hipStrea
 
 hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t stream) {
     const float4 *src = (vertex_passes(p) % 2 == 1) ? w.alt : w.pos;
-    hipLaunchKernelGGL(k_curvature_gs, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.st, w.g, src, w.pos,
-                       w.refpos, w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness,
-                       p.curvature_margin, 0.5f * w.skin, w.track_walls);
+    if (w.max_fiber_vertices <= kMaxLdsFiberVertices) {
+        hipLaunchKernelGGL(k_curvature_gs_wave, dim3(w.F), dim3(64), curvature_lds_bytes(w.max_fiber_vertices),
+                           stream, w.st, w.g, src, w.pos, w.refpos, w.vA, w.vB, w.fspan, w.wall, p.sweeps,
+                           p.curvature_stiffness, p.curvature_margin, 0.5f * w.skin, w.track_walls,
+                           w.max_fiber_vertices);
+    } else {
+        hipLaunchKernelGGL(k_curvature_gs_lane, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.st, w.g, src,
+                           w.pos, w.refpos, w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness,
+                           p.curvature_margin, 0.5f * w.skin, w.track_walls);
+    }
     return hipGetLastError();
 }
 

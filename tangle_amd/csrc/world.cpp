@@ -361,6 +361,7 @@ int upload_static(tgh_world *w) {
     HIP_TRY(hipMemset(d.corr0, 0, S * sizeof(float4)));
     HIP_TRY(hipMemset(d.corr1, 0, S * sizeof(float4)));
     HIP_TRY(hipMemset(d.ncount, 0, S * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(d.red, 0, 3 * tgh::kRedSlots * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_counts, 0, d.C * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_offsets, 0, d.C * sizeof(uint32_t)));
     DevState init;
@@ -402,6 +403,8 @@ int create_world(const tgh_assembly *a, const tgh_config *c, tgh_world **out) {
     d.C = w->size.C;
     d.skin = w->size.skin;
     d.block = w->size.block;
+    d.ovf_cap = std::max<uint32_t>(tgh::kOverflowListCapacity, d.block);
+    d.max_fiber_vertices = *std::max_element(p.fcount.begin(), p.fcount.end());
     d.track_walls = (p.periodic[0] && p.periodic[1] && p.periodic[2]) ? 0u : 1u;
     for (int k = 0; k < 3; ++k) {
         d.g.lower[k] = p.lower[k];
@@ -414,6 +417,8 @@ int create_world(const tgh_assembly *a, const tgh_config *c, tgh_world **out) {
     const size_t V = p.V, S = p.S, C = w->size.C;
     e = hipSuccess;
     if (e == hipSuccess) e = w->alloc(&d.st, 1);
+    if (e == hipSuccess) e = w->alloc(&d.red, 3 * tgh::kRedSlots);
+    if (e == hipSuccess) e = w->alloc(&d.ovf, std::max<size_t>(1, (size_t)d.ovf_cap * S));
     if (e == hipSuccess) e = w->alloc(&d.pos, V);
     if (e == hipSuccess) e = w->alloc(&d.alt, V);
     if (e == hipSuccess) e = w->alloc(&d.refpos, V);
@@ -665,7 +670,7 @@ int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t 
     if (!world || !counts || !lists) return set_error(TGH_ERR_INVALID, "null argument");
     const size_t S = world->d.S, B = world->d.block;
     if (list_capacity < S * B) return set_error(TGH_ERR_SIZE, "list buffer smaller than segment_count*neighbor_block");
-    std::vector<uint32_t> slot_seg(S), slot_count(S), cols(std::max<size_t>(1, S * B));
+    std::vector<uint32_t> slot_seg(S), slot_count(S), cols(std::max<size_t>(1, S * B));  // rows by slot
     HIP_TRY(hipStreamSynchronize(world->stream));
     HIP_TRY(hipMemcpy(slot_seg.data(), world->d.cell_segs, S * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(slot_count.data(), world->d.ncount, S * 4, hipMemcpyDeviceToHost));
@@ -675,7 +680,7 @@ int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t 
         const uint32_t seg = slot_seg[slot];
         counts[seg] = slot_count[slot];
         const size_t keep = std::min<size_t>(slot_count[slot], B);
-        for (size_t k = 0; k < keep; ++k) lists[(size_t)seg * B + k] = cols[k * S + slot];
+        for (size_t k = 0; k < keep; ++k) lists[(size_t)seg * B + k] = cols[slot * B + k];
     }
     return TGH_OK;
 }
