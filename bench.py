@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=8, help="timed relaxation batches of 128 iterations")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=128, help="iterations per step")
-    ap.add_argument("--tta-max-iterations", type=int, default=20000)
+    ap.add_argument("--tta-max-iterations", type=int, default=40000, help="budget of the final hard-gate stage")
+    ap.add_argument("--tta-warm", type=int, default=2560, help="default-settings iterations before the final stage")
     ap.add_argument("--skip-tta", action="store_true", help="skip the time-to-admissibility run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iterations", type=int, default=40, help="iterations in the CPU baseline sample")
@@ -176,16 +177,21 @@ def main():
     }
 
     if not args.skip_tta:
-        r = fx.bimodal_recipe(scale=args.scale)
-        r.relax_until_converged(max_iterations=args.tta_max_iterations)
-        tta_settings = fx.hard_gate_settings(d_min, max_iterations=args.tta_max_iterations)
-        tta = {"plain": None, "learned": None}
+        # plain HIP PBD plan to the hard gates: default relaxation, then a
+        # final SolvePolicy at the hard limits under the reference's
+        # contact_cleanup preset (the best of the schedules tried)
+        r = fx.staged_hard_gate_recipe(fx.bimodal_recipe(scale=args.scale), warm_iterations=args.tta_warm,
+                                       cleanup_iterations=args.tta_max_iterations)
+        tta_settings = fx.hard_gate_settings(d_min, max_iterations=args.tta_warm + args.tta_max_iterations + 1)
+        tta = {"plain": None, "learned": None, "gates": {"max_penetration_m": fx.HARD_PENETRATION,
+                                                           "max_curvature_ratio": fx.HARD_CURVATURE_RATIO}}
+        t0 = time.perf_counter()
         try:
-            t0 = time.perf_counter()
             res = r.run(tta_settings)
             wall = time.perf_counter() - t0
             tta["plain"] = {"seconds": round(wall, 3), "iterations": res.iterations, "converged": res.converged,
-                            "max_penetration": res.max_penetration, "max_curvature_ratio": res.max_curvature_ratio}
+                            "max_penetration": res.max_penetration, "max_curvature_ratio": res.max_curvature_ratio,
+                            "plan": r.operations()[2:]}
             positions = res.positions
         except tg.RecipeError as err:
             wall = time.perf_counter() - t0

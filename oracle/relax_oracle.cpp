@@ -833,25 +833,28 @@ void apply_contact(World &w, const tgh_config *c) {  // kernels.rs:1455-1537
 
 // ---- curvature Gauss–Seidel: kernels.rs:1165-1415 --------------------------
 
-void project_triplet(World &w, uint32_t prev, uint32_t center, uint32_t next, float stiffness, float margin) {
+// Diagnostics of the curvature cleanup (projections applied, sweeps run).
+uint64_t g_gs_projections = 0, g_gs_calls = 0, g_gs_fiber_sweeps = 0, g_gs_max_fiber_projections = 0;
+
+bool project_triplet(World &w, uint32_t prev, uint32_t center, uint32_t next, float stiffness, float margin) {
     float *P = w.pos.data();
     float kmax = w.vtx_kmax[center] * (1.0f - margin);
-    if (!(kmax > 0.f)) return;
+    if (!(kmax > 0.f)) return false;
     float ux = P[3 * center] - P[3 * prev], uy = P[3 * center + 1] - P[3 * prev + 1], uz = P[3 * center + 2] - P[3 * prev + 2];
     float vx = P[3 * next] - P[3 * center], vy = P[3 * next + 1] - P[3 * center + 1], vz = P[3 * next + 2] - P[3 * center + 2];
     float l1 = std::sqrt(ux * ux + uy * uy + uz * uz);
     float l2 = std::sqrt(vx * vx + vy * vy + vz * vz);
-    if (!(l1 > 1.0e-7f && l2 > 1.0e-7f)) return;
+    if (!(l1 > 1.0e-7f && l2 > 1.0e-7f)) return false;
     float unx = ux / l1, uny = uy / l1, unz = uz / l1;
     float vnx = vx / l2, vny = vy / l2, vnz = vz / l2;
     float cosine = rclamp(unx * vnx + uny * vny + unz * vnz, -1.f, 1.f);
     float sine_half = 0.25f * kmax * (l1 + l2);
-    if (!(sine_half < 1.0f)) return;
+    if (!(sine_half < 1.0f)) return false;
     float min_cos = 1.0f - 2.0f * sine_half * sine_half;
     float violation = min_cos - cosine;
     float sine = std::sqrt(rmax(1.0f - cosine * cosine, 0.f));
     if (violation > 0.f && sine > 1.0e-3f) violation = (std::acos(cosine) - std::acos(min_cos)) * sine;
-    if (!(violation > 0.f)) return;
+    if (!(violation > 0.f)) return false;
     float gux = (vnx - cosine * unx) / l1, guy = (vny - cosine * uny) / l1, guz = (vnz - cosine * unz) / l1;
     float gvx = (unx - cosine * vnx) / l2, gvy = (uny - cosine * vny) / l2, gvz = (unz - cosine * vnz) / l2;
     float gcx = gux - gvx, gcy = guy - gvy, gcz = guz - gvz;
@@ -860,7 +863,7 @@ void project_triplet(World &w, uint32_t prev, uint32_t center, uint32_t next, fl
     float wn = w.vtx_pinned[next] ? 0.f : 1.f;
     float den = wp * (gux * gux + guy * guy + guz * guz) + wc * (gcx * gcx + gcy * gcy + gcz * gcz) +
                 wn * (gvx * gvx + gvy * gvy + gvz * gvz);
-    if (!(den > 1.0e-12f)) return;
+    if (!(den > 1.0e-12f)) return false;
     float scale = stiffness * violation / den;
     float gp[3] = {gux, guy, guz}, gc[3] = {gcx, gcy, gcz}, gn[3] = {gvx, gvy, gvz};
     for (int k = 0; k < 3; ++k) {
@@ -881,21 +884,33 @@ void project_triplet(World &w, uint32_t prev, uint32_t center, uint32_t next, fl
         if (wc > 0.f) P[ic] = vc;
         if (wn > 0.f) P[in] = vn;
     }
+    return true;
 }
 
 void curvature_cleanup(World &w, const tgh_config *c) {
+    ++g_gs_calls;
     const float st = c->curvature_limit_stiffness, mg = c->curvature_limit_safety_margin;
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t fi = 0; fi < (int64_t)w.F; ++fi) {
         uint32_t start = w.fiber_span[2 * fi], count = w.fiber_span[2 * fi + 1];
         if (count < 3) continue;
+        uint64_t projections = 0, sweeps_run = 0;
         for (uint32_t sweep = 0; sweep < c->curvature_cleanup_sweeps; ++sweep) {
-            for (uint32_t v = start + 2; v < start + count; ++v) project_triplet(w, v - 2, v - 1, v, st, mg);
+            uint64_t forward = 0;
+            for (uint32_t v = start + 2; v < start + count; ++v) forward += project_triplet(w, v - 2, v - 1, v, st, mg);
             for (uint32_t l = 2; l < count; ++l) {
                 uint32_t v = start + count - 1 - l;
-                project_triplet(w, v, v + 1, v + 2, st, mg);
+                projections += project_triplet(w, v, v + 1, v + 2, st, mg);
             }
+            projections += forward;
+            if (forward != 0) ++sweeps_run;  // sweeps the GPU kernel's early exit still runs
         }
+#pragma omp atomic
+        g_gs_projections += projections;
+#pragma omp atomic
+        g_gs_fiber_sweeps += sweeps_run;
+#pragma omp critical
+        g_gs_max_fiber_projections = std::max(g_gs_max_fiber_projections, projections);
     }
 }
 
@@ -1040,6 +1055,17 @@ int orc_world_contact_state(void *h, float *corrections, float *segment_max, flo
     if (segment_max) std::memcpy(segment_max, w.seg_max.data(), w.seg_max.size() * 4);
     if (curvature) std::memcpy(curvature, w.curv.data(), w.curv.size() * 4);
     return TGH_OK;
+}
+
+// Curvature-cleanup counters since the last call: calls, projections
+// applied, fiber sweeps whose forward pass projected, most projections of
+// one fiber in one call.
+void orc_gs_counters(uint64_t *out) {
+    out[0] = g_gs_calls;
+    out[1] = g_gs_projections;
+    out[2] = g_gs_fiber_sweeps;
+    out[3] = g_gs_max_fiber_projections;
+    g_gs_calls = g_gs_projections = g_gs_fiber_sweeps = g_gs_max_fiber_projections = 0;
 }
 
 void orc_world_destroy(void *h) { delete (World *)h; }

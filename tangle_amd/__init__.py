@@ -37,6 +37,8 @@ __all__ = [
     "UniformPosition",
     "generate_fiber_population",
     "RelaxationSettings",
+    "RelaxationOverrides",
+    "SolvePolicy",
     "Recipe",
     "RecipeError",
     "RunResult",
@@ -403,6 +405,124 @@ class RelaxationSettings:
         return c
 
 
+_EXHAUSTION = ("fail", "continue_if_hard_ok")
+
+
+@dataclass
+class SolvePolicy:
+    """A relaxation gate of ``Recipe.solve`` (settings.rs:800-900,
+    tangle_generate/src/recipe.rs:380-481): the kernels solve toward the
+    targets, the stage is accepted when the measured residuals are within the
+    max_* limits, and ``max_iterations`` (plus ``max_extra_iterations``) bound it."""
+
+    name: str = "relaxation"
+    target_penetration: float = 1.0e-4
+    target_curvature_ratio: float = 1.00001
+    max_penetration: float | None = None
+    max_curvature_ratio: float | None = None
+    hard_penetration: bool = True
+    hard_curvature: bool = True
+    max_iterations: int = 2000
+    max_extra_iterations: int | None = None
+    on_budget_exhausted: str = "fail"
+
+    def __post_init__(self):
+        if not (math.isfinite(self.target_penetration) and self.target_penetration >= 0.0):
+            raise ValueError("target_penetration must be nonnegative and finite")
+        if not (math.isfinite(self.target_curvature_ratio) and self.target_curvature_ratio >= 1.0):
+            raise ValueError("target_curvature_ratio must be at least 1")
+        if self.max_iterations <= 0:
+            raise ValueError("max_iterations must be positive")
+        if self.on_budget_exhausted not in _EXHAUSTION:
+            raise ValueError(f"on_budget_exhausted must be one of {list(_EXHAUSTION)}")
+
+    @property
+    def penetration_limit(self) -> float:
+        return self.target_penetration if self.max_penetration is None else self.max_penetration
+
+    @property
+    def curvature_limit(self) -> float:
+        return self.target_curvature_ratio if self.max_curvature_ratio is None else self.max_curvature_ratio
+
+    @property
+    def extra_iterations(self) -> int:
+        return self.max_iterations // 2 if self.max_extra_iterations is None else self.max_extra_iterations
+
+    def accepts(self, penetration: float, curvature: float) -> bool:
+        return penetration <= self.penetration_limit and curvature <= self.curvature_limit
+
+    def hard_limits_hold(self, penetration: float, curvature: float) -> bool:
+        return ((not self.hard_penetration or penetration <= self.penetration_limit)
+                and (not self.hard_curvature or curvature <= self.curvature_limit))
+
+    def copy(self) -> "SolvePolicy":
+        return _dc_replace(self)
+
+    def replace(self, **changes) -> "SolvePolicy":
+        return _dc_replace(self, **changes)
+
+
+# (correction_fraction, stretch, curvature_limit_stiffness, constraint_iterations,
+# curvature_cleanup_sweeps); every preset uses deepest-only contact aggregation
+# and no rest-chord bending (settings.rs:645-690).
+_PRESETS = {
+    "contact_first": (0.8, 0.05, 0.15, 1, 1),
+    "curvature_cleanup": (0.05, 0.35, 1.0, 8, 16),
+    "contact_cleanup": (0.8, 0.35, 0.75, 8, 16),
+}
+
+
+@dataclass
+class RelaxationOverrides:
+    """Per-stage replacements for RelaxationSettings fields, layered onto each
+    batch of the stage (plugin.rs:902-942, 1069-1102)."""
+
+    motion_model: str | None = None
+    correction_fraction: float | None = None
+    contact_aggregation: str | None = None
+    stretch_stiffness: float | None = None
+    bend_stiffness: float | None = None
+    curvature_limit_stiffness: float | None = None
+    constraint_iterations: int | None = None
+    curvature_cleanup_sweeps: int | None = None
+
+    @classmethod
+    def preset(cls, name: str, **changes) -> "RelaxationOverrides":
+        if name not in _PRESETS:
+            raise ValueError(f"overrides preset must be one of {list(_PRESETS)}")
+        cf, stretch, limit, iterations, sweeps = _PRESETS[name]
+        base = cls(motion_model="flexible", correction_fraction=cf, contact_aggregation="deepest_only",
+                   stretch_stiffness=stretch, bend_stiffness=0.0, curvature_limit_stiffness=limit,
+                   constraint_iterations=iterations, curvature_cleanup_sweeps=sweeps)
+        return _dc_replace(base, **changes)
+
+    def copy(self) -> "RelaxationOverrides":
+        return _dc_replace(self)
+
+    def replace(self, **changes) -> "RelaxationOverrides":
+        return _dc_replace(self, **changes)
+
+    def apply(self, config: "_native.Config") -> "_native.Config":
+        out = _native.Config.from_buffer_copy(config)
+        if self.motion_model not in (None, "flexible"):
+            raise ValueError("the hip backend implements motion_model='flexible'")
+        if self.correction_fraction is not None:
+            out.correction_fraction = self.correction_fraction
+        if self.contact_aggregation is not None:
+            out.contact_aggregation = _AGGREGATIONS[self.contact_aggregation]
+        if self.stretch_stiffness is not None:
+            out.stretch_stiffness = self.stretch_stiffness
+        if self.bend_stiffness is not None:
+            out.bend_stiffness = self.bend_stiffness
+        if self.curvature_limit_stiffness is not None:
+            out.curvature_limit_stiffness = self.curvature_limit_stiffness
+        if self.constraint_iterations is not None:
+            out.constraint_iterations = self.constraint_iterations
+        if self.curvature_cleanup_sweeps is not None:
+            out.curvature_cleanup_sweeps = self.curvature_cleanup_sweeps
+        return out
+
+
 class RecipeError(RuntimeError):
     """A recipe operation failed (tangle_python/src/recipe.rs:705-722)."""
 
@@ -449,6 +569,101 @@ class RunResult:
                 f"max_penetration={self.max_penetration:.3e}, max_curvature_ratio={self.max_curvature_ratio:.6f})")
 
 
+class _RunState:
+    """Batch controller of one Recipe.run: sizes each device batch, layers
+    stage settings onto it and applies the stage's gate (plugin.rs:1041-1181,
+    tangle_generate/src/recipe.rs:1311-1518)."""
+
+    def __init__(self, settings: RelaxationSettings, config: "_native.Config"):
+        self.settings = settings
+        self.config = config
+        self.total = 0
+        self.status = None
+        self.converged = False
+        self.events: list[str] = []
+        self.warnings: list[str] = []
+        self.uploaded = 0
+        self.downloaded = 0
+
+    def _batch(self, world, config, iterations):
+        status = world.run_batch(config, iterations)
+        self.downloaded += 11 * 4 + 3 * 4
+        self.total = int(status.total_iterations)
+        self.status = status
+        self.converged = bool(status.converged)
+        return status
+
+    def _budget(self, index, description):
+        if self.total >= self.settings.max_iterations:
+            raise RecipeError(index, description, self.total,
+                              f"iteration budget {self.settings.max_iterations} exhausted")
+        return self.settings.max_iterations - self.total
+
+    def _residuals(self):
+        s = self.status
+        return (f"penetration {s.max_penetration:.3e}, curvature ratio {s.max_curvature_ratio:.6f}"
+                if s is not None else "no measurement")
+
+    def relax(self, world, index, description, kind, amount):
+        begin = self.total
+        config = _native.Config.from_buffer_copy(self.config)
+        force_full = kind == "relax_for"
+        config.force_full_iterations = 1 if (force_full or self.settings.force_full_iterations) else 0
+        self.converged = False
+        while True:
+            completed = self.total - begin
+            if kind == "relax_until_converged" and self.converged:
+                self.events.append(f"relax to convergence in {completed} iterations ({self._residuals()})")
+                return
+            if completed >= amount:
+                if force_full:
+                    self.events.append(f"relax for {amount} iterations")
+                    return
+                raise RecipeError(index, description, self.total,
+                                  f"failed to converge in {amount} iterations: {self._residuals()}")
+            remaining = self._budget(index, description)
+            batch = min(remaining, self.settings.iterations_per_batch, amount - completed)
+            status = self._batch(world, config, batch)
+            if status.cell_list_overflow:
+                raise RecipeError(index, description, self.total, "cell list overflow")
+
+    def solve(self, world, index, description, policy: "SolvePolicy", overrides):
+        begin = self.total
+        config = _native.Config.from_buffer_copy(self.config)
+        if overrides is not None:
+            config = overrides.apply(config)
+        config.force_full_iterations = 0
+        config.penetration_tolerance = policy.target_penetration
+        config.curvature_ratio_tolerance = max(policy.target_curvature_ratio - 1.0, 0.0)
+        measured = False
+        while True:
+            completed = self.total - begin
+            s = self.status
+            if measured and s is not None and policy.accepts(s.max_penetration, s.max_curvature_ratio):
+                self.events.append(f"{policy.name} accepted in {completed} iterations ({self._residuals()})")
+                return
+            if completed < policy.max_iterations:
+                remaining = policy.max_iterations - completed
+            elif (policy.on_budget_exhausted == "continue_if_hard_ok" and s is not None
+                  and policy.hard_limits_hold(s.max_penetration, s.max_curvature_ratio)):
+                self.warnings.append(f"{policy.name} exhausted {policy.max_iterations} iterations with deferred "
+                                     f"soft limits: {self._residuals()}")
+                return
+            elif completed < policy.max_iterations + policy.extra_iterations and \
+                    self.settings.max_iterations - self.total > 1:
+                remaining = min(policy.max_iterations + policy.extra_iterations - completed,
+                                self.settings.max_iterations - self.total - 1)
+            else:
+                raise RecipeError(index, description, self.total,
+                                  f"{policy.name} failed after {completed} iterations: {self._residuals()}")
+            run_remaining = self._budget(index, description)
+            batch = max(1, min(run_remaining, self.settings.iterations_per_batch, remaining))
+            status = self._batch(world, config, batch)
+            measured = True
+            if status.cell_list_overflow:
+                raise RecipeError(index, description, self.total, "cell list overflow")
+
+
 class Recipe:
     """An ordered list of steps on one cell; nothing runs until ``run``."""
 
@@ -487,6 +702,16 @@ class Recipe:
         self._operations.append(("relax_for", int(iterations)))
         self._descriptions.append(f"relax for {iterations} iterations")
 
+    def solve(self, policy: SolvePolicy, overrides: RelaxationOverrides | None = None) -> None:
+        """Relax under `policy`'s targets and gate, with `overrides` layered
+        onto every batch of the stage (recipe.rs:1398-1518)."""
+        if not isinstance(policy, SolvePolicy):
+            raise TypeError("solve takes a SolvePolicy")
+        if overrides is not None and not isinstance(overrides, RelaxationOverrides):
+            raise TypeError("overrides must be RelaxationOverrides")
+        self._operations.append(("solve", (policy, overrides)))
+        self._descriptions.append(f"solve {policy.name}")
+
     def operations(self) -> list[str]:
         return list(self._descriptions)
 
@@ -521,57 +746,29 @@ class Recipe:
             operations.append(("relax_until_converged", settings.max_iterations))
             descriptions.append("relax until converged")
         packed = self._packed(settings.pin_fiber_ends)
-        events: list[str] = []
+        run = _RunState(settings, config)
         with _native.World(packed, config) as world:
             info = world.info()
-            uploaded = int(info.device_bytes)
-            downloaded = 0
-            total = 0
-            status = None
-            converged = False
+            run.uploaded = int(info.device_bytes)
             for index, (kind, amount) in enumerate(operations):
                 if kind == "insert":
                     continue
-                begin = total
-                force_full = kind == "relax_for"
-                batch_config = _native.Config.from_buffer_copy(config)
-                batch_config.force_full_iterations = 1 if (force_full or settings.force_full_iterations) else 0
-                converged = False
-                while True:
-                    completed = total - begin
-                    if kind == "relax_until_converged" and converged:
-                        events.append(f"relax to convergence in {completed} iterations (penetration "
-                                      f"{status.max_penetration:.3e}, curvature ratio {status.max_curvature_ratio:.6f})")
-                        break
-                    if completed >= amount:
-                        if kind == "relax_for":
-                            events.append(f"relax for {amount} iterations")
-                            break
-                        raise RecipeError(index, descriptions[index], total,
-                                          f"failed to converge in {amount} iterations: penetration "
-                                          f"{status.max_penetration:.3e}, curvature ratio "
-                                          f"{status.max_curvature_ratio:.6f}")
-                    if total >= settings.max_iterations:
-                        raise RecipeError(index, descriptions[index], total,
-                                          f"iteration budget {settings.max_iterations} exhausted")
-                    batch = min(settings.max_iterations - total, settings.iterations_per_batch, amount - completed)
-                    status = world.run_batch(batch_config, batch)
-                    downloaded += 11 * 4 + 3 * 4
-                    total = int(status.total_iterations)
-                    converged = bool(status.converged)
-                    if status.cell_list_overflow:
-                        raise RecipeError(index, descriptions[index], total, "cell list overflow")
+                if kind == "solve":
+                    run.solve(world, index, descriptions[index], *amount)
+                else:
+                    run.relax(world, index, descriptions[index], kind, amount)
             positions = world.download_positions()
-            downloaded += positions.nbytes
+            run.downloaded += positions.nbytes
             rebuilds = int(world.info().neighbor_list_rebuilds)
+        status = run.status
         wall = time.perf_counter() - started
         return RunResult(
-            fiber_count=len(self._fibers), iterations=total, converged=converged,
+            fiber_count=len(self._fibers), iterations=run.total, converged=run.converged,
             max_penetration=float(status.max_penetration) if status else float("inf"),
             max_curvature_ratio=float(status.max_curvature_ratio) if status else float("inf"),
             max_displacement=float(status.max_displacement) if status else 0.0,
             active_segments=int(info.segment_count), active_vertices=int(info.vertex_count),
-            cell_count=int(info.cell_count), uploaded_bytes=uploaded, downloaded_bytes=downloaded,
-            events=events, warnings=[], neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
+            cell_count=int(info.cell_count), uploaded_bytes=run.uploaded, downloaded_bytes=run.downloaded,
+            events=run.events, warnings=run.warnings, neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
             positions=positions, vertex_counts=packed.counts.copy(),
         )

@@ -182,3 +182,20 @@ def bimodal_recipe(scale=1.0, box=1e-3, length=1e-3, beta=1.0 / 0.3):
 def hard_gate_settings(d_min, **changes):
     """Hard admissibility: penetration <= 1e-3 d_min, curvature ratio <= 1 + 1e-5."""
     return tg.RelaxationSettings(penetration_tolerance=1e-3 * d_min, curvature_ratio_tolerance=1e-5, **changes)
+
+
+HARD_PENETRATION = 1e-3 * 7e-6   # 1e-3 d_min
+HARD_CURVATURE_RATIO = 1.0 + 1e-5
+
+
+def staged_hard_gate_recipe(recipe, warm_iterations=1280, cleanup_iterations=40000, preset="contact_cleanup"):
+    """The plain path's plan to the hard gates, in the reference's recipe
+    vocabulary: `warm_iterations` of default relaxation, then a final
+    SolvePolicy at the hard limits under a cleanup preset (as the reference's
+    tutorials finish their felts, crates/tangle_python/python/tutorials/generate.py:745-753)."""
+    if warm_iterations:
+        recipe.relax_for(warm_iterations)
+    policy = tg.SolvePolicy("final", target_penetration=HARD_PENETRATION, target_curvature_ratio=HARD_CURVATURE_RATIO,
+                            max_iterations=cleanup_iterations, max_extra_iterations=0)
+    recipe.solve(policy, tg.RelaxationOverrides.preset(preset) if preset else None)
+    return recipe
