@@ -149,6 +149,46 @@ int tgh_world_timing(tgh_world *world, double *group_ms, uint64_t *group_launche
 void tgh_world_destroy(tgh_world *world);
 const char *tgh_last_error(void);
 
+/* ---- learned warm start (no counterpart in the reference) -------------
+ * A per-segment MLP 64 -> 64 -> 64 -> 16 (51 inputs used: for the 16 nearest
+ * listed neighbours the unit closest-point direction times
+ * min((r1 + r2) / distance, 8), then the unit tangent; 3 outputs used: the
+ * segment's predicted displacement in diameters), BF16 operands with FP32
+ * accumulation on MFMA, trained on the GPU from trajectories recorded from
+ * the plain relaxer. Parameters are one f32 vector: W1 [64][64], b1 [64],
+ * W2 [64][64], b2 [64], W3 [16][64], b3 [16] (weights [out][in]). */
+typedef struct tgh_mlp tgh_mlp;
+size_t tgh_mlp_parameter_count(void);
+int tgh_mlp_create(uint64_t seed, tgh_mlp **out);
+void tgh_mlp_destroy(tgh_mlp *mlp);
+int tgh_mlp_get_parameters(const tgh_mlp *mlp, float *out, size_t count);
+int tgh_mlp_set_parameters(tgh_mlp *mlp, const float *in, size_t count);
+/* features: rows x 64 bf16 bit patterns; out: rows x 3 */
+int tgh_mlp_forward(tgh_mlp *mlp, const uint16_t *features, size_t rows, float *out);
+/* loss = mean over rows of the squared error summed over the 3 outputs;
+ * grad: tgh_mlp_parameter_count() floats */
+int tgh_mlp_gradients(tgh_mlp *mlp, const uint16_t *features, const float *targets, size_t rows, float *grad,
+                      float *loss);
+/* Trajectory recording: features and segment midpoints of `world` now, then
+ * (after the caller relaxes it) the midpoint displacement in diameters as
+ * the targets; the pair of calls appends segment_count rows. */
+int tgh_mlp_record_begin(tgh_mlp *mlp, tgh_world *world);
+int tgh_mlp_record_end(tgh_mlp *mlp, tgh_world *world);
+uint64_t tgh_mlp_dataset_rows(const tgh_mlp *mlp);
+int tgh_mlp_dataset_clear(tgh_mlp *mlp);
+/* Adam over shuffled minibatches of the recorded rows; epoch_losses: [epochs] or NULL */
+int tgh_mlp_train(tgh_mlp *mlp, uint32_t epochs, uint32_t batch_rows, float learning_rate, uint64_t seed,
+                  float *epoch_losses);
+/* features of every segment at the world's current positions: segment_count x 64 bf16 */
+int tgh_world_features(tgh_world *world, uint16_t *out, size_t count);
+/* Moves every unpinned vertex by `scale` times the mean predicted
+ * displacement of its segments, keeping the previous positions for
+ * tgh_world_revert_proposal; both request a neighbour-list rebuild. */
+int tgh_world_propose(tgh_world *world, tgh_mlp *mlp, float scale);
+int tgh_world_revert_proposal(tgh_world *world);
+/* Matrix-core layout check: d (16x16 f32) = a (16x16 bf16) * b (16x16 bf16), row-major. */
+int tgh_mfma_probe(const uint16_t *a, const uint16_t *b, float *d);
+
 /* ---- seeded population generator (population.rs:246-357, 554-813) ---- */
 
 #define TGH_ORIENT_ISOTROPIC 0   /* OrientationDistribution::Isotropic3d           */

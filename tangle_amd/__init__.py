@@ -574,7 +574,7 @@ class _RunState:
     stage settings onto it and applies the stage's gate (plugin.rs:1041-1181,
     tangle_generate/src/recipe.rs:1311-1518)."""
 
-    def __init__(self, settings: RelaxationSettings, config: "_native.Config"):
+    def __init__(self, settings: RelaxationSettings, config: "_native.Config", warm_start=None):
         self.settings = settings
         self.config = config
         self.total = 0
@@ -584,6 +584,44 @@ class _RunState:
         self.warnings: list[str] = []
         self.uploaded = 0
         self.downloaded = 0
+        self.warm_start = warm_start
+        self.last_proposal = 0
+        self.proposals_kept = 0
+        self.proposals_reverted = 0
+
+    def _propose(self, world, config):
+        """Learned warm start: every `interval` iterations of the first
+        `active_iterations`, move the vertices by the network's prediction,
+        run `settle` plain iterations, and revert the move (keeping the
+        iteration count) if the normalised residual max(penetration /
+        tolerance, curvature excess / tolerance) rose above `tolerance` times
+        its value before the move. Plain PBD still runs every later pass."""
+        ws = self.warm_start
+        if ws is None or self.total >= ws.active_iterations or self.total - self.last_proposal < ws.interval:
+            return
+        from . import learned
+        self.last_proposal = self.total
+        pen_tol = max(config.penetration_tolerance, 1e-30)
+        curv_tol = max(config.curvature_ratio_tolerance, 1e-30)
+
+        def residual(st):
+            return max(st.max_penetration / pen_tol, (st.max_curvature_ratio - 1.0) / curv_tol)
+
+        before = residual(learned.measure(world, config))
+        learned.propose(world, ws.model, ws.scale)
+        if ws.settle > 0:
+            settle = _native.Config.from_buffer_copy(config)
+            settle.force_full_iterations = 1
+            self._batch(world, settle, ws.settle)
+        after = residual(learned.measure(world, config))
+        if after > ws.tolerance * before:
+            learned.revert(world)
+            self.proposals_reverted += 1
+            # the last measurement describes the reverted positions again
+            self.status = learned.measure(world, config)
+            self.converged = False
+        else:
+            self.proposals_kept += 1
 
     def _batch(self, world, config, iterations):
         status = world.run_batch(config, iterations)
@@ -592,6 +630,15 @@ class _RunState:
         self.status = status
         self.converged = bool(status.converged)
         return status
+
+    def _proposal_window(self, world, config, batch):
+        """Proposes when one is due and shortens the batch so that the next
+        proposal falls on a batch boundary."""
+        ws = self.warm_start
+        if ws is None or self.total >= ws.active_iterations:
+            return batch
+        self._propose(world, config)
+        return max(1, min(batch, self.last_proposal + ws.interval - self.total))
 
     def _budget(self, index, description):
         if self.total >= self.settings.max_iterations:
@@ -623,6 +670,7 @@ class _RunState:
                                   f"failed to converge in {amount} iterations: {self._residuals()}")
             remaining = self._budget(index, description)
             batch = min(remaining, self.settings.iterations_per_batch, amount - completed)
+            batch = self._proposal_window(world, config, batch)
             status = self._batch(world, config, batch)
             if status.cell_list_overflow:
                 raise RecipeError(index, description, self.total, "cell list overflow")
@@ -658,6 +706,7 @@ class _RunState:
                                   f"{policy.name} failed after {completed} iterations: {self._residuals()}")
             run_remaining = self._budget(index, description)
             batch = max(1, min(run_remaining, self.settings.iterations_per_batch, remaining))
+            batch = self._proposal_window(world, config, batch)
             status = self._batch(world, config, batch)
             measured = True
             if status.cell_list_overflow:
@@ -729,9 +778,11 @@ class Recipe:
         return _native.PackedAssembly(counts, placed, intrinsic, radius, bend, self._cell.origin,
                                       self._cell.lengths, self._cell.periodic, pin_fiber_ends=pin_fiber_ends)
 
-    def run(self, settings: RelaxationSettings | None = None) -> RunResult:
+    def run(self, settings: RelaxationSettings | None = None, *, warm_start=None) -> RunResult:
         """Simulates every recorded step (recipe.rs:618-723 and the batch
-        controller plugin.rs:1041-1181, tangle_generate/src/recipe.rs:1311-1397)."""
+        controller plugin.rs:1041-1181, tangle_generate/src/recipe.rs:1311-1397).
+        `warm_start` (a tangle_amd.learned.WarmStart) adds learned proposals,
+        which have no counterpart in the reference."""
         if not self._fibers:
             raise ValueError("recipe contains no fibers")
         if not self._operations:
@@ -746,7 +797,7 @@ class Recipe:
             operations.append(("relax_until_converged", settings.max_iterations))
             descriptions.append("relax until converged")
         packed = self._packed(settings.pin_fiber_ends)
-        run = _RunState(settings, config)
+        run = _RunState(settings, config, warm_start)
         with _native.World(packed, config) as world:
             info = world.info()
             run.uploaded = int(info.device_bytes)
@@ -769,6 +820,8 @@ class Recipe:
             max_displacement=float(status.max_displacement) if status else 0.0,
             active_segments=int(info.segment_count), active_vertices=int(info.vertex_count),
             cell_count=int(info.cell_count), uploaded_bytes=run.uploaded, downloaded_bytes=run.downloaded,
-            events=run.events, warnings=run.warnings, neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
+            events=run.events + ([f"warm start: {run.proposals_kept} proposals kept, "
+                                  f"{run.proposals_reverted} reverted"] if warm_start else []),
+            warnings=run.warnings, neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
             positions=positions, vertex_counts=packed.counts.copy(),
         )

@@ -138,6 +138,22 @@ SIGNATURES = {
     "tgh_world_destroy": (None, [C.c_void_p]),
     "tgh_last_error": (C.c_char_p, []),
     "tgh_generate_population": (C.c_int, [C.POINTER(PopulationSpec), _f64p, _f64p, _f64p, C.c_size_t]),
+    "tgh_mlp_parameter_count": (C.c_size_t, []),
+    "tgh_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "tgh_mlp_destroy": (None, [C.c_void_p]),
+    "tgh_mlp_get_parameters": (C.c_int, [C.c_void_p, _f32p, C.c_size_t]),
+    "tgh_mlp_set_parameters": (C.c_int, [C.c_void_p, _f32p, C.c_size_t]),
+    "tgh_mlp_forward": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint16), C.c_size_t, _f32p]),
+    "tgh_mlp_gradients": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint16), _f32p, C.c_size_t, _f32p, _f32p]),
+    "tgh_mlp_record_begin": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tgh_mlp_record_end": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tgh_mlp_dataset_rows": (C.c_uint64, [C.c_void_p]),
+    "tgh_mlp_dataset_clear": (C.c_int, [C.c_void_p]),
+    "tgh_mlp_train": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _f32p]),
+    "tgh_world_features": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint16), C.c_size_t]),
+    "tgh_world_propose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float]),
+    "tgh_world_revert_proposal": (C.c_int, [C.c_void_p]),
+    "tgh_mfma_probe": (C.c_int, [C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), _f32p]),
 }
 
 _lock = threading.Lock()

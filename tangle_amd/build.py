@@ -23,7 +23,13 @@ LIB_DIR = PACKAGE / "_lib"
 LIBRARY = LIB_DIR / "libtangle_hip.so"
 
 # (source, language): host code that calls the HIP runtime is compiled as HIP
-SOURCES = [("relax_kernels.hip", "hip"), ("world.cpp", "hip"), ("population.cpp", "c++")]
+SOURCES = [
+    ("relax_kernels.hip", "hip"),
+    ("learned.hip", "hip"),
+    ("world.cpp", "hip"),
+    ("learned_host.cpp", "hip"),
+    ("population.cpp", "c++"),
+]
 ARCH = os.environ.get("TANGLE_HIP_ARCH", "gfx950")
 
 COMMON_FLAGS = [

@@ -1,0 +1,409 @@
+// Host side of the learned warm start: parameters, the trajectory dataset
+// kept in device memory, minibatch training with Adam, and proposals applied
+// to a world. See learned.hip for the network and its features.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/tangle_hip.h"
+#include "errors.h"
+#include "learned_device.h"
+#include "world_internal.h"
+
+using tgh::MlpDevice;
+
+#define HIP_CHECK(expr)                                                                             \
+    do {                                                                                            \
+        hipError_t _e = (expr);                                                                     \
+        if (_e != hipSuccess)                                                                       \
+            return tgh::fail(TGH_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));   \
+    } while (0)
+
+struct tgh_mlp {
+    hipStream_t stream = nullptr;
+    MlpDevice m{};
+    float *grad = nullptr;     // kMlpPartialStride
+    float *partial = nullptr;  // kMaxSlots * kMlpPartialStride
+    uint64_t step = 0;
+    // dataset
+    uint16_t *X = nullptr;
+    float4 *T = nullptr;
+    uint64_t rows = 0, capacity = 0;
+    // pending recording
+    uint16_t *pending_x = nullptr;
+    float4 *pending_mid = nullptr;
+    uint32_t pending_rows = 0, pending_capacity = 0;
+    // scratch for standalone forward/gradient calls
+    std::vector<void *> owned;
+
+    ~tgh_mlp() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : owned) (void)hipFree(p);
+        if (X) (void)hipFree(X);
+        if (T) (void)hipFree(T);
+        if (pending_x) (void)hipFree(pending_x);
+        if (pending_mid) (void)hipFree(pending_mid);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+constexpr uint32_t kMaxSlots = 512;
+
+template <typename T>
+hipError_t owned_alloc(tgh_mlp *m, T **p, size_t count) {
+    hipError_t e = hipMalloc((void **)p, std::max<size_t>(1, count) * sizeof(T));
+    if (e == hipSuccess) m->owned.push_back(*p);
+    return e;
+}
+
+int init_parameters(std::vector<float> &w, uint64_t seed) {
+    std::mt19937_64 rng(seed);
+    w.assign(tgh::kMlpParams, 0.0f);
+    auto fill = [&](uint32_t off, uint32_t out, uint32_t in, float gain) {
+        std::uniform_real_distribution<float> u(-1.0f, 1.0f);
+        const float bound = gain * std::sqrt(6.0f / (float)in);  // He-uniform for ReLU layers
+        for (uint32_t i = 0; i < out * in; ++i) w[off + i] = bound * u(rng);
+    };
+    fill(tgh::kOffW1, 64, 64, 1.0f);
+    fill(tgh::kOffW2, 64, 64, 1.0f);
+    fill(tgh::kOffW3, 16, 64, 0.05f);  // small first proposals
+    for (uint32_t o = 3; o < 16; ++o)
+        for (uint32_t i = 0; i < 64; ++i) w[tgh::kOffW3 + o * 64 + i] = 0.0f;  // unused outputs
+    return TGH_OK;
+}
+
+int upload_parameters(tgh_mlp *m, const float *params) {
+    HIP_CHECK(hipMemcpyAsync(m->m.master, params, tgh::kMlpParams * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    HIP_CHECK(tgh::launch_mlp_publish(m->m, m->stream));
+    HIP_CHECK(hipStreamSynchronize(m->stream));
+    return TGH_OK;
+}
+
+int ensure_dataset(tgh_mlp *m, uint64_t rows) {
+    if (rows <= m->capacity) return TGH_OK;
+    uint64_t cap = std::max<uint64_t>(rows, 2 * m->capacity);
+    uint16_t *X = nullptr;
+    float4 *T = nullptr;
+    HIP_CHECK(hipMalloc((void **)&X, cap * tgh::kFeatureWidth * sizeof(uint16_t)));
+    HIP_CHECK(hipMalloc((void **)&T, cap * sizeof(float4)));
+    if (m->rows) {
+        HIP_CHECK(hipMemcpyAsync(X, m->X, m->rows * tgh::kFeatureWidth * sizeof(uint16_t), hipMemcpyDeviceToDevice,
+                                 m->stream));
+        HIP_CHECK(hipMemcpyAsync(T, m->T, m->rows * sizeof(float4), hipMemcpyDeviceToDevice, m->stream));
+        HIP_CHECK(hipStreamSynchronize(m->stream));
+    }
+    if (m->X) (void)hipFree(m->X);
+    if (m->T) (void)hipFree(m->T);
+    m->X = X;
+    m->T = T;
+    m->capacity = cap;
+    return TGH_OK;
+}
+
+int ensure_pending(tgh_mlp *m, uint32_t rows) {
+    if (rows <= m->pending_capacity) return TGH_OK;
+    if (m->pending_x) (void)hipFree(m->pending_x);
+    if (m->pending_mid) (void)hipFree(m->pending_mid);
+    m->pending_x = nullptr;
+    m->pending_mid = nullptr;
+    HIP_CHECK(hipMalloc((void **)&m->pending_x, (size_t)rows * tgh::kFeatureWidth * sizeof(uint16_t)));
+    HIP_CHECK(hipMalloc((void **)&m->pending_mid, (size_t)rows * sizeof(float4)));
+    m->pending_capacity = rows;
+    return TGH_OK;
+}
+
+// Per-world scratch for proposals: features, predictions, position backup.
+struct ProposalScratch {
+    uint16_t *x = nullptr;
+    float4 *y = nullptr;
+    float4 *backup = nullptr;
+};
+
+}  // namespace
+
+extern "C" {
+
+size_t tgh_mlp_parameter_count(void) { return tgh::kMlpParams; }
+
+int tgh_mlp_create(uint64_t seed, tgh_mlp **out) {
+    if (!out) return tgh::fail(TGH_ERR_INVALID, "output pointer is null");
+    *out = nullptr;
+    tgh_mlp *m = new tgh_mlp();
+    auto bail = [&](int code) {
+        delete m;
+        return code;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e)));
+    float *master = nullptr, *am = nullptr, *av = nullptr;
+    uint16_t *w1, *w2, *w3, *w2n, *w3n;
+    e = owned_alloc(m, &master, tgh::kMlpParams);
+    if (e == hipSuccess) e = owned_alloc(m, &am, tgh::kMlpParams);
+    if (e == hipSuccess) e = owned_alloc(m, &av, tgh::kMlpParams);
+    if (e == hipSuccess) e = owned_alloc(m, &w1, 64 * 64);
+    if (e == hipSuccess) e = owned_alloc(m, &w2, 64 * 64);
+    if (e == hipSuccess) e = owned_alloc(m, &w3, 16 * 64);
+    if (e == hipSuccess) e = owned_alloc(m, &w2n, 64 * 64);
+    if (e == hipSuccess) e = owned_alloc(m, &w3n, 64 * 16);
+    if (e == hipSuccess) e = owned_alloc(m, &m->grad, tgh::kMlpPartialStride);
+    if (e == hipSuccess) e = owned_alloc(m, &m->partial, (size_t)kMaxSlots * tgh::kMlpPartialStride);
+    if (e != hipSuccess) return bail(tgh::fail(TGH_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e)));
+    m->m.master = master;
+    m->m.adam_m = am;
+    m->m.adam_v = av;
+    m->m.b1 = master + tgh::kOffB1;
+    m->m.b2 = master + tgh::kOffB2;
+    m->m.b3 = master + tgh::kOffB3;
+    m->m.w1 = w1;
+    m->m.w2 = w2;
+    m->m.w3 = w3;
+    m->m.w2n = w2n;
+    m->m.w3n = w3n;
+    e = hipMemsetAsync(am, 0, tgh::kMlpParams * sizeof(float), m->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(av, 0, tgh::kMlpParams * sizeof(float), m->stream);
+    if (e != hipSuccess) return bail(tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e)));
+    std::vector<float> w;
+    init_parameters(w, seed);
+    int rc = upload_parameters(m, w.data());
+    if (rc) return bail(rc);
+    *out = m;
+    return TGH_OK;
+}
+
+void tgh_mlp_destroy(tgh_mlp *mlp) { delete mlp; }
+
+int tgh_mlp_get_parameters(const tgh_mlp *mlp, float *out, size_t count) {
+    if (!mlp || !out) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (count < tgh::kMlpParams) return tgh::fail(TGH_ERR_SIZE, "parameter buffer too small");
+    HIP_CHECK(hipStreamSynchronize(mlp->stream));
+    HIP_CHECK(hipMemcpy(out, mlp->m.master, tgh::kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    return TGH_OK;
+}
+
+int tgh_mlp_set_parameters(tgh_mlp *mlp, const float *in, size_t count) {
+    if (!mlp || !in) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (count != tgh::kMlpParams) return tgh::fail(TGH_ERR_SIZE, "expected tgh_mlp_parameter_count() floats");
+    mlp->step = 0;
+    HIP_CHECK(hipMemsetAsync(mlp->m.adam_m, 0, tgh::kMlpParams * sizeof(float), mlp->stream));
+    HIP_CHECK(hipMemsetAsync(mlp->m.adam_v, 0, tgh::kMlpParams * sizeof(float), mlp->stream));
+    return upload_parameters(mlp, in);
+}
+
+int tgh_mlp_forward(tgh_mlp *mlp, const uint16_t *features, size_t rows, float *out) {
+    if (!mlp || !features || !out) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (rows == 0) return TGH_OK;
+    uint16_t *X = nullptr;
+    float4 *Y = nullptr;
+    HIP_CHECK(hipMalloc((void **)&X, rows * tgh::kFeatureWidth * sizeof(uint16_t)));
+    hipError_t e = hipMalloc((void **)&Y, rows * sizeof(float4));
+    if (e != hipSuccess) {
+        (void)hipFree(X);
+        return tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e));
+    }
+    std::vector<float4> host(rows);
+    e = hipMemcpyAsync(X, features, rows * tgh::kFeatureWidth * sizeof(uint16_t), hipMemcpyHostToDevice, mlp->stream);
+    if (e == hipSuccess) e = tgh::launch_mlp_forward(mlp->m, X, Y, (uint32_t)rows, mlp->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host.data(), Y, rows * sizeof(float4), hipMemcpyDeviceToHost, mlp->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(mlp->stream);
+    (void)hipFree(X);
+    (void)hipFree(Y);
+    if (e != hipSuccess) return tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e));
+    for (size_t r = 0; r < rows; ++r) {
+        out[3 * r] = host[r].x;
+        out[3 * r + 1] = host[r].y;
+        out[3 * r + 2] = host[r].z;
+    }
+    return TGH_OK;
+}
+
+int tgh_mlp_gradients(tgh_mlp *mlp, const uint16_t *features, const float *targets, size_t rows, float *grad,
+                      float *loss) {
+    if (!mlp || !features || !targets || !grad || !loss) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (rows == 0) return tgh::fail(TGH_ERR_INVALID, "rows must be positive");
+    uint16_t *X = nullptr;
+    float4 *T = nullptr;
+    HIP_CHECK(hipMalloc((void **)&X, rows * tgh::kFeatureWidth * sizeof(uint16_t)));
+    hipError_t e = hipMalloc((void **)&T, rows * sizeof(float4));
+    if (e != hipSuccess) {
+        (void)hipFree(X);
+        return tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e));
+    }
+    std::vector<float4> t4(rows);
+    for (size_t r = 0; r < rows; ++r) t4[r] = make_float4(targets[3 * r], targets[3 * r + 1], targets[3 * r + 2], 0.f);
+    std::vector<float> g(tgh::kMlpPartialStride);
+    const uint32_t tiles = (uint32_t)((rows + 15) / 16);
+    const uint32_t slots = std::min<uint32_t>(tiles, kMaxSlots);
+    e = hipMemcpyAsync(X, features, rows * tgh::kFeatureWidth * sizeof(uint16_t), hipMemcpyHostToDevice, mlp->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(T, t4.data(), rows * sizeof(float4), hipMemcpyHostToDevice, mlp->stream);
+    if (e == hipSuccess)
+        e = tgh::launch_mlp_gradients(mlp->m, X, T, nullptr, (uint32_t)rows, slots, mlp->partial, mlp->grad, mlp->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g.data(), mlp->grad, g.size() * sizeof(float), hipMemcpyDeviceToHost, mlp->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(mlp->stream);
+    (void)hipFree(X);
+    (void)hipFree(T);
+    if (e != hipSuccess) return tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e));
+    std::memcpy(grad, g.data(), tgh::kMlpParams * sizeof(float));
+    *loss = g[tgh::kOffLoss];
+    return TGH_OK;
+}
+
+int tgh_world_features(tgh_world *world, uint16_t *out, size_t count) {
+    if (!world || !out) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    const size_t S = world->d.S;
+    if (count < S * tgh::kFeatureWidth) return tgh::fail(TGH_ERR_SIZE, "feature buffer holds fewer than 64 per segment");
+    uint16_t *X = nullptr;
+    HIP_CHECK(hipMalloc((void **)&X, S * tgh::kFeatureWidth * sizeof(uint16_t)));
+    hipError_t e = tgh::launch_refresh_lists(world->d, world->stream);
+    if (e == hipSuccess) e = tgh::launch_features(world->d, X, world->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, X, S * tgh::kFeatureWidth * sizeof(uint16_t), hipMemcpyDeviceToHost,
+                                            world->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(world->stream);
+    (void)hipFree(X);
+    if (e != hipSuccess) return tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e));
+    return TGH_OK;
+}
+
+int tgh_mlp_record_begin(tgh_mlp *mlp, tgh_world *world) {
+    if (!mlp || !world) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    int rc = ensure_pending(mlp, world->d.S);
+    if (rc) return rc;
+    HIP_CHECK(hipStreamSynchronize(mlp->stream));
+    HIP_CHECK(tgh::launch_refresh_lists(world->d, world->stream));
+    HIP_CHECK(tgh::launch_features(world->d, mlp->pending_x, world->stream));
+    HIP_CHECK(tgh::launch_midpoints(world->d, mlp->pending_mid, world->stream));
+    HIP_CHECK(hipStreamSynchronize(world->stream));
+    mlp->pending_rows = world->d.S;
+    return TGH_OK;
+}
+
+int tgh_mlp_record_end(tgh_mlp *mlp, tgh_world *world) {
+    if (!mlp || !world) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (mlp->pending_rows == 0 || mlp->pending_rows != world->d.S)
+        return tgh::fail(TGH_ERR_INVALID, "record_end without a matching record_begin on this world");
+    int rc = ensure_dataset(mlp, mlp->rows + mlp->pending_rows);
+    if (rc) return rc;
+    HIP_CHECK(hipMemcpyAsync(mlp->X + mlp->rows * tgh::kFeatureWidth, mlp->pending_x,
+                             (size_t)mlp->pending_rows * tgh::kFeatureWidth * sizeof(uint16_t),
+                             hipMemcpyDeviceToDevice, world->stream));
+    HIP_CHECK(tgh::launch_targets(world->d, mlp->pending_mid, mlp->T + mlp->rows, world->stream));
+    HIP_CHECK(hipStreamSynchronize(world->stream));
+    mlp->rows += mlp->pending_rows;
+    mlp->pending_rows = 0;
+    return TGH_OK;
+}
+
+uint64_t tgh_mlp_dataset_rows(const tgh_mlp *mlp) { return mlp ? mlp->rows : 0; }
+
+int tgh_mlp_dataset_clear(tgh_mlp *mlp) {
+    if (!mlp) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    mlp->rows = 0;
+    return TGH_OK;
+}
+
+int tgh_mlp_train(tgh_mlp *mlp, uint32_t epochs, uint32_t batch_rows, float learning_rate, uint64_t seed,
+                  float *epoch_losses) {
+    if (!mlp) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (mlp->rows == 0) return tgh::fail(TGH_ERR_INVALID, "the dataset is empty");
+    if (batch_rows == 0 || epochs == 0) return tgh::fail(TGH_ERR_INVALID, "epochs and batch_rows must be positive");
+    const uint64_t rows = mlp->rows;
+    uint32_t *idx = nullptr;
+    HIP_CHECK(hipMalloc((void **)&idx, rows * sizeof(uint32_t)));
+    std::vector<uint32_t> perm(rows);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::mt19937_64 rng(seed);
+    const uint64_t batches = (rows + batch_rows - 1) / batch_rows;
+    float *losses = nullptr;
+    hipError_t e = hipHostMalloc((void **)&losses, batches * sizeof(float), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipFree(idx);
+        return tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e));
+    }
+    for (uint32_t epoch = 0; epoch < epochs && e == hipSuccess; ++epoch) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        e = hipMemcpyAsync(idx, perm.data(), rows * sizeof(uint32_t), hipMemcpyHostToDevice, mlp->stream);
+        for (uint64_t b = 0; b < batches && e == hipSuccess; ++b) {
+            const uint64_t begin = b * batch_rows;
+            const uint32_t n = (uint32_t)std::min<uint64_t>(batch_rows, rows - begin);
+            const uint32_t slots = std::min<uint32_t>((n + 15) / 16, kMaxSlots);
+            e = tgh::launch_mlp_gradients(mlp->m, mlp->X, mlp->T, idx + begin, n, slots, mlp->partial, mlp->grad,
+                                          mlp->stream);
+            if (e == hipSuccess) e = tgh::launch_mlp_adam(mlp->m, mlp->grad, learning_rate, ++mlp->step, mlp->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(&losses[b], mlp->grad + tgh::kOffLoss, sizeof(float), hipMemcpyDeviceToHost,
+                                   mlp->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(mlp->stream);
+        if (e == hipSuccess && epoch_losses) {
+            double s = 0.0;
+            for (uint64_t b = 0; b < batches; ++b) s += losses[b];
+            epoch_losses[epoch] = (float)(s / (double)batches);
+        }
+    }
+    (void)hipHostFree(losses);
+    (void)hipFree(idx);
+    if (e != hipSuccess) return tgh::fail(TGH_ERR_DEVICE, std::string("training: ") + hipGetErrorString(e));
+    return TGH_OK;
+}
+
+int tgh_world_propose(tgh_world *world, tgh_mlp *mlp, float scale) {
+    if (!world || !mlp) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (!(std::isfinite(scale))) return tgh::fail(TGH_ERR_INVALID, "scale must be finite");
+    const size_t S = world->d.S, V = world->d.V;
+    if (!world->proposal_x) {
+        HIP_CHECK(world->alloc(&world->proposal_x, S * tgh::kFeatureWidth));
+        HIP_CHECK(world->alloc(&world->proposal_y, S));
+        HIP_CHECK(world->alloc(&world->proposal_backup, V));
+    }
+    HIP_CHECK(hipStreamSynchronize(mlp->stream));
+    HIP_CHECK(tgh::launch_refresh_lists(world->d, world->stream));
+    HIP_CHECK(tgh::launch_features(world->d, world->proposal_x, world->stream));
+    HIP_CHECK(tgh::launch_mlp_forward(mlp->m, world->proposal_x, world->proposal_y, (uint32_t)S, world->stream));
+    HIP_CHECK(tgh::launch_apply_proposal(world->d, world->proposal_y, world->proposal_backup, scale, world->stream));
+    // the proposal moved vertices outside the displacement-tracked loop
+    HIP_CHECK(tgh::launch_request_rebuild(world->d, world->stream));
+    HIP_CHECK(hipStreamSynchronize(world->stream));
+    world->proposal_pending = true;
+    return TGH_OK;
+}
+
+int tgh_world_revert_proposal(tgh_world *world) {
+    if (!world) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (!world->proposal_pending) return tgh::fail(TGH_ERR_INVALID, "no proposal to revert");
+    HIP_CHECK(tgh::launch_restore(world->d, world->proposal_backup, world->stream));
+    HIP_CHECK(tgh::launch_request_rebuild(world->d, world->stream));
+    HIP_CHECK(hipStreamSynchronize(world->stream));
+    world->proposal_pending = false;
+    return TGH_OK;
+}
+
+int tgh_mfma_probe(const uint16_t *a, const uint16_t *b, float *d) {
+    if (!a || !b || !d) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    uint16_t *A = nullptr, *B = nullptr;
+    float *D = nullptr;
+    HIP_CHECK(hipMalloc((void **)&A, 256 * sizeof(uint16_t)));
+    HIP_CHECK(hipMalloc((void **)&B, 256 * sizeof(uint16_t)));
+    HIP_CHECK(hipMalloc((void **)&D, 256 * sizeof(float)));
+    hipError_t e = hipMemcpy(A, a, 512, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B, b, 512, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = tgh::launch_mfma_probe(A, B, D, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(d, D, 1024, hipMemcpyDeviceToHost);
+    (void)hipFree(A);
+    (void)hipFree(B);
+    (void)hipFree(D);
+    if (e != hipSuccess) return tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e));
+    return TGH_OK;
+}
+
+}  // extern "C"

@@ -89,6 +89,11 @@ struct IterParams {
 hipError_t launch_begin_batch(const WorldDev &w, hipStream_t stream);
 hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream);
 hipError_t launch_rebuild_now(const WorldDev &w, hipStream_t stream);
+// Finishes a pending rebuild (if any) now, from the current positions: the
+// same lists, snapshot and rebuild count the next pass would have produced.
+hipError_t launch_refresh_lists(const WorldDev &w, hipStream_t stream);
+// Marks the lists stale so the next pass rebuilds them.
+hipError_t launch_request_rebuild(const WorldDev &w, hipStream_t stream);
 // One relaxation iteration is these five groups in order (world.rs:1038-1501).
 hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t stream);
 hipError_t launch_measure(const WorldDev &w, const IterParams &p, hipStream_t stream);

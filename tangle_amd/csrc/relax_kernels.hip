@@ -1106,6 +1106,18 @@ hipError_t launch_rebuild_now(const WorldDev &w, hipStream_t stream) {
     return hipGetLastError();
 }
 
+hipError_t launch_refresh_lists(const WorldDev &w, hipStream_t stream) {
+    hipError_t e = launch_rebuild(w, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finish_rebuild, dim3(1), dim3(64), 0, stream, w.st);
+    return hipGetLastError();
+}
+
+hipError_t launch_request_rebuild(const WorldDev &w, hipStream_t stream) {
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(64), 0, stream, w.st);
+    return hipGetLastError();
+}
+
 hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t stream) {
     constexpr uint32_t per_block = 256 / kContactGroup;
     hipLaunchKernelGGL(k_contact<kContactGroup>, dim3(blocks_for(w.S, per_block)), dim3(256), 0, stream, w.st, w.g,

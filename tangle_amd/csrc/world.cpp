@@ -25,11 +25,14 @@
 #include "../../include/tangle_hip.h"
 #include "errors.h"
 #include "relax_device.h"
+#include "world_internal.h"
 
 using tgh::DevState;
 using tgh::GridParams;
 using tgh::IterParams;
 using tgh::WorldDev;
+using tgh::Packed;
+using tgh::Sizing;
 
 namespace {
 
@@ -65,22 +68,6 @@ inline uint32_t to_u32_saturating(float x) {
     if (x >= 4294967296.0f) return 0xffffffffu;
     return (uint32_t)x;
 }
-
-// Host copy of the packed assembly in the reference's f32 representation.
-struct Packed {
-    uint32_t F = 0, V = 0, S = 0;
-    std::vector<float> pos;        // 3V
-    std::vector<float> intr;       // 3V
-    std::vector<uint32_t> fstart;  // F
-    std::vector<uint32_t> fcount;  // F
-    std::vector<float> radius;     // F
-    std::vector<float> rest;       // S
-    std::vector<float> kmax;       // V
-    std::vector<uint8_t> pinned;   // V
-    std::vector<float> arc;        // V
-    float lower[3], upper[3];
-    uint32_t periodic[3];
-};
 
 int pack_assembly(const tgh_assembly *a, Packed &p) {
     if (!a) return set_error(TGH_ERR_INVALID, "assembly is null");
@@ -162,14 +149,6 @@ int pack_assembly(const tgh_assembly *a, Packed &p) {
     }
     return TGH_OK;
 }
-
-struct Sizing {
-    float skin = 0.f, cell_size = 0.f;
-    uint32_t cells[3] = {1, 1, 1};
-    uint32_t C = 1;
-    size_t slots = 1;
-    uint32_t block = 48;
-};
 
 int validate_config(const tgh_config *c) {
     if (!c) return set_error(TGH_ERR_INVALID, "config is null");
@@ -253,47 +232,6 @@ IterParams iter_params(const tgh_config *c, uint32_t iterations) {
 }
 
 }  // namespace
-
-struct tgh_world {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    Packed host;
-    Sizing size;
-    WorldDev d{};
-    DevState *host_state = nullptr;  // pinned
-    uint64_t device_bytes = 0;
-    std::vector<void *> allocations;
-    std::map<std::string, hipGraphExec_t> graphs;
-    std::vector<hipGraph_t> graph_templates;
-    bool use_graphs = true;
-    // Kernel-group timing: events bracket every group of every iteration of
-    // a batch, launched directly on the stream.
-    bool timing = false;
-    std::vector<hipEvent_t> direct_events;
-    double group_ms[tgh::kGroupCount] = {0, 0, 0, 0, 0};
-    uint64_t group_launches[tgh::kGroupCount] = {0, 0, 0, 0, 0};
-
-    template <typename T>
-    hipError_t alloc(T **out, size_t count) {
-        size_t bytes = std::max<size_t>(1, count) * sizeof(T);
-        hipError_t e = hipMalloc((void **)out, bytes);
-        if (e == hipSuccess) {
-            allocations.push_back(*out);
-            device_bytes += bytes;
-        }
-        return e;
-    }
-
-    ~tgh_world() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
-        for (hipGraph_t g : graph_templates) (void)hipGraphDestroy(g);
-        for (hipEvent_t e : direct_events) (void)hipEventDestroy(e);
-        for (void *p : allocations) (void)hipFree(p);
-        if (host_state) (void)hipHostFree(host_state);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
 
 namespace {
 

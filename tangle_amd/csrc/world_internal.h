@@ -1,0 +1,88 @@
+// Host-side state of one world, shared by the relaxation and learned
+// warm-start host sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "relax_device.h"
+
+namespace tgh {
+
+// Host copy of the packed assembly in the reference's f32 representation.
+struct Packed {
+    uint32_t F = 0, V = 0, S = 0;
+    std::vector<float> pos;        // 3V
+    std::vector<float> intr;       // 3V
+    std::vector<uint32_t> fstart;  // F
+    std::vector<uint32_t> fcount;  // F
+    std::vector<float> radius;     // F
+    std::vector<float> rest;       // S
+    std::vector<float> kmax;       // V
+    std::vector<uint8_t> pinned;   // V
+    std::vector<float> arc;        // V
+    float lower[3], upper[3];
+    uint32_t periodic[3];
+};
+
+struct Sizing {
+    float skin = 0.f, cell_size = 0.f;
+    uint32_t cells[3] = {1, 1, 1};
+    uint32_t C = 1;
+    size_t slots = 1;
+    uint32_t block = 48;
+};
+
+}  // namespace tgh
+
+struct tgh_world {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    tgh::Packed host;
+    tgh::Sizing size;
+    tgh::WorldDev d{};
+    tgh::DevState *host_state = nullptr;  // pinned
+    uint64_t device_bytes = 0;
+    std::vector<void *> allocations;
+    std::map<std::string, hipGraphExec_t> graphs;
+    std::vector<hipGraph_t> graph_templates;
+    bool use_graphs = true;
+    // learned warm start: features, predictions and the pre-proposal positions
+    uint16_t *proposal_x = nullptr;
+    float4 *proposal_y = nullptr;
+    float4 *proposal_backup = nullptr;
+    bool proposal_pending = false;
+    // Kernel-group timing: events bracket every group of every iteration of
+    // a batch, launched directly on the stream.
+    bool timing = false;
+    std::vector<hipEvent_t> direct_events;
+    double group_ms[tgh::kGroupCount] = {0, 0, 0, 0, 0};
+    uint64_t group_launches[tgh::kGroupCount] = {0, 0, 0, 0, 0};
+
+    template <typename T>
+    hipError_t alloc(T **out, size_t count) {
+        size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+        hipError_t e = hipMalloc((void **)out, bytes);
+        if (e == hipSuccess) {
+            allocations.push_back(*out);
+            device_bytes += bytes;
+        }
+        return e;
+    }
+
+    ~tgh_world() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (hipGraph_t g : graph_templates) (void)hipGraphDestroy(g);
+        for (hipEvent_t e : direct_events) (void)hipEventDestroy(e);
+        for (void *p : allocations) (void)hipFree(p);
+        if (host_state) (void)hipHostFree(host_state);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+

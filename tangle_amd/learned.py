@@ -1,0 +1,172 @@
+"""Learned warm start for the relaxation loop (an addition; the reference has none).
+
+A per-segment MLP (51 -> 64 -> 64 -> 3, BF16 operands with FP32 accumulation
+on the MI355X matrix cores) predicts each segment's displacement over the
+next `horizon` iterations of the plain relaxer from its 16 nearest neighbour
+capsules and its tangent. It is trained on the GPU, by the library's own
+forward and backward kernels, on trajectories the plain relaxer generates.
+During a run it proposes a displacement every `interval` iterations; a
+proposal that raises the measured residual is reverted, and the plain PBD
+projector always runs the final passes and judges admissibility.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+
+FEATURE_WIDTH = 64
+
+
+class WarmStartModel:
+    """The per-segment MLP with its device-resident trajectory dataset."""
+
+    def __init__(self, seed: int = 0):
+        lib = _native.library()
+        handle = C.c_void_p()
+        _native.check(lib.tgh_mlp_create(int(seed), C.byref(handle)))
+        self._handle = handle
+        self.parameter_count = int(lib.tgh_mlp_parameter_count())
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            _native.library().tgh_mlp_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- parameters -------------------------------------------------------
+    def parameters(self) -> np.ndarray:
+        out = np.empty(self.parameter_count, dtype=np.float32)
+        _native.check(_native.library().tgh_mlp_get_parameters(self._handle, _native.ptr(out, C.c_float), out.size))
+        return out
+
+    def set_parameters(self, values: np.ndarray) -> None:
+        arr = np.ascontiguousarray(values, dtype=np.float32)
+        _native.check(_native.library().tgh_mlp_set_parameters(self._handle, _native.ptr(arr, C.c_float), arr.size))
+
+    def save(self, path) -> None:
+        np.savez(path, parameters=self.parameters())
+
+    @classmethod
+    def load(cls, path) -> "WarmStartModel":
+        model = cls()
+        with np.load(path, allow_pickle=False) as data:
+            model.set_parameters(data["parameters"])
+        return model
+
+    # -- direct evaluation --------------------------------------------------
+    def forward(self, features: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(features, dtype=np.uint16).reshape(-1, FEATURE_WIDTH)
+        out = np.empty((x.shape[0], 3), dtype=np.float32)
+        _native.check(_native.library().tgh_mlp_forward(self._handle, _native.ptr(x, C.c_uint16), x.shape[0],
+                                                        _native.ptr(out, C.c_float)))
+        return out
+
+    def gradients(self, features: np.ndarray, targets: np.ndarray):
+        x = np.ascontiguousarray(features, dtype=np.uint16).reshape(-1, FEATURE_WIDTH)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 3)
+        grad = np.empty(self.parameter_count, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        _native.check(_native.library().tgh_mlp_gradients(self._handle, _native.ptr(x, C.c_uint16),
+                                                          _native.ptr(t, C.c_float), x.shape[0],
+                                                          _native.ptr(grad, C.c_float), _native.ptr(loss, C.c_float)))
+        return grad, float(loss[0])
+
+    # -- dataset and training ---------------------------------------------
+    def record_begin(self, world: _native.World) -> None:
+        _native.check(_native.library().tgh_mlp_record_begin(self._handle, world._handle))
+
+    def record_end(self, world: _native.World) -> None:
+        _native.check(_native.library().tgh_mlp_record_end(self._handle, world._handle))
+
+    @property
+    def dataset_rows(self) -> int:
+        return int(_native.library().tgh_mlp_dataset_rows(self._handle))
+
+    def clear_dataset(self) -> None:
+        _native.check(_native.library().tgh_mlp_dataset_clear(self._handle))
+
+    def train(self, epochs: int = 4, batch_rows: int = 8192, learning_rate: float = 1e-3, seed: int = 0):
+        losses = np.zeros(epochs, dtype=np.float32)
+        _native.check(_native.library().tgh_mlp_train(self._handle, int(epochs), int(batch_rows),
+                                                      float(learning_rate), int(seed),
+                                                      _native.ptr(losses, C.c_float)))
+        return losses
+
+
+def world_features(world: _native.World) -> np.ndarray:
+    """Features of every segment at the world's current positions (bf16 bits)."""
+    info = world.info()
+    out = np.empty((info.segment_count, FEATURE_WIDTH), dtype=np.uint16)
+    _native.check(_native.library().tgh_world_features(world._handle, _native.ptr(out, C.c_uint16), out.size))
+    return out
+
+
+def propose(world: _native.World, model: WarmStartModel, scale: float) -> None:
+    _native.check(_native.library().tgh_world_propose(world._handle, model._handle, float(scale)))
+
+
+def revert(world: _native.World) -> None:
+    _native.check(_native.library().tgh_world_revert_proposal(world._handle))
+
+
+def measure(world: _native.World, config: _native.Config) -> _native.BatchStatus:
+    """Residuals of the current positions, without moving anything: a batch
+    whose tolerances every state meets stops at its first (measuring) pass."""
+    probe = _native.Config.from_buffer_copy(config)
+    probe.penetration_tolerance = 3.0e38
+    probe.curvature_ratio_tolerance = 3.0e38
+    probe.force_full_iterations = 0
+    return world.run_batch(probe, 1)
+
+
+def collect_trajectories(model: WarmStartModel, packed: _native.PackedAssembly, config: _native.Config,
+                         iterations: int, interval: int, horizon: int) -> int:
+    """Relaxes `packed` with the plain loop for `iterations` iterations and,
+    every `interval` iterations, records (features now, midpoint displacement
+    after `horizon` more iterations) for every segment. Returns rows added."""
+    before = model.dataset_rows
+    cfg = _native.Config.from_buffer_copy(config)
+    cfg.force_full_iterations = 1
+    with _native.World(packed, cfg) as world:
+        done = 0
+        while done + horizon <= iterations:
+            model.record_begin(world)
+            world.run_batch(cfg, horizon)
+            model.record_end(world)
+            done += horizon
+            rest = interval - horizon
+            if rest > 0 and done + rest <= iterations:
+                world.run_batch(cfg, rest)
+                done += rest
+    return model.dataset_rows - before
+
+
+@dataclass
+class WarmStart:
+    """How Recipe.run uses a trained model: every `interval` iterations of the
+    first `active_iterations`, move each vertex by `scale` times the predicted
+    displacement, run `settle` plain iterations, and revert the move when the
+    normalised residual is then above `tolerance` times its value before."""
+
+    model: WarmStartModel
+    interval: int = 64
+    scale: float = 1.0
+    active_iterations: int = 2048
+    settle: int = 8
+    tolerance: float = 1.0
+
+    def __post_init__(self):
+        if self.interval <= 0 or self.active_iterations < 0 or self.settle < 0:
+            raise ValueError("interval must be positive; active_iterations and settle nonnegative")
+        if not (self.scale == self.scale and abs(self.scale) < 1e6) or not self.tolerance > 0:
+            raise ValueError("scale must be finite and tolerance positive")
