@@ -82,3 +82,94 @@ def finite_difference(params, x, t, index, h=1e-6):
     p[index] -= 2 * h
     lm, _ = loss_and_gradients(p, x, t, bf16=False)
     return (lp - lm) / (2 * h)
+
+
+# ---------------------------------------------------------------------------
+# Segment features (k_features in tangle_amd/csrc/learned.hip), float32
+# arithmetic in the kernel's order; the closest points follow the capsule
+# pair routine of the contact pass (contact/geometry.rs closest points of two
+# segments, minimum image per axis).
+
+def _f(x):
+    return np.float32(x)
+
+
+def closest_offset(p1, q1, p2, q2, periodic, lengths):
+    from .cell_hash import rust_round
+    eps_len, eps_par = _f(1.0e-20), _f(1.0e-6)
+    p1 = [_f(v) for v in p1]
+    q1 = [_f(v) for v in q1]
+    p2 = [_f(v) for v in p2]
+    q2 = [_f(v) for v in q2]
+    half = _f(0.5)
+    for k in range(3):
+        mid = half * (((p2[k] + q2[k]) - p1[k]) - q1[k])
+        if periodic[k]:
+            sh = _f(rust_round(np.float32(mid / _f(lengths[k])))) * _f(lengths[k])
+            p2[k] = _f(p2[k] - sh)
+            q2[k] = _f(q2[k] - sh)
+    d1 = [q1[k] - p1[k] for k in range(3)]
+    d2 = [q2[k] - p2[k] for k in range(3)]
+    r = [p1[k] - p2[k] for k in range(3)]
+
+    def dot(u, v):
+        return _f(_f(_f(u[0] * v[0]) + _f(u[1] * v[1])) + _f(u[2] * v[2]))
+
+    def clamp01(x):
+        return _f(min(max(x, _f(0.0)), _f(1.0)))
+
+    a, e, f = dot(d1, d1), dot(d2, d2), dot(d2, r)
+    s = t = _f(0.0)
+    if a <= eps_len and e > eps_len:
+        t = clamp01(f / e)
+    elif a > eps_len:
+        c = dot(d1, r)
+        if e <= eps_len:
+            s = clamp01(-c / a)
+        else:
+            b = dot(d1, d2)
+            den = _f(a * e) - _f(b * b)
+            if abs(den) > _f(_f(eps_par * a) * e):
+                s = clamp01(_f(_f(b * f) - _f(c * e)) / den)
+            proj = _f(_f(b * s) + f) / e
+            if proj < 0:
+                s = clamp01(-c / a)
+            elif proj > 1:
+                t = _f(1.0)
+                s = clamp01(_f(b - c) / a)
+            else:
+                t = proj
+    o = [_f(_f(p2[k] + _f(d2[k] * t)) - _f(p1[k] + _f(d1[k] * s))) for k in range(3)]
+    dist = _f(np.sqrt(dot(o, o)))
+    return o, dist
+
+
+def segment_features(positions, seg_v0, seg_radius, lists, counts, periodic, lengths, neighbors=16):
+    """Feature rows (bf16 bits, 64 per segment) from per-segment candidate
+    lists: the `neighbors` candidates of smallest surface gap (earlier list
+    entries first on ties), each as unit direction times
+    min((r1 + r2) / distance, 8), then the unit tangent, then zeros."""
+    pos = np.asarray(positions, dtype=np.float32).reshape(-1, 3)
+    S = len(seg_v0)
+    out = np.zeros((S, 64), dtype=np.float32)
+    for s in range(S):
+        v = int(seg_v0[s])
+        p1, q1 = pos[v], pos[v + 1]
+        r1 = _f(seg_radius[s])
+        cands = []
+        for k in range(int(counts[s])):
+            o_seg = int(lists[s][k])
+            w = int(seg_v0[o_seg])
+            o, dist = closest_offset(p1, q1, pos[w], pos[w + 1], periodic, lengths)
+            gap = _f(dist - _f(r1 + _f(seg_radius[o_seg])))
+            cands.append((gap, k, o, dist, _f(r1 + _f(seg_radius[o_seg]))))
+        cands.sort(key=lambda c: (c[0], c[1]))
+        for j, (gap, _, o, dist, contact) in enumerate(cands[:neighbors]):
+            d = _f(max(dist, _f(_f(1.0e-3) * contact)))
+            w = _f(_f(min(_f(contact / d), _f(8.0))) / d)
+            out[s, 3 * j:3 * j + 3] = [_f(o[0] * w), _f(o[1] * w), _f(o[2] * w)]
+        t = [_f(q1[k] - p1[k]) for k in range(3)]
+        n = _f(np.sqrt(_f(_f(_f(t[0] * t[0]) + _f(t[1] * t[1])) + _f(t[2] * t[2]))))
+        inv = _f(_f(1.0) / _f(max(n, _f(1.0e-30))))
+        out[s, 48:51] = [_f(t[0] * inv), _f(t[1] * inv), _f(t[2] * inv)]
+    return f32_to_bf16_bits(out)

@@ -212,7 +212,7 @@ __global__ void k_targets(const float4 *pos, const uint4 *sstat, const float4 *m
 // ---------------------------------------------------------------------------
 // MLP on matrix cores. One wave processes 16-row tiles.
 
-struct TileLds {
+struct __align__(16) TileLds {
     uint16_t x[16 * 64];
     uint16_t h1[16 * 64];
     uint16_t h2[16 * 64];

@@ -163,8 +163,9 @@ BIMODAL_FINE = dict(diameter=7e-6, count=1763, segments=71, bend=200e-6, seed=20
 BIMODAL_COARSE = dict(diameter=19e-6, count=237, segments=26, bend=60e-6, seed=20_260_922)
 
 
-def bimodal_recipe(scale=1.0, box=1e-3, length=1e-3, beta=1.0 / 0.3):
-    """scale < 1 keeps the volume fraction and shrinks the box and fiber counts."""
+def bimodal_recipe(scale=1.0, box=1e-3, length=1e-3, beta=1.0 / 0.3, seed_offset=0):
+    """scale < 1 keeps the volume fraction and shrinks the box and fiber counts;
+    `seed_offset` draws another specimen of the same distribution."""
     side = box * scale ** (1.0 / 3.0) if scale != 1.0 else box
     cell = tg.Cell([side] * 3, periodic="xyz")
     recipe = tg.Recipe(cell)
@@ -174,7 +175,7 @@ def bimodal_recipe(scale=1.0, box=1e-3, length=1e-3, beta=1.0 / 0.3):
         flen = min(length, side)
         segments = max(2, int(round(spec["segments"] * flen / length)))
         pop = tg.FiberPopulation(material=material, count=count, length=flen, segments_per_fiber=segments,
-                                 seed=spec["seed"], orientation=tg.SchladitzOrientation(beta=beta, axis="z"))
+                                 seed=spec["seed"] + seed_offset, orientation=tg.SchladitzOrientation(beta=beta, axis="z"))
         recipe.insert(tg.generate_fiber_population(cell, pop))
     return recipe
 
