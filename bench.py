@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--cpu-iterations", type=int, default=40, help="iterations in the CPU baseline sample")
     ap.add_argument("--scale", type=float, default=1.0, help="specimen scale (1.0 = the 2,000-fiber workload)")
     ap.add_argument("--dump", type=str, default="", help="write final tta positions (.npy) here")
+    ap.add_argument("--curvature-lanes", type=int, default=-1,
+                    help="lanes per fiber in the curvature sweep (0, 16, 32, 64; default: the world's choice)")
     return ap.parse_args()
 
 
@@ -88,6 +90,9 @@ def main():
     import ctypes as C
 
     world = _native.World(packed, cfg)
+    if args.curvature_lanes >= 0:
+        world.set_tuning(curvature_lanes=args.curvature_lanes)
+    lanes = int(world.tuning().curvature_lanes)
     info = world.info()
     # warmup: builds and caches the batch graph
     for _ in range(args.warmup):
@@ -172,6 +177,7 @@ def main():
             "neighbor_block": block,
             "mean_list_length": round(mean_list, 2),
             "rebuilds_in_timed_region": int(rebuilds),
+            "curvature_lanes": lanes,
         },
         "roofline": roofline,
     }

@@ -146,6 +146,19 @@ int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t 
 int tgh_world_set_timing(tgh_world *world, uint32_t enable);
 int tgh_world_timing(tgh_world *world, double *group_ms, uint64_t *group_launches, size_t groups,
                      uint32_t reset);
+/* Launch shape of the relaxation kernels (no counterpart in the reference,
+ * whose launch shapes are fixed). Any setting gives the same positions bit
+ * for bit; tgh_world_set_tuning validates the settings and drops captured
+ * graphs so the next batch is captured with the new shape.
+ *   curvature_lanes: lanes that sweep one fiber in the curvature Gauss-Seidel
+ *     pass: 4, 8, 16, 32 or 64 (fibers staged in LDS), or 0 for one lane per
+ *     fiber walking it in global memory. */
+typedef struct tgh_tuning {
+    uint32_t curvature_lanes;
+    uint32_t reserved[7];
+} tgh_tuning;
+int tgh_world_get_tuning(const tgh_world *world, tgh_tuning *out);
+int tgh_world_set_tuning(tgh_world *world, const tgh_tuning *tuning);
 void tgh_world_destroy(tgh_world *world);
 const char *tgh_last_error(void);
 

@@ -99,6 +99,13 @@ class WorldInfo(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("curvature_lanes", C.c_uint32),
+        ("reserved", C.c_uint32 * 7),
+    ]
+
+
 class PopulationSpec(C.Structure):
     _fields_ = [
         ("count", C.c_uint32),
@@ -134,6 +141,8 @@ SIGNATURES = {
     "tgh_world_cell_list": (C.c_int, [C.c_void_p, _u32p, _u32p, _u32p, C.c_size_t, C.c_size_t]),
     "tgh_world_neighbor_lists": (C.c_int, [C.c_void_p, _u32p, _u32p, C.c_size_t]),
     "tgh_world_set_timing": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "tgh_world_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "tgh_world_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "tgh_world_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32]),
     "tgh_world_destroy": (None, [C.c_void_p]),
     "tgh_last_error": (C.c_char_p, []),
@@ -293,6 +302,17 @@ class World:
     def upload_positions(self, positions: np.ndarray) -> None:
         arr = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
         check(library().tgh_world_upload_positions(self._handle, ptr(arr, C.c_float), arr.size))
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        check(library().tgh_world_get_tuning(self._handle, C.byref(t)))
+        return t
+
+    def set_tuning(self, **changes) -> None:
+        t = self.tuning()
+        for key, value in changes.items():
+            setattr(t, key, value)
+        check(library().tgh_world_set_tuning(self._handle, C.byref(t)))
 
     def rebuild_lists(self) -> None:
         check(library().tgh_world_rebuild_lists(self._handle))

@@ -55,6 +55,7 @@ struct WorldDev {
     uint32_t ovf_cap;
     uint32_t track_walls;
     uint32_t max_fiber_vertices;
+    uint32_t gs_lanes;  // lanes per fiber in the curvature sweep (16, 32 or 64)
     DevState *st;
     uint32_t *red;  // 3 * kRedSlots u32 bit patterns of nonnegative floats
     float4 *pos, *alt, *refpos, *wall, *vA, *vB;
@@ -83,6 +84,9 @@ struct IterParams {
     uint32_t force_full;
     uint32_t iterations;  // batch size, compared with control[2]
 };
+
+// Dynamic LDS of one curvature-sweep workgroup (64 / lanes fibers).
+size_t curvature_lds_bytes(uint32_t max_vertices, uint32_t lanes, uint32_t track_walls);
 
 // Enqueue work on `stream`; every launcher checks its grid against the
 // world's sizes and returns a HIP error code (hipSuccess when enqueued).

@@ -895,96 +895,376 @@ __device__ __forceinline__ bool triplet_projection(float3 pp, float3 pc, float3 
     return true;
 }
 
-// One wave per fiber with the fiber staged in LDS. In sweep order the lanes
-// evaluate the next 64 triplets at once from the current positions; the
-// first one whose projection runs (a ballot) is applied by its lane, and the
-// scan resumes just after it. Triplets before it in the sweep would not move
-// anything, and each evaluation reads the same values the sequential sweep
-// would, so the result is the Gauss–Seidel sweep's, bit for bit. A forward
-// sweep that projects nothing ends the cleanup: later sweeps would revisit
+// Curvature sweep with L lanes per fiber (64 / L fibers per wave), each
+// fiber staged in LDS as float4 (x, y, z, wall extent) plus (kmax, weight).
+//
+// A group alternates between two modes. Screening: the next L triplets of
+// the sweep are tested from the current positions with a cheap bound; a
+// triplet is passed over only when its cosine exceeds the limit cosine by
+// more than 1e-5, far beyond the few-ulp error of the bound, so the exact
+// test would find no violation either. Exact: the triplet at the cursor is
+// evaluated with the sequential sweep's f32 operations in its order by three
+// lanes, one per axis (cross-axis sums in x, y, z order through quad
+// broadcasts); when its projection runs, those lanes move its three vertices.
+// A projection at sweep position o changes the triplets at o + 1 and o + 2,
+// which are then evaluated exactly in turn with the positions carried in
+// registers; screening results for positions past o + 2 stay valid and pick
+// the next exact candidate. Every triplet the sweep would project is thus
+// projected in the same order from the same values: the result is the
+// sequential Gauss–Seidel sweep's, bit for bit. A sweep direction that
+// projects nothing ends the cleanup, since every later pass would revisit
 // the same unchanged triplets.
-__global__ __launch_bounds__(64) void k_curvature_gs_wave(DevState *st, GridParams g, const float4 *in, float4 *pos,
-                                                          const float4 *refpos, const float4 *vA, const float4 *vB,
-                                                          const uint2 *fspan, float4 *wall, uint32_t sweeps,
-                                                          float stiffness, float margin, float half_skin,
-                                                          uint32_t track_walls, uint32_t max_vertices) {
-    if (st->control[0] == 0) return;
-    extern __shared__ float lds[];
-    float *X = lds, *Y = X + max_vertices, *Z = Y + max_vertices, *K = Z + max_vertices, *E = K + max_vertices;
-    uint32_t *F = (uint32_t *)(E + max_vertices);
-    float *WX = (float *)(F + max_vertices), *WY = WX + max_vertices, *WZ = WY + max_vertices;
-    const uint32_t lane = threadIdx.x;
-    const uint2 span = fspan[blockIdx.x];
-    const uint32_t start = span.x, n = span.y;
-    for (uint32_t i = lane; i < n; i += 64) {
-        float4 p = in[start + i];
-        float4 a = vA[start + i];
-        X[i] = p.x;
-        Y[i] = p.y;
-        Z[i] = p.z;
-        K[i] = a.x;
-        E[i] = a.w;
-        F[i] = __float_as_uint(vB[start + i].z);
-        if (track_walls) {
-            float4 w = wall[start + i];
-            WX[i] = w.x;
-            WY[i] = w.y;
-            WZ[i] = w.z;
-        }
+__device__ __forceinline__ float quad_bcast(float x, int k) {
+    // v_mov_b32 with a DPP quad permutation: every lane of a quad reads lane k
+    switch (k) {
+        case 0: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x00, 0xF, 0xF, false));
+        case 1: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x55, 0xF, 0xF, false));
+        default: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xAA, 0xF, 0xF, false));
     }
+}
+__device__ __forceinline__ float axis_sum(float x) {  // (x + y) + z over lanes 0, 1, 2 of the quad
+    return (quad_bcast(x, 0) + quad_bcast(x, 1)) + quad_bcast(x, 2);
+}
+
+__device__ __forceinline__ void lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+__host__ __device__ inline size_t curvature_group_stride_floats(uint32_t max_vertices, uint32_t track_walls) {
+    const size_t n = (max_vertices + 1u) & ~1u;  // keeps every region 16-byte aligned
+    return n * (4u + 2u + (track_walls ? 4u : 0u));
+}
+
+template <int L>
+__global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, GridParams g, const float4 *in, float4 *pos,
+                                                           const float4 *refpos, const float4 *vA, const float4 *vB,
+                                                           const uint2 *fspan, float4 *wall, uint32_t F,
+                                                           uint32_t sweeps, float stiffness, float margin,
+                                                           float half_skin, uint32_t track_walls,
+                                                           uint32_t max_vertices) {
+    static_assert(L == 4 || L == 8 || L == 16 || L == 32, "group width");
+    if (st->control[0] == 0) return;
+    constexpr uint32_t G = 64 / L;
+    extern __shared__ float4 lds4[];
+    const uint32_t lane = threadIdx.x, group = lane / L, gl = lane % L, gbase = group * L;
+    const uint32_t nv = (max_vertices + 1u) & ~1u;
+    float4 *P = (float4 *)((float *)lds4 + group * curvature_group_stride_floats(max_vertices, track_walls));
+    float4 *WR = P + nv;                                     // wall reactions (when tracked)
+    float2 *KW = (float2 *)(track_walls ? WR + nv : WR);     // (kmax, 1 - pinned)
+    float *Pf = (float *)P;
+    const uint32_t fiber = blockIdx.x * G + group;
+    uint2 span = make_uint2(0u, 0u);
+    if (fiber < F) span = fspan[fiber];
+    const uint32_t start = span.x, n = span.y;
+    for (uint32_t i = gl; i < n; i += L) {
+        const float4 p = in[start + i];
+        const float4 a = vA[start + i];
+        P[i] = make_float4(p.x, p.y, p.z, a.w);
+        KW[i] = make_float2(a.x, (__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f);
+        if (track_walls) WR[i] = wall[start + i];
+    }
+    // the axis of an exact lane and that axis's walls, held in registers
+    const uint32_t k = gl < 3 ? gl : 0u;
+    const bool walled = !g.periodic[k];
+    const float lo = g.lower[k], hi = g.upper[k];
     __syncthreads();
     const uint32_t trip = n >= 3 ? n - 2 : 0;  // triplet t has center t + 1
-    bool done = trip == 0;
-    for (uint32_t sweep = 0; sweep < sweeps && !done; ++sweep) {
-        bool any = false;
-        for (int dir = 0; dir < 2 && !done; ++dir) {
+    const uint64_t gmask = (1ull << (L % 64)) - 1ull;
+    bool done = trip == 0 || sweeps == 0;
+    bool any = false;
+    uint32_t sweep = 0, dir = 0, cursor = 0;
+    while (true) {
+        if (!done && cursor >= trip) {
+            if (!any || (dir == 1 && sweep + 1 >= sweeps)) {
+                done = true;
+            } else {
+                any = false;
+                cursor = 0;
+                if (dir == 0) {
+                    dir = 1;
+                } else {
+                    dir = 0;
+                    ++sweep;
+                }
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        // screen the next L triplets
+        bool maybe = false;
+        const uint32_t order = cursor + gl;
+        if (!done && order < trip) {
+            const uint32_t t = dir == 0 ? order : trip - 1 - order;
+            const float4 a = P[t], b = P[t + 1], c = P[t + 2];
+            const float kmax = KW[t + 1].x * (1.0f - margin);
+            const float ux = b.x - a.x, uy = b.y - a.y, uz = b.z - a.z;
+            const float vx = c.x - b.x, vy = c.y - b.y, vz = c.z - b.z;
+            const float uu = ux * ux + uy * uy + uz * uz, vv = vx * vx + vy * vy + vz * vz;
+            const float uv = ux * vx + uy * vy + uz * vz;
+            const float ru = __builtin_amdgcn_rsqf(uu), rv = __builtin_amdgcn_rsqf(vv);
+            const float cos_est = uv * ru * rv;
+            const float half_est = 0.25f * kmax * (uu * ru + vv * rv);
+            const float min_cos_est = 1.0f - 2.0f * half_est * half_est;
+            maybe = kmax > 0.f && !(cos_est - min_cos_est > 1.0e-5f);
+        }
+        const uint64_t ballot = (uint64_t)__ballot(maybe);
+        const uint64_t mine = (ballot >> gbase) & gmask;
+        if (!done && mine == 0) cursor += L;
+        if (ballot == 0) continue;
+        // exact evaluation of each group's first candidate on lanes 0..2
+        const uint32_t j = mine ? (uint32_t)__builtin_ctzll(mine) : 0u;
+        const uint32_t t = dir == 0 ? cursor + j : trip - 1 - (cursor + j);
+        bool hit = false;
+        if (mine != 0 && gl < 3) {
+            float pp = Pf[4 * t + k], pc = Pf[4 * (t + 1) + k], pn = Pf[4 * (t + 2) + k];
+            const float ep = P[t].w, ec = P[t + 1].w, en = P[t + 2].w;
+            const float2 kwp = KW[t], kwc = KW[t + 1], kwn = KW[t + 2];
+            const float kmax = kwc.x * (1.0f - margin);
+            const float u = pc - pp, v = pn - pc;
+            const float l1 = sqrtf(axis_sum(u * u));
+            const float l2 = sqrtf(axis_sum(v * v));
+            const float un = u / l1, vn = v / l2;
+            const float cosine = rclamp(axis_sum(un * vn), -1.f, 1.f);
+            const float sine_half = 0.25f * kmax * (l1 + l2);
+            const float min_cos = 1.0f - 2.0f * sine_half * sine_half;
+            float violation = min_cos - cosine;
+            bool ok = kmax > 0.f && l1 > 1.0e-7f && l2 > 1.0e-7f && sine_half < 1.0f && violation > 0.f;
+            const float sine = sqrtf(fmaxf(1.0f - cosine * cosine, 0.f));
+            if (sine > 1.0e-3f) violation = (acosf(cosine) - acosf(min_cos)) * sine;
+            ok = ok && violation > 0.f;
+            const float gu = (vn - cosine * un) / l1;
+            const float gv = (un - cosine * vn) / l2;
+            const float gc = gu - gv;
+            const float fwp = kwp.y, fwc = kwc.y, fwn = kwn.y;
+            const float den = fwp * axis_sum(gu * gu) + fwc * axis_sum(gc * gc) + fwn * axis_sum(gv * gv);
+            ok = ok && den > 1.0e-12f;
+            if (ok) {
+                const float scale = stiffness * violation / den;
+                const float m_p = pp - fwp * scale * gu;
+                const float m_c = pc + fwc * scale * gc;
+                const float m_n = pn + fwn * scale * gv;
+                float v_p = m_p, v_c = m_c, v_n = m_n;
+                if (walled) {
+                    v_p = rclamp(m_p, lo + ep, hi - ep);
+                    v_c = rclamp(m_c, lo + ec, hi - ec);
+                    v_n = rclamp(m_n, lo + en, hi - en);
+                    if (track_walls) {
+                        float *W = (float *)WR;
+                        if (fwp > 0.f) W[4 * t + k] += m_p - v_p;
+                        if (fwc > 0.f) W[4 * (t + 1) + k] += m_c - v_c;
+                        if (fwn > 0.f) W[4 * (t + 2) + k] += m_n - v_n;
+                    }
+                }
+                if (fwp > 0.f) Pf[4 * t + k] = v_p;
+                if (fwc > 0.f) Pf[4 * (t + 1) + k] = v_c;
+                if (fwn > 0.f) Pf[4 * (t + 2) + k] = v_n;
+            }
+            hit = ok;
+        }
+        const uint64_t hits = (uint64_t)__ballot(hit && gl == 0);
+        if (mine != 0) {
+            cursor += j + 1;
+            if ((hits >> gbase) & 1ull) any = true;
+        }
+        // one wave: its LDS accesses complete in order; keep the compiler's
+        lds_order();
+    }
+    __syncthreads();
+    const float limit = half_skin * half_skin;
+    bool flag = false;
+    for (uint32_t i = gl; i < n; i += L) {
+        const uint32_t v = start + i;
+        const float4 old = in[v], q = P[i];
+        pos[v] = make_float4(q.x, q.y, q.z, old.w);
+        if (track_walls) wall[v] = make_float4(WR[i].x, WR[i].y, WR[i].z, 0.f);
+        const float4 r = refpos[v];
+        const float dx = q.x - r.x, dy = q.y - r.y, dz = q.z - r.z;
+        if (dx * dx + dy * dy + dz * dz > limit) flag = true;
+    }
+    if (__ballot(flag) != 0 && lane == 0) st->nstate[0] = 1u;
+}
+
+// One fiber per wave: the same screening and exact evaluation as
+// k_curvature_gs_group, with every decision wave-uniform. After a screen
+// the wave's three axis lanes run the exact chain in a tight loop: the
+// triplet positions slide through registers, the next vertex is loaded
+// while the current triplet is evaluated, and the loop leaves only when the
+// next position is neither changed by the last projection nor a screened
+// candidate. LDS accesses of one wave complete in order, so the screen that
+// follows reads the chain's writes without a barrier.
+__global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, GridParams g, const float4 *in, float4 *pos,
+                                                            const float4 *refpos, const float4 *vA, const float4 *vB,
+                                                            const uint2 *fspan, float4 *wall, uint32_t F,
+                                                            uint32_t sweeps, float stiffness, float margin,
+                                                            float half_skin, uint32_t track_walls,
+                                                            uint32_t max_vertices) {
+    if (st->control[0] == 0) return;
+    extern __shared__ float4 lds4[];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nv = (max_vertices + 1u) & ~1u;
+    float4 *P = lds4;
+    float4 *WR = P + nv;
+    float2 *KW = (float2 *)(track_walls ? WR + nv : WR);
+    float *Pf = (float *)P;
+    const uint32_t fiber = blockIdx.x;
+    if (fiber >= F) return;
+    const uint2 span = fspan[fiber];
+    const uint32_t start = span.x, n = span.y;
+    for (uint32_t i = lane; i < n; i += 64) {
+        const float4 p = in[start + i];
+        const float4 a = vA[start + i];
+        P[i] = make_float4(p.x, p.y, p.z, a.w);
+        KW[i] = make_float2(a.x, (__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f);
+        if (track_walls) WR[i] = wall[start + i];
+    }
+    __syncthreads();
+    const uint32_t trip = n >= 3 ? n - 2 : 0;
+    const uint32_t k = lane < 3 ? lane : 0u;
+    const bool walled = !g.periodic[k];
+    const float lo = g.lower[k], hi = g.upper[k];
+    for (uint32_t sweep = 0; sweep < sweeps && trip > 0; ++sweep) {
+        bool stop = false;
+        for (uint32_t dir = 0; dir < 2; ++dir) {
+            bool any = false;
             uint32_t cursor = 0;
             while (cursor < trip) {
+                // screen the next 64 positions
+                bool maybe = false;
                 const uint32_t order = cursor + lane;
-                bool hit = false;
-                float3 mp, mc, mn, rp, rc, rn;
-                uint32_t t = 0;
                 if (order < trip) {
-                    t = dir == 0 ? order : trip - 1 - order;
-                    hit = triplet_projection(make_float3(X[t], Y[t], Z[t]), make_float3(X[t + 1], Y[t + 1], Z[t + 1]),
-                                             make_float3(X[t + 2], Y[t + 2], Z[t + 2]), K[t + 1], F[t] & kPinned,
-                                             F[t + 1] & kPinned, F[t + 2] & kPinned, E[t], E[t + 1], E[t + 2], g,
-                                             stiffness, margin, mp, mc, mn, rp, rc, rn);
+                    const uint32_t t = dir == 0 ? order : trip - 1 - order;
+                    const float4 a = P[t], b = P[t + 1], c = P[t + 2];
+                    const float kmax = KW[t + 1].x * (1.0f - margin);
+                    const float ux = b.x - a.x, uy = b.y - a.y, uz = b.z - a.z;
+                    const float vx = c.x - b.x, vy = c.y - b.y, vz = c.z - b.z;
+                    const float uu = ux * ux + uy * uy + uz * uz, vv = vx * vx + vy * vy + vz * vz;
+                    const float uv = ux * vx + uy * vy + uz * vz;
+                    const float ru = __builtin_amdgcn_rsqf(uu), rv = __builtin_amdgcn_rsqf(vv);
+                    const float cos_est = uv * ru * rv;
+                    const float half_est = 0.25f * kmax * (uu * ru + vv * rv);
+                    const float min_cos_est = 1.0f - 2.0f * half_est * half_est;
+                    maybe = kmax > 0.f && !(cos_est - min_cos_est > 1.0e-5f);
                 }
-                const uint64_t mask = (uint64_t)__ballot(hit);
-                if (mask == 0) {
+                const uint64_t cand = (uint64_t)__ballot(maybe);
+                if (cand == 0) {
                     cursor += 64;
                     continue;
                 }
-                const uint32_t j = (uint32_t)__builtin_ctzll(mask);
-                if (lane == j) {
-                    const bool pp = F[t] & kPinned, pc = F[t + 1] & kPinned, pn = F[t + 2] & kPinned;
-                    X[t] = mp.x; Y[t] = mp.y; Z[t] = mp.z;
-                    X[t + 1] = mc.x; Y[t + 1] = mc.y; Z[t + 1] = mc.z;
-                    X[t + 2] = mn.x; Y[t + 2] = mn.y; Z[t + 2] = mn.z;
-                    if (track_walls) {
-                        if (!pp) { WX[t] += rp.x; WY[t] += rp.y; WZ[t] += rp.z; }
-                        if (!pc) { WX[t + 1] += rc.x; WY[t + 1] += rc.y; WZ[t + 1] += rc.z; }
-                        if (!pn) { WX[t + 2] += rn.x; WY[t + 2] += rn.y; WZ[t + 2] += rn.z; }
+                const uint32_t wbase = cursor;
+                cursor += (uint32_t)__builtin_ctzll(cand);
+                // exact chain on the three axis lanes; control stays uniform
+                uint32_t next_cursor = cursor;
+                bool chain_any = false;
+                if (lane < 3) {
+                    uint32_t t = dir == 0 ? cursor : trip - 1 - cursor;
+                    float pp = Pf[4 * t + k], pc = Pf[4 * (t + 1) + k], pn = Pf[4 * (t + 2) + k];
+                    float ep = P[t].w, ec = P[t + 1].w, en = P[t + 2].w;
+                    float2 kwp = KW[t], kwc = KW[t + 1], kwn = KW[t + 2];
+                    int32_t last_hit = -3;
+                    while (true) {
+                        // the vertex that enters when the triplet slides by one
+                        const bool more = cursor + 1 < trip;
+                        const uint32_t tn = more ? (dir == 0 ? t + 3 : t - 1) : t;
+                        const float pnext = Pf[4 * tn + k];
+                        const float enext = P[tn].w;
+                        const float2 kwnext = KW[tn];
+                        const float kmax = kwc.x * (1.0f - margin);
+                        const float u = pc - pp, v = pn - pc;
+                        const float l1 = sqrtf(axis_sum(u * u));
+                        const float l2 = sqrtf(axis_sum(v * v));
+                        const float un = u / l1, vn = v / l2;
+                        const float cosine = rclamp(axis_sum(un * vn), -1.f, 1.f);
+                        const float sine_half = 0.25f * kmax * (l1 + l2);
+                        const float min_cos = 1.0f - 2.0f * sine_half * sine_half;
+                        float violation = min_cos - cosine;
+                        bool ok = kmax > 0.f && l1 > 1.0e-7f && l2 > 1.0e-7f && sine_half < 1.0f && violation > 0.f;
+                        const float sine = sqrtf(fmaxf(1.0f - cosine * cosine, 0.f));
+                        if (sine > 1.0e-3f) violation = (acosf(cosine) - acosf(min_cos)) * sine;
+                        ok = ok && violation > 0.f;
+                        const float gu = (vn - cosine * un) / l1;
+                        const float gv = (un - cosine * vn) / l2;
+                        const float gc = gu - gv;
+                        const float fwp = kwp.y, fwc = kwc.y, fwn = kwn.y;
+                        const float den =
+                            fwp * axis_sum(gu * gu) + fwc * axis_sum(gc * gc) + fwn * axis_sum(gv * gv);
+                        ok = ok && den > 1.0e-12f;
+                        // lanes 0..2 hold identical scalars: the decision is uniform
+                        const bool hit = __builtin_amdgcn_readfirstlane(ok ? 1 : 0) != 0;
+                        if (hit) {
+                            const float scale = stiffness * violation / den;
+                            const float m_p = pp - fwp * scale * gu;
+                            const float m_c = pc + fwc * scale * gc;
+                            const float m_n = pn + fwn * scale * gv;
+                            float v_p = m_p, v_c = m_c, v_n = m_n;
+                            if (walled) {
+                                v_p = rclamp(m_p, lo + ep, hi - ep);
+                                v_c = rclamp(m_c, lo + ec, hi - ec);
+                                v_n = rclamp(m_n, lo + en, hi - en);
+                                if (track_walls) {
+                                    float *W = (float *)WR;
+                                    if (fwp > 0.f) W[4 * t + k] += m_p - v_p;
+                                    if (fwc > 0.f) W[4 * (t + 1) + k] += m_c - v_c;
+                                    if (fwn > 0.f) W[4 * (t + 2) + k] += m_n - v_n;
+                                }
+                            }
+                            if (fwp > 0.f) pp = v_p;
+                            if (fwc > 0.f) pc = v_c;
+                            if (fwn > 0.f) pn = v_n;
+                            Pf[4 * t + k] = pp;
+                            Pf[4 * (t + 1) + k] = pc;
+                            Pf[4 * (t + 2) + k] = pn;
+                            chain_any = true;
+                            last_hit = (int32_t)cursor;
+                        }
+                        ++cursor;
+                        if (!more) break;
+                        // continue while the next position changed with a
+                        // projection or was screened as a candidate
+                        const uint32_t rel = cursor - wbase;
+                        const bool screened = rel < 64 && ((cand >> rel) & 1ull);
+                        if (!((int32_t)cursor <= last_hit + 2 || screened)) break;
+                        if (dir == 0) {
+                            pp = pc; pc = pn; pn = pnext;
+                            ep = ec; ec = en; en = enext;
+                            kwp = kwc; kwc = kwn; kwn = kwnext;
+                            t = t + 1;
+                        } else {
+                            pn = pc; pc = pp; pp = pnext;
+                            en = ec; ec = ep; ep = enext;
+                            kwn = kwc; kwc = kwp; kwp = kwnext;
+                            t = t - 1;
+                        }
                     }
+                    next_cursor = cursor;
                 }
-                any = true;
-                cursor += j + 1;
-                __syncthreads();
+                cursor = (uint32_t)__shfl(next_cursor, 0, 64);
+                any = any || __shfl(chain_any ? 1 : 0, 0, 64) != 0;
+                lds_order();
+                // resume at the next screened candidate of this window, if any
+                const uint32_t rel = cursor - wbase;
+                const uint64_t rest = rel < 64 ? (cand >> rel) : 0ull;
+                if (rest != 0) {
+                    cursor += (uint32_t)__builtin_ctzll(rest);
+                    // re-screen from there: the window's other flags still hold,
+                    // and a fresh screen of the same positions gives them again
+                } else if (cursor < wbase + 64) {
+                    cursor = wbase + 64;
+                }
             }
-            if (dir == 0 && !any) done = true;
+            if (!any) {
+                stop = true;
+                break;
+            }
         }
+        if (stop) break;
     }
     __syncthreads();
     const float limit = half_skin * half_skin;
     bool flag = false;
     for (uint32_t i = lane; i < n; i += 64) {
         const uint32_t v = start + i;
-        const float4 old = in[v];
-        pos[v] = make_float4(X[i], Y[i], Z[i], old.w);
-        if (track_walls) wall[v] = make_float4(WX[i], WY[i], WZ[i], 0.f);
+        const float4 old = in[v], q = P[i];
+        pos[v] = make_float4(q.x, q.y, q.z, old.w);
+        if (track_walls) wall[v] = make_float4(WR[i].x, WR[i].y, WR[i].z, 0.f);
         const float4 r = refpos[v];
-        float dx = X[i] - r.x, dy = Y[i] - r.y, dz = Z[i] - r.z;
+        const float dx = q.x - r.x, dy = q.y - r.y, dz = q.z - r.z;
         if (dx * dx + dy * dy + dz * dz > limit) flag = true;
     }
     if (__ballot(flag) != 0 && lane == 0) st->nstate[0] = 1u;
@@ -1067,7 +1347,9 @@ inline uint32_t blocks_for(uint64_t n, uint32_t threads) { return (uint32_t)((n 
 constexpr int kContactGroup = 16;
 }  // namespace
 
-size_t curvature_lds_bytes(uint32_t max_vertices) { return (size_t)max_vertices * 9u * sizeof(float); }
+size_t curvature_lds_bytes(uint32_t max_vertices, uint32_t lanes, uint32_t track_walls) {
+    return (64u / lanes) * curvature_group_stride_floats(max_vertices, track_walls) * sizeof(float);
+}
 
 hipError_t launch_begin_batch(const WorldDev &w, hipStream_t stream) {
     hipLaunchKernelGGL(k_begin_batch, dim3(1), dim3(64), 0, stream, w.st);
@@ -1154,11 +1436,33 @@ hipError_t launch_vertex_passes(const WorldDev &w, const IterParams &p, hipStrea
 
 hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t stream) {
     const float4 *src = (vertex_passes(p) % 2 == 1) ? w.alt : w.pos;
-    if (w.max_fiber_vertices <= kMaxLdsFiberVertices) {
-        hipLaunchKernelGGL(k_curvature_gs_wave, dim3(w.F), dim3(64), curvature_lds_bytes(w.max_fiber_vertices),
-                           stream, w.st, w.g, src, w.pos, w.refpos, w.vA, w.vB, w.fspan, w.wall, p.sweeps,
-                           p.curvature_stiffness, p.curvature_margin, 0.5f * w.skin, w.track_walls,
-                           w.max_fiber_vertices);
+    if (w.gs_lanes != 0 && w.max_fiber_vertices <= kMaxLdsFiberVertices) {
+        const size_t lds = curvature_lds_bytes(w.max_fiber_vertices, w.gs_lanes, w.track_walls);
+        const uint32_t per_block = 64u / w.gs_lanes;
+        const dim3 grid(blocks_for(w.F, per_block));
+        if (w.gs_lanes == 4) {
+            hipLaunchKernelGGL(k_curvature_gs_group<4>, grid, dim3(64), lds, stream, w.st, w.g, src, w.pos, w.refpos,
+                               w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, p.curvature_margin,
+                               0.5f * w.skin, w.track_walls, w.max_fiber_vertices);
+        } else if (w.gs_lanes == 8) {
+            hipLaunchKernelGGL(k_curvature_gs_group<8>, grid, dim3(64), lds, stream, w.st, w.g, src, w.pos, w.refpos,
+                               w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, p.curvature_margin,
+                               0.5f * w.skin, w.track_walls, w.max_fiber_vertices);
+        } else if (w.gs_lanes == 16) {
+            hipLaunchKernelGGL(k_curvature_gs_group<16>, grid, dim3(64), lds, stream, w.st, w.g, src, w.pos, w.refpos,
+                               w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, p.curvature_margin,
+                               0.5f * w.skin, w.track_walls, w.max_fiber_vertices);
+        } else if (w.gs_lanes == 32) {
+            hipLaunchKernelGGL(k_curvature_gs_group<32>, grid, dim3(64), lds, stream, w.st, w.g, src, w.pos, w.refpos,
+                               w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, p.curvature_margin,
+                               0.5f * w.skin, w.track_walls, w.max_fiber_vertices);
+        } else if (w.gs_lanes == 64) {
+            hipLaunchKernelGGL(k_curvature_gs_wave64, grid, dim3(64), lds, stream, w.st, w.g, src, w.pos, w.refpos,
+                               w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, p.curvature_margin,
+                               0.5f * w.skin, w.track_walls, w.max_fiber_vertices);
+        } else {
+            return hipErrorInvalidValue;
+        }
     } else {
         hipLaunchKernelGGL(k_curvature_gs_lane, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.st, w.g, src,
                            w.pos, w.refpos, w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness,

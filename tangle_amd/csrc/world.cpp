@@ -344,6 +344,16 @@ int create_world(const tgh_assembly *a, const tgh_config *c, tgh_world **out) {
     d.ovf_cap = std::max<uint32_t>(tgh::kOverflowListCapacity, d.block);
     d.max_fiber_vertices = *std::max_element(p.fcount.begin(), p.fcount.end());
     d.track_walls = (p.periodic[0] && p.periodic[1] && p.periodic[2]) ? 0u : 1u;
+    // lanes per fiber in the curvature sweep: 8 (eight fibers per wave, the
+    // fastest measured on the bimodal felt) unless a workgroup's LDS would
+    // exceed 64 KiB, then wider groups
+    d.gs_lanes = 0u;
+    for (uint32_t lanes : {8u, 16u, 32u, 64u})
+        if (d.max_fiber_vertices <= tgh::kMaxLdsFiberVertices &&
+            tgh::curvature_lds_bytes(d.max_fiber_vertices, lanes, d.track_walls) <= 64u * 1024u) {
+            d.gs_lanes = lanes;
+            break;
+        }
     for (int k = 0; k < 3; ++k) {
         d.g.lower[k] = p.lower[k];
         d.g.upper[k] = p.upper[k];
@@ -640,6 +650,32 @@ int tgh_world_timing(tgh_world *world, double *group_ms, uint64_t *group_launche
             world->group_launches[g] = 0;
         }
     }
+    return TGH_OK;
+}
+
+int tgh_world_get_tuning(const tgh_world *world, tgh_tuning *out) {
+    if (!world || !out) return set_error(TGH_ERR_INVALID, "null argument");
+    *out = tgh_tuning{};
+    out->curvature_lanes = world->d.gs_lanes;
+    return TGH_OK;
+}
+
+int tgh_world_set_tuning(tgh_world *world, const tgh_tuning *tuning) {
+    if (!world || !tuning) return set_error(TGH_ERR_INVALID, "null argument");
+    const uint32_t lanes = tuning->curvature_lanes;
+    if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+        return set_error(TGH_ERR_INVALID, "curvature_lanes must be 0, 4, 8, 16, 32 or 64");
+    if (lanes != 0) {
+        const size_t lds = tgh::curvature_lds_bytes(world->d.max_fiber_vertices, lanes, world->d.track_walls);
+        if (world->d.max_fiber_vertices > tgh::kMaxLdsFiberVertices || lds > 64u * 1024u)
+            return set_error(TGH_ERR_SIZE, "the longest fiber does not fit in LDS at this lane count");
+    }
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    for (auto &kv : world->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (hipGraph_t g : world->graph_templates) (void)hipGraphDestroy(g);
+    world->graphs.clear();
+    world->graph_templates.clear();
+    world->d.gs_lanes = lanes;
     return TGH_OK;
 }
 

@@ -111,6 +111,45 @@ def test_bitwise_repeatable(hip_device):
         assert np.array_equal(runs[0][0], runs[1][0])
 
 
+def _walled_bimodal():
+    """The bimodal mix in a closed box: wall clamps and reactions are live."""
+    recipe = fx.bimodal_recipe(scale=0.02)
+    packed = recipe._packed(False)
+    return _native.PackedAssembly(packed.counts, packed.placed, packed.intrinsic, packed.radius,
+                                  packed.min_bend_radius, [packed.struct.cell_origin[k] for k in range(3)],
+                                  [packed.struct.cell_lengths[k] for k in range(3)], [False] * 3)
+
+
+@pytest.mark.parametrize("case", ["bimodal", "walled", "zigzag"])
+def test_curvature_launch_shapes_agree_bitwise(hip_device, case):
+    """The grouped curvature sweep (screen, then exact projection on three
+    axis lanes) at 4 to 64 lanes per fiber, and the sequential walk
+    with one lane per fiber, give the same positions bit for bit."""
+    if case == "zigzag":
+        packed = fx.zigzag()
+        cfg = fx.config(penetration_tolerance=0.0, stretch_stiffness=0.0, bend_stiffness=0.0, force_full_iterations=1,
+                        constraint_iterations=1, curvature_cleanup_sweeps=3, max_step=0.02)
+    else:
+        packed = fx.bimodal_recipe(scale=0.02)._packed(False) if case == "bimodal" else _walled_bimodal()
+        cfg = fx.hard_gate_settings(7e-6, force_full_iterations=True).to_config(7e-6)
+    results = {}
+    for lanes in (0, 4, 8, 16, 32, 64):
+        print(f"[case] {case} curvature_lanes={lanes}", flush=True)
+        with _native.World(packed, cfg) as w:
+            w.set_tuning(curvature_lanes=lanes)
+            assert w.tuning().curvature_lanes == lanes
+            w.run_batch(cfg, 40)
+            results[lanes] = w.download_positions().view(np.uint32).copy()
+    for lanes in (4, 8, 16, 32, 64):
+        assert np.array_equal(results[lanes], results[0]), lanes
+
+
+def test_tuning_rejects_bad_lanes(hip_device):
+    with _native.World(fx.zigzag(), fx.config()) as w:
+        with pytest.raises(ValueError):
+            w.set_tuning(curvature_lanes=8)
+
+
 def _lists_equal(gpu, ref):
     gc, go, gs = gpu.cell_list()
     oc, oo, os_ = ref.cell_list()
