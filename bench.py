@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--cpu-iterations", type=int, default=40, help="iterations in the CPU baseline sample")
     ap.add_argument("--scale", type=float, default=1.0, help="specimen scale (1.0 = the 2,000-fiber workload)")
     ap.add_argument("--dump", type=str, default="", help="write final tta positions (.npy) here")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 5 times (reference kernel for the HBM counters)")
     ap.add_argument("--curvature-lanes", type=int, default=-1,
                     help="lanes per fiber in the curvature sweep (0, 16, 32, 64; default: the world's choice)")
     return ap.parse_args()
@@ -72,6 +74,10 @@ def main():
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of this "
                                    "build (see DESIGN.md for how the path would shard)", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import fixtures as fx
     import tangle_amd as tg
@@ -212,6 +218,21 @@ def main():
         result["cpu_baseline"] = cpu_baseline(packed, cfg, S, args.cpu_iterations)
 
     print(json.dumps(result), flush=True)
+
+
+def copy_calibration(mib: int) -> None:
+    """A plain device-to-device copy of known size: its FETCH_SIZE and
+    WRITE_SIZE counters per dispatch give the scale between those counters
+    and bytes moved."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "dispatches": 5}), flush=True)
 
 
 def cpu_baseline(packed, cfg, S, iterations):

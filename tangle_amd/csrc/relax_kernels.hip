@@ -11,23 +11,25 @@
 //     test (crates/tangle_contact/src/device/cell_list.rs:99-110) reads its
 //     arc lengths from the same load.
 //   * Contact and list-build work is laid out in cell-list slot order, so the
-//     64 lanes of a wave hold spatially adjacent segments and share their
-//     candidate gathers in L1/L2; neighbour lists are stored column-major per
-//     slot (nbr[k * S + slot]), so list reads are coalesced across lanes.
+//     lanes of a wave hold spatially adjacent segments and share their
+//     candidate gathers in L2; each slot's list is one contiguous row
+//     (nbr[slot * block + k]) read by a group of lanes, one candidate each.
 //   * Each segment's list is kept sorted while it is built, in LDS, which
 //     replaces the separate sort pass (neighbor.rs:366-408).
 //   * The Jacobi stretch/bend passes and the contact application read one
 //     position buffer and write the other (kernels.rs:1027-1163, 1417-1537):
 //     a find/apply pair becomes one launch with identical per-vertex
 //     arithmetic, and the last pass also applies the contact corrections.
-//   * The curvature Gauss–Seidel keeps a sliding three-vertex window of each
-//     fiber in registers (kernels.rs:1165-1415) and stops after a forward
-//     sweep that projected nothing, which leaves the result unchanged: the
-//     remaining sweeps would revisit the same unchanged triplets.
-//   * Max penetration / curvature ratio / vertex step reduce with wave64
-//     shuffles and one u32 atomicMax per wave (order-free, so exact), and the
-//     convergence assessment runs in the last workgroup of the curvature
-//     kernel instead of a separate single-thread launch.
+//   * The curvature Gauss–Seidel stages each fiber in LDS, screens the next
+//     triplets of the sweep in parallel with a conservative bound and
+//     projects the first candidate exactly, one lane per axis
+//     (kernels.rs:1165-1415); it stops after a sweep direction that
+//     projected nothing, which leaves the result unchanged.
+//   * Max penetration / curvature ratio / vertex step reduce within each
+//     workgroup and then with one u32 atomicMax per workgroup into one of
+//     256 slots (order-free, so exact); the last workgroup of the measuring
+//     kernel folds the slots and assesses convergence, replacing the
+//     single-thread assess launch.
 //
 // All float arithmetic must be compiled without contraction
 // (-ffp-contract=off) and with IEEE division and square root, as the Rust
