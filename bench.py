@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--tta-warm", type=int, default=2560, help="default-settings iterations before the final stage")
     ap.add_argument("--skip-tta", action="store_true", help="skip the time-to-admissibility run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-iterations", type=int, default=40, help="iterations in the CPU baseline sample")
+    ap.add_argument("--cpu-iterations", type=int, default=384, help="iterations in the CPU baseline sample")
     ap.add_argument("--scale", type=float, default=1.0, help="specimen scale (1.0 = the 2,000-fiber workload)")
     ap.add_argument("--dump", type=str, default="", help="write final tta positions (.npy) here")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -58,6 +58,26 @@ def parse():
     ap.add_argument("--curvature-lanes", type=int, default=-1,
                     help="lanes per fiber in the curvature sweep (0, 16, 32, 64; default: the world's choice)")
     return ap.parse_args()
+
+
+def curvature_bytes_per_launch(V: int) -> float:
+    """Algorithmic bytes of one curvature-sweep launch: each vertex's position
+    (16 B), static records vA and vB (32 B) and list snapshot (16 B) read once,
+    its position written once (16 B)."""
+    return 80.0 * V
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration); None when the summary does not cover the kernel."""
+    path = ROOT / "profiles" / "hbm_counters_bimodal2000.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch ") and line.split()[1] == kernel:
+            return int(line.split("=")[1])
+    return None
 
 
 def contact_bytes_per_launch(S: int, V: int, mean_list: float) -> float:
@@ -141,23 +161,37 @@ def main():
 
     # dominant kernel group by event time
     dominant = max(GROUPS, key=lambda g: group_ms[g])
-    contact_avg_ms = group_ms["contact"] / max(1, group_launches["contact"])
-    contact_bytes = contact_bytes_per_launch(S, V, mean_list)
-    achieved = contact_bytes / (contact_avg_ms * 1e-3) / 1e9
-    roofline = {
-        "bound": "hbm",
-        "kernel": "tgh::k_contact",
-        "achieved": round(achieved, 1),
-        "peak": PEAK_HBM_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / PEAK_HBM_GBS, 4),
-        "traffic": None,
-        "avg_launch_us": round(1000.0 * contact_avg_ms, 2),
-        "bytes_per_launch": int(contact_bytes),
-        "dominant_group": dominant,
-        "group_ms_per_iteration": {g: round(group_ms[g] / max(1, total_iterations + args.steps), 4) for g in GROUPS},
-        "timing_replay_bitwise_equal": replay_matches,
+    kernels = {
+        "curvature_gs": (f"tgh::k_curvature_gs_group<{lanes}>" if lanes in (4, 8, 16, 32)
+                         else ("tgh::k_curvature_gs_wave64" if lanes == 64 else "tgh::k_curvature_gs_lane"),
+                         curvature_bytes_per_launch(V)),
+        "contact": ("tgh::k_contact<16>", contact_bytes_per_launch(S, V, mean_list)),
     }
+
+    def kernel_roofline(group):
+        name, nbytes = kernels[group]
+        avg_ms = group_ms[group] / max(1, group_launches[group])
+        achieved = nbytes / (avg_ms * 1e-3) / 1e9
+        return {
+            "bound": "hbm",
+            "kernel": name,
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 5),
+            "traffic": profiled_traffic(name) if args.scale == 1.0 else None,
+            "avg_launch_us": round(1000.0 * avg_ms, 2),
+            "bytes_per_launch": int(nbytes),
+        }
+
+    roofline = kernel_roofline("curvature_gs")
+    roofline["limiter"] = ("the sequential Gauss-Seidel projection chain of the slowest fiber "
+                           "(one exact projection per step), not memory")
+    roofline["contact"] = kernel_roofline("contact")
+    roofline["dominant_group"] = dominant
+    roofline["group_ms_per_iteration"] = {g: round(group_ms[g] / max(1, total_iterations + args.steps), 4)
+                                          for g in GROUPS}
+    roofline["timing_replay_bitwise_equal"] = replay_matches
 
     result = {
         "metric": METRIC,

@@ -163,12 +163,15 @@ void tgh_world_destroy(tgh_world *world);
 const char *tgh_last_error(void);
 
 /* ---- learned warm start (no counterpart in the reference) -------------
- * A per-segment MLP 64 -> 64 -> 64 -> 16 (51 inputs used: for the 16 nearest
- * listed neighbours the unit closest-point direction times
- * min((r1 + r2) / distance, 8), then the unit tangent; 3 outputs used: the
- * segment's predicted displacement in diameters), BF16 operands with FP32
+ * A per-segment MLP 64 -> 64 -> 64 -> 16, ReLU, BF16 operands with FP32
  * accumulation on MFMA, trained on the GPU from trajectories recorded from
- * the plain relaxer. Parameters are one f32 vector: W1 [64][64], b1 [64],
+ * the plain relaxer. 55 inputs are used: for the 16 nearest listed
+ * neighbours the unit closest-point direction times min((r1 + r2) /
+ * distance, 8); the unit tangent; and the recent motion since the world's
+ * last history mark (h = midpoint displacement / diameter, m = |h|): h / m
+ * and log10(m) / 8 + 1, zero without a mark or motion. 3 outputs are used:
+ * the predicted midpoint displacement over the next interval in units of
+ * m diameters. Parameters are one f32 vector: W1 [64][64], b1 [64],
  * W2 [64][64], b2 [64], W3 [16][64], b3 [16] (weights [out][in]). */
 typedef struct tgh_mlp tgh_mlp;
 size_t tgh_mlp_parameter_count(void);
@@ -182,9 +185,13 @@ int tgh_mlp_forward(tgh_mlp *mlp, const uint16_t *features, size_t rows, float *
  * grad: tgh_mlp_parameter_count() floats */
 int tgh_mlp_gradients(tgh_mlp *mlp, const uint16_t *features, const float *targets, size_t rows, float *grad,
                       float *loss);
+/* Marks the world's current segment midpoints as the reference of the
+ * motion features (history); proposals and recordings read the motion since
+ * the last mark. */
+int tgh_world_mark_history(tgh_world *world);
 /* Trajectory recording: features and segment midpoints of `world` now, then
- * (after the caller relaxes it) the midpoint displacement in diameters as
- * the targets; the pair of calls appends segment_count rows. */
+ * (after the caller relaxes it) the midpoint displacement in units of the
+ * recent motion as the targets; the pair of calls appends segment_count rows. */
 int tgh_mlp_record_begin(tgh_mlp *mlp, tgh_world *world);
 int tgh_mlp_record_end(tgh_mlp *mlp, tgh_world *world);
 uint64_t tgh_mlp_dataset_rows(const tgh_mlp *mlp);
@@ -195,8 +202,9 @@ int tgh_mlp_train(tgh_mlp *mlp, uint32_t epochs, uint32_t batch_rows, float lear
 /* features of every segment at the world's current positions: segment_count x 64 bf16 */
 int tgh_world_features(tgh_world *world, uint16_t *out, size_t count);
 /* Moves every unpinned vertex by `scale` times the mean predicted
- * displacement of its segments, keeping the previous positions for
- * tgh_world_revert_proposal; both request a neighbour-list rebuild. */
+ * displacement of its segments (prediction times m diameters), keeping the
+ * previous positions for tgh_world_revert_proposal; both request a
+ * neighbour-list rebuild. Segments without recent motion do not move. */
 int tgh_world_propose(tgh_world *world, tgh_mlp *mlp, float scale);
 int tgh_world_revert_proposal(tgh_world *world);
 /* Matrix-core layout check: d (16x16 f32) = a (16x16 bf16) * b (16x16 bf16), row-major. */

@@ -144,11 +144,14 @@ def closest_offset(p1, q1, p2, q2, periodic, lengths):
     return o, dist
 
 
-def segment_features(positions, seg_v0, seg_radius, lists, counts, periodic, lengths, neighbors=16):
+def segment_features(positions, seg_v0, seg_radius, lists, counts, periodic, lengths, neighbors=16, history=None):
     """Feature rows (bf16 bits, 64 per segment) from per-segment candidate
     lists: the `neighbors` candidates of smallest surface gap (earlier list
     entries first on ties), each as unit direction times
-    min((r1 + r2) / distance, 8), then the unit tangent, then zeros."""
+    min((r1 + r2) / distance, 8); the unit tangent; then, from `history`
+    (segment midpoints at the last mark, S x 3, or None), h = (midpoint -
+    history) / diameter, m = |h|: h / m and log10(m) / 8 + 1 (zero when
+    m <= 1e-12 or no history); then zeros."""
     pos = np.asarray(positions, dtype=np.float32).reshape(-1, 3)
     S = len(seg_v0)
     out = np.zeros((S, 64), dtype=np.float32)
@@ -172,4 +175,12 @@ def segment_features(positions, seg_v0, seg_radius, lists, counts, periodic, len
         n = _f(np.sqrt(_f(_f(_f(t[0] * t[0]) + _f(t[1] * t[1])) + _f(t[2] * t[2]))))
         inv = _f(_f(1.0) / _f(max(n, _f(1.0e-30))))
         out[s, 48:51] = [_f(t[0] * inv), _f(t[1] * inv), _f(t[2] * inv)]
+        if history is not None:
+            hist = np.asarray(history, dtype=np.float32).reshape(-1, 3)[s]
+            diameter = _f(_f(2.0) * r1)
+            h = [_f(_f(_f(_f(0.5) * _f(p1[k] + q1[k])) - _f(hist[k])) / diameter) for k in range(3)]
+            m = _f(np.sqrt(_f(_f(_f(h[0] * h[0]) + _f(h[1] * h[1])) + _f(h[2] * h[2]))))
+            if m > _f(1.0e-12):
+                out[s, 51:54] = [_f(h[0] / m), _f(h[1] / m), _f(h[2] / m)]
+                out[s, 54] = _f(_f(np.log10(m)) * _f(0.125)) + _f(1.0)
     return f32_to_bf16_bits(out)

@@ -586,6 +586,7 @@ class _RunState:
         self.downloaded = 0
         self.warm_start = warm_start
         self.last_proposal = 0
+        self.history_marked = False
         self.proposals_kept = 0
         self.proposals_reverted = 0
 
@@ -597,9 +598,17 @@ class _RunState:
         tolerance, curvature excess / tolerance) rose above `tolerance` times
         its value before the move. Plain PBD still runs every later pass."""
         ws = self.warm_start
-        if ws is None or self.total >= ws.active_iterations or self.total - self.last_proposal < ws.interval:
+        if ws is None or self.total >= ws.active_iterations:
             return
         from . import learned
+        if not self.history_marked:
+            # the first interval only provides the motion history
+            learned.mark_history(world)
+            self.history_marked = True
+            self.last_proposal = self.total
+            return
+        if self.total - self.last_proposal < ws.interval:
+            return
         self.last_proposal = self.total
         pen_tol = max(config.penetration_tolerance, 1e-30)
         curv_tol = max(config.curvature_ratio_tolerance, 1e-30)
@@ -622,6 +631,8 @@ class _RunState:
             self.converged = False
         else:
             self.proposals_kept += 1
+        learned.mark_history(world)
+        self.last_proposal = self.total
 
     def _batch(self, world, config, iterations):
         status = world.run_batch(config, iterations)

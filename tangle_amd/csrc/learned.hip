@@ -9,8 +9,13 @@
 // Features of a segment (row of 64 bf16): for its 16 nearest listed
 // neighbours by surface gap, the unit direction from its closest point to
 // the neighbour's times min((r1 + r2) / distance, 8) (zero when absent;
-// above one when overlapping), then the unit tangent. Target: the segment
-// midpoint's displacement over the recorded interval, in segment diameters.
+// above one when overlapping); the unit tangent; and the segment's recent
+// motion: with h = (midpoint now - midpoint at the last history mark) /
+// diameter and m = |h|, the direction h / m and log10(m) / 8 + 1 (all zero
+// when m is below 1e-12 or no mark exists). Target: the midpoint's
+// displacement over the next interval in units of m diameters, so the
+// network predicts the coming motion relative to the recent one, at every
+// stage of the relaxation alike.
 //
 // Matrix-core tiles: v_mfma_f32_16x16x16_bf16 with, for lane l,
 //   A (16x16, MxK): row l%16, k = 4*(l/16) + j       (4 bf16 per lane)
@@ -124,8 +129,9 @@ __device__ __forceinline__ void closest_offset(const GridParams &g, float4 P1, f
 
 __global__ __launch_bounds__(256) void k_features(GridParams g, const float4 *pos, const uint4 *sstat,
                                                   const uint32_t *cell_segs, const uint32_t *ncount,
-                                                  const uint32_t *nbr, const uint32_t *ovf, uint16_t *X,
-                                                  uint32_t block, uint32_t ovf_cap, uint32_t S) {
+                                                  const uint32_t *nbr, const uint32_t *ovf, const float4 *history,
+                                                  uint16_t *X, float *scale, uint32_t block, uint32_t ovf_cap,
+                                                  uint32_t S) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= S) return;
     const uint32_t seg = cell_segs[slot];
@@ -183,30 +189,65 @@ __global__ __launch_bounds__(256) void k_features(GridParams g, const float4 *po
     }
     float tx = Q1.x - P1.x, ty = Q1.y - P1.y, tz = Q1.z - P1.z;
     float inv = 1.0f / fmaxf(sqrtf(tx * tx + ty * ty + tz * tz), 1.0e-30f);
-    row[48] = f2bf(tx * inv);
-    row[49] = f2bf(ty * inv);
-    row[50] = f2bf(tz * inv);
-    for (uint32_t k = 51; k < kFeatureWidth; ++k) row[k] = 0;
+    row[kFeatTangent] = f2bf(tx * inv);
+    row[kFeatTangent + 1] = f2bf(ty * inv);
+    row[kFeatTangent + 2] = f2bf(tz * inv);
+    float hx = 0.f, hy = 0.f, hz = 0.f, hl = 0.f, m = 0.f;
+    if (history) {
+        const float4 h = history[seg];
+        const float diameter = 2.0f * r1;
+        hx = (0.5f * (P1.x + Q1.x) - h.x) / diameter;
+        hy = (0.5f * (P1.y + Q1.y) - h.y) / diameter;
+        hz = (0.5f * (P1.z + Q1.z) - h.z) / diameter;
+        m = sqrtf(hx * hx + hy * hy + hz * hz);
+        if (m > kMinMotion) {
+            hx /= m;
+            hy /= m;
+            hz /= m;
+            hl = log10f(m) * 0.125f + 1.0f;
+        } else {
+            hx = hy = hz = 0.f;
+            m = 0.f;
+        }
+    }
+    row[kFeatMotion] = f2bf(hx);
+    row[kFeatMotion + 1] = f2bf(hy);
+    row[kFeatMotion + 2] = f2bf(hz);
+    row[kFeatMotionLog] = f2bf(hl);
+    for (uint32_t k = kFeatMotionLog + 1; k < kFeatureWidth; ++k) row[k] = 0;
+    if (scale) scale[seg] = m * 2.0f * r1;
 }
 
-// Segment midpoints, for recording displacement targets.
-__global__ void k_midpoints(const float4 *pos, const uint4 *sstat, float4 *mid, uint32_t S) {
+// Segment midpoints; .w = the unit targets are measured in (the diameter
+// times the segment's recent motion when `scale` is given).
+__global__ void k_midpoints(const float4 *pos, const uint4 *sstat, const float *scale, float4 *mid, uint32_t S) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     const uint4 me = sstat[s];
     const float4 p = pos[me.x], q = pos[me.x + 1];
-    mid[s] = make_float4(0.5f * (p.x + q.x), 0.5f * (p.y + q.y), 0.5f * (p.z + q.z), 2.0f * __uint_as_float(me.z));
+    const float unit = scale ? scale[s] : 2.0f * __uint_as_float(me.z);
+    mid[s] = make_float4(0.5f * (p.x + q.x), 0.5f * (p.y + q.y), 0.5f * (p.z + q.z), unit);
 }
 
-// Target row = (midpoint now - midpoint then) / diameter.
+// Target row = (midpoint now - midpoint then) / unit, each component
+// clamped to [-kTargetClamp, kTargetClamp] (a segment that barely moved
+// before may move far more next: such rows would swamp the loss); zero when
+// unit is 0.
+constexpr float kTargetClamp = 4.0f;
 __global__ void k_targets(const float4 *pos, const uint4 *sstat, const float4 *mid_then, float4 *T, uint32_t S) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     const uint4 me = sstat[s];
     const float4 p = pos[me.x], q = pos[me.x + 1], m = mid_then[s];
+    if (!(m.w > 0.f)) {
+        T[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
     const float inv = 1.0f / m.w;
-    T[s] = make_float4((0.5f * (p.x + q.x) - m.x) * inv, (0.5f * (p.y + q.y) - m.y) * inv,
-                       (0.5f * (p.z + q.z) - m.z) * inv, 0.f);
+    const float c = kTargetClamp;
+    T[s] = make_float4(fminf(fmaxf((0.5f * (p.x + q.x) - m.x) * inv, -c), c),
+                       fminf(fmaxf((0.5f * (p.y + q.y) - m.y) * inv, -c), c),
+                       fminf(fmaxf((0.5f * (p.z + q.z) - m.z) * inv, -c), c), 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -475,8 +516,9 @@ __global__ void k_mlp_publish(MlpDevice m) {
 }
 
 // Proposal: each vertex moves by the mean of its segments' predicted
-// displacements (diameters times prediction), scaled, clamped to walls.
-__global__ void k_apply_proposal(GridParams g, float4 *pos, const float4 *backup_src, float4 *backup,
+// displacements (prediction times the segment's unit), scaled, clamped to
+// walls.
+__global__ void k_apply_proposal(GridParams g, float4 *pos, const float *seg_scale, float4 *backup,
                                  const float4 *Y, const float4 *vA, const float4 *vB, float scale, uint32_t V) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
@@ -486,18 +528,19 @@ __global__ void k_apply_proposal(GridParams g, float4 *pos, const float4 *backup
     const uint32_t flags = __float_as_uint(sb.z);
     if (flags & kPinned) return;
     const uint32_t next_seg = v - __float_as_uint(sb.w);
-    const float diameter = 2.0f * vA[v].w;
     float dx = 0.f, dy = 0.f, dz = 0.f, n = 0.f;
     if (flags & kHasPrev) {
-        float4 y = Y[next_seg - 1];
-        dx += y.x; dy += y.y; dz += y.z; n += 1.f;
+        const float4 y = Y[next_seg - 1];
+        const float u = seg_scale[next_seg - 1];
+        dx += y.x * u; dy += y.y * u; dz += y.z * u; n += 1.f;
     }
     if (flags & kHasNext) {
-        float4 y = Y[next_seg];
-        dx += y.x; dy += y.y; dz += y.z; n += 1.f;
+        const float4 y = Y[next_seg];
+        const float u = seg_scale[next_seg];
+        dx += y.x * u; dy += y.y * u; dz += y.z * u; n += 1.f;
     }
     if (n == 0.f) return;
-    const float k = scale * diameter / n;
+    const float k = scale / n;
     float x = p.x + dx * k, y = p.y + dy * k, z = p.z + dz * k;
     const float e = vA[v].w;
     if (!g.periodic[0]) x = fminf(fmaxf(x, g.lower[0] + e), g.upper[0] - e);
@@ -523,14 +566,14 @@ hipError_t launch_mfma_probe(const uint16_t *A, const uint16_t *B, float *D, hip
     return hipGetLastError();
 }
 
-hipError_t launch_features(const WorldDev &w, uint16_t *X, hipStream_t s) {
+hipError_t launch_features(const WorldDev &w, const float4 *history, uint16_t *X, float *scale, hipStream_t s) {
     hipLaunchKernelGGL(k_features, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.g, w.pos, w.sstat, w.cell_segs,
-                       w.ncount, w.nbr, w.ovf, X, w.block, w.ovf_cap, w.S);
+                       w.ncount, w.nbr, w.ovf, history, X, scale, w.block, w.ovf_cap, w.S);
     return hipGetLastError();
 }
 
-hipError_t launch_midpoints(const WorldDev &w, float4 *mid, hipStream_t s) {
-    hipLaunchKernelGGL(k_midpoints, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.pos, w.sstat, mid, w.S);
+hipError_t launch_midpoints(const WorldDev &w, const float *scale, float4 *mid, hipStream_t s) {
+    hipLaunchKernelGGL(k_midpoints, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.pos, w.sstat, scale, mid, w.S);
     return hipGetLastError();
 }
 
@@ -566,9 +609,10 @@ hipError_t launch_mlp_publish(const MlpDevice &m, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_apply_proposal(const WorldDev &w, const float4 *Y, float4 *backup, float scale, hipStream_t s) {
-    hipLaunchKernelGGL(k_apply_proposal, dim3(nblocks(w.V, 256)), dim3(256), 0, s, w.g, w.pos, w.pos, backup, Y, w.vA,
-                       w.vB, scale, w.V);
+hipError_t launch_apply_proposal(const WorldDev &w, const float4 *Y, const float *seg_scale, float4 *backup,
+                                 float scale, hipStream_t s) {
+    hipLaunchKernelGGL(k_apply_proposal, dim3(nblocks(w.V, 256)), dim3(256), 0, s, w.g, w.pos, seg_scale, backup, Y,
+                       w.vA, w.vB, scale, w.V);
     return hipGetLastError();
 }
 

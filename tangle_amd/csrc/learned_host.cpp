@@ -40,6 +40,7 @@ struct tgh_mlp {
     // pending recording
     uint16_t *pending_x = nullptr;
     float4 *pending_mid = nullptr;
+    float *pending_scale = nullptr;
     uint32_t pending_rows = 0, pending_capacity = 0;
     // scratch for standalone forward/gradient calls
     std::vector<void *> owned;
@@ -51,6 +52,7 @@ struct tgh_mlp {
         if (T) (void)hipFree(T);
         if (pending_x) (void)hipFree(pending_x);
         if (pending_mid) (void)hipFree(pending_mid);
+        if (pending_scale) (void)hipFree(pending_scale);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -114,10 +116,14 @@ int ensure_pending(tgh_mlp *m, uint32_t rows) {
     if (rows <= m->pending_capacity) return TGH_OK;
     if (m->pending_x) (void)hipFree(m->pending_x);
     if (m->pending_mid) (void)hipFree(m->pending_mid);
+    if (m->pending_scale) (void)hipFree(m->pending_scale);
     m->pending_x = nullptr;
     m->pending_mid = nullptr;
+    m->pending_scale = nullptr;
+    m->pending_capacity = 0;
     HIP_CHECK(hipMalloc((void **)&m->pending_x, (size_t)rows * tgh::kFeatureWidth * sizeof(uint16_t)));
     HIP_CHECK(hipMalloc((void **)&m->pending_mid, (size_t)rows * sizeof(float4)));
+    HIP_CHECK(hipMalloc((void **)&m->pending_scale, (size_t)rows * sizeof(float)));
     m->pending_capacity = rows;
     return TGH_OK;
 }
@@ -265,12 +271,23 @@ int tgh_world_features(tgh_world *world, uint16_t *out, size_t count) {
     uint16_t *X = nullptr;
     HIP_CHECK(hipMalloc((void **)&X, S * tgh::kFeatureWidth * sizeof(uint16_t)));
     hipError_t e = tgh::launch_refresh_lists(world->d, world->stream);
-    if (e == hipSuccess) e = tgh::launch_features(world->d, X, world->stream);
+    if (e == hipSuccess)
+        e = tgh::launch_features(world->d, world->history_valid ? world->history : nullptr, X, nullptr,
+                                 world->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out, X, S * tgh::kFeatureWidth * sizeof(uint16_t), hipMemcpyDeviceToHost,
                                             world->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(world->stream);
     (void)hipFree(X);
     if (e != hipSuccess) return tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e));
+    return TGH_OK;
+}
+
+int tgh_world_mark_history(tgh_world *world) {
+    if (!world) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (!world->history) HIP_CHECK(world->alloc(&world->history, world->d.S));
+    HIP_CHECK(tgh::launch_midpoints(world->d, nullptr, world->history, world->stream));
+    HIP_CHECK(hipStreamSynchronize(world->stream));
+    world->history_valid = true;
     return TGH_OK;
 }
 
@@ -280,8 +297,9 @@ int tgh_mlp_record_begin(tgh_mlp *mlp, tgh_world *world) {
     if (rc) return rc;
     HIP_CHECK(hipStreamSynchronize(mlp->stream));
     HIP_CHECK(tgh::launch_refresh_lists(world->d, world->stream));
-    HIP_CHECK(tgh::launch_features(world->d, mlp->pending_x, world->stream));
-    HIP_CHECK(tgh::launch_midpoints(world->d, mlp->pending_mid, world->stream));
+    HIP_CHECK(tgh::launch_features(world->d, world->history_valid ? world->history : nullptr, mlp->pending_x,
+                                   mlp->pending_scale, world->stream));
+    HIP_CHECK(tgh::launch_midpoints(world->d, mlp->pending_scale, mlp->pending_mid, world->stream));
     HIP_CHECK(hipStreamSynchronize(world->stream));
     mlp->pending_rows = world->d.S;
     return TGH_OK;
@@ -364,12 +382,15 @@ int tgh_world_propose(tgh_world *world, tgh_mlp *mlp, float scale) {
         HIP_CHECK(world->alloc(&world->proposal_x, S * tgh::kFeatureWidth));
         HIP_CHECK(world->alloc(&world->proposal_y, S));
         HIP_CHECK(world->alloc(&world->proposal_backup, V));
+        HIP_CHECK(world->alloc(&world->proposal_scale, S));
     }
     HIP_CHECK(hipStreamSynchronize(mlp->stream));
     HIP_CHECK(tgh::launch_refresh_lists(world->d, world->stream));
-    HIP_CHECK(tgh::launch_features(world->d, world->proposal_x, world->stream));
+    HIP_CHECK(tgh::launch_features(world->d, world->history_valid ? world->history : nullptr, world->proposal_x,
+                                   world->proposal_scale, world->stream));
     HIP_CHECK(tgh::launch_mlp_forward(mlp->m, world->proposal_x, world->proposal_y, (uint32_t)S, world->stream));
-    HIP_CHECK(tgh::launch_apply_proposal(world->d, world->proposal_y, world->proposal_backup, scale, world->stream));
+    HIP_CHECK(tgh::launch_apply_proposal(world->d, world->proposal_y, world->proposal_scale, world->proposal_backup,
+                                         scale, world->stream));
     // the proposal moved vertices outside the displacement-tracked loop
     HIP_CHECK(tgh::launch_request_rebuild(world->d, world->stream));
     HIP_CHECK(hipStreamSynchronize(world->stream));

@@ -576,6 +576,8 @@ int tgh_world_upload_positions(tgh_world *world, const float *positions, size_t 
         tmp[v].z = positions[3 * v + 2];
     }
     HIP_TRY(hipMemcpy(world->d.pos, tmp.data(), V * sizeof(float4), hipMemcpyHostToDevice));
+    world->history_valid = false;  // the motion history described the replaced positions
+    world->proposal_pending = false;
     DevState s;
     HIP_TRY(hipMemcpy(&s, world->d.st, sizeof(s), hipMemcpyDeviceToHost));
     s.nstate[0] = 1;

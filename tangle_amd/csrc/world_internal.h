@@ -56,6 +56,9 @@ struct tgh_world {
     uint16_t *proposal_x = nullptr;
     float4 *proposal_y = nullptr;
     float4 *proposal_backup = nullptr;
+    float *proposal_scale = nullptr;  // per segment: history length (m) x diameter
+    float4 *history = nullptr;        // per segment: midpoint at the last history mark
+    bool history_valid = false;
     bool proposal_pending = false;
     // Kernel-group timing: events bracket every group of every iteration of
     // a batch, launched directly on the stream.

@@ -1,13 +1,15 @@
 """Learned warm start for the relaxation loop (an addition; the reference has none).
 
-A per-segment MLP (51 -> 64 -> 64 -> 3, BF16 operands with FP32 accumulation
-on the MI355X matrix cores) predicts each segment's displacement over the
-next `horizon` iterations of the plain relaxer from its 16 nearest neighbour
-capsules and its tangent. It is trained on the GPU, by the library's own
-forward and backward kernels, on trajectories the plain relaxer generates.
-During a run it proposes a displacement every `interval` iterations; a
-proposal that raises the measured residual is reverted, and the plain PBD
-projector always runs the final passes and judges admissibility.
+A per-segment MLP (55 -> 64 -> 64 -> 3, BF16 operands with FP32 accumulation
+on the MI355X matrix cores) predicts each segment's midpoint displacement
+over the next interval of the plain relaxer from its 16 nearest neighbour
+capsules, its tangent and its own motion over the previous interval, in
+units of that recent motion: it learns to extrapolate the relaxer's
+trajectory. It is trained on the GPU, by the library's own forward and
+backward kernels, on trajectories the plain relaxer generates. During a run
+it proposes a displacement every `interval` iterations; a proposal that
+raises the measured residual is reverted, and the plain PBD projector always
+runs the later passes and judges admissibility.
 """
 
 from __future__ import annotations
@@ -111,6 +113,11 @@ def world_features(world: _native.World) -> np.ndarray:
     return out
 
 
+def mark_history(world: _native.World) -> None:
+    """Makes the current midpoints the reference of the motion features."""
+    _native.check(_native.library().tgh_world_mark_history(world._handle))
+
+
 def propose(world: _native.World, model: WarmStartModel, scale: float) -> None:
     _native.check(_native.library().tgh_world_propose(world._handle, model._handle, float(scale)))
 
@@ -130,33 +137,37 @@ def measure(world: _native.World, config: _native.Config) -> _native.BatchStatus
 
 
 def collect_trajectories(model: WarmStartModel, packed: _native.PackedAssembly, config: _native.Config,
-                         iterations: int, interval: int, horizon: int) -> int:
+                         iterations: int, interval: int) -> int:
     """Relaxes `packed` with the plain loop for `iterations` iterations and,
-    every `interval` iterations, records (features now, midpoint displacement
-    after `horizon` more iterations) for every segment. Returns rows added."""
+    at every multiple of `interval` past the first, records one row per
+    segment: the features (with the motion over the previous interval) and
+    the midpoint displacement over the next interval. Returns rows added."""
+    if interval <= 0 or iterations < 2 * interval:
+        raise ValueError("need interval > 0 and at least two intervals of iterations")
     before = model.dataset_rows
     cfg = _native.Config.from_buffer_copy(config)
     cfg.force_full_iterations = 1
     with _native.World(packed, cfg) as world:
-        done = 0
-        while done + horizon <= iterations:
+        mark_history(world)
+        world.run_batch(cfg, interval)
+        done = interval
+        while done + interval <= iterations:
             model.record_begin(world)
-            world.run_batch(cfg, horizon)
+            mark_history(world)
+            world.run_batch(cfg, interval)
             model.record_end(world)
-            done += horizon
-            rest = interval - horizon
-            if rest > 0 and done + rest <= iterations:
-                world.run_batch(cfg, rest)
-                done += rest
+            done += interval
     return model.dataset_rows - before
 
 
 @dataclass
 class WarmStart:
     """How Recipe.run uses a trained model: every `interval` iterations of the
-    first `active_iterations`, move each vertex by `scale` times the predicted
+    first `active_iterations` (after a first interval that provides the
+    motion history), move each vertex by `scale` times the predicted
     displacement, run `settle` plain iterations, and revert the move when the
-    normalised residual is then above `tolerance` times its value before."""
+    normalised residual is then above `tolerance` times its value before.
+    `interval` should be the one the model was trained with."""
 
     model: WarmStartModel
     interval: int = 64

@@ -147,7 +147,7 @@ def test_curvature_launch_shapes_agree_bitwise(hip_device, case):
 def test_tuning_rejects_bad_lanes(hip_device):
     with _native.World(fx.zigzag(), fx.config()) as w:
         with pytest.raises(ValueError):
-            w.set_tuning(curvature_lanes=8)
+            w.set_tuning(curvature_lanes=12)
 
 
 def _lists_equal(gpu, ref):

@@ -88,3 +88,19 @@ def test_features_keep_sixteen_smallest_gaps_in_order():
     ys = f[0, 1:48:3]
     expect = [0.2 / (0.25 + 0.05 * k) for k in range(16)]
     assert np.allclose(ys, expect, rtol=4e-3)
+
+
+def test_features_motion_history_closed_form():
+    # segment 0 moved +0.002 along y since the mark; diameter 0.2: h = 0.01,
+    # direction +y, log10(0.01) / 8 + 1 = 0.75; segment 1 did not move
+    pos = np.array([[0, 1, 1], [1, 1, 1], [0, 1.3, 1], [1, 1.3, 1]], dtype=np.float32)
+    v0 = np.array([0, 2])
+    radius = np.array([0.1, 0.1], dtype=np.float32)
+    mids = np.array([[0.5, 1.0 - 0.002, 1.0], [0.5, 1.3, 1.0]], dtype=np.float32)
+    f = mlp_ref.bf16_bits_to_f32(mlp_ref.segment_features(
+        pos, v0, radius, np.zeros((2, 0), dtype=np.int64), np.zeros(2, dtype=np.int64), [False] * 3, [4.0] * 3,
+        history=mids))
+    assert f[0, 51] == 0 and f[0, 53] == 0
+    assert f[0, 52] == pytest.approx(1.0, rel=4e-3)
+    assert f[0, 54] == pytest.approx(0.75, abs=4e-3)
+    assert not np.any(f[1, 51:55])
