@@ -49,6 +49,10 @@ def parse():
     ap.add_argument("--tta-max-iterations", type=int, default=40000, help="budget of the final hard-gate stage")
     ap.add_argument("--tta-warm", type=int, default=2560, help="default-settings iterations before the final stage")
     ap.add_argument("--skip-tta", action="store_true", help="skip the time-to-admissibility run")
+    ap.add_argument("--skip-learned", action="store_true", help="skip the learned warm-start run")
+    ap.add_argument("--learned-warm", type=int, default=960, help="default-settings iterations of the learned plan")
+    ap.add_argument("--learned-scale", type=float, default=4.0, help="proposal scale (intervals extrapolated)")
+    ap.add_argument("--learned-interval", type=int, default=32, help="iterations between proposals")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iterations", type=int, default=384, help="iterations in the CPU baseline sample")
     ap.add_argument("--scale", type=float, default=1.0, help="specimen scale (1.0 = the 2,000-fiber workload)")
@@ -244,6 +248,10 @@ def main():
             tta["plain"] = {"seconds": None, "iterations": err.iteration, "converged": False,
                             "error": str(err), "elapsed_s": round(wall, 3)}
             positions = None
+        if not args.skip_learned:
+            tta["learned"] = learned_tta(args, fx, tg, d_min)
+            if tta["plain"] and tta["plain"].get("seconds") and tta["learned"].get("seconds"):
+                tta["learned_speedup"] = round(tta["plain"]["seconds"] / tta["learned"]["seconds"], 3)
         result["time_to_hard_admissibility"] = tta
         if args.dump and positions is not None:
             np.save(args.dump, positions)
@@ -252,6 +260,42 @@ def main():
         result["cpu_baseline"] = cpu_baseline(packed, cfg, S, args.cpu_iterations)
 
     print(json.dumps(result), flush=True)
+
+
+def learned_tta(args, fx, tg, d_min):
+    """The learned warm start: a model trained on the GPU from plain-relaxer
+    trajectories of two other specimens of the same distribution (seed
+    offsets 7 and 13; training time reported, not counted in the run), then
+    Recipe.run on the benchmark specimen with proposals every
+    `--learned-interval` iterations, a shorter default-settings stage and the
+    same final hard-gate stage as the plain plan."""
+    from tangle_amd import learned
+
+    t0 = time.perf_counter()
+    specimens = [fx.bimodal_recipe(scale=args.scale, seed_offset=o)._packed(False) for o in (7, 13)]
+    model, losses = learned.train_warm_start(specimens, d_min, interval=args.learned_interval)
+    rows = model.dataset_rows
+    train_s = time.perf_counter() - t0
+    ws = learned.WarmStart(model, interval=args.learned_interval, scale=args.learned_scale, settle=4,
+                           active_iterations=10 ** 9)
+    r = fx.staged_hard_gate_recipe(fx.bimodal_recipe(scale=args.scale), warm_iterations=args.learned_warm,
+                                   cleanup_iterations=args.tta_max_iterations)
+    settings = fx.hard_gate_settings(d_min, max_iterations=args.learned_warm + args.tta_max_iterations + 1)
+    out = {"training_seconds": round(train_s, 3), "training_rows": int(rows),
+           "training_loss": [round(float(losses[0]), 5), round(float(losses[-1]), 5)],
+           "interval": args.learned_interval, "scale": args.learned_scale}
+    t0 = time.perf_counter()
+    try:
+        res = r.run(settings, warm_start=ws)
+        wall = time.perf_counter() - t0
+        out.update({"seconds": round(wall, 3), "iterations": res.iterations, "converged": res.converged,
+                    "max_penetration": res.max_penetration, "max_curvature_ratio": res.max_curvature_ratio,
+                    "plan": r.operations()[2:], "events": res.events[2:]})
+    except tg.RecipeError as err:
+        out.update({"seconds": None, "iterations": err.iteration, "converged": False, "error": str(err),
+                    "elapsed_s": round(time.perf_counter() - t0, 3)})
+    model.close()
+    return out
 
 
 def copy_calibration(mib: int) -> None:

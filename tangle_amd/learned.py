@@ -181,3 +181,37 @@ class WarmStart:
             raise ValueError("interval must be positive; active_iterations and settle nonnegative")
         if not (self.scale == self.scale and abs(self.scale) < 1e6) or not self.tolerance > 0:
             raise ValueError("scale must be finite and tolerance positive")
+
+
+def train_warm_start(specimens, d_min: float, *, interval: int = 32, default_iterations: int = 1024,
+                     cleanup_iterations: int = 1024, cleanup_preset: str | None = "contact_cleanup",
+                     cleanup_after: int = 2048, epochs: int = 12, batch_rows: int = 16384,
+                     learning_rate: float = 1e-3, seed: int = 0, model: WarmStartModel | None = None):
+    """Trains a warm-start model on the GPU from plain-relaxer trajectories of
+    `specimens` (PackedAssembly objects): `default_iterations` from each
+    generated state under the default settings, then, from the state after
+    `cleanup_after` default iterations, `cleanup_iterations` under the
+    `cleanup_preset` overrides (the final stage of a hard-gate plan). Returns
+    (model, epoch_losses)."""
+    from . import RelaxationOverrides, RelaxationSettings
+
+    model = model or WarmStartModel(seed=seed)
+    base = RelaxationSettings().to_config(d_min)
+    cleanup = RelaxationOverrides.preset(cleanup_preset).apply(base) if cleanup_preset else base
+    for packed in specimens:
+        collect_trajectories(model, packed, base, iterations=default_iterations, interval=interval)
+        if cleanup_iterations <= 0:
+            continue
+        cfg = _native.Config.from_buffer_copy(base)
+        cfg.force_full_iterations = 1
+        with _native.World(packed, cfg) as world:
+            world.run_batch(cfg, cleanup_after)
+            warm = world.download_positions().astype(np.float64)
+        s = packed.struct
+        relaxed = _native.PackedAssembly(packed.counts, warm, packed.intrinsic, packed.radius, packed.min_bend_radius,
+                                         [s.cell_origin[k] for k in range(3)], [s.cell_lengths[k] for k in range(3)],
+                                         [bool(s.cell_periodic[k]) for k in range(3)],
+                                         pin_fiber_ends=bool(s.pin_fiber_ends), vertex_pinned=packed.vertex_pinned)
+        collect_trajectories(model, relaxed, cleanup, iterations=cleanup_iterations, interval=interval)
+    losses = model.train(epochs=epochs, batch_rows=batch_rows, learning_rate=learning_rate, seed=seed)
+    return model, losses
