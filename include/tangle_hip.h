@@ -109,6 +109,8 @@ typedef struct tgh_world_info {
     uint32_t neighbor_block;             /* list slots per segment actually used */
     uint32_t neighbor_list_rebuilds;     /* neighbor_state[1] */
     uint64_t device_bytes;               /* device memory held by the world */
+    uint32_t specimen_count;             /* specimens relaxed together (1 unless batched) */
+    uint32_t reserved;
 } tgh_world_info;
 
 typedef struct tgh_world tgh_world;
@@ -117,6 +119,18 @@ typedef struct tgh_world tgh_world;
 void tgh_config_default(tgh_config *config);
 
 int tgh_world_create(const tgh_assembly *assembly, const tgh_config *config, tgh_world **out);
+/* Batched specimens (no counterpart in the reference, which relaxes one
+ * world per DeviceWorld): `count` independent assemblies, each with its own
+ * cell, grid, skin and convergence, relaxed together by the same launches.
+ * Vertices, fibers and segments are numbered specimen after specimen, and
+ * tgh_world_download_positions returns them in that order. Each specimen
+ * stops on its own gate; tgh_world_run_batch reports the worst of each
+ * field (converged only when all converged) and tgh_world_specimen_status
+ * each specimen's status of the last batch. The neighbour-list rebuild is
+ * shared: any specimen's displacement trigger rebuilds every list. */
+int tgh_world_create_batch(const tgh_assembly *specimens, uint32_t count, const tgh_config *config,
+                           tgh_world **out);
+int tgh_world_specimen_status(const tgh_world *world, tgh_batch_status *out, size_t count);
 int tgh_world_run_batch(tgh_world *world, const tgh_config *config, uint32_t iterations,
                         tgh_batch_status *status);
 int tgh_world_download_positions(const tgh_world *world, float *out, size_t count);

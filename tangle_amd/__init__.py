@@ -836,3 +836,58 @@ class Recipe:
             warnings=run.warnings, neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
             positions=positions, vertex_counts=packed.counts.copy(),
         )
+
+
+def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iterations: int | None = None,
+                iterations: int | None = None) -> list[RunResult]:
+    """Relaxes independent specimens together (no counterpart in the
+    reference, which runs one world at a time): every recipe's fibers and
+    cell become one specimen of a batched device world, all specimens step
+    through the same launches, and each stops on its own convergence gate.
+
+    With `iterations`, every specimen runs exactly that many iterations
+    (force_full_iterations); otherwise each relaxes until converged, within
+    `max_iterations` (default: settings.max_iterations). Returns one
+    RunResult per recipe, in order; a specimen that does not converge in
+    the budget is returned with converged=False rather than raising."""
+    recipes = list(recipes)
+    if not recipes:
+        raise ValueError("relax_batch needs at least one recipe")
+    for r in recipes:
+        if not isinstance(r, Recipe):
+            raise TypeError("relax_batch takes Recipe objects")
+        if not r._fibers:
+            raise ValueError("a recipe in the batch contains no fibers")
+    settings = settings or RelaxationSettings()
+    budget = int(max_iterations if max_iterations is not None else settings.max_iterations)
+    if budget <= 0 or (iterations is not None and iterations <= 0):
+        raise ValueError("iteration counts must be positive")
+    started = time.perf_counter()
+    d_min = min(2.0 * f.radius for r in recipes for f in r._fibers)
+    config = settings.to_config(d_min)
+    if iterations is not None:
+        config.force_full_iterations = 1
+        budget = int(iterations)
+    packed = [r._packed(settings.pin_fiber_ends) for r in recipes]
+    with _native.World.batch(packed, config) as world:
+        info = world.info()
+        done = 0
+        while done < budget:
+            status = world.run_batch(config, min(settings.iterations_per_batch, budget - done))
+            done = int(status.total_iterations)
+            if status.converged or status.batch_iterations == 0:
+                break
+        statuses = world.specimen_status()
+        positions = world.specimen_positions()
+        rebuilds = int(world.info().neighbor_list_rebuilds)
+    wall = time.perf_counter() - started
+    out = []
+    for r, p, st, pos in zip(recipes, packed, statuses, positions):
+        out.append(RunResult(
+            fiber_count=len(r._fibers), iterations=int(st.total_iterations), converged=bool(st.converged),
+            max_penetration=float(st.max_penetration), max_curvature_ratio=float(st.max_curvature_ratio),
+            max_displacement=float(st.max_displacement), active_segments=int(p.vertex_count - p.counts.size),
+            active_vertices=int(p.vertex_count), cell_count=int(info.cell_count), uploaded_bytes=0,
+            downloaded_bytes=int(pos.nbytes), events=[f"batched with {len(recipes)} specimens"], warnings=[],
+            neighbor_list_rebuilds=rebuilds, wall_seconds=wall, positions=pos, vertex_counts=p.counts.copy()))
+    return out

@@ -96,6 +96,8 @@ class WorldInfo(C.Structure):
         ("neighbor_block", C.c_uint32),
         ("neighbor_list_rebuilds", C.c_uint32),
         ("device_bytes", C.c_uint64),
+        ("specimen_count", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -133,6 +135,8 @@ class PopulationSpec(C.Structure):
 SIGNATURES = {
     "tgh_config_default": (None, [C.POINTER(Config)]),
     "tgh_world_create": (C.c_int, [C.POINTER(Assembly), C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    "tgh_world_create_batch": (C.c_int, [C.POINTER(Assembly), C.c_uint32, C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    "tgh_world_specimen_status": (C.c_int, [C.c_void_p, C.POINTER(BatchStatus), C.c_size_t]),
     "tgh_world_run_batch": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_uint32, C.POINTER(BatchStatus)]),
     "tgh_world_download_positions": (C.c_int, [C.c_void_p, _f32p, C.c_size_t]),
     "tgh_world_upload_positions": (C.c_int, [C.c_void_p, _f32p, C.c_size_t]),
@@ -258,14 +262,52 @@ class PackedAssembly:
 
 
 class World:
-    """One resident device world (DeviceFiberWorld on the HIP backend)."""
+    """One resident device world (DeviceFiberWorld on the HIP backend), or
+    several specimens relaxed together (World.batch)."""
 
     def __init__(self, packed: PackedAssembly, config: Config):
         lib = library()
         self._packed = packed
+        self._specimens = [packed]
+        self._vertex_count = packed.vertex_count
         handle = C.c_void_p()
         check(lib.tgh_world_create(C.byref(packed.struct), C.byref(config), C.byref(handle)))
         self._handle = handle
+
+    @classmethod
+    def batch(cls, specimens, config: Config) -> "World":
+        """Independent specimens (PackedAssembly each) in one world."""
+        specimens = list(specimens)
+        if not specimens:
+            raise ValueError("no specimens")
+        lib = library()
+        self = cls.__new__(cls)
+        self._packed = specimens[0]
+        self._specimens = specimens
+        self._vertex_count = sum(p.vertex_count for p in specimens)
+        arr = (Assembly * len(specimens))(*[p.struct for p in specimens])
+        handle = C.c_void_p()
+        check(lib.tgh_world_create_batch(arr, len(specimens), C.byref(config), C.byref(handle)))
+        self._handle = handle
+        return self
+
+    @property
+    def specimen_count(self) -> int:
+        return len(self._specimens)
+
+    def specimen_status(self):
+        out = (BatchStatus * len(self._specimens))()
+        check(library().tgh_world_specimen_status(self._handle, out, len(self._specimens)))
+        return list(out)
+
+    def specimen_positions(self):
+        """Positions split per specimen (each 3 * its vertex count floats)."""
+        flat = self.download_positions()
+        out, start = [], 0
+        for p in self._specimens:
+            out.append(flat[start:start + 3 * p.vertex_count])
+            start += 3 * p.vertex_count
+        return out
 
     def close(self) -> None:
         if getattr(self, "_handle", None):
@@ -295,7 +337,7 @@ class World:
         return info
 
     def download_positions(self) -> np.ndarray:
-        n = 3 * self._packed.vertex_count
+        n = 3 * self._vertex_count
         out = np.empty(n, dtype=np.float32)
         check(library().tgh_world_download_positions(self._handle, ptr(out, C.c_float), n))
         return out

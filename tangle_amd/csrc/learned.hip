@@ -127,7 +127,7 @@ __device__ __forceinline__ void closest_offset(const GridParams &g, float4 P1, f
     dist = sqrtf(ox * ox + oy * oy + oz * oz);
 }
 
-__global__ __launch_bounds__(256) void k_features(GridParams g, const float4 *pos, const uint4 *sstat,
+__global__ __launch_bounds__(256) void k_features(const GridParams *grids, const float4 *pos, const uint4 *sstat,
                                                   const uint32_t *cell_segs, const uint32_t *ncount,
                                                   const uint32_t *nbr, const uint32_t *ovf, const float4 *history,
                                                   uint16_t *X, float *scale, uint32_t block, uint32_t ovf_cap,
@@ -136,6 +136,7 @@ __global__ __launch_bounds__(256) void k_features(GridParams g, const float4 *po
     if (slot >= S) return;
     const uint32_t seg = cell_segs[slot];
     const uint4 me = sstat[seg];
+    const GridParams &g = grids[me.w];
     const float4 P1 = pos[me.x], Q1 = pos[me.x + 1];
     const float r1 = __uint_as_float(me.z);
     const uint32_t count = ncount[slot];
@@ -518,7 +519,7 @@ __global__ void k_mlp_publish(MlpDevice m) {
 // Proposal: each vertex moves by the mean of its segments' predicted
 // displacements (prediction times the segment's unit), scaled, clamped to
 // walls.
-__global__ void k_apply_proposal(GridParams g, float4 *pos, const float *seg_scale, float4 *backup,
+__global__ void k_apply_proposal(const GridParams *grids, float4 *pos, const float *seg_scale, float4 *backup,
                                  const float4 *Y, const float4 *vA, const float4 *vB, float scale, uint32_t V) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
@@ -527,6 +528,7 @@ __global__ void k_apply_proposal(GridParams g, float4 *pos, const float *seg_sca
     const float4 sb = vB[v];
     const uint32_t flags = __float_as_uint(sb.z);
     if (flags & kPinned) return;
+    const GridParams &g = grids[flags >> kSpecShift];
     const uint32_t next_seg = v - __float_as_uint(sb.w);
     float dx = 0.f, dy = 0.f, dz = 0.f, n = 0.f;
     if (flags & kHasPrev) {
@@ -567,7 +569,7 @@ hipError_t launch_mfma_probe(const uint16_t *A, const uint16_t *B, float *D, hip
 }
 
 hipError_t launch_features(const WorldDev &w, const float4 *history, uint16_t *X, float *scale, hipStream_t s) {
-    hipLaunchKernelGGL(k_features, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.g, w.pos, w.sstat, w.cell_segs,
+    hipLaunchKernelGGL(k_features, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.grids, w.pos, w.sstat, w.cell_segs,
                        w.ncount, w.nbr, w.ovf, history, X, scale, w.block, w.ovf_cap, w.S);
     return hipGetLastError();
 }
@@ -611,7 +613,7 @@ hipError_t launch_mlp_publish(const MlpDevice &m, hipStream_t s) {
 
 hipError_t launch_apply_proposal(const WorldDev &w, const float4 *Y, const float *seg_scale, float4 *backup,
                                  float scale, hipStream_t s) {
-    hipLaunchKernelGGL(k_apply_proposal, dim3(nblocks(w.V, 256)), dim3(256), 0, s, w.g, w.pos, seg_scale, backup, Y,
+    hipLaunchKernelGGL(k_apply_proposal, dim3(nblocks(w.V, 256)), dim3(256), 0, s, w.grids, w.pos, seg_scale, backup, Y,
                        w.vA, w.vB, scale, w.V);
     return hipGetLastError();
 }

@@ -19,12 +19,21 @@ constexpr uint32_t kLdsListCapacity = 64;
 constexpr uint32_t kOverflowListCapacity = 256;
 // Longest fiber the per-wave curvature kernel stages in LDS (9 floats each).
 constexpr uint32_t kMaxLdsFiberVertices = 1536;
-// Distributed atomic-max slots per metric (penetration, step, curvature).
+// Distributed atomic-max slots per metric (penetration, step, curvature) of
+// a single-specimen world; a world of several specimens uses kRedSlotsBatch
+// per specimen.
 constexpr uint32_t kRedSlots = 256;
-constexpr uint32_t kRedPen = 0, kRedStep = kRedSlots, kRedCurv = 2 * kRedSlots;
+constexpr uint32_t kRedSlotsBatch = 8;
+constexpr uint32_t kMetricPen = 0, kMetricStep = 1, kMetricCurv = 2;
 
-// Uniform-grid parameters: widths are (upper - lower) / cells computed once
-// in f32, exactly as the reference's proxy_cell computes them per call.
+// Vertex flag bits above kSpecShift hold the vertex's specimen index.
+constexpr uint32_t kSpecShift = 8;
+constexpr uint32_t kFlagMask = (1u << kSpecShift) - 1u;
+
+// One specimen's uniform grid: widths are (upper - lower) / cells computed
+// once in f32, exactly as the reference's proxy_cell computes them per call.
+// The specimen's cells occupy [cell_base, cell_base + cells product) of the
+// world's cell list; `skin` is its Verlet skin.
 struct GridParams {
     float lower[3];
     float upper[3];
@@ -32,35 +41,49 @@ struct GridParams {
     float length[3];
     uint32_t cells[3];
     uint32_t periodic[3];
+    uint32_t cell_base;
+    float skin;
+    uint32_t pad[2];
 };
 
-// Device-resident control block. `control` and `nstate` follow the
-// reference's buffers (world.rs:466-467, neighbor.rs:1-9): control =
-// {running, lifetime iterations, batch iterations, terminal failure},
-// nstate = {rebuild pending, rebuild count}.
-struct DevState {
+// Per-specimen control block. `control` follows the reference's buffer
+// (world.rs:466-467): {running, lifetime iterations, batch iterations,
+// terminal failure}; `metrics` = {max penetration, max vertex step, max
+// curvature ratio} measured at the specimen's last assessment.
+struct SpecState {
     uint32_t control[4];
     float metrics[3];
-    uint32_t nstate[2];
-    uint32_t done_blocks;
-    uint32_t pad[6];
+    uint32_t pad;
 };
 
-// Device pointers and sizes of one world, passed to the launchers.
+// World-wide state: nstate = {rebuild pending, rebuild count} (neighbor.rs:1-9)
+// and the workgroup counter of the measuring kernel's last-block assessment.
+struct DevState {
+    uint32_t nstate[2];
+    uint32_t done_blocks;
+    uint32_t pad[5];
+};
+
+// Device pointers and sizes of one world (one or more specimens relaxed
+// together), passed to the launchers. Specimens are stored one after
+// another: fibers, vertices, segments and cells of specimen s follow those
+// of specimen s - 1.
 struct WorldDev {
     uint32_t F, V, S, C;
-    GridParams g;
-    float skin;
+    uint32_t nspec;
+    uint32_t red_slots;  // atomic-max slots per specimen and metric
+    const GridParams *grids;  // [nspec], device
+    SpecState *ss;            // [nspec], device
     uint32_t block;  // neighbour-list slots per segment
     uint32_t ovf_cap;
     uint32_t track_walls;
     uint32_t max_fiber_vertices;
-    uint32_t gs_lanes;  // lanes per fiber in the curvature sweep (16, 32 or 64)
+    uint32_t gs_lanes;  // lanes per fiber in the curvature sweep (0, 4, 8, 16, 32 or 64)
     DevState *st;
-    uint32_t *red;  // 3 * kRedSlots u32 bit patterns of nonnegative floats
+    uint32_t *red;  // [nspec][3][red_slots] u32 bit patterns of nonnegative floats
     float4 *pos, *alt, *refpos, *wall, *vA, *vB;
-    uint4 *sstat;
-    uint2 *fspan;
+    uint4 *sstat;   // per segment: first vertex, fiber, radius bits, specimen
+    uint4 *fspan;   // per fiber: first vertex, vertex count, specimen, 0
     uint32_t *cell_counts, *cell_offsets, *cell_cursor, *cell_segs, *seg_cell, *scattered;
     float4 *slotA, *slotB;
     uint4 *slotC;

@@ -147,11 +147,14 @@ __device__ __forceinline__ float axis_distance(float p1x, float p1y, float p1z, 
 // ---------------------------------------------------------------------------
 // batch begin (kernels.rs:3-12)
 
-__global__ void k_begin_batch(DevState *st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    st->control[2] = 0;
-    if (st->control[3] == 0) st->control[0] = 1;
+__global__ void k_begin_batch(SpecState *ss, uint32_t nspec) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nspec) return;
+    ss[s].control[2] = 0;
+    if (ss[s].control[3] == 0) ss[s].control[0] = 1;
 }
+
+__device__ __forceinline__ bool running(const SpecState *ss, uint32_t spec) { return ss[spec].control[0] != 0u; }
 
 // ---------------------------------------------------------------------------
 // neighbour-list rebuild, every kernel gated on the device-side request flag
@@ -170,13 +173,15 @@ __global__ void k_rebuild_clear(const DevState *st, uint32_t *cell_counts, uint3
     if (i < V) refpos[i] = pos[i];
 }
 
-__global__ void k_bin_segments(const DevState *st, GridParams g, const float4 *pos, const uint4 *sstat,
+__global__ void k_bin_segments(const DevState *st, const GridParams *grids, const float4 *pos, const uint4 *sstat,
                                uint32_t *seg_cell, uint32_t *cell_counts, uint32_t S) {
     if (st->nstate[0] == 0) return;
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
-    uint32_t v0 = sstat[s].x;
-    uint32_t cell = cell_index(g, pos[v0], pos[v0 + 1]);
+    const uint4 me = sstat[s];
+    const GridParams &g = grids[me.w];
+    uint32_t v0 = me.x;
+    uint32_t cell = g.cell_base + cell_index(g, pos[v0], pos[v0 + 1]);
     seg_cell[s] = cell;
     atomicAdd(&cell_counts[cell], 1u);
 }
@@ -242,7 +247,7 @@ __global__ __launch_bounds__(256) void k_rank_cells(const DevState *st, const ui
         float4 p = pos[ss.x], q = pos[ss.x + 1];
         slotA[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(ss.z));
         slotB[slot] = make_float4(q.x, q.y, q.z, p.w);
-        slotC[slot] = make_uint4(mine, ss.y, __float_as_uint(q.w), 0u);
+        slotC[slot] = make_uint4(mine, ss.y, __float_as_uint(q.w), ss.w);
     }
 }
 
@@ -256,12 +261,12 @@ __global__ __launch_bounds__(256) void k_rank_cells(const DevState *st, const ui
 // the pairs within the skin, the only ones that can touch before the next
 // rebuild. The first kLdsListCapacity entries are staged in LDS.
 template <bool Lds>
-__global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *st, GridParams g,
+__global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *st, const GridParams *grids,
                                                                const float4 *slotA, const float4 *slotB,
                                                                const uint4 *slotC, const uint32_t *seg_cell,
                                                                const uint32_t *cell_counts,
                                                                const uint32_t *cell_offsets, uint32_t *ncount,
-                                                               uint32_t *nbr, uint32_t *ovf, float skin,
+                                                               uint32_t *nbr, uint32_t *ovf,
                                                                uint32_t block, uint32_t ovf_cap, uint32_t S) {
     if (st->nstate[0] == 0) return;
     __shared__ uint32_t lds[Lds ? kLdsListCapacity * kBuildThreads : 1];
@@ -271,17 +276,20 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
     float4 A = slotA[slot], B = slotB[slot];
     uint4 M = slotC[slot];
     const uint32_t seg = M.x, owner = M.y;
+    const GridParams &g = grids[M.w];
+    const float skin = g.skin;
     const float arc_p = B.w, arc_q = __uint_as_float(M.z);
     const float p1x = A.x, p1y = A.y, p1z = A.z, radius = A.w;
     const float q1x = B.x, q1y = B.y, q1z = B.z;
     const float d1x = q1x - p1x, d1y = q1y - p1y, d1z = q1z - p1z;
     const float half_length = 0.5f * sqrtf(d1x * d1x + d1y * d1y + d1z * d1z);
-    const uint32_t home = seg_cell[seg];
+    const uint32_t home = seg_cell[seg] - g.cell_base;
     uint32_t *row = ovf + (size_t)slot * ovf_cap;  // encounter order
     uint32_t count = 0;
     for (uint32_t n = 0; n < 27; ++n) {
         uint32_t cell;
         if (!stencil_cell(g, home, n, cell)) continue;
+        cell += g.cell_base;
         uint32_t cs = cell_offsets[cell], cn = cell_counts[cell];
         for (uint32_t k = 0; k < cn; ++k) {
             uint32_t o = cs + k;
@@ -532,6 +540,39 @@ struct Accum {
     }
 };
 
+__device__ __forceinline__ uint32_t block_max_256(uint32_t v, uint32_t *scratch4) {
+    uint32_t m = wave_max_u32(v);
+    if ((threadIdx.x & 63u) == 0) scratch4[threadIdx.x >> 6] = m;
+    __syncthreads();
+    uint32_t b = max(max(scratch4[0], scratch4[1]), max(scratch4[2], scratch4[3]));
+    __syncthreads();
+    return b;
+}
+
+// Slot of (specimen, metric) that workgroup `block` adds its maximum to.
+__device__ __forceinline__ uint32_t *red_slot(uint32_t *red, uint32_t spec, uint32_t metric, uint32_t red_slots,
+                                              uint32_t block) {
+    return red + ((size_t)spec * 3u + metric) * red_slots + block % red_slots;
+}
+
+// Workgroup maximum of `bits` (256 threads) for the specimen of thread 0's
+// element, added to one of that specimen's slots with one atomicMax; the
+// elements of a later specimen (a workgroup straddling a specimen boundary)
+// add theirs directly. Elements are ordered by specimen, so thread 0 holds
+// the workgroup's first; inactive elements pass 0.
+__device__ __forceinline__ void spec_block_max(uint32_t bits, uint32_t spec, uint32_t *red, uint32_t metric,
+                                               uint32_t red_slots, uint32_t *scratch5) {
+    if (threadIdx.x == 0) scratch5[4] = spec;
+    __syncthreads();
+    const uint32_t first = scratch5[4];
+    if (spec != first && bits) {
+        atomicMax(red_slot(red, spec, metric, red_slots, blockIdx.x), bits);
+        __threadfence();
+    }
+    const uint32_t b = block_max_256(spec == first ? bits : 0u, scratch5);
+    if (threadIdx.x == 0 && b) atomicMax(red_slot(red, first, metric, red_slots, blockIdx.x), b);
+}
+
 // Logical block for an XCD-aware dispatch: the hardware deals workgroups to
 // the eight XCDs round-robin, so physical block b runs on XCD b % 8. Giving
 // each XCD a contiguous range of logical blocks keeps spatially adjacent
@@ -550,7 +591,8 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // sequentially in lane 0 of its group. Thread 0 of the grid first closes a
 // pending rebuild (finish_neighbor_list_rebuild, neighbor.rs:424-432).
 template <int G>
-__global__ __launch_bounds__(256) void k_contact(DevState *st, GridParams g, const float4 *pos, const uint4 *sstat,
+__global__ __launch_bounds__(256) void k_contact(DevState *st, const SpecState *ss_all, const GridParams *grids,
+                                                 uint32_t red_slots, const float4 *pos, const uint4 *sstat,
                                                  const uint32_t *cell_segs, const uint32_t *seg_cell,
                                                  const uint32_t *cell_counts, const uint32_t *cell_offsets,
                                                  const uint32_t *ncount, const uint32_t *nbr, const uint32_t *ovf,
@@ -561,19 +603,24 @@ __global__ __launch_bounds__(256) void k_contact(DevState *st, GridParams g, con
         st->nstate[0] = 0;
         st->nstate[1] += 1;
     }
-    if (st->control[0] == 0) return;
     constexpr uint32_t groups = 256 / G;
     const uint32_t lane = threadIdx.x % G, group = threadIdx.x / G;
     const uint32_t wave_lane = threadIdx.x & 63u, group_base = wave_lane - lane;
     const uint32_t slot = xcd_block(blockIdx.x, gridDim.x) * groups + group;
-    const bool active = slot < S;
+    bool active = slot < S;
     SelfSeg me;
-    uint32_t count = 0;
+    uint32_t count = 0, spec = 0;
     const uint32_t *list = nullptr;
     bool slow = false;
+    uint4 ss = make_uint4(0u, 0u, 0u, 0u);
     if (active) {
         me.seg = cell_segs[slot];
-        uint4 ss = sstat[me.seg];
+        ss = sstat[me.seg];
+        spec = ss.w;
+        active = running(ss_all, spec);
+    }
+    const GridParams &g = grids[spec];
+    if (active) {
         me.p = pos[ss.x];
         me.q = pos[ss.x + 1];
         me.owner = ss.y;
@@ -618,10 +665,11 @@ __global__ __launch_bounds__(256) void k_contact(DevState *st, GridParams g, con
     } else if (active && slow && lane == 0) {
         // the reference's overflow scan: 27 stencil cells of the home cell at
         // the list-build positions, each cell in ascending segment order
-        const uint32_t home = seg_cell[me.seg];
+        const uint32_t home = seg_cell[me.seg] - g.cell_base;
         for (uint32_t n = 0; n < 27; ++n) {
             uint32_t cell;
             if (!stencil_cell(g, home, n, cell)) continue;
+            cell += g.cell_base;
             const uint32_t cs = cell_offsets[cell], cn = cell_counts[cell];
             for (uint32_t k = 0; k < cn; ++k) {
                 Term t;
@@ -637,15 +685,10 @@ __global__ __launch_bounds__(256) void k_contact(DevState *st, GridParams g, con
         corr0[me.seg] = make_float4(acc.c0x, acc.c0y, acc.c0z, acc.max_pen);
         corr1[me.seg] = make_float4(acc.c1x, acc.c1y, acc.c1z, 0.f);
     }
-    // segment max penetration, max(0) as bits, reduced per workgroup
-    __shared__ uint32_t wave_max[4];
-    uint32_t m = wave_max_u32(active && lane == 0 ? __float_as_uint(fmaxf(acc.max_pen, 0.f)) : 0u);
-    if (wave_lane == 0) wave_max[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t b = max(max(wave_max[0], wave_max[1]), max(wave_max[2], wave_max[3]));
-        if (b) atomicMax(&red[kRedPen + blockIdx.x % kRedSlots], b);
-    }
+    // segment max penetration, max(0) as bits, reduced per workgroup and specimen
+    __shared__ uint32_t scratch[5];
+    spec_block_max(active && lane == 0 ? __float_as_uint(fmaxf(acc.max_pen, 0.f)) : 0u, spec, red, kMetricPen,
+                   red_slots, scratch);
 }
 
 // ---------------------------------------------------------------------------
@@ -653,28 +696,46 @@ __global__ __launch_bounds__(256) void k_contact(DevState *st, GridParams g, con
 // metric reductions (kernels.rs:95-127) and, in the last workgroup to finish,
 // the convergence assessment (assess_reduced_metrics, kernels.rs:129-162).
 
-__device__ __forceinline__ uint32_t block_max_256(uint32_t v, uint32_t *scratch4) {
-    uint32_t m = wave_max_u32(v);
-    if ((threadIdx.x & 63u) == 0) scratch4[threadIdx.x >> 6] = m;
-    __syncthreads();
-    uint32_t b = max(max(scratch4[0], scratch4[1]), max(scratch4[2], scratch4[3]));
-    __syncthreads();
-    return b;
+
+__device__ __forceinline__ void assess_specimen(SpecState &sp, uint32_t pen_bits, uint32_t step_bits,
+                                                uint32_t curv_bits, float penetration_tolerance,
+                                                float curvature_ratio_tolerance, uint32_t iterations,
+                                                uint32_t force_full, bool &keep_going) {
+    const float pen = __uint_as_float(pen_bits), curv = __uint_as_float(curv_bits);
+    sp.metrics[0] = pen;
+    sp.metrics[1] = __uint_as_float(step_bits);
+    sp.metrics[2] = curv;
+    const bool stop = (force_full == 0u && pen <= penetration_tolerance && curv <= 1.0f + curvature_ratio_tolerance) ||
+                      sp.control[2] >= iterations;
+    if (stop) {
+        sp.control[0] = 0;
+    } else {
+        sp.control[1] += 1;
+        sp.control[2] += 1;
+    }
+    keep_going = !stop;
 }
 
-__global__ __launch_bounds__(256) void k_measure_assess(DevState *st, const float4 *pos, const float4 *vA,
+__device__ __forceinline__ uint32_t load_slot(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void k_measure_assess(DevState *st, SpecState *ss, uint32_t nspec,
+                                                        uint32_t red_slots, const float4 *pos, const float4 *vA,
                                                         const float4 *vB, uint32_t *red, uint32_t V,
                                                         float penetration_tolerance, float curvature_ratio_tolerance,
                                                         uint32_t iterations, uint32_t force_full) {
-    if (st->control[0] == 0) return;
-    __shared__ uint32_t scratch[4];
+    __shared__ uint32_t scratch[5];
     __shared__ uint32_t last;
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     float ratio = 0.f;
+    uint32_t spec = 0;
     if (v < V) {
-        float kmax = vA[v].x;
-        uint32_t flags = __float_as_uint(vB[v].z);
-        if (kmax > 0.f && (flags & kHasPrev) && (flags & kHasNext)) {
+        const float4 b4 = vB[v];
+        const uint32_t flags = __float_as_uint(b4.z);
+        spec = flags >> kSpecShift;
+        const float kmax = vA[v].x;
+        if (running(ss, spec) && kmax > 0.f && (flags & kHasPrev) && (flags & kHasNext)) {
             float4 a = pos[v - 1], b = pos[v], c = pos[v + 1];
             float ax = b.x - a.x, ay = b.y - a.y, az = b.z - a.z;
             float bx = c.x - b.x, by = c.y - b.y, bz = c.z - b.z;
@@ -688,43 +749,56 @@ __global__ __launch_bounds__(256) void k_measure_assess(DevState *st, const floa
             }
         }
     }
-    uint32_t b = block_max_256(__float_as_uint(fmaxf(ratio, 0.f)), scratch);
+    spec_block_max(__float_as_uint(fmaxf(ratio, 0.f)), spec, red, kMetricCurv, red_slots, scratch);
     if (threadIdx.x == 0) {
-        if (b) atomicMax(&red[kRedCurv + blockIdx.x % kRedSlots], b);
         __threadfence();
         last = atomicAdd(&st->done_blocks, 1u) == gridDim.x - 1 ? 1u : 0u;
     }
     __syncthreads();
     if (last == 0u) return;
     __threadfence();
-    // the last workgroup: reduce the distributed slots and assess
-    const uint32_t t = threadIdx.x;  // blockDim == kRedSlots
-    uint32_t pen_bits = block_max_256(__hip_atomic_load(&red[kRedPen + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), scratch);
-    uint32_t step_bits = block_max_256(__hip_atomic_load(&red[kRedStep + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), scratch);
-    uint32_t curv_bits = block_max_256(__hip_atomic_load(&red[kRedCurv + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), scratch);
-    __shared__ uint32_t keep_going;
-    if (t == 0) {
-        float pen = __uint_as_float(pen_bits), curv = __uint_as_float(curv_bits);
-        st->metrics[0] = pen;
-        st->metrics[1] = __uint_as_float(step_bits);
-        st->metrics[2] = curv;
-        bool stop = (force_full == 0u && pen <= penetration_tolerance && curv <= 1.0f + curvature_ratio_tolerance) ||
-                    st->control[2] >= iterations;
-        if (stop) {
-            st->control[0] = 0;
-        } else {
-            st->control[1] += 1;
-            st->control[2] += 1;
+    // the last workgroup: reduce the distributed slots and assess each
+    // running specimen
+    const uint32_t t = threadIdx.x;
+    if (nspec == 1u && red_slots == 256u) {
+        uint32_t pen_bits = block_max_256(load_slot(red_slot(red, 0, kMetricPen, 256u, t)), scratch);
+        uint32_t step_bits = block_max_256(load_slot(red_slot(red, 0, kMetricStep, 256u, t)), scratch);
+        uint32_t curv_bits = block_max_256(load_slot(red_slot(red, 0, kMetricCurv, 256u, t)), scratch);
+        __shared__ uint32_t keep;
+        if (t == 0) {
+            bool keep_going = false;
+            if (running(ss, 0)) {
+                assess_specimen(ss[0], pen_bits, step_bits, curv_bits, penetration_tolerance,
+                                curvature_ratio_tolerance, iterations, force_full, keep_going);
+            }
+            keep = keep_going ? 1u : 0u;
+            st->done_blocks = 0u;
         }
-        keep_going = stop ? 0u : 1u;
-        st->done_blocks = 0u;
+        __syncthreads();
+        *red_slot(red, 0, kMetricPen, 256u, t) = 0u;
+        *red_slot(red, 0, kMetricCurv, 256u, t) = 0u;
+        // the step maxima are recomputed by this iteration's contact
+        // application; a stopping pass leaves them for the next batch's
+        // first assessment
+        if (keep) *red_slot(red, 0, kMetricStep, 256u, t) = 0u;
+        return;
+    }
+    for (uint32_t sp = t; sp < nspec; sp += blockDim.x) {
+        if (!running(ss, sp)) continue;
+        uint32_t m[3] = {0u, 0u, 0u};
+        for (uint32_t metric = 0; metric < 3; ++metric)
+            for (uint32_t k = 0; k < red_slots; ++k) m[metric] = max(m[metric], load_slot(red_slot(red, sp, metric, red_slots, k)));
+        bool keep_going = false;
+        assess_specimen(ss[sp], m[kMetricPen], m[kMetricStep], m[kMetricCurv], penetration_tolerance,
+                        curvature_ratio_tolerance, iterations, force_full, keep_going);
+        for (uint32_t k = 0; k < red_slots; ++k) {
+            *red_slot(red, sp, kMetricPen, red_slots, k) = 0u;
+            *red_slot(red, sp, kMetricCurv, red_slots, k) = 0u;
+            if (keep_going) *red_slot(red, sp, kMetricStep, red_slots, k) = 0u;
+        }
     }
     __syncthreads();
-    red[kRedPen + t] = 0u;
-    red[kRedCurv + t] = 0u;
-    // the step maxima are recomputed by this iteration's contact application;
-    // a stopping pass leaves them for the next batch's first assessment
-    if (keep_going) red[kRedStep + t] = 0u;
+    if (t == 0) st->done_blocks = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -749,18 +823,25 @@ __device__ __forceinline__ float wall_move(float moved, uint32_t periodic, float
     return periodic ? moved : rclamp(moved, lo + extent, hi - extent);
 }
 
-__global__ __launch_bounds__(256) void k_vertex_pass(DevState *st, GridParams g, const float4 *in, float4 *out,
+__global__ __launch_bounds__(256) void k_vertex_pass(const SpecState *ss, const GridParams *grids,
+                                                     uint32_t red_slots, const float4 *in, float4 *out,
                                                      const float4 *vA, const float4 *vB, const float4 *corr0,
                                                      const float4 *corr1, float4 *wall, uint32_t *red, uint32_t V,
                                                      float stretch, float bend, float max_step, uint32_t do_internal,
                                                      uint32_t do_contact, uint32_t first_pass, uint32_t track_walls) {
-    if (st->control[0] == 0) return;
-    __shared__ uint32_t scratch[4];
+    __shared__ uint32_t scratch[5];
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     float step = 0.f;
+    uint32_t spec = 0;
+    float4 sb = make_float4(0.f, 0.f, 0.f, 0.f);
     if (v < V) {
+        sb = vB[v];
+        spec = __float_as_uint(sb.z) >> kSpecShift;
+    }
+    if (v < V && running(ss, spec)) {
+        const GridParams &g = grids[spec];
         float4 me = in[v];
-        float4 sa = vA[v], sb = vB[v];
+        float4 sa = vA[v];
         uint32_t flags = __float_as_uint(sb.z);
         float4 result = me;
         // cleared once per iteration (clear_wall_reactions, kernels.rs:14-24)
@@ -820,10 +901,7 @@ __global__ __launch_bounds__(256) void k_vertex_pass(DevState *st, GridParams g,
         out[v] = result;
         if (track_walls) wall[v] = w;
     }
-    if (do_contact) {
-        uint32_t b = block_max_256(__float_as_uint(fmaxf(step, 0.f)), scratch);
-        if (threadIdx.x == 0 && b) atomicMax(&red[kRedStep + blockIdx.x % kRedSlots], b);
-    }
+    if (do_contact) spec_block_max(__float_as_uint(fmaxf(step, 0.f)), spec, red, kMetricStep, red_slots, scratch);
 }
 
 // ---------------------------------------------------------------------------
@@ -936,14 +1014,13 @@ __host__ __device__ inline size_t curvature_group_stride_floats(uint32_t max_ver
 }
 
 template <int L>
-__global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, GridParams g, const float4 *in, float4 *pos,
-                                                           const float4 *refpos, const float4 *vA, const float4 *vB,
-                                                           const uint2 *fspan, float4 *wall, uint32_t F,
-                                                           uint32_t sweeps, float stiffness, float margin,
-                                                           float half_skin, uint32_t track_walls,
+__global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, const SpecState *ss, const GridParams *grids,
+                                                           const float4 *in, float4 *pos, const float4 *refpos,
+                                                           const float4 *vA, const float4 *vB, const uint4 *fspan,
+                                                           float4 *wall, uint32_t F, uint32_t sweeps, float stiffness,
+                                                           float margin, uint32_t track_walls,
                                                            uint32_t max_vertices) {
     static_assert(L == 4 || L == 8 || L == 16 || L == 32, "group width");
-    if (st->control[0] == 0) return;
     constexpr uint32_t G = 64 / L;
     extern __shared__ float4 lds4[];
     const uint32_t lane = threadIdx.x, group = lane / L, gl = lane % L, gbase = group * L;
@@ -953,9 +1030,12 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, GridPar
     float2 *KW = (float2 *)(track_walls ? WR + nv : WR);     // (kmax, 1 - pinned)
     float *Pf = (float *)P;
     const uint32_t fiber = blockIdx.x * G + group;
-    uint2 span = make_uint2(0u, 0u);
+    uint4 span = make_uint4(0u, 0u, 0u, 0u);
     if (fiber < F) span = fspan[fiber];
+    if (!running(ss, span.z)) span.y = 0u;  // a specimen that stopped keeps its positions
     const uint32_t start = span.x, n = span.y;
+    const GridParams &g = grids[span.z];
+    const float half_skin = 0.5f * g.skin;
     for (uint32_t i = gl; i < n; i += L) {
         const float4 p = in[start + i];
         const float4 a = vA[start + i];
@@ -1091,13 +1171,12 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, GridPar
 // next position is neither changed by the last projection nor a screened
 // candidate. LDS accesses of one wave complete in order, so the screen that
 // follows reads the chain's writes without a barrier.
-__global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, GridParams g, const float4 *in, float4 *pos,
+__global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, const SpecState *ss,
+                                                            const GridParams *grids, const float4 *in, float4 *pos,
                                                             const float4 *refpos, const float4 *vA, const float4 *vB,
-                                                            const uint2 *fspan, float4 *wall, uint32_t F,
+                                                            const uint4 *fspan, float4 *wall, uint32_t F,
                                                             uint32_t sweeps, float stiffness, float margin,
-                                                            float half_skin, uint32_t track_walls,
-                                                            uint32_t max_vertices) {
-    if (st->control[0] == 0) return;
+                                                            uint32_t track_walls, uint32_t max_vertices) {
     extern __shared__ float4 lds4[];
     const uint32_t lane = threadIdx.x;
     const uint32_t nv = (max_vertices + 1u) & ~1u;
@@ -1107,8 +1186,11 @@ __global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, GridPa
     float *Pf = (float *)P;
     const uint32_t fiber = blockIdx.x;
     if (fiber >= F) return;
-    const uint2 span = fspan[fiber];
+    const uint4 span = fspan[fiber];
+    if (!running(ss, span.z)) return;
     const uint32_t start = span.x, n = span.y;
+    const GridParams &g = grids[span.z];
+    const float half_skin = 0.5f * g.skin;
     for (uint32_t i = lane; i < n; i += 64) {
         const float4 p = in[start + i];
         const float4 a = vA[start + i];
@@ -1274,16 +1356,18 @@ __global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, GridPa
 
 // Long fibers that do not fit in LDS: one lane per fiber walks the chain in
 // global memory with the same evaluate-then-apply projection.
-__global__ __launch_bounds__(64) void k_curvature_gs_lane(DevState *st, GridParams g, const float4 *in, float4 *pos,
-                                                          const float4 *refpos, const float4 *vA, const float4 *vB,
-                                                          const uint2 *fspan, float4 *wall, uint32_t F,
-                                                          uint32_t sweeps, float stiffness, float margin,
-                                                          float half_skin, uint32_t track_walls) {
-    if (st->control[0] == 0) return;
+__global__ __launch_bounds__(64) void k_curvature_gs_lane(DevState *st, const SpecState *ss, const GridParams *grids,
+                                                          const float4 *in, float4 *pos, const float4 *refpos,
+                                                          const float4 *vA, const float4 *vB, const uint4 *fspan,
+                                                          float4 *wall, uint32_t F, uint32_t sweeps, float stiffness,
+                                                          float margin, uint32_t track_walls) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
-    const uint2 span = fspan[f];
+    const uint4 span = fspan[f];
+    if (!running(ss, span.z)) return;
     const uint32_t start = span.x, n = span.y;
+    const GridParams &g = grids[span.z];
+    const float half_skin = 0.5f * g.skin;
     if (in != pos)
         for (uint32_t v = start; v < start + n; ++v) pos[v] = in[v];
     const uint32_t trip = n >= 3 ? n - 2 : 0;
@@ -1354,7 +1438,7 @@ size_t curvature_lds_bytes(uint32_t max_vertices, uint32_t lanes, uint32_t track
 }
 
 hipError_t launch_begin_batch(const WorldDev &w, hipStream_t stream) {
-    hipLaunchKernelGGL(k_begin_batch, dim3(1), dim3(64), 0, stream, w.st);
+    hipLaunchKernelGGL(k_begin_batch, dim3(blocks_for(w.nspec, 256)), dim3(256), 0, stream, w.ss, w.nspec);
     return hipGetLastError();
 }
 
@@ -1363,8 +1447,8 @@ hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
     const uint32_t clear_n = w.C > w.V ? w.C : w.V;
     hipLaunchKernelGGL(k_rebuild_clear, dim3(blocks_for(clear_n, 256)), dim3(256), 0, stream, w.st, w.cell_counts,
                        w.cell_cursor, w.C, w.pos, w.refpos, w.V);
-    hipLaunchKernelGGL(k_bin_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.g, w.pos, w.sstat,
-                       w.seg_cell, w.cell_counts, w.S);
+    hipLaunchKernelGGL(k_bin_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.grids, w.pos,
+                       w.sstat, w.seg_cell, w.cell_counts, w.S);
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, w.st, w.cell_counts, w.cell_offsets, w.C);
     hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.seg_cell,
                        w.cell_offsets, w.cell_cursor, w.scattered, w.S);
@@ -1372,12 +1456,12 @@ hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
                        w.cell_offsets, w.scattered, w.cell_segs, w.pos, w.sstat, w.slotA, w.slotB, w.slotC, w.C);
     if (w.block <= kLdsListCapacity) {
         hipLaunchKernelGGL(k_build_lists<true>, dim3(blocks_for(w.S, kBuildThreads)), dim3(kBuildThreads), 0, stream,
-                           w.st, w.g, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets, w.ncount,
-                           w.nbr, w.ovf, w.skin, w.block, w.ovf_cap, w.S);
+                           w.st, w.grids, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets,
+                           w.ncount, w.nbr, w.ovf, w.block, w.ovf_cap, w.S);
     } else {
         hipLaunchKernelGGL(k_build_lists<false>, dim3(blocks_for(w.S, kBuildThreads)), dim3(kBuildThreads), 0,
-                           stream, w.st, w.g, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets,
-                           w.ncount, w.nbr, w.ovf, w.skin, w.block, w.ovf_cap, w.S);
+                           stream, w.st, w.grids, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts,
+                           w.cell_offsets, w.ncount, w.nbr, w.ovf, w.block, w.ovf_cap, w.S);
     }
     return hipGetLastError();
 }
@@ -1404,15 +1488,17 @@ hipError_t launch_request_rebuild(const WorldDev &w, hipStream_t stream) {
 
 hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t stream) {
     constexpr uint32_t per_block = 256 / kContactGroup;
-    hipLaunchKernelGGL(k_contact<kContactGroup>, dim3(blocks_for(w.S, per_block)), dim3(256), 0, stream, w.st, w.g,
-                       w.pos, w.sstat, w.cell_segs, w.seg_cell, w.cell_counts, w.cell_offsets, w.ncount, w.nbr, w.ovf,
-                       w.corr0, w.corr1, w.red, p.correction_fraction, p.aggregation, w.block, w.ovf_cap, w.S);
+    hipLaunchKernelGGL(k_contact<kContactGroup>, dim3(blocks_for(w.S, per_block)), dim3(256), 0, stream, w.st, w.ss,
+                       w.grids, w.red_slots, w.pos, w.sstat, w.cell_segs, w.seg_cell, w.cell_counts, w.cell_offsets,
+                       w.ncount, w.nbr, w.ovf, w.corr0, w.corr1, w.red, p.correction_fraction, p.aggregation, w.block,
+                       w.ovf_cap, w.S);
     return hipGetLastError();
 }
 
 hipError_t launch_measure(const WorldDev &w, const IterParams &p, hipStream_t stream) {
-    hipLaunchKernelGGL(k_measure_assess, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.st, w.pos, w.vA, w.vB,
-                       w.red, w.V, p.penetration_tolerance, p.curvature_ratio_tolerance, p.iterations, p.force_full);
+    hipLaunchKernelGGL(k_measure_assess, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.st, w.ss, w.nspec,
+                       w.red_slots, w.pos, w.vA, w.vB, w.red, w.V, p.penetration_tolerance,
+                       p.curvature_ratio_tolerance, p.iterations, p.force_full);
     return hipGetLastError();
 }
 
@@ -1425,10 +1511,10 @@ hipError_t launch_vertex_passes(const WorldDev &w, const IterParams &p, hipStrea
     const uint32_t passes = vertex_passes(p);
     float4 *src = w.pos, *dst = w.alt;
     for (uint32_t i = 0; i < passes; ++i) {
-        hipLaunchKernelGGL(k_vertex_pass, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.st, w.g, src, dst,
-                           w.vA, w.vB, w.corr0, w.corr1, w.wall, w.red, w.V, p.stretch, p.bend, p.max_step,
-                           p.constraint_iterations > 0 ? 1u : 0u, i + 1 == passes ? 1u : 0u, i == 0 ? 1u : 0u,
-                           w.track_walls);
+        hipLaunchKernelGGL(k_vertex_pass, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.ss, w.grids,
+                           w.red_slots, src, dst, w.vA, w.vB, w.corr0, w.corr1, w.wall, w.red, w.V, p.stretch, p.bend,
+                           p.max_step, p.constraint_iterations > 0 ? 1u : 0u, i + 1 == passes ? 1u : 0u,
+                           i == 0 ? 1u : 0u, w.track_walls);
         float4 *t = src;
         src = dst;
         dst = t;
@@ -1442,33 +1528,22 @@ hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t 
         const size_t lds = curvature_lds_bytes(w.max_fiber_vertices, w.gs_lanes, w.track_walls);
         const uint32_t per_block = 64u / w.gs_lanes;
         const dim3 grid(blocks_for(w.F, per_block));
-        if (w.gs_lanes == 4) {
-            hipLaunchKernelGGL(k_curvature_gs_group<4>, grid, dim3(64), lds, stream, w.st, w.g, src, w.pos, w.refpos,
-                               w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, p.curvature_margin,
-                               0.5f * w.skin, w.track_walls, w.max_fiber_vertices);
-        } else if (w.gs_lanes == 8) {
-            hipLaunchKernelGGL(k_curvature_gs_group<8>, grid, dim3(64), lds, stream, w.st, w.g, src, w.pos, w.refpos,
-                               w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, p.curvature_margin,
-                               0.5f * w.skin, w.track_walls, w.max_fiber_vertices);
-        } else if (w.gs_lanes == 16) {
-            hipLaunchKernelGGL(k_curvature_gs_group<16>, grid, dim3(64), lds, stream, w.st, w.g, src, w.pos, w.refpos,
-                               w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, p.curvature_margin,
-                               0.5f * w.skin, w.track_walls, w.max_fiber_vertices);
-        } else if (w.gs_lanes == 32) {
-            hipLaunchKernelGGL(k_curvature_gs_group<32>, grid, dim3(64), lds, stream, w.st, w.g, src, w.pos, w.refpos,
-                               w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, p.curvature_margin,
-                               0.5f * w.skin, w.track_walls, w.max_fiber_vertices);
-        } else if (w.gs_lanes == 64) {
-            hipLaunchKernelGGL(k_curvature_gs_wave64, grid, dim3(64), lds, stream, w.st, w.g, src, w.pos, w.refpos,
-                               w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, p.curvature_margin,
-                               0.5f * w.skin, w.track_walls, w.max_fiber_vertices);
-        } else {
-            return hipErrorInvalidValue;
+#define TGH_GS_ARGS                                                                                              \
+    w.st, w.ss, w.grids, src, w.pos, w.refpos, w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness, \
+        p.curvature_margin, w.track_walls, w.max_fiber_vertices
+        switch (w.gs_lanes) {
+            case 4: hipLaunchKernelGGL(k_curvature_gs_group<4>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
+            case 8: hipLaunchKernelGGL(k_curvature_gs_group<8>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
+            case 16: hipLaunchKernelGGL(k_curvature_gs_group<16>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
+            case 32: hipLaunchKernelGGL(k_curvature_gs_group<32>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
+            case 64: hipLaunchKernelGGL(k_curvature_gs_wave64, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
+            default: return hipErrorInvalidValue;
         }
+#undef TGH_GS_ARGS
     } else {
-        hipLaunchKernelGGL(k_curvature_gs_lane, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.st, w.g, src,
-                           w.pos, w.refpos, w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness,
-                           p.curvature_margin, 0.5f * w.skin, w.track_walls);
+        hipLaunchKernelGGL(k_curvature_gs_lane, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.st, w.ss, w.grids,
+                           src, w.pos, w.refpos, w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness,
+                           p.curvature_margin, w.track_walls);
     }
     return hipGetLastError();
 }

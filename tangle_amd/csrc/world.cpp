@@ -28,6 +28,7 @@
 #include "world_internal.h"
 
 using tgh::DevState;
+using tgh::SpecState;
 using tgh::GridParams;
 using tgh::IterParams;
 using tgh::WorldDev;
@@ -236,56 +237,62 @@ IterParams iter_params(const tgh_config *c, uint32_t iterations) {
 namespace {
 
 int upload_static(tgh_world *w) {
-    const Packed &p = w->host;
     WorldDev &d = w->d;
-    const uint32_t V = p.V, S = p.S, F = p.F;
+    const uint32_t V = d.V, S = d.S, F = d.F;
     std::vector<float4> pos4(V), vA(V), vB(V);
     std::vector<uint4> sstat(S);
-    std::vector<uint2> fspan(F);
-    for (uint32_t f = 0; f < F; ++f) {
-        const uint32_t st = p.fstart[f], n = p.fcount[f];
-        fspan[f] = make_uint2(st, n);
-        for (uint32_t l = 0; l < n; ++l) {
-            const uint32_t v = st + l;
-            pos4[v] = make_float4(p.pos[3 * (size_t)v], p.pos[3 * (size_t)v + 1], p.pos[3 * (size_t)v + 2], p.arc[v]);
-            uint32_t flags = 0;
-            float rest_prev = 0.f, rest_next = 0.f, chord_prev = 0.f, chord_next = 0.f;
-            if (l > 0) {
-                flags |= tgh::kHasPrev;
-                rest_prev = p.rest[v - f - 1];
+    std::vector<uint4> fspan(F);
+    for (uint32_t sp = 0; sp < d.nspec; ++sp) {
+        const Packed &p = w->specimens[sp];
+        const uint32_t vb = w->vbase[sp], fb = w->fbase[sp], sb = w->sbase[sp];
+        for (uint32_t f = 0; f < p.F; ++f) {
+            const uint32_t st = p.fstart[f], n = p.fcount[f];
+            const uint32_t gf = fb + f;  // world fiber index
+            fspan[gf] = make_uint4(vb + st, n, sp, 0u);
+            for (uint32_t l = 0; l < n; ++l) {
+                const uint32_t v = st + l, gv = vb + v;
+                pos4[gv] = make_float4(p.pos[3 * (size_t)v], p.pos[3 * (size_t)v + 1], p.pos[3 * (size_t)v + 2],
+                                       p.arc[v]);
+                uint32_t flags = 0;
+                float rest_prev = 0.f, rest_next = 0.f, chord_prev = 0.f, chord_next = 0.f;
+                if (l > 0) {
+                    flags |= tgh::kHasPrev;
+                    rest_prev = p.rest[v - f - 1];
+                }
+                if (l + 1 < n) {
+                    flags |= tgh::kHasNext;
+                    rest_next = p.rest[v - f];
+                }
+                // rest chords to the second neighbours, f32 as in kernels.rs:1098-1101
+                auto chord = [&](uint32_t other) {
+                    const float *I = p.intr.data();
+                    float rx = I[3 * (size_t)other] - I[3 * (size_t)v];
+                    float ry = I[3 * (size_t)other + 1] - I[3 * (size_t)v + 1];
+                    float rz = I[3 * (size_t)other + 2] - I[3 * (size_t)v + 2];
+                    return std::sqrt(rx * rx + ry * ry + rz * rz);
+                };
+                if (l >= 2) {
+                    flags |= tgh::kHasPrev2;
+                    chord_prev = chord(v - 2);
+                }
+                if (l + 2 < n) {
+                    flags |= tgh::kHasNext2;
+                    chord_next = chord(v + 2);
+                }
+                if (p.pinned[v]) flags |= tgh::kPinned;
+                flags |= sp << tgh::kSpecShift;
+                float flag_bits, fiber_bits;
+                std::memcpy(&flag_bits, &flags, 4);
+                std::memcpy(&fiber_bits, &gf, 4);
+                vA[gv] = make_float4(p.kmax[v], rest_prev, rest_next, p.radius[f]);
+                vB[gv] = make_float4(chord_prev, chord_next, flag_bits, fiber_bits);
             }
-            if (l + 1 < n) {
-                flags |= tgh::kHasNext;
-                rest_next = p.rest[v - f];
+            for (uint32_t l = 0; l + 1 < n; ++l) {
+                const uint32_t seg = sb + (st - f + l);
+                uint32_t rbits;
+                std::memcpy(&rbits, &p.radius[f], 4);
+                sstat[seg] = make_uint4(vb + st + l, gf, rbits, sp);
             }
-            // rest chords to the second neighbours, f32 as in kernels.rs:1098-1101
-            auto chord = [&](uint32_t other) {
-                const float *I = p.intr.data();
-                float rx = I[3 * (size_t)other] - I[3 * (size_t)v];
-                float ry = I[3 * (size_t)other + 1] - I[3 * (size_t)v + 1];
-                float rz = I[3 * (size_t)other + 2] - I[3 * (size_t)v + 2];
-                return std::sqrt(rx * rx + ry * ry + rz * rz);
-            };
-            if (l >= 2) {
-                flags |= tgh::kHasPrev2;
-                chord_prev = chord(v - 2);
-            }
-            if (l + 2 < n) {
-                flags |= tgh::kHasNext2;
-                chord_next = chord(v + 2);
-            }
-            if (p.pinned[v]) flags |= tgh::kPinned;
-            float flag_bits, fiber_bits;
-            std::memcpy(&flag_bits, &flags, 4);
-            std::memcpy(&fiber_bits, &f, 4);
-            vA[v] = make_float4(p.kmax[v], rest_prev, rest_next, p.radius[f]);
-            vB[v] = make_float4(chord_prev, chord_next, flag_bits, fiber_bits);
-        }
-        for (uint32_t l = 0; l + 1 < n; ++l) {
-            uint32_t s = st - f + l;
-            uint32_t rbits;
-            std::memcpy(&rbits, &p.radius[f], 4);
-            sstat[s] = make_uint4(st + l, f, rbits, 0u);
         }
     }
     HIP_TRY(hipMemcpy(d.pos, pos4.data(), V * sizeof(float4), hipMemcpyHostToDevice));
@@ -294,56 +301,103 @@ int upload_static(tgh_world *w) {
     HIP_TRY(hipMemcpy(d.vA, vA.data(), V * sizeof(float4), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.vB, vB.data(), V * sizeof(float4), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.sstat, sstat.data(), S * sizeof(uint4), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.fspan, fspan.data(), F * sizeof(uint2), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.fspan, fspan.data(), F * sizeof(uint4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)d.grids, w->grids.data(), d.nspec * sizeof(GridParams), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.wall, 0, V * sizeof(float4)));
     HIP_TRY(hipMemset(d.corr0, 0, S * sizeof(float4)));
     HIP_TRY(hipMemset(d.corr1, 0, S * sizeof(float4)));
     HIP_TRY(hipMemset(d.ncount, 0, S * sizeof(uint32_t)));
-    HIP_TRY(hipMemset(d.red, 0, 3 * tgh::kRedSlots * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(d.red, 0, (size_t)d.nspec * 3 * d.red_slots * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_counts, 0, d.C * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_offsets, 0, d.C * sizeof(uint32_t)));
+    std::vector<SpecState> spec(d.nspec);
+    for (SpecState &sp : spec) {
+        std::memset(&sp, 0, sizeof(sp));
+        sp.metrics[0] = std::numeric_limits<float>::infinity();
+        sp.metrics[2] = std::numeric_limits<float>::infinity();
+    }
+    HIP_TRY(hipMemcpy(d.ss, spec.data(), d.nspec * sizeof(SpecState), hipMemcpyHostToDevice));
     DevState init;
     std::memset(&init, 0, sizeof(init));
-    init.metrics[0] = std::numeric_limits<float>::infinity();
-    init.metrics[2] = std::numeric_limits<float>::infinity();
     init.nstate[0] = 1;  // lists start stale so the first contact pass builds them (world.rs:510-511)
     HIP_TRY(hipMemcpy(d.st, &init, sizeof(init), hipMemcpyHostToDevice));
     return TGH_OK;
 }
 
-int create_world(const tgh_assembly *a, const tgh_config *c, tgh_world **out) {
+int create_world(const tgh_assembly *a, uint32_t count, const tgh_config *c, tgh_world **out) {
     int rc = validate_config(c);
     if (rc) return rc;
+    if (!a || count == 0) return set_error(TGH_ERR_INVALID, "no assembly");
+    if (count > (1u << (32 - tgh::kSpecShift))) return set_error(TGH_ERR_INVALID, "too many specimens");
     tgh_world *w = new tgh_world();
-    rc = pack_assembly(a, w->host);
-    if (!rc) rc = size_grid(w->host, c, w->size);
-    if (rc) {
-        delete w;
-        return rc;
-    }
-    const char *env = std::getenv("TANGLE_HIP_GRAPHS");
-    if (env && env[0] == '0') w->use_graphs = false;
     auto fail = [&](int code) {
         delete w;
         return code;
     };
+    w->specimens.resize(count);
+    w->sizes.resize(count);
+    w->vbase.assign(count + 1, 0);
+    w->fbase.assign(count + 1, 0);
+    w->sbase.assign(count + 1, 0);
+    w->cbase.assign(count + 1, 0);
+    uint64_t V = 0, F = 0, S = 0, C = 0;
+    for (uint32_t sp = 0; sp < count; ++sp) {
+        rc = pack_assembly(&a[sp], w->specimens[sp]);
+        if (!rc) rc = size_grid(w->specimens[sp], c, w->sizes[sp]);
+        if (rc) {
+            if (count > 1) set_error(rc, "specimen " + std::to_string(sp) + ": " + tgh_last_error());
+            return fail(rc);
+        }
+        V += w->specimens[sp].V;
+        F += w->specimens[sp].F;
+        S += w->specimens[sp].S;
+        C += w->sizes[sp].C;
+        if (V > 0xffffffffull || C > (uint64_t(1) << 28))
+            return fail(set_error(TGH_ERR_SIZE, "the specimens exceed 2^32 vertices or 2^28 cells"));
+        w->vbase[sp + 1] = (uint32_t)V;
+        w->fbase[sp + 1] = (uint32_t)F;
+        w->sbase[sp + 1] = (uint32_t)S;
+        w->cbase[sp + 1] = (uint32_t)C;
+    }
+    const char *env = std::getenv("TANGLE_HIP_GRAPHS");
+    if (env && env[0] == '0') w->use_graphs = false;
     hipError_t e = hipGetDevice(&w->device);
     if (e != hipSuccess) return fail(set_error(TGH_ERR_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e)));
     e = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(set_error(TGH_ERR_DEVICE, hipGetErrorString(e)));
-    e = hipHostMalloc((void **)&w->host_state, sizeof(DevState), hipHostMallocDefault);
+    e = hipHostMalloc((void **)&w->host_spec, count * sizeof(SpecState), hipHostMallocDefault);
     if (e != hipSuccess) return fail(set_error(TGH_ERR_DEVICE, hipGetErrorString(e)));
-    const Packed &p = w->host;
     WorldDev &d = w->d;
-    d.F = p.F;
-    d.V = p.V;
-    d.S = p.S;
-    d.C = w->size.C;
-    d.skin = w->size.skin;
-    d.block = w->size.block;
+    d.F = (uint32_t)F;
+    d.V = (uint32_t)V;
+    d.S = (uint32_t)S;
+    d.C = (uint32_t)C;
+    d.nspec = count;
+    d.red_slots = count == 1 ? tgh::kRedSlots : tgh::kRedSlotsBatch;
+    // one list width for all specimens: the narrowest any specimen's budget allows
+    d.block = w->sizes[0].block;
+    for (const Sizing &z : w->sizes) d.block = std::min(d.block, z.block);
     d.ovf_cap = std::max<uint32_t>(tgh::kOverflowListCapacity, d.block);
-    d.max_fiber_vertices = *std::max_element(p.fcount.begin(), p.fcount.end());
-    d.track_walls = (p.periodic[0] && p.periodic[1] && p.periodic[2]) ? 0u : 1u;
+    d.max_fiber_vertices = 0;
+    d.track_walls = 0;
+    w->grids.resize(count);
+    for (uint32_t sp = 0; sp < count; ++sp) {
+        const Packed &p = w->specimens[sp];
+        d.max_fiber_vertices = std::max(d.max_fiber_vertices, *std::max_element(p.fcount.begin(), p.fcount.end()));
+        if (!(p.periodic[0] && p.periodic[1] && p.periodic[2])) d.track_walls = 1u;
+        GridParams &g = w->grids[sp];
+        std::memset(&g, 0, sizeof(g));
+        for (int k = 0; k < 3; ++k) {
+            g.lower[k] = p.lower[k];
+            g.upper[k] = p.upper[k];
+            g.cells[k] = w->sizes[sp].cells[k];
+            g.periodic[k] = p.periodic[k];
+            g.width[k] = (p.upper[k] - p.lower[k]) / (float)w->sizes[sp].cells[k];
+            g.length[k] = p.upper[k] - p.lower[k];
+        }
+        g.cell_base = w->cbase[sp];
+        g.skin = w->sizes[sp].skin;
+    }
     // lanes per fiber in the curvature sweep: 8 (eight fibers per wave, the
     // fastest measured on the bimodal felt) unless a workgroup's LDS would
     // exceed 64 KiB, then wider groups
@@ -354,18 +408,12 @@ int create_world(const tgh_assembly *a, const tgh_config *c, tgh_world **out) {
             d.gs_lanes = lanes;
             break;
         }
-    for (int k = 0; k < 3; ++k) {
-        d.g.lower[k] = p.lower[k];
-        d.g.upper[k] = p.upper[k];
-        d.g.cells[k] = w->size.cells[k];
-        d.g.periodic[k] = p.periodic[k];
-        d.g.width[k] = (p.upper[k] - p.lower[k]) / (float)w->size.cells[k];
-        d.g.length[k] = p.upper[k] - p.lower[k];
-    }
-    const size_t V = p.V, S = p.S, C = w->size.C;
     e = hipSuccess;
+    GridParams *grids = nullptr;
     if (e == hipSuccess) e = w->alloc(&d.st, 1);
-    if (e == hipSuccess) e = w->alloc(&d.red, 3 * tgh::kRedSlots);
+    if (e == hipSuccess) e = w->alloc(&d.ss, count);
+    if (e == hipSuccess) e = w->alloc(&grids, count);
+    if (e == hipSuccess) e = w->alloc(&d.red, (size_t)count * 3 * d.red_slots);
     if (e == hipSuccess) e = w->alloc(&d.ovf, std::max<size_t>(1, (size_t)d.ovf_cap * S));
     if (e == hipSuccess) e = w->alloc(&d.pos, V);
     if (e == hipSuccess) e = w->alloc(&d.alt, V);
@@ -374,7 +422,7 @@ int create_world(const tgh_assembly *a, const tgh_config *c, tgh_world **out) {
     if (e == hipSuccess) e = w->alloc(&d.vA, V);
     if (e == hipSuccess) e = w->alloc(&d.vB, V);
     if (e == hipSuccess) e = w->alloc(&d.sstat, S);
-    if (e == hipSuccess) e = w->alloc(&d.fspan, p.F);
+    if (e == hipSuccess) e = w->alloc(&d.fspan, F);
     if (e == hipSuccess) e = w->alloc(&d.cell_counts, C);
     if (e == hipSuccess) e = w->alloc(&d.cell_offsets, C);
     if (e == hipSuccess) e = w->alloc(&d.cell_cursor, C);
@@ -389,6 +437,7 @@ int create_world(const tgh_assembly *a, const tgh_config *c, tgh_world **out) {
     if (e == hipSuccess) e = w->alloc(&d.corr0, S);
     if (e == hipSuccess) e = w->alloc(&d.corr1, S);
     if (e != hipSuccess) return fail(set_error(TGH_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e)));
+    d.grids = grids;
     rc = upload_static(w);
     if (rc) return fail(rc);
     *out = w;
@@ -456,6 +505,21 @@ int accumulate_timing(tgh_world *w, const IterParams &q, const std::vector<hipEv
     return TGH_OK;
 }
 
+// BatchStatus of one specimen (read_status, world.rs:2337-2369).
+void specimen_status(const SpecState &s, const tgh_config &c, tgh_batch_status &st) {
+    st.total_iterations = s.control[1];
+    st.batch_iterations = s.control[2];
+    st.max_penetration = s.metrics[0];
+    st.max_displacement = s.metrics[1];
+    st.max_curvature_ratio = s.metrics[2];
+    const bool overflow = s.control[3] != 0;
+    st.cell_list_overflow = overflow ? 1u : 0u;
+    st.converged = (!c.force_full_iterations && s.metrics[0] <= c.penetration_tolerance &&
+                    s.metrics[2] <= 1.0f + c.curvature_ratio_tolerance && !overflow)
+                       ? 1u
+                       : 0u;
+}
+
 int run_batch(tgh_world *w, const tgh_config *c, uint32_t iterations, tgh_batch_status *st) {
     int rc = validate_config(c);
     if (rc) return rc;
@@ -491,24 +555,31 @@ int run_batch(tgh_world *w, const tgh_config *c, uint32_t iterations, tgh_batch_
         rc = enqueue_batch(w, q, events);
         if (rc) return rc;
     }
-    HIP_TRY(hipMemcpyAsync(w->host_state, w->d.st, sizeof(DevState), hipMemcpyDeviceToHost, w->stream));
+    HIP_TRY(hipMemcpyAsync(w->host_spec, w->d.ss, w->d.nspec * sizeof(SpecState), hipMemcpyDeviceToHost, w->stream));
     HIP_TRY(hipStreamSynchronize(w->stream));
     if (events) {
         rc = accumulate_timing(w, q, *events);
         if (rc) return rc;
     }
-    const DevState &s = *w->host_state;
-    st->total_iterations = s.control[1];
-    st->batch_iterations = s.control[2];
-    st->max_penetration = s.metrics[0];
-    st->max_displacement = s.metrics[1];
-    st->max_curvature_ratio = s.metrics[2];
-    const bool overflow = s.control[3] != 0;
-    st->cell_list_overflow = overflow ? 1u : 0u;
-    st->converged = (!c->force_full_iterations && s.metrics[0] <= c->penetration_tolerance &&
-                     s.metrics[2] <= 1.0f + c->curvature_ratio_tolerance && !overflow)
-                        ? 1u
-                        : 0u;
+    w->last_config = *c;
+    w->ran = true;
+    // one specimen: its status; several: the worst of each field, converged
+    // only when every specimen converged
+    for (uint32_t sp = 0; sp < w->d.nspec; ++sp) {
+        tgh_batch_status one;
+        specimen_status(w->host_spec[sp], *c, one);
+        if (sp == 0) {
+            *st = one;
+            continue;
+        }
+        st->total_iterations = std::max(st->total_iterations, one.total_iterations);
+        st->batch_iterations = std::max(st->batch_iterations, one.batch_iterations);
+        st->max_penetration = std::fmax(st->max_penetration, one.max_penetration);
+        st->max_displacement = std::fmax(st->max_displacement, one.max_displacement);
+        st->max_curvature_ratio = std::fmax(st->max_curvature_ratio, one.max_curvature_ratio);
+        st->converged = st->converged && one.converged;
+        st->cell_list_overflow = st->cell_list_overflow || one.cell_list_overflow;
+    }
     return TGH_OK;
 }
 
@@ -540,7 +611,21 @@ void tgh_config_default(tgh_config *c) {
 int tgh_world_create(const tgh_assembly *assembly, const tgh_config *config, tgh_world **out) {
     if (!out) return set_error(TGH_ERR_INVALID, "output pointer is null");
     *out = nullptr;
-    return create_world(assembly, config, out);
+    return create_world(assembly, 1, config, out);
+}
+
+int tgh_world_create_batch(const tgh_assembly *specimens, uint32_t count, const tgh_config *config, tgh_world **out) {
+    if (!out) return set_error(TGH_ERR_INVALID, "output pointer is null");
+    *out = nullptr;
+    return create_world(specimens, count, config, out);
+}
+
+int tgh_world_specimen_status(const tgh_world *world, tgh_batch_status *out, size_t count) {
+    if (!world || !out) return set_error(TGH_ERR_INVALID, "null argument");
+    if (count < world->d.nspec) return set_error(TGH_ERR_SIZE, "status buffer holds fewer than specimen_count entries");
+    if (!world->ran) return set_error(TGH_ERR_INVALID, "no batch has run yet");
+    for (uint32_t sp = 0; sp < world->d.nspec; ++sp) specimen_status(world->host_spec[sp], world->last_config, out[sp]);
+    return TGH_OK;
 }
 
 int tgh_world_run_batch(tgh_world *world, const tgh_config *config, uint32_t iterations, tgh_batch_status *status) {
@@ -591,10 +676,11 @@ int tgh_world_info_get(const tgh_world *world, tgh_world_info *info) {
     info->fiber_count = world->d.F;
     info->segment_count = world->d.S;
     info->vertex_count = world->d.V;
-    for (int k = 0; k < 3; ++k) info->cells[k] = world->d.g.cells[k];
+    for (int k = 0; k < 3; ++k) info->cells[k] = world->grids[0].cells[k];
     info->cell_count = world->d.C;
-    info->cell_size = world->size.cell_size;
-    info->neighbor_skin = world->d.skin;
+    info->cell_size = world->sizes[0].cell_size;
+    info->neighbor_skin = world->grids[0].skin;
+    info->specimen_count = world->d.nspec;
     info->neighbor_block = world->d.block;
     DevState s;
     HIP_TRY(hipStreamSynchronize(world->stream));

@@ -43,10 +43,16 @@ struct Sizing {
 struct tgh_world {
     int device = 0;
     hipStream_t stream = nullptr;
-    tgh::Packed host;
-    tgh::Sizing size;
+    // specimens relaxed together, each with its own grid; their fibers,
+    // vertices, segments and cells follow one another on the device
+    std::vector<tgh::Packed> specimens;
+    std::vector<tgh::Sizing> sizes;
+    std::vector<uint32_t> vbase, fbase, sbase, cbase;  // [nspec + 1] prefix offsets
+    std::vector<tgh::GridParams> grids;
     tgh::WorldDev d{};
-    tgh::DevState *host_state = nullptr;  // pinned
+    tgh::SpecState *host_spec = nullptr;  // pinned [nspec], read after every batch
+    tgh_config last_config{};
+    bool ran = false;
     uint64_t device_bytes = 0;
     std::vector<void *> allocations;
     std::map<std::string, hipGraphExec_t> graphs;
@@ -84,7 +90,7 @@ struct tgh_world {
         for (hipGraph_t g : graph_templates) (void)hipGraphDestroy(g);
         for (hipEvent_t e : direct_events) (void)hipEventDestroy(e);
         for (void *p : allocations) (void)hipFree(p);
-        if (host_state) (void)hipHostFree(host_state);
+        if (host_spec) (void)hipHostFree(host_spec);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

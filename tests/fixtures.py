@@ -200,3 +200,22 @@ def staged_hard_gate_recipe(recipe, warm_iterations=1280, cleanup_iterations=400
                             max_iterations=cleanup_iterations, max_extra_iterations=0)
     recipe.solve(policy, tg.RelaxationOverrides.preset(preset) if preset else None)
     return recipe
+
+
+# Batched specimens: independent cubes of straight-ish 10 um fibers 0.5 mm
+# long (25 segments of 2 d), isotropic, each cube sized for 10 % fill, with
+# fiber counts log-uniform in [n_min, n_max] (seeded).
+def batched_specimen_recipes(count=256, n_min=50, n_max=5000, seed=20_260_930, fill=0.10):
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(n_min), np.log(n_max), count)).round().astype(int)
+    d, length = 10e-6, 0.5e-3
+    recipes = []
+    for i, n in enumerate(counts):
+        side = (n * np.pi * (d / 2) ** 2 * length / fill) ** (1.0 / 3.0)
+        cell = tg.Cell([side] * 3, periodic="xyz")
+        pop = tg.FiberPopulation(material=tg.Material("fiber", d), count=int(n), length=length,
+                                 segments_per_fiber=25, seed=seed + 1 + i)
+        recipe = tg.Recipe(cell)
+        recipe.insert(tg.generate_fiber_population(cell, pop))
+        recipes.append(recipe)
+    return recipes

@@ -1,0 +1,92 @@
+"""Batched specimens (no counterpart in the reference): several independent
+specimens, each with its own cell, grid, skin and convergence gate, relaxed
+by the same launches. A specimen in a batch must follow exactly the
+trajectory it follows alone: the contact sums run over the same ascending
+lists (a shared rebuild only changes when lists are rebuilt, not what a
+listed segment sees), and every other kernel is per vertex or per fiber."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import fixtures as fx
+import tangle_amd as tg
+from tangle_amd import _native
+
+pytestmark = pytest.mark.gpu
+
+
+def _specimens():
+    # periodic straight fibers, a walled dense mat, a pinned zigzag with a bend
+    # limit, and a bimodal felt with bend limits: different grids, walls,
+    # pins and curvature work in one world
+    return [fx.config1_recipe(count=12)._packed(False), fx.dense_mat(), fx.zigzag(),
+            fx.bimodal_recipe(scale=0.02)._packed(False)]
+
+
+def _config(**changes):
+    cfg = fx.config(penetration_tolerance=1e-9, max_step=0.01 * 10e-6, neighbor_capacity=64,
+                    curvature_ratio_tolerance=1e-5)
+    for k, v in changes.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def _alone(packed, cfg, iterations):
+    with _native.World(packed, cfg) as w:
+        st = w.run_batch(cfg, iterations)
+        counts, _ = w.neighbor_lists()
+        return w.download_positions(), st, int(counts.max()), int(w.info().neighbor_block)
+
+
+def test_batched_specimens_follow_their_solo_trajectories(hip_device):
+    specimens = _specimens()
+    cfg = _config(force_full_iterations=1)
+    with _native.World.batch(specimens, cfg) as w:
+        assert w.info().specimen_count == len(specimens)
+        w.run_batch(cfg, 40)
+        statuses = w.specimen_status()
+        together = w.specimen_positions()
+    for i, packed in enumerate(specimens):
+        print(f"[case] specimen {i}", flush=True)
+        alone, st, max_count, block = _alone(packed, cfg, 40)
+        assert max_count <= block, "an overflowed list sums in stencil order, which depends on rebuild times"
+        assert statuses[i].total_iterations == st.total_iterations == 40
+        assert np.array_equal(together[i].view(np.uint32), alone.view(np.uint32)), i
+        assert statuses[i].max_penetration == st.max_penetration
+        assert statuses[i].max_curvature_ratio == st.max_curvature_ratio
+
+
+def test_each_specimen_stops_on_its_own_gate(hip_device):
+    specimens = _specimens()
+    cfg = _config(penetration_tolerance=1e-3 * 10e-6)
+    with _native.World.batch(specimens, cfg) as w:
+        for _ in range(10):
+            agg = w.run_batch(cfg, 200)
+            if agg.converged:
+                break
+        statuses = w.specimen_status()
+        together = w.specimen_positions()
+    for i, packed in enumerate(specimens):
+        alone_world = _native.World(packed, cfg)
+        with alone_world:
+            for _ in range(10):
+                st = alone_world.run_batch(cfg, 200)
+                if st.converged:
+                    break
+            alone = alone_world.download_positions()
+        print(f"[case] specimen {i}: {statuses[i].total_iterations} vs {st.total_iterations}", flush=True)
+        assert statuses[i].total_iterations == st.total_iterations
+        assert bool(statuses[i].converged) == bool(st.converged)
+        assert np.array_equal(together[i].view(np.uint32), alone.view(np.uint32)), i
+
+
+def test_relax_batch_facade(hip_device):
+    recipes = fx.batched_specimen_recipes(count=6, n_min=20, n_max=200)
+    results = tg.relax_batch(recipes, tg.RelaxationSettings(max_iterations=4000))
+    assert len(results) == 6
+    for r, recipe in zip(results, recipes):
+        assert r.converged, r
+        assert r.positions.size == 3 * sum(len(f.placed) for f in recipe._fibers)
+        assert r.max_penetration <= 0.01 * 10e-6
