@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--dump", type=str, default="", help="write final tta positions (.npy) here")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (reference kernel for the HBM counters)")
+    ap.add_argument("--skip-batched", action="store_true", help="skip the batched-specimens throughput run")
+    ap.add_argument("--batched-specimens", type=int, default=256, help="specimens in the batched run")
+    ap.add_argument("--batched-steps", type=int, default=2, help="timed 128-iteration batches of the batched run")
     ap.add_argument("--curvature-lanes", type=int, default=-1,
                     help="lanes per fiber in the curvature sweep (0, 16, 32, 64; default: the world's choice)")
     return ap.parse_args()
@@ -256,6 +259,9 @@ def main():
         if args.dump and positions is not None:
             np.save(args.dump, positions)
 
+    if not args.skip_batched:
+        result["batched"] = batched_throughput(args, fx, tg, _native)
+
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(packed, cfg, S, args.cpu_iterations)
 
@@ -296,6 +302,77 @@ def learned_tta(args, fx, tg, d_min):
                     "elapsed_s": round(time.perf_counter() - t0, 3)})
     model.close()
     return out
+
+
+def batched_throughput(args, fx, tg, _native):
+    """Independent specimens in one batched world: fiber counts log-uniform
+    in [50, 5,000], 10 um x 0.5 mm fibers of 25 segments, isotropic, each
+    cube at 10 % fill. Every kernel launch serves all specimens; each step
+    is a 128-iteration batch with force_full_iterations from the generated
+    state. Reports segment-iterations/s and the event-timed kernel groups."""
+    import ctypes as C
+
+    recipes = fx.batched_specimen_recipes(count=args.batched_specimens)
+    settings = tg.RelaxationSettings(force_full_iterations=True)
+    packed = [r._packed(False) for r in recipes]
+    cfg = settings.to_config(10e-6)
+    S = sum(p.vertex_count - p.counts.size for p in packed)
+    V = sum(p.vertex_count for p in packed)
+    lib = _native.library()
+    from tangle_amd.autotune import autotune
+    t0 = time.perf_counter()
+    tuning = autotune(packed, cfg, iterations=32)
+    tune_s = time.perf_counter() - t0
+    default_tuning = {"cell_size_scale": cfg.cell_size_scale, "neighbor_skin_scale": cfg.neighbor_skin_scale}
+    cfg = tuning.configure(cfg)
+    t0 = time.perf_counter()
+    world = _native.World.batch(packed, cfg)
+    tuning.apply(world)
+    create_s = time.perf_counter() - t0
+    initial = world.download_positions()
+    world.run_batch(cfg, args.batch)  # warmup: captures the batch graph
+    world.upload_positions(initial)
+    t0 = time.perf_counter()
+    iterations = 0
+    for _ in range(args.batched_steps):
+        iterations += int(world.run_batch(cfg, args.batch).batch_iterations)
+    seconds = time.perf_counter() - t0
+    counts, _ = world.neighbor_lists()
+    info = world.info()
+    mean_list = float(np.mean(np.minimum(counts, info.neighbor_block)))
+    world.upload_positions(initial)
+    _native.check(lib.tgh_world_set_timing(world._handle, 1))
+    ms = (C.c_double * 5)()
+    launches = (C.c_uint64 * 5)()
+    _native.check(lib.tgh_world_timing(world._handle, ms, launches, 5, 1))
+    for _ in range(args.batched_steps):
+        world.run_batch(cfg, args.batch)
+    _native.check(lib.tgh_world_timing(world._handle, ms, launches, 5, 1))
+    world.close()
+    group_ms = {GROUPS[g]: ms[g] for g in range(5)}
+    contact_avg = group_ms["contact"] / max(1, launches[1])
+    contact_bytes = contact_bytes_per_launch(S, V, mean_list)
+    per_iter = {g: round(group_ms[g] / max(1, iterations + args.batched_steps), 4) for g in GROUPS}
+    return {
+        "workload": f"{len(packed)} independent specimens, fiber counts log-uniform in [50, 5000], 10 um x 0.5 mm, "
+                    "25 segments, isotropic, 10 % fill each",
+        "specimens": len(packed), "fibers": int(sum(p.counts.size for p in packed)), "segments": int(S),
+        "vertices": int(V), "value": round(S * iterations / seconds, 1), "unit": "segment-iters/s",
+        "ms_per_iteration": round(1000.0 * seconds / iterations, 3), "iterations": iterations,
+        "create_seconds": round(create_s, 3), "mean_list_length": round(mean_list, 2),
+        "autotune": {"seconds": round(tune_s, 2), "default": default_tuning,
+                     "chosen": {"cell_size_scale": tuning.cell_size_scale,
+                                "neighbor_skin_scale": tuning.neighbor_skin_scale,
+                                "contact_lanes": tuning.contact_lanes, "curvature_lanes": tuning.curvature_lanes},
+                     "trial_ms": {k: {str(kk): round(vv, 3) for kk, vv in v.items()}
+                                  for k, v in tuning.trials.items()}},
+        "group_ms_per_iteration": per_iter,
+        "contact_roofline": {"bound": "hbm", "kernel": "tgh::k_contact<16>", "bytes_per_launch": int(contact_bytes),
+                             "avg_launch_us": round(1000.0 * contact_avg, 2),
+                             "achieved": round(contact_bytes / (contact_avg * 1e-3) / 1e9, 1),
+                             "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": round(contact_bytes / (contact_avg * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
+    }
 
 
 def copy_calibration(mib: int) -> None:

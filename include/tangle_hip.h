@@ -166,10 +166,13 @@ int tgh_world_timing(tgh_world *world, double *group_ms, uint64_t *group_launche
  * graphs so the next batch is captured with the new shape.
  *   curvature_lanes: lanes that sweep one fiber in the curvature Gauss-Seidel
  *     pass: 4, 8, 16, 32 or 64 (fibers staged in LDS), or 0 for one lane per
- *     fiber walking it in global memory. */
+ *     fiber walking it in global memory.
+ *   contact_lanes: lanes that evaluate one segment's neighbour list in the
+ *     contact pass, one candidate each: 4, 8, 16 or 32. */
 typedef struct tgh_tuning {
     uint32_t curvature_lanes;
-    uint32_t reserved[7];
+    uint32_t contact_lanes;
+    uint32_t reserved[6];
 } tgh_tuning;
 int tgh_world_get_tuning(const tgh_world *world, tgh_tuning *out);
 int tgh_world_set_tuning(tgh_world *world, const tgh_tuning *tuning);

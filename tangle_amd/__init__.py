@@ -839,7 +839,7 @@ class Recipe:
 
 
 def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iterations: int | None = None,
-                iterations: int | None = None) -> list[RunResult]:
+                iterations: int | None = None, autotune: bool = False) -> list[RunResult]:
     """Relaxes independent specimens together (no counterpart in the
     reference, which runs one world at a time): every recipe's fibers and
     cell become one specimen of a batched device world, all specimens step
@@ -847,9 +847,12 @@ def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iter
 
     With `iterations`, every specimen runs exactly that many iterations
     (force_full_iterations); otherwise each relaxes until converged, within
-    `max_iterations` (default: settings.max_iterations). Returns one
-    RunResult per recipe, in order; a specimen that does not converge in
-    the budget is returned with converged=False rather than raising."""
+    `max_iterations` (default: settings.max_iterations). With `autotune`,
+    the cell size, skin and launch shapes are first picked by
+    tangle_amd.autotune on a scratch copy (same positions bit for bit,
+    fewer seconds). Returns one RunResult per recipe, in order; a specimen
+    that does not converge in the budget is returned with converged=False
+    rather than raising."""
     recipes = list(recipes)
     if not recipes:
         raise ValueError("relax_batch needs at least one recipe")
@@ -869,7 +872,14 @@ def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iter
         config.force_full_iterations = 1
         budget = int(iterations)
     packed = [r._packed(settings.pin_fiber_ends) for r in recipes]
+    tuning = None
+    if autotune:
+        from .autotune import autotune as _autotune
+        tuning = _autotune(packed, config)
+        config = tuning.configure(config)
     with _native.World.batch(packed, config) as world:
+        if tuning is not None:
+            tuning.apply(world)
         info = world.info()
         done = 0
         while done < budget:

@@ -104,7 +104,8 @@ class WorldInfo(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [
         ("curvature_lanes", C.c_uint32),
-        ("reserved", C.c_uint32 * 7),
+        ("contact_lanes", C.c_uint32),
+        ("reserved", C.c_uint32 * 6),
     ]
 
 

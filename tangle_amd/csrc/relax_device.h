@@ -56,12 +56,10 @@ struct SpecState {
     uint32_t pad;
 };
 
-// World-wide state: nstate = {rebuild pending, rebuild count} (neighbor.rs:1-9)
-// and the workgroup counter of the measuring kernel's last-block assessment.
+// World-wide state: nstate = {rebuild pending, rebuild count} (neighbor.rs:1-9).
 struct DevState {
     uint32_t nstate[2];
-    uint32_t done_blocks;
-    uint32_t pad[5];
+    uint32_t pad[6];
 };
 
 // Device pointers and sizes of one world (one or more specimens relaxed
@@ -78,7 +76,8 @@ struct WorldDev {
     uint32_t ovf_cap;
     uint32_t track_walls;
     uint32_t max_fiber_vertices;
-    uint32_t gs_lanes;  // lanes per fiber in the curvature sweep (0, 4, 8, 16, 32 or 64)
+    uint32_t gs_lanes;       // lanes per fiber in the curvature sweep (0, 4, 8, 16, 32 or 64)
+    uint32_t contact_lanes;  // lanes per segment in the contact pass (4, 8, 16 or 32)
     DevState *st;
     uint32_t *red;  // [nspec][3][red_slots] u32 bit patterns of nonnegative floats
     float4 *pos, *alt, *refpos, *wall, *vA, *vB;

@@ -64,6 +64,22 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
+// The minimum-image data of one specimen, held in registers for the
+// per-candidate loops.
+struct Image {
+    float length[3];
+    uint32_t periodic[3];
+};
+__device__ __forceinline__ Image image_of(const GridParams &g) {
+    Image m;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        m.length[k] = g.length[k];
+        m.periodic[k] = g.periodic[k];
+    }
+    return m;
+}
+
 // Cell of a segment midpoint (proxy_cell, cell_list.rs:32-79, one proxy).
 __device__ __forceinline__ void cell_coords(const GridParams &g, float px, float py, float pz, float qx,
                                             float qy, float qz, uint32_t &x, uint32_t &y, uint32_t &z) {
@@ -276,8 +292,9 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
     float4 A = slotA[slot], B = slotB[slot];
     uint4 M = slotC[slot];
     const uint32_t seg = M.x, owner = M.y;
-    const GridParams &g = grids[M.w];
+    const GridParams g = grids[M.w];
     const float skin = g.skin;
+    const Image im = image_of(g);
     const float arc_p = B.w, arc_q = __uint_as_float(M.z);
     const float p1x = A.x, p1y = A.y, p1z = A.z, radius = A.w;
     const float q1x = B.x, q1y = B.y, q1z = B.z;
@@ -300,18 +317,18 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
             bool near = OM.y == owner && gap < radius + OA.w;
             if (other == seg || near) continue;
             float p2x = OA.x, p2y = OA.y, p2z = OA.z, q2x = OB.x, q2y = OB.y, q2z = OB.z;
-            if (g.periodic[0]) {
-                float sh = roundf(0.5f * (p2x + q2x - p1x - q1x) / g.length[0]) * g.length[0];
+            if (im.periodic[0]) {
+                float sh = roundf(0.5f * (p2x + q2x - p1x - q1x) / im.length[0]) * im.length[0];
                 p2x -= sh;
                 q2x -= sh;
             }
-            if (g.periodic[1]) {
-                float sh = roundf(0.5f * (p2y + q2y - p1y - q1y) / g.length[1]) * g.length[1];
+            if (im.periodic[1]) {
+                float sh = roundf(0.5f * (p2y + q2y - p1y - q1y) / im.length[1]) * im.length[1];
                 p2y -= sh;
                 q2y -= sh;
             }
-            if (g.periodic[2]) {
-                float sh = roundf(0.5f * (p2z + q2z - p1z - q1z) / g.length[2]) * g.length[2];
+            if (im.periodic[2]) {
+                float sh = roundf(0.5f * (p2z + q2z - p1z - q1z) / im.length[2]) * im.length[2];
                 p2z -= sh;
                 q2z -= sh;
             }
@@ -382,7 +399,7 @@ struct Term {
     float aw;
 };
 
-__device__ __forceinline__ bool contact_term(const GridParams &g, const SelfSeg &me, uint32_t other,
+__device__ __forceinline__ bool contact_term(const Image &g, const SelfSeg &me, uint32_t other,
                                              const float4 *pos, const uint4 *sstat, float cf, uint32_t aggregation,
                                              Term &t) {
     const float eps_len = 1.0e-20f, eps_par = 1.0e-6f;
@@ -593,6 +610,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 template <int G>
 __global__ __launch_bounds__(256) void k_contact(DevState *st, const SpecState *ss_all, const GridParams *grids,
                                                  uint32_t red_slots, const float4 *pos, const uint4 *sstat,
+                                                 const uint4 *slotC,
                                                  const uint32_t *cell_segs, const uint32_t *seg_cell,
                                                  const uint32_t *cell_counts, const uint32_t *cell_offsets,
                                                  const uint32_t *ncount, const uint32_t *nbr, const uint32_t *ovf,
@@ -614,12 +632,16 @@ __global__ __launch_bounds__(256) void k_contact(DevState *st, const SpecState *
     bool slow = false;
     uint4 ss = make_uint4(0u, 0u, 0u, 0u);
     if (active) {
-        me.seg = cell_segs[slot];
+        // the slot record carries the segment and its specimen, so the
+        // segment record and the specimen's state load side by side
+        const uint4 sc = slotC[slot];
+        me.seg = sc.x;
+        spec = sc.w;
         ss = sstat[me.seg];
-        spec = ss.w;
         active = running(ss_all, spec);
     }
     const GridParams &g = grids[spec];
+    const Image im = image_of(g);
     if (active) {
         me.p = pos[ss.x];
         me.q = pos[ss.x + 1];
@@ -643,7 +665,7 @@ __global__ __launch_bounds__(256) void k_contact(DevState *st, const SpecState *
         for (uint32_t base = 0; base < count; base += G) {
             const uint32_t k = base + lane;
             Term t;
-            bool hit = k < count && contact_term(g, me, list[k], pos, sstat, cf, aggregation, t);
+            bool hit = k < count && contact_term(im, me, list[k], pos, sstat, cf, aggregation, t);
             uint64_t mask = (uint64_t)__ballot(hit);
             uint32_t mine = (uint32_t)((mask >> group_base) & (G == 64 ? ~0ull : ((1ull << G) - 1ull)));
             while (mine) {
@@ -673,7 +695,7 @@ __global__ __launch_bounds__(256) void k_contact(DevState *st, const SpecState *
             const uint32_t cs = cell_offsets[cell], cn = cell_counts[cell];
             for (uint32_t k = 0; k < cn; ++k) {
                 Term t;
-                if (contact_term(g, me, cell_segs[cs + k], pos, sstat, cf, aggregation, t)) acc.add(t, aggregation);
+                if (contact_term(im, me, cell_segs[cs + k], pos, sstat, cf, aggregation, t)) acc.add(t, aggregation);
             }
         }
     }
@@ -716,17 +738,11 @@ __device__ __forceinline__ void assess_specimen(SpecState &sp, uint32_t pen_bits
     keep_going = !stop;
 }
 
-__device__ __forceinline__ uint32_t load_slot(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ __launch_bounds__(256) void k_measure_assess(DevState *st, SpecState *ss, uint32_t nspec,
-                                                        uint32_t red_slots, const float4 *pos, const float4 *vA,
-                                                        const float4 *vB, uint32_t *red, uint32_t V,
-                                                        float penetration_tolerance, float curvature_ratio_tolerance,
-                                                        uint32_t iterations, uint32_t force_full) {
+// Curvature ratio of every vertex of a running specimen, reduced per
+// workgroup into the specimen's slots.
+__global__ __launch_bounds__(256) void k_measure(const SpecState *ss, uint32_t red_slots, const float4 *pos,
+                                                 const float4 *vA, const float4 *vB, uint32_t *red, uint32_t V) {
     __shared__ uint32_t scratch[5];
-    __shared__ uint32_t last;
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     float ratio = 0.f;
     uint32_t spec = 0;
@@ -750,29 +766,28 @@ __global__ __launch_bounds__(256) void k_measure_assess(DevState *st, SpecState 
         }
     }
     spec_block_max(__float_as_uint(fmaxf(ratio, 0.f)), spec, red, kMetricCurv, red_slots, scratch);
-    if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(&st->done_blocks, 1u) == gridDim.x - 1 ? 1u : 0u;
-    }
-    __syncthreads();
-    if (last == 0u) return;
-    __threadfence();
-    // the last workgroup: reduce the distributed slots and assess each
-    // running specimen
+}
+
+// One workgroup of 256 threads: folds the slots of every running specimen
+// and assesses it (assess_reduced_metrics, kernels.rs:129-162). Runs after
+// the measuring kernel in stream order, so every slot is final.
+__global__ __launch_bounds__(256) void k_assess(SpecState *ss, uint32_t nspec, uint32_t red_slots, uint32_t *red,
+                                                float penetration_tolerance, float curvature_ratio_tolerance,
+                                                uint32_t iterations, uint32_t force_full) {
+    __shared__ uint32_t scratch[5];
     const uint32_t t = threadIdx.x;
     if (nspec == 1u && red_slots == 256u) {
-        uint32_t pen_bits = block_max_256(load_slot(red_slot(red, 0, kMetricPen, 256u, t)), scratch);
-        uint32_t step_bits = block_max_256(load_slot(red_slot(red, 0, kMetricStep, 256u, t)), scratch);
-        uint32_t curv_bits = block_max_256(load_slot(red_slot(red, 0, kMetricCurv, 256u, t)), scratch);
+        const bool run = running(ss, 0);
+        if (!run) return;  // uniform across the workgroup
+        uint32_t pen_bits = block_max_256(*red_slot(red, 0, kMetricPen, 256u, t), scratch);
+        uint32_t step_bits = block_max_256(*red_slot(red, 0, kMetricStep, 256u, t), scratch);
+        uint32_t curv_bits = block_max_256(*red_slot(red, 0, kMetricCurv, 256u, t), scratch);
         __shared__ uint32_t keep;
         if (t == 0) {
             bool keep_going = false;
-            if (running(ss, 0)) {
-                assess_specimen(ss[0], pen_bits, step_bits, curv_bits, penetration_tolerance,
-                                curvature_ratio_tolerance, iterations, force_full, keep_going);
-            }
+            assess_specimen(ss[0], pen_bits, step_bits, curv_bits, penetration_tolerance, curvature_ratio_tolerance,
+                            iterations, force_full, keep_going);
             keep = keep_going ? 1u : 0u;
-            st->done_blocks = 0u;
         }
         __syncthreads();
         *red_slot(red, 0, kMetricPen, 256u, t) = 0u;
@@ -787,7 +802,7 @@ __global__ __launch_bounds__(256) void k_measure_assess(DevState *st, SpecState 
         if (!running(ss, sp)) continue;
         uint32_t m[3] = {0u, 0u, 0u};
         for (uint32_t metric = 0; metric < 3; ++metric)
-            for (uint32_t k = 0; k < red_slots; ++k) m[metric] = max(m[metric], load_slot(red_slot(red, sp, metric, red_slots, k)));
+            for (uint32_t k = 0; k < red_slots; ++k) m[metric] = max(m[metric], *red_slot(red, sp, metric, red_slots, k));
         bool keep_going = false;
         assess_specimen(ss[sp], m[kMetricPen], m[kMetricStep], m[kMetricCurv], penetration_tolerance,
                         curvature_ratio_tolerance, iterations, force_full, keep_going);
@@ -797,8 +812,6 @@ __global__ __launch_bounds__(256) void k_measure_assess(DevState *st, SpecState 
             if (keep_going) *red_slot(red, sp, kMetricStep, red_slots, k) = 0u;
         }
     }
-    __syncthreads();
-    if (t == 0) st->done_blocks = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1430,7 +1443,6 @@ namespace tgh {
 
 namespace {
 inline uint32_t blocks_for(uint64_t n, uint32_t threads) { return (uint32_t)((n + threads - 1) / threads); }
-constexpr int kContactGroup = 16;
 }  // namespace
 
 size_t curvature_lds_bytes(uint32_t max_vertices, uint32_t lanes, uint32_t track_walls) {
@@ -1487,18 +1499,26 @@ hipError_t launch_request_rebuild(const WorldDev &w, hipStream_t stream) {
 }
 
 hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t stream) {
-    constexpr uint32_t per_block = 256 / kContactGroup;
-    hipLaunchKernelGGL(k_contact<kContactGroup>, dim3(blocks_for(w.S, per_block)), dim3(256), 0, stream, w.st, w.ss,
-                       w.grids, w.red_slots, w.pos, w.sstat, w.cell_segs, w.seg_cell, w.cell_counts, w.cell_offsets,
-                       w.ncount, w.nbr, w.ovf, w.corr0, w.corr1, w.red, p.correction_fraction, p.aggregation, w.block,
-                       w.ovf_cap, w.S);
+#define TGH_CONTACT_ARGS                                                                                           \
+    w.st, w.ss, w.grids, w.red_slots, w.pos, w.sstat, w.slotC, w.cell_segs, w.seg_cell, w.cell_counts, w.cell_offsets, w.ncount, \
+        w.nbr, w.ovf, w.corr0, w.corr1, w.red, p.correction_fraction, p.aggregation, w.block, w.ovf_cap, w.S
+    const dim3 grid(blocks_for(w.S, 256u / w.contact_lanes));
+    switch (w.contact_lanes) {
+        case 4: hipLaunchKernelGGL(k_contact<4>, grid, dim3(256), 0, stream, TGH_CONTACT_ARGS); break;
+        case 8: hipLaunchKernelGGL(k_contact<8>, grid, dim3(256), 0, stream, TGH_CONTACT_ARGS); break;
+        case 16: hipLaunchKernelGGL(k_contact<16>, grid, dim3(256), 0, stream, TGH_CONTACT_ARGS); break;
+        case 32: hipLaunchKernelGGL(k_contact<32>, grid, dim3(256), 0, stream, TGH_CONTACT_ARGS); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef TGH_CONTACT_ARGS
     return hipGetLastError();
 }
 
 hipError_t launch_measure(const WorldDev &w, const IterParams &p, hipStream_t stream) {
-    hipLaunchKernelGGL(k_measure_assess, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.st, w.ss, w.nspec,
-                       w.red_slots, w.pos, w.vA, w.vB, w.red, w.V, p.penetration_tolerance,
-                       p.curvature_ratio_tolerance, p.iterations, p.force_full);
+    hipLaunchKernelGGL(k_measure, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.ss, w.red_slots, w.pos, w.vA,
+                       w.vB, w.red, w.V);
+    hipLaunchKernelGGL(k_assess, dim3(1), dim3(256), 0, stream, w.ss, w.nspec, w.red_slots, w.red,
+                       p.penetration_tolerance, p.curvature_ratio_tolerance, p.iterations, p.force_full);
     return hipGetLastError();
 }
 

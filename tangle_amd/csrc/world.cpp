@@ -408,6 +408,10 @@ int create_world(const tgh_assembly *a, uint32_t count, const tgh_config *c, tgh
             d.gs_lanes = lanes;
             break;
         }
+    // lanes per segment in the contact pass, one candidate each: 16 suits the
+    // ~20-candidate lists of the bimodal felt; the autotuner picks narrower
+    // groups for shorter lists
+    d.contact_lanes = 16u;
     e = hipSuccess;
     GridParams *grids = nullptr;
     if (e == hipSuccess) e = w->alloc(&d.st, 1);
@@ -745,6 +749,7 @@ int tgh_world_get_tuning(const tgh_world *world, tgh_tuning *out) {
     if (!world || !out) return set_error(TGH_ERR_INVALID, "null argument");
     *out = tgh_tuning{};
     out->curvature_lanes = world->d.gs_lanes;
+    out->contact_lanes = world->d.contact_lanes;
     return TGH_OK;
 }
 
@@ -753,6 +758,9 @@ int tgh_world_set_tuning(tgh_world *world, const tgh_tuning *tuning) {
     const uint32_t lanes = tuning->curvature_lanes;
     if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
         return set_error(TGH_ERR_INVALID, "curvature_lanes must be 0, 4, 8, 16, 32 or 64");
+    const uint32_t cl = tuning->contact_lanes;
+    if (cl != 4 && cl != 8 && cl != 16 && cl != 32)
+        return set_error(TGH_ERR_INVALID, "contact_lanes must be 4, 8, 16 or 32");
     if (lanes != 0) {
         const size_t lds = tgh::curvature_lds_bytes(world->d.max_fiber_vertices, lanes, world->d.track_walls);
         if (world->d.max_fiber_vertices > tgh::kMaxLdsFiberVertices || lds > 64u * 1024u)
@@ -764,6 +772,7 @@ int tgh_world_set_tuning(tgh_world *world, const tgh_tuning *tuning) {
     world->graphs.clear();
     world->graph_templates.clear();
     world->d.gs_lanes = lanes;
+    world->d.contact_lanes = cl;
     return TGH_OK;
 }
 

@@ -144,10 +144,29 @@ def test_curvature_launch_shapes_agree_bitwise(hip_device, case):
         assert np.array_equal(results[lanes], results[0]), lanes
 
 
+@pytest.mark.parametrize("capacity", [48, 4])
+def test_contact_launch_shapes_agree_bitwise(hip_device, capacity):
+    """4, 8, 16 and 32 lanes per segment in the contact pass give the same
+    positions bit for bit, on listed, overflow-row and 27-cell paths."""
+    packed = fx.bimodal_recipe(scale=0.02)._packed(False)
+    cfg = fx.hard_gate_settings(7e-6, force_full_iterations=True, neighbor_capacity=capacity).to_config(7e-6)
+    results = {}
+    for lanes in (4, 8, 16, 32):
+        print(f"[case] contact_lanes={lanes} capacity={capacity}", flush=True)
+        with _native.World(packed, cfg) as w:
+            w.set_tuning(contact_lanes=lanes)
+            w.run_batch(cfg, 30)
+            results[lanes] = w.download_positions().view(np.uint32).copy()
+    for lanes in (4, 8, 32):
+        assert np.array_equal(results[lanes], results[16]), lanes
+
+
 def test_tuning_rejects_bad_lanes(hip_device):
     with _native.World(fx.zigzag(), fx.config()) as w:
         with pytest.raises(ValueError):
             w.set_tuning(curvature_lanes=12)
+        with pytest.raises(ValueError):
+            w.set_tuning(contact_lanes=64)
 
 
 def _lists_equal(gpu, ref):
