@@ -27,9 +27,10 @@
 //     projected nothing, which leaves the result unchanged.
 //   * Max penetration / curvature ratio / vertex step reduce within each
 //     workgroup and then with one u32 atomicMax per workgroup into one of
-//     256 slots (order-free, so exact); the last workgroup of the measuring
-//     kernel folds the slots and assesses convergence, replacing the
-//     single-thread assess launch.
+//     the specimen's slots (order-free, so exact); one small workgroup then
+//     folds the slots and assesses every specimen's convergence.
+//   * Several independent specimens can share one world: every element
+//     reads its specimen's grid and state, so one launch serves them all.
 //
 // All float arithmetic must be compiled without contraction
 // (-ffp-contract=off) and with IEEE division and square root, as the Rust
@@ -715,8 +716,8 @@ __global__ __launch_bounds__(256) void k_contact(DevState *st, const SpecState *
 
 // ---------------------------------------------------------------------------
 // curvature ratio (measure_curvature_ratio, kernels.rs:925-980) with the
-// metric reductions (kernels.rs:95-127) and, in the last workgroup to finish,
-// the convergence assessment (assess_reduced_metrics, kernels.rs:129-162).
+// metric reductions (kernels.rs:95-127), then the convergence assessment
+// (assess_reduced_metrics, kernels.rs:129-162).
 
 
 __device__ __forceinline__ void assess_specimen(SpecState &sp, uint32_t pen_bits, uint32_t step_bits,
