@@ -224,6 +224,10 @@ int tgh_world_features(tgh_world *world, uint16_t *out, size_t count);
  * neighbour-list rebuild. Segments without recent motion do not move. */
 int tgh_world_propose(tgh_world *world, tgh_mlp *mlp, float scale);
 int tgh_world_revert_proposal(tgh_world *world);
+/* Average device time of one forward pass and of one gradient pass (forward,
+ * backward and the fixed-order reduction) over `rows` feature rows, each
+ * timed over `repeats` launches with HIP events. */
+int tgh_mlp_time_kernels(tgh_mlp *mlp, uint32_t rows, uint32_t repeats, float *forward_ms, float *gradient_ms);
 /* Matrix-core layout check: d (16x16 f32) = a (16x16 bf16) * b (16x16 bf16), row-major. */
 int tgh_mfma_probe(const uint16_t *a, const uint16_t *b, float *d);
 

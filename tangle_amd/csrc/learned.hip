@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "learned_device.h"
 #include "relax_device.h"
 
@@ -295,31 +297,90 @@ __device__ __forceinline__ void dense(const uint16_t *a_lds, const uint16_t *W, 
     }
 }
 
-__global__ __launch_bounds__(64) void k_mlp_forward(const MlpDevice m, const uint16_t *X, float4 *Y, uint32_t rows) {
-    __shared__ TileLds t;
-    const uint32_t l = threadIdx.x, r = l % 16, q = l / 16;
-    for (uint32_t tile = blockIdx.x; tile * 16 < rows; tile += gridDim.x) {
-        const uint32_t base = tile * 16;
-        for (uint32_t i = l; i < 16 * 64; i += 64) {
-            const uint32_t row = base + i / 64;
-            t.x[i] = row < rows ? X[(size_t)row * 64 + i % 64] : (uint16_t)0;
-        }
-        __syncthreads();
-        v4f a1[4], a2[4], a3[1];
-        dense<4>(t.x, m.w1, m.b1, true, t.h1, a1);
-        __syncthreads();
-        dense<4>(t.h1, m.w2, m.b2, true, t.h2, a2);
-        __syncthreads();
-        dense<1>(t.h2, m.w3, m.b3, false, nullptr, a3);
-        // lane holds output column r (< 3 used) for rows 4q + j
-        if (r < 3) {
+// Inference in the transposed form: each layer computes Z^T = W * H^T on
+// 16-row tiles, so a layer's accumulators (lane: column = tile row l%16,
+// rows = outputs 4(l/16)+j) are already the next layer's B operand (k =
+// outputs 4(l/16)+j, column = tile row), converted to bf16 in registers.
+// Every weight sits in registers for the life of the wave (the A operands,
+// loaded once), so a tile costs one 2 KiB read of its feature rows, 36
+// MFMAs and the 3 outputs of each row; no LDS. Layer 1's inputs are taken
+// in the order lane (r, q) reads them — its row's 16 contiguous features
+// 16q .. 16q+15 — and its weights in the same order, which leaves the
+// product unchanged.
+__device__ __forceinline__ v4s pack4(float a, float b, float c, float d) {
+    v4s r;
+    r[0] = (short)f2bf(a);
+    r[1] = (short)f2bf(b);
+    r[2] = (short)f2bf(c);
+    r[3] = (short)f2bf(d);
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_forward(const MlpDevice m, const uint16_t *X, float4 *Y, uint32_t rows) {
+    const uint32_t l = threadIdx.x & 63u, r = l % 16, q = l / 16;
+    const uint32_t wave = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u;
+    const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+    // A operands: W1 with layer 1's input order, W2, W3; biases per output row
+    v4s w1[4][4], w2[4][4], w3[4];
+    float b1[4][4], b2[4][4], b3[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t row = base + 4 * q + j;
-                if (row < rows) ((float *)&Y[row])[r] = a3[0][j];
-            }
+    for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            w1[mt][kt] = load4(m.w1 + (16 * mt + r) * 64 + 16 * q + 4 * kt);
+            w2[mt][kt] = load4(m.w2 + (16 * mt + r) * 64 + 16 * kt + 4 * q);
         }
-        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            b1[mt][j] = m.b1[16 * mt + 4 * q + j];
+            b2[mt][j] = m.b2[16 * mt + 4 * q + j];
+        }
+    }
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) w3[kt] = load4(m.w3 + r * 64 + 16 * kt + 4 * q);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b3[j] = m.b3[4 * q + j];
+    const uint32_t tiles = (rows + 15u) / 16u;
+    for (uint32_t tile = wave; tile < tiles; tile += waves) {
+        const uint32_t row = tile * 16u + r;
+        v4s x[4];
+        if (row < rows) {
+            const uint4 *src = (const uint4 *)(X + (size_t)row * 64 + 16 * q);
+            const uint4 u0 = src[0], u1 = src[1];
+            const uint32_t w[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) {
+                x[kt][0] = (short)(w[2 * kt] & 0xffffu);
+                x[kt][1] = (short)(w[2 * kt] >> 16);
+                x[kt][2] = (short)(w[2 * kt + 1] & 0xffffu);
+                x[kt][3] = (short)(w[2 * kt + 1] >> 16);
+            }
+        } else {
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) x[kt] = (v4s){0, 0, 0, 0};
+        }
+        v4s h1[4], h2[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) acc = mfma(w1[mt][kt], x[kt], acc);
+            h1[mt] = pack4(fmaxf(acc[0] + b1[mt][0], 0.f), fmaxf(acc[1] + b1[mt][1], 0.f),
+                           fmaxf(acc[2] + b1[mt][2], 0.f), fmaxf(acc[3] + b1[mt][3], 0.f));
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) acc = mfma(w2[mt][kt], h1[kt], acc);
+            h2[mt] = pack4(fmaxf(acc[0] + b2[mt][0], 0.f), fmaxf(acc[1] + b2[mt][1], 0.f),
+                           fmaxf(acc[2] + b2[mt][2], 0.f), fmaxf(acc[3] + b2[mt][3], 0.f));
+        }
+        v4f y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) y = mfma(w3[kt], h2[kt], y);
+        // lanes 0..15 hold outputs 0..3 of tile row r
+        if (q == 0 && row < rows) Y[row] = make_float4(y[0] + b3[0], y[1] + b3[1], y[2] + b3[2], 0.f);
     }
 }
 
@@ -585,8 +646,10 @@ hipError_t launch_targets(const WorldDev &w, const float4 *mid_then, float4 *T, 
 }
 
 hipError_t launch_mlp_forward(const MlpDevice &m, const uint16_t *X, float4 *Y, uint32_t rows, hipStream_t s) {
+    // persistent waves: 2 workgroups of 4 waves per CU, each wave looping over tiles
     const uint32_t tiles = nblocks(rows, 16);
-    hipLaunchKernelGGL(k_mlp_forward, dim3(tiles < 65536 ? tiles : 65536), dim3(64), 0, s, m, X, Y, rows);
+    const uint32_t blocks = std::min<uint32_t>(nblocks(tiles, 4), 512u);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(256), 0, s, m, X, Y, rows);
     return hipGetLastError();
 }
 

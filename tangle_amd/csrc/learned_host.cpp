@@ -408,6 +408,43 @@ int tgh_world_revert_proposal(tgh_world *world) {
     return TGH_OK;
 }
 
+int tgh_mlp_time_kernels(tgh_mlp *mlp, uint32_t rows, uint32_t repeats, float *forward_ms, float *gradient_ms) {
+    if (!mlp || !forward_ms || !gradient_ms) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (rows == 0 || repeats == 0) return tgh::fail(TGH_ERR_INVALID, "rows and repeats must be positive");
+    uint16_t *X = nullptr;
+    float4 *Y = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    hipError_t e = hipMalloc((void **)&X, (size_t)rows * tgh::kFeatureWidth * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&Y, (size_t)rows * sizeof(float4));
+    // inputs: every feature 0.25 (bf16 0x3e80); the targets are the zeroed outputs
+    if (e == hipSuccess) e = hipMemsetD16Async((hipDeviceptr_t)X, 0x3e80, (size_t)rows * tgh::kFeatureWidth, mlp->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(Y, 0, (size_t)rows * sizeof(float4), mlp->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventCreate(&e2);
+    const uint32_t slots = std::min<uint32_t>((rows + 15) / 16, kMaxSlots);
+    if (e == hipSuccess) e = tgh::launch_mlp_forward(mlp->m, X, Y, rows, mlp->stream);  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, mlp->stream);
+    for (uint32_t i = 0; i < repeats && e == hipSuccess; ++i) e = tgh::launch_mlp_forward(mlp->m, X, Y, rows, mlp->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, mlp->stream);
+    for (uint32_t i = 0; i < repeats && e == hipSuccess; ++i)
+        e = tgh::launch_mlp_gradients(mlp->m, X, Y, nullptr, rows, slots, mlp->partial, mlp->grad, mlp->stream);
+    if (e == hipSuccess) e = hipEventRecord(e2, mlp->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e2);
+    float f = 0.f, g = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&g, e1, e2);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e2) (void)hipEventDestroy(e2);
+    if (X) (void)hipFree(X);
+    if (Y) (void)hipFree(Y);
+    if (e != hipSuccess) return tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e));
+    *forward_ms = f / (float)repeats;
+    *gradient_ms = g / (float)repeats;
+    return TGH_OK;
+}
+
 int tgh_mfma_probe(const uint16_t *a, const uint16_t *b, float *d) {
     if (!a || !b || !d) return tgh::fail(TGH_ERR_INVALID, "null argument");
     uint16_t *A = nullptr, *B = nullptr;

@@ -83,6 +83,13 @@ class WarmStartModel:
                                                           _native.ptr(grad, C.c_float), _native.ptr(loss, C.c_float)))
         return grad, float(loss[0])
 
+    def time_kernels(self, rows: int, repeats: int = 10):
+        """(forward ms, gradient ms) per launch over `rows` rows."""
+        f, g = C.c_float(), C.c_float()
+        _native.check(_native.library().tgh_mlp_time_kernels(self._handle, int(rows), int(repeats), C.byref(f),
+                                                             C.byref(g)))
+        return float(f.value), float(g.value)
+
     # -- dataset and training ---------------------------------------------
     def record_begin(self, world: _native.World) -> None:
         _native.check(_native.library().tgh_mlp_record_begin(self._handle, world._handle))

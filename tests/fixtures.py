@@ -219,3 +219,19 @@ def batched_specimen_recipes(count=256, n_min=50, n_max=5000, seed=20_260_930, f
         recipe.insert(tg.generate_fiber_population(cell, pop))
         recipes.append(recipe)
     return recipes
+
+
+# The bimodal mix at scale: 20,000 fibers (17,630 x 7 um + 2,370 x 19 um)
+# 1 mm long in a 3 x 3 x 1 mm slab, periodic in x and y, walled in z.
+def slab_recipe(fibers=20_000, beta=1.0 / 0.3, seed_offset=0):
+    cell = tg.Cell([3e-3, 3e-3, 1e-3], periodic="xy")
+    recipe = tg.Recipe(cell)
+    total = BIMODAL_FINE["count"] + BIMODAL_COARSE["count"]
+    for spec in (BIMODAL_FINE, BIMODAL_COARSE):
+        material = tg.Material("fine" if spec is BIMODAL_FINE else "coarse", spec["diameter"], spec["bend"])
+        count = int(round(spec["count"] * fibers / total))
+        pop = tg.FiberPopulation(material=material, count=count, length=1e-3, segments_per_fiber=spec["segments"],
+                                 seed=spec["seed"] + 1000 + seed_offset,
+                                 orientation=tg.SchladitzOrientation(beta=beta, axis="z"))
+        recipe.insert(tg.generate_fiber_population(cell, pop))
+    return recipe
