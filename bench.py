@@ -282,6 +282,16 @@ def learned_tta(args, fx, tg, d_min):
     model, losses = learned.train_warm_start(specimens, d_min, interval=args.learned_interval)
     rows = model.dataset_rows
     train_s = time.perf_counter() - t0
+    # matrix-core kernels of the network, at the specimen's row count
+    S = sum(p.vertex_count - p.counts.size for p in specimens[:1])
+    fwd_ms, grad_ms = model.time_kernels(S, 20)
+    tile_flops = 2.0 * (64 * 64 + 64 * 64 + 16 * 64)  # per row, as the 16x16x16 tiles compute it
+    mlp = {"rows": S, "forward_us": round(1000 * fwd_ms, 2), "gradient_us": round(1000 * grad_ms, 2),
+           "roofline": {"bound": "mfma", "kernel": "tgh::k_mlp_forward", "dtype": "bf16 (f32 accumulate)",
+                        "achieved": round(tile_flops * S / (fwd_ms * 1e-3) / 1e12, 2), "peak": 2500.0,
+                        "unit": "TFLOP/s",
+                        "frac": round(tile_flops * S / (fwd_ms * 1e-3) / 1e12 / 2500.0, 4),
+                        "input_GBps": round(128.0 * S / (fwd_ms * 1e-3) / 1e9, 1)}}
     ws = learned.WarmStart(model, interval=args.learned_interval, scale=args.learned_scale, settle=4,
                            active_iterations=10 ** 9)
     r = fx.staged_hard_gate_recipe(fx.bimodal_recipe(scale=args.scale), warm_iterations=args.learned_warm,
@@ -289,7 +299,7 @@ def learned_tta(args, fx, tg, d_min):
     settings = fx.hard_gate_settings(d_min, max_iterations=args.learned_warm + args.tta_max_iterations + 1)
     out = {"training_seconds": round(train_s, 3), "training_rows": int(rows),
            "training_loss": [round(float(losses[0]), 5), round(float(losses[-1]), 5)],
-           "interval": args.learned_interval, "scale": args.learned_scale}
+           "interval": args.learned_interval, "scale": args.learned_scale, "mlp": mlp}
     t0 = time.perf_counter()
     try:
         res = r.run(settings, warm_start=ws)
