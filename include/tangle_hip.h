@@ -64,6 +64,11 @@ typedef struct tgh_assembly {
     double cell_lengths[3];
     uint32_t cell_periodic[3];
     uint32_t pin_fiber_ends;              /* pin first/last vertex of every fiber */
+    /* formation (crates/tangle_relax/src/device/packed.rs:94, 202-210):
+     * the recipe step that inserts each fiber (NULL: all 0) and its
+     * manufacturing layer (NULL, or 0xffffffff for a fiber: none) */
+    const uint32_t *fiber_formation_step;
+    const uint32_t *fiber_formation_layer;
 } tgh_assembly;
 
 /* RelaxationConfig (plugin.rs:77-175) restricted to the flexible, round,
@@ -160,6 +165,34 @@ int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t 
 int tgh_world_set_timing(tgh_world *world, uint32_t enable);
 int tgh_world_timing(tgh_world *world, double *group_ms, uint64_t *group_launches, size_t groups,
                      uint32_t reset);
+/* ---- formation targets (kernels.rs:164-221, 538-710; world.rs:1424-1482,
+ * 1723-1927, 2139-2180) -------------------------------------------------
+ * tgh_world_activate_formation_step: every fiber whose formation step is at
+ * most `maximum_step` becomes active, the others inactive (activate_formation
+ * _step); inactive fibers are left out of the cell list, the neighbour lists,
+ * contact and every projection. Returns the active segment and vertex counts.
+ * tgh_world_set_layer_targets: fibers of layers first..last are translated
+ * along `axis` toward targets[layer] (their vertex-mean coordinate) by
+ * clamp((target - mean) * stiffness, +-max_translation), once now and after
+ * every iteration's projections (apply_layer_target_range, world.rs:1752-1803).
+ * tgh_world_set_vertex_targets: each listed vertex is commanded from its
+ * current coordinate toward coordinate + displacement along `axis`, by at
+ * most max_translation per iteration, and set to the command
+ * (apply_vertex_displacement_targets, world.rs:1813-1858).
+ * tgh_world_formation_target_error: the largest |target - mean| over held
+ * layers and |target - coordinate| over held vertices (formation_target_error,
+ * world.rs:1867-1927); has_layer / has_vertex tell which targets exist. */
+int tgh_world_activate_formation_step(tgh_world *world, uint32_t maximum_step, uint64_t *active_segments,
+                                      uint64_t *active_vertices);
+int tgh_world_set_layer_targets(tgh_world *world, uint32_t axis, const float *targets, size_t count,
+                                uint32_t first_layer, uint32_t last_layer, float stiffness, float max_translation);
+int tgh_world_clear_layer_targets(tgh_world *world);
+int tgh_world_set_vertex_targets(tgh_world *world, uint32_t axis, const uint32_t *vertices, size_t count,
+                                 float displacement, float stiffness, float max_translation);
+int tgh_world_clear_vertex_targets(tgh_world *world);
+int tgh_world_formation_target_error(tgh_world *world, float *layer_error, float *vertex_error,
+                                     uint32_t *has_layer, uint32_t *has_vertex);
+
 /* Launch shape of the relaxation kernels (no counterpart in the reference,
  * whose launch shapes are fixed). Any setting gives the same positions bit
  * for bit; tgh_world_set_tuning validates the settings and drops captured

@@ -48,6 +48,15 @@ def library() -> C.CDLL:
             "orc_world_cell_list": (C.c_int, [C.c_void_p, u32p, u32p, u32p]),
             "orc_world_neighbor_lists": (C.c_int, [C.c_void_p, u32p, u32p]),
             "orc_world_contact_state": (C.c_int, [C.c_void_p, f32p, f32p, f32p]),
+            "orc_world_activate": (None, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+            "orc_world_set_layer_targets": (None, [C.c_void_p, C.c_uint32, f32p, C.c_size_t, C.c_uint32, C.c_uint32,
+                                                   C.c_float, C.c_float]),
+            "orc_world_clear_layer_targets": (None, [C.c_void_p]),
+            "orc_world_set_vertex_targets": (None, [C.c_void_p, C.c_uint32, u32p, C.c_size_t, C.c_float, C.c_float,
+                                                    C.c_float]),
+            "orc_world_clear_vertex_targets": (None, [C.c_void_p]),
+            "orc_world_formation_target_error": (None, [C.c_void_p, f32p, f32p, C.POINTER(C.c_uint32),
+                                                  C.POINTER(C.c_uint32)]),
             "orc_world_destroy": (None, [C.c_void_p]),
             "orc_last_error": (C.c_char_p, []),
         }
@@ -115,6 +124,37 @@ class OracleWorld:
 
     def rebuild_lists(self) -> None:
         _check(library().orc_world_rebuild_now(self._h))
+
+    # formation, as _native.World
+    def activate_formation_step(self, maximum_step):
+        s, v = C.c_uint64(), C.c_uint64()
+        library().orc_world_activate(self._h, int(maximum_step), C.byref(s), C.byref(v))
+        return int(s.value), int(v.value)
+
+    def set_layer_targets(self, axis, targets, first_layer=0, last_layer=None, stiffness=0.25, max_translation=1.0e-5):
+        arr = np.ascontiguousarray(targets, dtype=np.float32)
+        last = arr.size - 1 if last_layer is None else int(last_layer)
+        library().orc_world_set_layer_targets(self._h, int(axis), _p(arr, C.c_float), arr.size, int(first_layer), last,
+                                              float(stiffness), float(max_translation))
+
+    def clear_layer_targets(self):
+        library().orc_world_clear_layer_targets(self._h)
+
+    def set_vertex_targets(self, axis, vertices, displacement, stiffness=0.25, max_translation=1.0e-5):
+        arr = np.ascontiguousarray(vertices, dtype=np.uint32)
+        library().orc_world_set_vertex_targets(self._h, int(axis), _p(arr, C.c_uint32), arr.size, float(displacement),
+                                               float(stiffness), float(max_translation))
+
+    def clear_vertex_targets(self):
+        library().orc_world_clear_vertex_targets(self._h)
+
+    def formation_target_error(self):
+        """(layer error or None, vertex error or None), as the reference's
+        Option results (world.rs:1867-1927)."""
+        le, ve = C.c_float(), C.c_float()
+        hl, hv = C.c_uint32(), C.c_uint32()
+        library().orc_world_formation_target_error(self._h, C.byref(le), C.byref(ve), C.byref(hl), C.byref(hv))
+        return (float(le.value) if hl.value else None, float(ve.value) if hv.value else None)
 
     def cell_list(self):
         info = self.info()

@@ -95,6 +95,16 @@ struct World {
     std::vector<float> wall_extent;        // 3V
     float lower[3], upper[3];
     uint32_t periodic[3];
+    // formation (packed.rs:94, 202-210, 289, 303): every fiber active at upload
+    std::vector<uint32_t> fiber_step, fiber_layer, fiber_active;  // F
+    // installed targets (world.rs:1723-1927)
+    std::vector<float> layer_targets;
+    uint32_t layer_axis = 0, first_layer = 0, last_layer = 0;
+    float layer_stiffness = 0, layer_max = 0;
+    std::vector<uint32_t> vt_index;
+    std::vector<float> vt_target, vt_command;
+    uint32_t vt_axis = 0;
+    float vt_stiffness = 0, vt_max = 0;
 
     // sizing (world.rs:322-417)
     float skin = 0, cell_size = 0;
@@ -215,8 +225,22 @@ int pack(World &w, const tgh_assembly *a) {
         w.upper[k] = (float)(a->cell_origin[k] + a->cell_lengths[k]);
         w.periodic[k] = a->cell_periodic[k] ? 1u : 0u;
     }
+    w.fiber_step.assign(w.F, 0u);
+    w.fiber_layer.assign(w.F, 0xffffffffu);
+    w.fiber_active.assign(w.F, 1u);
+    for (uint32_t f = 0; f < w.F; ++f) {
+        if (a->fiber_formation_step) w.fiber_step[f] = a->fiber_formation_step[f];
+        if (a->fiber_formation_layer) w.fiber_layer[f] = a->fiber_formation_layer[f];
+    }
     return TGH_OK;
 }
+
+inline bool seg_on(const World &w, uint32_t s) { return w.fiber_active[w.seg_fiber[s]] != 0; }
+inline uint32_t vertex_fiber(const World &w, uint32_t v) {
+    uint32_t s = w.vtx_seg[2 * v] != kNone ? w.vtx_seg[2 * v] : w.vtx_seg[2 * v + 1];
+    return w.seg_fiber[s];
+}
+inline bool vtx_on(const World &w, uint32_t v) { return w.fiber_active[vertex_fiber(w, v)] != 0; }
 
 // ---- upload sizing: world.rs:322-417, 496-509, 868-917 ---------------------
 
@@ -305,6 +329,7 @@ void rebuild_cell_list(World &w) {
     w.cell_segments.assign(w.S, 0);
     std::vector<uint32_t> cell_of(w.S);
     for (uint32_t s = 0; s < w.S; ++s) {
+        if (!seg_on(w, s)) continue;  // only active segments are binned
         cell_of[s] = segment_cell(w, w.pos.data(), s);
         w.cell_counts[cell_of[s]]++;
     }
@@ -315,6 +340,7 @@ void rebuild_cell_list(World &w) {
     }
     std::vector<uint32_t> cursor(w.C, 0);
     for (uint32_t s = 0; s < w.S; ++s) {
+        if (!seg_on(w, s)) continue;
         uint32_t c = cell_of[s];
         w.cell_segments[w.cell_offsets[c] + cursor[c]++] = s;
     }
@@ -394,6 +420,7 @@ void build_neighbor_lists(World &w) {
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t si = 0; si < (int64_t)w.S; ++si) {
         uint32_t seg = (uint32_t)si;
+        if (!seg_on(w, seg)) continue;  // inactive: no list
         uint32_t v1 = w.seg_vtx[2 * seg], v2 = w.seg_vtx[2 * seg + 1];
         float p1x = P[3 * v1], p1y = P[3 * v1 + 1], p1z = P[3 * v1 + 2];
         float q1x = P[3 * v2], q1y = P[3 * v2 + 1], q1z = P[3 * v2 + 2];
@@ -481,6 +508,11 @@ void find_segment_corrections(World &w, const tgh_config *c) {
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t si = 0; si < (int64_t)w.S; ++si) {
         uint32_t seg = (uint32_t)si;
+        if (!seg_on(w, seg)) {  // inactive segments take no contact
+            std::fill(w.corr.begin() + 6 * (size_t)seg, w.corr.begin() + 6 * (size_t)seg + 6, 0.f);
+            w.seg_max[seg] = 0.f;
+            continue;
+        }
         uint32_t v1 = w.seg_vtx[2 * seg], v2 = w.seg_vtx[2 * seg + 1];
         float p1x = P[3 * v1], p1y = P[3 * v1 + 1], p1z = P[3 * v1 + 2];
         float q1x = P[3 * v2], q1y = P[3 * v2 + 1], q1z = P[3 * v2 + 2];
@@ -673,7 +705,7 @@ void measure_curvature_ratio(World &w) {
         float kmax = w.vtx_kmax[v];
         float ratio = 0.f;
         uint32_t ps = w.vtx_seg[2 * v], ns = w.vtx_seg[2 * v + 1];
-        if (kmax > 0.f && ps != kNone && ns != kNone) {
+        if (kmax > 0.f && ps != kNone && ns != kNone && vtx_on(w, v)) {
             uint32_t pa = w.seg_vtx[2 * ps], pb = w.seg_vtx[2 * ps + 1];
             uint32_t prev = pa == v ? pb : pa;
             uint32_t na = w.seg_vtx[2 * ns], nb = w.seg_vtx[2 * ns + 1];
@@ -728,7 +760,7 @@ void find_internal(World &w, const tgh_config *c) {
     for (int64_t vi = 0; vi < (int64_t)w.V; ++vi) {
         uint32_t v = (uint32_t)vi;
         float *o = &w.icorr[3 * (size_t)v];
-        if (w.vtx_pinned[v]) {
+        if (w.vtx_pinned[v] || !vtx_on(w, v)) {
             o[0] = o[1] = o[2] = 0.f;
             continue;
         }
@@ -780,7 +812,7 @@ void apply_internal(World &w) {
 #pragma omp parallel for schedule(static)
     for (int64_t vi = 0; vi < (int64_t)w.V; ++vi) {
         uint32_t v = (uint32_t)vi;
-        if (w.vtx_pinned[v]) continue;
+        if (w.vtx_pinned[v] || !vtx_on(w, v)) continue;
         for (int k = 0; k < 3; ++k) {
             size_t i = 3 * (size_t)v + k;
             float r = w.wall_extent[i];
@@ -796,7 +828,7 @@ void apply_contact(World &w, const tgh_config *c) {  // kernels.rs:1455-1537
 #pragma omp parallel for schedule(static)
     for (int64_t vi = 0; vi < (int64_t)w.V; ++vi) {
         uint32_t v = (uint32_t)vi;
-        if (w.vtx_pinned[v]) {
+        if (w.vtx_pinned[v] || !vtx_on(w, v)) {
             w.vstep[v] = 0.f;
             continue;
         }
@@ -893,7 +925,7 @@ void curvature_cleanup(World &w, const tgh_config *c) {
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t fi = 0; fi < (int64_t)w.F; ++fi) {
         uint32_t start = w.fiber_span[2 * fi], count = w.fiber_span[2 * fi + 1];
-        if (count < 3) continue;
+        if (count < 3 || !w.fiber_active[fi]) continue;
         uint64_t projections = 0, sweeps_run = 0;
         for (uint32_t sweep = 0; sweep < c->curvature_cleanup_sweeps; ++sweep) {
             uint64_t forward = 0;
@@ -911,6 +943,49 @@ void curvature_cleanup(World &w, const tgh_config *c) {
         g_gs_fiber_sweeps += sweeps_run;
 #pragma omp critical
         g_gs_max_fiber_projections = std::max(g_gs_max_fiber_projections, projections);
+    }
+}
+
+// ---- formation: kernels.rs:164-221, 538-710 --------------------------------
+
+void activate(World &w, uint32_t maximum_step) {  // activate_formation_step
+    for (uint32_t f = 0; f < w.F; ++f) w.fiber_active[f] = w.fiber_step[f] > maximum_step ? 0u : 1u;
+    w.nstate[0] = 1;  // request_neighbor_list_rebuild (world.rs:2176-2178)
+}
+
+float fiber_center(const World &w, uint32_t f, uint32_t axis) {
+    uint32_t start = w.fiber_span[2 * f], count = w.fiber_span[2 * f + 1];
+    float center = 0.0f;
+    for (uint32_t i = 0; i < count; ++i) center += w.pos[3 * (size_t)(start + i) + axis];
+    return center / (float)count;
+}
+
+bool layer_held(const World &w, uint32_t f) {
+    uint32_t layer = w.fiber_layer[f];
+    return !(layer == 0xffffffffu || layer >= w.layer_targets.size() || layer < w.first_layer ||
+             layer > w.last_layer) &&
+           w.fiber_span[2 * f + 1] > 0 && w.fiber_active[f];
+}
+
+void apply_layer_targets(World &w) {  // apply_formation_layer_targets (kernels.rs:538-580)
+    if (w.layer_targets.empty()) return;
+    for (uint32_t f = 0; f < w.F; ++f) {
+        if (!layer_held(w, f)) continue;
+        float center = fiber_center(w, f, w.layer_axis);
+        float t = rclamp((w.layer_targets[w.fiber_layer[f]] - center) * w.layer_stiffness, -w.layer_max, w.layer_max);
+        uint32_t start = w.fiber_span[2 * f], count = w.fiber_span[2 * f + 1];
+        for (uint32_t i = 0; i < count; ++i) w.pos[3 * (size_t)(start + i) + w.layer_axis] += t;
+    }
+}
+
+void apply_vertex_targets(World &w) {  // apply_vertex_targets (kernels.rs:605-635)
+    for (size_t i = 0; i < w.vt_index.size(); ++i) {
+        uint32_t v = w.vt_index[i];
+        if (!vtx_on(w, v)) continue;
+        float c = w.vt_command[i];
+        float t = rclamp((w.vt_target[i] - c) * w.vt_stiffness, -w.vt_max, w.vt_max);
+        w.vt_command[i] = c + t;
+        w.pos[3 * (size_t)v + w.vt_axis] = c + t;
     }
 }
 
@@ -937,6 +1012,10 @@ int run_batch(World &w, const tgh_config *c, uint32_t iterations, tgh_batch_stat
             }
             apply_contact(w, c);
             curvature_cleanup(w, c);
+            if (step < iterations) {  // manufacturing targets (world.rs:1420-1482)
+                apply_layer_targets(w);
+                apply_vertex_targets(w);
+            }
         }
         flag_displacement(w);
     }
@@ -1066,6 +1145,68 @@ void orc_gs_counters(uint64_t *out) {
     out[2] = g_gs_fiber_sweeps;
     out[3] = g_gs_max_fiber_projections;
     g_gs_calls = g_gs_projections = g_gs_fiber_sweeps = g_gs_max_fiber_projections = 0;
+}
+
+void orc_world_activate(void *h, uint32_t maximum_step, uint64_t *segments, uint64_t *vertices) {
+    World &w = *(World *)h;
+    activate(w, maximum_step);
+    uint64_t s = 0, v = 0;
+    for (uint32_t f = 0; f < w.F; ++f)
+        if (w.fiber_active[f]) {
+            s += w.fiber_span[2 * f + 1] - 1;
+            v += w.fiber_span[2 * f + 1];
+        }
+    *segments = s;
+    *vertices = v;
+}
+
+void orc_world_set_layer_targets(void *h, uint32_t axis, const float *targets, size_t count, uint32_t first,
+                                 uint32_t last, float stiffness, float max_translation) {
+    World &w = *(World *)h;
+    w.layer_targets.assign(targets, targets + count);
+    w.layer_axis = axis;
+    w.first_layer = first;
+    w.last_layer = last;
+    w.layer_stiffness = stiffness;
+    w.layer_max = max_translation;
+    apply_layer_targets(w);  // the first correction applies at once (world.rs:1766-1795)
+}
+
+void orc_world_clear_layer_targets(void *h) { ((World *)h)->layer_targets.clear(); }
+
+void orc_world_set_vertex_targets(void *h, uint32_t axis, const uint32_t *vertices, size_t count, float displacement,
+                                  float stiffness, float max_translation) {
+    World &w = *(World *)h;  // initialize_vertex_displacement_targets (kernels.rs:582-603)
+    w.vt_index.assign(vertices, vertices + count);
+    w.vt_target.resize(count);
+    w.vt_command.resize(count);
+    for (size_t i = 0; i < count; ++i) {
+        float c = w.pos[3 * (size_t)vertices[i] + axis];
+        w.vt_target[i] = c + displacement;
+        w.vt_command[i] = c;
+    }
+    w.vt_axis = axis;
+    w.vt_stiffness = stiffness;
+    w.vt_max = max_translation;
+}
+
+void orc_world_clear_vertex_targets(void *h) { ((World *)h)->vt_index.clear(); }
+
+void orc_world_formation_target_error(void *h, float *layer_error, float *vertex_error, uint32_t *has_layer,
+                                      uint32_t *has_vertex) {
+    World &w = *(World *)h;  // measure_layer_target_error, measure_vertex_target_error
+    float le = 0.f, ve = 0.f;
+    if (!w.layer_targets.empty())
+        for (uint32_t f = 0; f < w.F; ++f)
+            if (layer_held(w, f))
+                le = std::max(le, std::fabs(w.layer_targets[w.fiber_layer[f]] - fiber_center(w, f, w.layer_axis)));
+    for (size_t i = 0; i < w.vt_index.size(); ++i)
+        if (vtx_on(w, w.vt_index[i]))
+            ve = std::max(ve, std::fabs(w.vt_target[i] - w.pos[3 * (size_t)w.vt_index[i] + w.vt_axis]));
+    *layer_error = le;
+    *vertex_error = ve;
+    *has_layer = w.layer_targets.empty() ? 0u : 1u;
+    *has_vertex = w.vt_index.empty() ? 0u : 1u;
 }
 
 void orc_world_destroy(void *h) { delete (World *)h; }

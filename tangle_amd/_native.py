@@ -45,6 +45,8 @@ class Assembly(C.Structure):
         ("cell_lengths", C.c_double * 3),
         ("cell_periodic", C.c_uint32 * 3),
         ("pin_fiber_ends", C.c_uint32),
+        ("fiber_formation_step", C.POINTER(C.c_uint32)),
+        ("fiber_formation_layer", C.POINTER(C.c_uint32)),
     ]
 
 
@@ -147,6 +149,16 @@ SIGNATURES = {
     "tgh_world_neighbor_lists": (C.c_int, [C.c_void_p, _u32p, _u32p, C.c_size_t]),
     "tgh_world_set_timing": (C.c_int, [C.c_void_p, C.c_uint32]),
     "tgh_world_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "tgh_world_activate_formation_step": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64),
+                                                    C.POINTER(C.c_uint64)]),
+    "tgh_world_set_layer_targets": (C.c_int, [C.c_void_p, C.c_uint32, _f32p, C.c_size_t, C.c_uint32, C.c_uint32,
+                                              C.c_float, C.c_float]),
+    "tgh_world_clear_layer_targets": (C.c_int, [C.c_void_p]),
+    "tgh_world_set_vertex_targets": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_size_t, C.c_float,
+                                               C.c_float, C.c_float]),
+    "tgh_world_clear_vertex_targets": (C.c_int, [C.c_void_p]),
+    "tgh_world_formation_target_error": (C.c_int, [C.c_void_p, _f32p, _f32p, C.POINTER(C.c_uint32),
+                                                   C.POINTER(C.c_uint32)]),
     "tgh_world_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "tgh_world_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32]),
     "tgh_world_destroy": (None, [C.c_void_p]),
@@ -223,7 +235,7 @@ class PackedAssembly:
     """Holds contiguous arrays alive for a tgh_assembly view."""
 
     def __init__(self, counts, placed, intrinsic, radius, min_bend_radius, origin, lengths, periodic,
-                 pin_fiber_ends=False, vertex_pinned=None):
+                 pin_fiber_ends=False, vertex_pinned=None, formation_steps=None, formation_layers=None):
         self.counts = np.ascontiguousarray(counts, dtype=np.uint32)
         self.placed = np.ascontiguousarray(placed, dtype=np.float64).reshape(-1)
         self.intrinsic = np.ascontiguousarray(intrinsic, dtype=np.float64).reshape(-1)
@@ -257,6 +269,16 @@ class PackedAssembly:
             self.struct.cell_lengths[k] = float(lengths[k])
             self.struct.cell_periodic[k] = 1 if periodic[k] else 0
         self.struct.pin_fiber_ends = 1 if pin_fiber_ends else 0
+        F = self.counts.size
+        self.formation_steps = None if formation_steps is None else np.ascontiguousarray(formation_steps, np.uint32)
+        self.formation_layers = None if formation_layers is None else np.ascontiguousarray(formation_layers, np.uint32)
+        for arr, name in ((self.formation_steps, "formation_steps"), (self.formation_layers, "formation_layers")):
+            if arr is not None and arr.size != F:
+                raise ValueError(f"one {name} entry per fiber is required")
+        self.struct.fiber_formation_step = (ptr(self.formation_steps, C.c_uint32)
+                                            if self.formation_steps is not None else None)
+        self.struct.fiber_formation_layer = (ptr(self.formation_layers, C.c_uint32)
+                                             if self.formation_layers is not None else None)
 
     @property
     def vertex_count(self) -> int:
@@ -358,6 +380,42 @@ class World:
         for key, value in changes.items():
             setattr(t, key, value)
         check(library().tgh_world_set_tuning(self._handle, C.byref(t)))
+
+    # -- formation (world.rs:1723-1927, 2139-2180) -----------------------
+    def activate_formation_step(self, maximum_step: int):
+        """Activates the fibers inserted at steps <= maximum_step; returns the
+        (active segments, active vertices) counts."""
+        segs, verts = C.c_uint64(), C.c_uint64()
+        check(library().tgh_world_activate_formation_step(self._handle, int(maximum_step), C.byref(segs),
+                                                           C.byref(verts)))
+        return int(segs.value), int(verts.value)
+
+    def set_layer_targets(self, axis, targets, first_layer=0, last_layer=None, stiffness=0.25,
+                          max_translation=1.0e-5):
+        arr = np.ascontiguousarray(targets, dtype=np.float32)
+        last = arr.size - 1 if last_layer is None else int(last_layer)
+        check(library().tgh_world_set_layer_targets(self._handle, int(axis), ptr(arr, C.c_float), arr.size,
+                                                     int(first_layer), last, float(stiffness),
+                                                     float(max_translation)))
+
+    def clear_layer_targets(self):
+        check(library().tgh_world_clear_layer_targets(self._handle))
+
+    def set_vertex_targets(self, axis, vertices, displacement, stiffness=0.25, max_translation=1.0e-5):
+        arr = np.ascontiguousarray(vertices, dtype=np.uint32)
+        check(library().tgh_world_set_vertex_targets(self._handle, int(axis), ptr(arr, C.c_uint32), arr.size,
+                                                      float(displacement), float(stiffness), float(max_translation)))
+
+    def clear_vertex_targets(self):
+        check(library().tgh_world_clear_vertex_targets(self._handle))
+
+    def formation_target_error(self):
+        """(layer error or None, vertex error or None)."""
+        le, ve = C.c_float(), C.c_float()
+        hl, hv = C.c_uint32(), C.c_uint32()
+        check(library().tgh_world_formation_target_error(self._handle, C.byref(le), C.byref(ve), C.byref(hl),
+                                                         C.byref(hv)))
+        return (float(le.value) if hl.value else None, float(ve.value) if hv.value else None)
 
     def rebuild_lists(self) -> None:
         check(library().tgh_world_rebuild_lists(self._handle))

@@ -129,13 +129,14 @@ __device__ __forceinline__ void closest_offset(const GridParams &g, float4 P1, f
     dist = sqrtf(ox * ox + oy * oy + oz * oz);
 }
 
-__global__ __launch_bounds__(256) void k_features(const GridParams *grids, const float4 *pos, const uint4 *sstat,
+__global__ __launch_bounds__(256) void k_features(const DevState *st, const GridParams *grids, const float4 *pos,
+                                                  const uint4 *sstat,
                                                   const uint32_t *cell_segs, const uint32_t *ncount,
                                                   const uint32_t *nbr, const uint32_t *ovf, const float4 *history,
                                                   uint16_t *X, float *scale, uint32_t block, uint32_t ovf_cap,
                                                   uint32_t S) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= S) return;
+    if (slot >= S || slot >= st->slots) return;  // inactive segments keep zero rows
     const uint32_t seg = cell_segs[slot];
     const uint4 me = sstat[seg];
     const GridParams &g = grids[me.w];
@@ -588,7 +589,7 @@ __global__ void k_apply_proposal(const GridParams *grids, float4 *pos, const flo
     backup[v] = p;
     const float4 sb = vB[v];
     const uint32_t flags = __float_as_uint(sb.z);
-    if (flags & kPinned) return;
+    if (flags & (kPinned | kInactive)) return;
     const GridParams &g = grids[flags >> kSpecShift];
     const uint32_t next_seg = v - __float_as_uint(sb.w);
     float dx = 0.f, dy = 0.f, dz = 0.f, n = 0.f;
@@ -630,7 +631,11 @@ hipError_t launch_mfma_probe(const uint16_t *A, const uint16_t *B, float *D, hip
 }
 
 hipError_t launch_features(const WorldDev &w, const float4 *history, uint16_t *X, float *scale, hipStream_t s) {
-    hipLaunchKernelGGL(k_features, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.grids, w.pos, w.sstat, w.cell_segs,
+    hipError_t e = hipMemsetAsync(X, 0, (size_t)w.S * kFeatureWidth * sizeof(uint16_t), s);
+    if (e != hipSuccess) return e;
+    if (scale) e = hipMemsetAsync(scale, 0, (size_t)w.S * sizeof(float), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_features, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.st, w.grids, w.pos, w.sstat, w.cell_segs,
                        w.ncount, w.nbr, w.ovf, history, X, scale, w.block, w.ovf_cap, w.S);
     return hipGetLastError();
 }

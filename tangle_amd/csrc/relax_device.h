@@ -12,6 +12,7 @@ constexpr uint32_t kHasNext = 2u;   // next segment exists
 constexpr uint32_t kHasPrev2 = 4u;  // second previous vertex exists (rest-chord bend)
 constexpr uint32_t kHasNext2 = 8u;  // second next vertex exists
 constexpr uint32_t kPinned = 16u;
+constexpr uint32_t kInactive = 32u;  // the vertex's fiber is not active (formation)
 
 constexpr uint32_t kBuildThreads = 128;
 constexpr uint32_t kLdsListCapacity = 64;
@@ -56,10 +57,27 @@ struct SpecState {
     uint32_t pad;
 };
 
-// World-wide state: nstate = {rebuild pending, rebuild count} (neighbor.rs:1-9).
+// World-wide state: nstate = {rebuild pending, rebuild count} (neighbor.rs:1-9);
+// slots = the segments in the cell list (the active ones) after a rebuild.
 struct DevState {
     uint32_t nstate[2];
-    uint32_t pad[6];
+    uint32_t slots;
+    uint32_t pad[5];
+};
+
+// Installed formation targets: layer targets (per layer coordinate along
+// `layer_axis`, fibers of layers first..last held) and vertex targets
+// (per vertex: index, target coordinate, command coordinate along
+// `vertex_axis`). Counts of 0 mean none.
+struct FormationDev {
+    uint32_t layer_count, layer_axis, first_layer, last_layer;
+    float layer_stiffness, layer_max_translation;
+    float *layer_targets;
+    uint32_t vertex_count, vertex_axis;
+    float vertex_stiffness, vertex_max_translation;
+    uint32_t *vertex_index;
+    float *vertex_target, *vertex_command;
+    float *error;  // [2]: layer, vertex
 };
 
 // Device pointers and sizes of one world (one or more specimens relaxed
@@ -83,6 +101,9 @@ struct WorldDev {
     float4 *pos, *alt, *refpos, *wall, *vA, *vB;
     uint4 *sstat;   // per segment: first vertex, fiber, radius bits, specimen
     uint4 *fspan;   // per fiber: first vertex, vertex count, specimen, 0
+    uint32_t *fstep;    // per fiber: formation step
+    uint32_t *flayer;   // per fiber: formation layer (0xffffffff: none)
+    uint32_t *factive;  // per fiber: 1 active, 0 not
     uint32_t *cell_counts, *cell_offsets, *cell_cursor, *cell_segs, *seg_cell, *scattered;
     float4 *slotA, *slotB;
     uint4 *slotC;
@@ -125,6 +146,13 @@ hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t st
 hipError_t launch_measure(const WorldDev &w, const IterParams &p, hipStream_t stream);
 hipError_t launch_vertex_passes(const WorldDev &w, const IterParams &p, hipStream_t stream);
 hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t stream);
+// Formation: activation of fibers through a step; layer and vertex targets.
+hipError_t launch_activate(const WorldDev &w, uint32_t maximum_step, hipStream_t stream);
+hipError_t launch_layer_targets(const WorldDev &w, const FormationDev &f, bool gate_on_running, hipStream_t stream);
+hipError_t launch_init_vertex_targets(const WorldDev &w, const FormationDev &f, float displacement,
+                                      hipStream_t stream);
+hipError_t launch_vertex_targets(const WorldDev &w, const FormationDev &f, hipStream_t stream);
+hipError_t launch_target_error(const WorldDev &w, const FormationDev &f, hipStream_t stream);
 
 // Kernel groups of one iteration, in launch order; used for event timing.
 enum KernelGroup { kGroupRebuild = 0, kGroupContact, kGroupMeasure, kGroupVertex, kGroupCurvature, kGroupCount };

@@ -191,11 +191,15 @@ __global__ void k_rebuild_clear(const DevState *st, uint32_t *cell_counts, uint3
 }
 
 __global__ void k_bin_segments(const DevState *st, const GridParams *grids, const float4 *pos, const uint4 *sstat,
-                               uint32_t *seg_cell, uint32_t *cell_counts, uint32_t S) {
+                               const uint32_t *factive, uint32_t *seg_cell, uint32_t *cell_counts, uint32_t S) {
     if (st->nstate[0] == 0) return;
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     const uint4 me = sstat[s];
+    if (factive[me.y] == 0u) {  // inactive fibers are not in the cell list
+        seg_cell[s] = 0xffffffffu;
+        return;
+    }
     const GridParams &g = grids[me.w];
     uint32_t v0 = me.x;
     uint32_t cell = g.cell_base + cell_index(g, pos[v0], pos[v0 + 1]);
@@ -206,7 +210,7 @@ __global__ void k_bin_segments(const DevState *st, const GridParams *grids, cons
 // Exclusive scan of the cell counts in one 1024-thread workgroup: each thread
 // sums a contiguous chunk, the 1024 chunk sums are scanned in LDS, then each
 // thread writes its chunk's offsets.
-__global__ __launch_bounds__(1024) void k_scan_cells(const DevState *st, const uint32_t *counts,
+__global__ __launch_bounds__(1024) void k_scan_cells(DevState *st, const uint32_t *counts,
                                                      uint32_t *offsets, uint32_t C) {
     if (st->nstate[0] == 0) return;
     __shared__ uint32_t sums[1024];
@@ -229,6 +233,7 @@ __global__ __launch_bounds__(1024) void k_scan_cells(const DevState *st, const u
         offsets[i] = run;
         run += counts[i];
     }
+    if (t == 1023) st->slots = sums[1023];  // segments in the cell list
 }
 
 __global__ void k_scatter_segments(const DevState *st, const uint32_t *seg_cell, const uint32_t *cell_offsets,
@@ -237,6 +242,7 @@ __global__ void k_scatter_segments(const DevState *st, const uint32_t *seg_cell,
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     uint32_t cell = seg_cell[s];
+    if (cell == 0xffffffffu) return;  // inactive
     uint32_t slot = cell_offsets[cell] + atomicAdd(&cell_cursor[cell], 1u);
     scattered[slot] = s;
 }
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
     if (st->nstate[0] == 0) return;
     __shared__ uint32_t lds[Lds ? kLdsListCapacity * kBuildThreads : 1];
     uint32_t slot = blockIdx.x * kBuildThreads + threadIdx.x;
-    if (slot >= S) return;
+    if (slot >= S || slot >= st->slots) return;
     const uint32_t tid = threadIdx.x;
     float4 A = slotA[slot], B = slotB[slot];
     uint4 M = slotC[slot];
@@ -626,7 +632,7 @@ __global__ __launch_bounds__(256) void k_contact(DevState *st, const SpecState *
     const uint32_t lane = threadIdx.x % G, group = threadIdx.x / G;
     const uint32_t wave_lane = threadIdx.x & 63u, group_base = wave_lane - lane;
     const uint32_t slot = xcd_block(blockIdx.x, gridDim.x) * groups + group;
-    bool active = slot < S;
+    bool active = slot < S && slot < st->slots;
     SelfSeg me;
     uint32_t count = 0, spec = 0;
     const uint32_t *list = nullptr;
@@ -752,7 +758,7 @@ __global__ __launch_bounds__(256) void k_measure(const SpecState *ss, uint32_t r
         const uint32_t flags = __float_as_uint(b4.z);
         spec = flags >> kSpecShift;
         const float kmax = vA[v].x;
-        if (running(ss, spec) && kmax > 0.f && (flags & kHasPrev) && (flags & kHasNext)) {
+        if (running(ss, spec) && !(flags & kInactive) && kmax > 0.f && (flags & kHasPrev) && (flags & kHasNext)) {
             float4 a = pos[v - 1], b = pos[v], c = pos[v + 1];
             float ax = b.x - a.x, ay = b.y - a.y, az = b.z - a.z;
             float bx = c.x - b.x, by = c.y - b.y, bz = c.z - b.z;
@@ -852,7 +858,7 @@ __global__ __launch_bounds__(256) void k_vertex_pass(const SpecState *ss, const 
         sb = vB[v];
         spec = __float_as_uint(sb.z) >> kSpecShift;
     }
-    if (v < V && running(ss, spec)) {
+    if (v < V && running(ss, spec) && !(__float_as_uint(sb.z) & kInactive)) {
         const GridParams &g = grids[spec];
         float4 me = in[v];
         float4 sa = vA[v];
@@ -1046,7 +1052,8 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, const S
     const uint32_t fiber = blockIdx.x * G + group;
     uint4 span = make_uint4(0u, 0u, 0u, 0u);
     if (fiber < F) span = fspan[fiber];
-    if (!running(ss, span.z)) span.y = 0u;  // a specimen that stopped keeps its positions
+    // a specimen that stopped, or an inactive fiber, keeps its positions
+    if (!running(ss, span.z) || (span.y && (__float_as_uint(vB[span.x].z) & kInactive))) span.y = 0u;
     const uint32_t start = span.x, n = span.y;
     const GridParams &g = grids[span.z];
     const float half_skin = 0.5f * g.skin;
@@ -1201,7 +1208,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, const 
     const uint32_t fiber = blockIdx.x;
     if (fiber >= F) return;
     const uint4 span = fspan[fiber];
-    if (!running(ss, span.z)) return;
+    if (!running(ss, span.z) || (__float_as_uint(vB[span.x].z) & kInactive)) return;
     const uint32_t start = span.x, n = span.y;
     const GridParams &g = grids[span.z];
     const float half_skin = 0.5f * g.skin;
@@ -1378,7 +1385,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_lane(DevState *st, const Sp
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     const uint4 span = fspan[f];
-    if (!running(ss, span.z)) return;
+    if (!running(ss, span.z) || (__float_as_uint(vB[span.x].z) & kInactive)) return;
     const uint32_t start = span.x, n = span.y;
     const GridParams &g = grids[span.z];
     const float half_skin = 0.5f * g.skin;
@@ -1435,6 +1442,130 @@ __global__ void k_finish_rebuild(DevState *st) {
     st->nstate[1] += 1u;
 }
 
+// ---------------------------------------------------------------------------
+// Formation (kernels.rs:164-221, 538-710): activation of fibers by recipe
+// step, manufacturing-layer targets and vertex (needle) targets.
+
+// activate_formation_step (kernels.rs:164-221) for non-adaptive layouts:
+// fibers past `maximum_step` become inactive, the others active. The flag is
+// kept per fiber and mirrored in the vertex flags the per-vertex kernels read.
+__global__ void k_activate(const uint4 *fspan, const uint32_t *fstep, uint32_t *factive, float4 *vB, uint32_t F,
+                           uint32_t maximum_step) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const uint32_t on = fstep[f] > maximum_step ? 0u : 1u;
+    if (factive[f] == on) return;
+    factive[f] = on;
+    const uint4 span = fspan[f];
+    for (uint32_t i = 0; i < span.y; ++i) {
+        float4 b = vB[span.x + i];
+        uint32_t flags = __float_as_uint(b.z);
+        flags = on ? (flags & ~kInactive) : (flags | kInactive);
+        b.z = __uint_as_float(flags);
+        vB[span.x + i] = b;
+    }
+}
+
+__device__ __forceinline__ float axis_of(const float4 &p, uint32_t axis) {
+    return axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
+}
+__device__ __forceinline__ void add_axis(float4 &p, uint32_t axis, float d) {
+    if (axis == 0) p.x += d;
+    else if (axis == 1) p.y += d;
+    else p.z += d;
+}
+__device__ __forceinline__ void set_axis(float4 &p, uint32_t axis, float v) {
+    if (axis == 0) p.x = v;
+    else if (axis == 1) p.y = v;
+    else p.z = v;
+}
+
+// The mean coordinate of a fiber along `axis`, summed in vertex order.
+__device__ __forceinline__ float fiber_center(const float4 *pos, uint4 span, uint32_t axis) {
+    float center = 0.0f;
+    for (uint32_t i = 0; i < span.y; ++i) center += axis_of(pos[span.x + i], axis);
+    return center / (float)span.y;
+}
+
+// apply_formation_layer_targets (kernels.rs:538-580): one lane per fiber.
+__global__ void k_layer_targets(const SpecState *ss, float4 *pos, const uint4 *fspan, const uint32_t *flayer,
+                                const uint32_t *factive, const float *targets, uint32_t layers, uint32_t axis,
+                                uint32_t first, uint32_t last, float stiffness, float max_translation, uint32_t F,
+                                uint32_t gate) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const uint4 span = fspan[f];
+    if (gate && !running(ss, span.z)) return;
+    const uint32_t layer = flayer[f];
+    if (layer == 0xffffffffu || layer >= layers || layer < first || layer > last) return;
+    if (span.y == 0 || factive[f] == 0u) return;
+    const float center = fiber_center(pos, span, axis);
+    const float translation = rclamp((targets[layer] - center) * stiffness, -max_translation, max_translation);
+    for (uint32_t i = 0; i < span.y; ++i) {
+        float4 p = pos[span.x + i];
+        add_axis(p, axis, translation);
+        pos[span.x + i] = p;
+    }
+}
+
+// initialize_vertex_displacement_targets (kernels.rs:582-603).
+__global__ void k_init_vertex_targets(const float4 *pos, const uint32_t *index, float *target, float *command,
+                                      uint32_t count, uint32_t axis, float displacement) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float c = axis_of(pos[index[i]], axis);
+    target[i] = c + displacement;
+    command[i] = c;
+}
+
+// apply_vertex_targets (kernels.rs:605-635).
+__global__ void k_vertex_targets(const SpecState *ss, float4 *pos, const float4 *vB, const uint32_t *index,
+                                 const float *target, float *command, uint32_t count, uint32_t axis, float stiffness,
+                                 float max_translation) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t v = index[i];
+    const uint32_t flags = __float_as_uint(vB[v].z);
+    if (!running(ss, flags >> kSpecShift) || (flags & kInactive)) return;
+    const float c = command[i];
+    const float translation = rclamp((target[i] - c) * stiffness, -max_translation, max_translation);
+    command[i] = c + translation;
+    float4 p = pos[v];
+    set_axis(p, axis, c + translation);
+    pos[v] = p;
+}
+
+// measure_layer_target_error and measure_vertex_target_error
+// (kernels.rs:637-710): maxima, order-free, over one workgroup.
+__global__ __launch_bounds__(256) void k_target_error(const float4 *pos, const uint4 *fspan, const uint32_t *flayer,
+                                                      const uint32_t *factive, const float *targets, uint32_t layers,
+                                                      uint32_t layer_axis, uint32_t first, uint32_t last, uint32_t F,
+                                                      const float4 *vB, const uint32_t *index, const float *vtarget,
+                                                      uint32_t vcount, uint32_t vertex_axis, float *error) {
+    __shared__ uint32_t scratch[5];
+    float le = 0.f, ve = 0.f;
+    if (layers) {
+        for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) {
+            const uint32_t layer = flayer[f];
+            const uint4 span = fspan[f];
+            if (layer == 0xffffffffu || layer >= layers || layer < first || layer > last) continue;
+            if (span.y == 0 || factive[f] == 0u) continue;
+            le = fmaxf(le, fabsf(targets[layer] - fiber_center(pos, span, layer_axis)));
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < vcount; i += blockDim.x) {
+        const uint32_t v = index[i];
+        if (__float_as_uint(vB[v].z) & kInactive) continue;
+        ve = fmaxf(ve, fabsf(vtarget[i] - axis_of(pos[v], vertex_axis)));
+    }
+    const uint32_t lb = block_max_256(__float_as_uint(le), scratch);
+    const uint32_t vb = block_max_256(__float_as_uint(ve), scratch);
+    if (threadIdx.x == 0) {
+        error[0] = __uint_as_float(lb);
+        error[1] = __uint_as_float(vb);
+    }
+}
+
 }  // namespace tgh
 
 // ---------------------------------------------------------------------------
@@ -1461,7 +1592,7 @@ hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
     hipLaunchKernelGGL(k_rebuild_clear, dim3(blocks_for(clear_n, 256)), dim3(256), 0, stream, w.st, w.cell_counts,
                        w.cell_cursor, w.C, w.pos, w.refpos, w.V);
     hipLaunchKernelGGL(k_bin_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.grids, w.pos,
-                       w.sstat, w.seg_cell, w.cell_counts, w.S);
+                       w.sstat, w.factive, w.seg_cell, w.cell_counts, w.S);
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, w.st, w.cell_counts, w.cell_offsets, w.C);
     hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.seg_cell,
                        w.cell_offsets, w.cell_cursor, w.scattered, w.S);
@@ -1566,6 +1697,44 @@ hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t 
                            src, w.pos, w.refpos, w.vA, w.vB, w.fspan, w.wall, w.F, p.sweeps, p.curvature_stiffness,
                            p.curvature_margin, w.track_walls);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_activate(const WorldDev &w, uint32_t maximum_step, hipStream_t stream) {
+    hipLaunchKernelGGL(k_activate, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.fspan, w.fstep, w.factive, w.vB,
+                       w.F, maximum_step);
+    return hipGetLastError();
+}
+
+hipError_t launch_layer_targets(const WorldDev &w, const FormationDev &f, bool gate_on_running, hipStream_t stream) {
+    if (f.layer_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_layer_targets, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.ss, w.pos, w.fspan,
+                       w.flayer, w.factive, f.layer_targets, f.layer_count, f.layer_axis, f.first_layer,
+                       f.last_layer, f.layer_stiffness, f.layer_max_translation, w.F, gate_on_running ? 1u : 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_vertex_targets(const WorldDev &w, const FormationDev &f, float displacement,
+                                      hipStream_t stream) {
+    if (f.vertex_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init_vertex_targets, dim3(blocks_for(f.vertex_count, 64)), dim3(64), 0, stream, w.pos,
+                       f.vertex_index, f.vertex_target, f.vertex_command, f.vertex_count, f.vertex_axis,
+                       displacement);
+    return hipGetLastError();
+}
+
+hipError_t launch_vertex_targets(const WorldDev &w, const FormationDev &f, hipStream_t stream) {
+    if (f.vertex_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vertex_targets, dim3(blocks_for(f.vertex_count, 64)), dim3(64), 0, stream, w.ss, w.pos,
+                       w.vB, f.vertex_index, f.vertex_target, f.vertex_command, f.vertex_count, f.vertex_axis,
+                       f.vertex_stiffness, f.vertex_max_translation);
+    return hipGetLastError();
+}
+
+hipError_t launch_target_error(const WorldDev &w, const FormationDev &f, hipStream_t stream) {
+    hipLaunchKernelGGL(k_target_error, dim3(1), dim3(256), 0, stream, w.pos, w.fspan, w.flayer, w.factive,
+                       f.layer_targets, f.layer_count, f.layer_axis, f.first_layer, f.last_layer, w.F, w.vB,
+                       f.vertex_index, f.vertex_target, f.vertex_count, f.vertex_axis, f.error);
     return hipGetLastError();
 }
 

@@ -129,6 +129,12 @@ int pack_assembly(const tgh_assembly *a, Packed &p) {
     }
     if (a->vertex_pinned)
         for (uint32_t v = 0; v < p.V; ++v) p.pinned[v] |= a->vertex_pinned[v] ? 1 : 0;
+    p.fstep.assign(p.F, 0u);
+    p.flayer.assign(p.F, 0xffffffffu);
+    for (uint32_t f = 0; f < p.F; ++f) {
+        if (a->fiber_formation_step) p.fstep[f] = a->fiber_formation_step[f];
+        if (a->fiber_formation_layer) p.flayer[f] = a->fiber_formation_layer[f];
+    }
     // rest arc length along the intrinsic centreline, f64 sums of f32 data
     p.arc.assign(p.V, 0.0f);
     for (uint32_t f = 0; f < p.F; ++f) {
@@ -302,6 +308,16 @@ int upload_static(tgh_world *w) {
     HIP_TRY(hipMemcpy(d.vB, vB.data(), V * sizeof(float4), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.sstat, sstat.data(), S * sizeof(uint4), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.fspan, fspan.data(), F * sizeof(uint4), hipMemcpyHostToDevice));
+    {
+        std::vector<uint32_t> fstep, flayer, factive(F, 1u);  // every fiber active at upload (packed.rs:289, 303)
+        for (const Packed &p : w->specimens) {
+            fstep.insert(fstep.end(), p.fstep.begin(), p.fstep.end());
+            flayer.insert(flayer.end(), p.flayer.begin(), p.flayer.end());
+        }
+        HIP_TRY(hipMemcpy(d.fstep, fstep.data(), F * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.flayer, flayer.data(), F * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.factive, factive.data(), F * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMemcpy((void *)d.grids, w->grids.data(), d.nspec * sizeof(GridParams), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.wall, 0, V * sizeof(float4)));
     HIP_TRY(hipMemset(d.corr0, 0, S * sizeof(float4)));
@@ -427,6 +443,10 @@ int create_world(const tgh_assembly *a, uint32_t count, const tgh_config *c, tgh
     if (e == hipSuccess) e = w->alloc(&d.vB, V);
     if (e == hipSuccess) e = w->alloc(&d.sstat, S);
     if (e == hipSuccess) e = w->alloc(&d.fspan, F);
+    if (e == hipSuccess) e = w->alloc(&d.fstep, F);
+    if (e == hipSuccess) e = w->alloc(&d.flayer, F);
+    if (e == hipSuccess) e = w->alloc(&d.factive, F);
+    if (e == hipSuccess) e = w->alloc(&w->formation.error, 2);
     if (e == hipSuccess) e = w->alloc(&d.cell_counts, C);
     if (e == hipSuccess) e = w->alloc(&d.cell_offsets, C);
     if (e == hipSuccess) e = w->alloc(&d.cell_cursor, C);
@@ -491,6 +511,12 @@ int enqueue_batch(tgh_world *w, const IterParams &q, std::vector<hipEvent_t> *ev
         if (e == hipSuccess) e = tgh::launch_vertex_passes(d, q, s);
         mark(base + tgh::kGroupCurvature);
         if (e == hipSuccess) e = tgh::launch_curvature(d, q, s);
+        // manufacturing targets end each iteration at the commanded position;
+        // the extra pass only measures (world.rs:1420-1482)
+        if (step < q.iterations) {
+            if (e == hipSuccess) e = tgh::launch_layer_targets(d, w->formation, true, s);
+            if (e == hipSuccess) e = tgh::launch_vertex_targets(d, w->formation, s);
+        }
     }
     mark((size_t)(q.iterations + 1) * G);
     if (e != hipSuccess) return set_error(TGH_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -711,12 +737,16 @@ int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t 
     const size_t S = world->d.S, B = world->d.block;
     if (list_capacity < S * B) return set_error(TGH_ERR_SIZE, "list buffer smaller than segment_count*neighbor_block");
     std::vector<uint32_t> slot_seg(S), slot_count(S), cols(std::max<size_t>(1, S * B));  // rows by slot
+    DevState st;
     HIP_TRY(hipStreamSynchronize(world->stream));
+    HIP_TRY(hipMemcpy(&st, world->d.st, sizeof(st), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(slot_seg.data(), world->d.cell_segs, S * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(slot_count.data(), world->d.ncount, S * 4, hipMemcpyDeviceToHost));
     if (S * B) HIP_TRY(hipMemcpy(cols.data(), world->d.nbr, S * B * 4, hipMemcpyDeviceToHost));
     std::memset(lists, 0, S * B * 4);
-    for (size_t slot = 0; slot < S; ++slot) {
+    std::memset(counts, 0, S * 4);  // segments outside the cell list (inactive) have no list
+    const size_t listed_slots = std::min<size_t>(S, st.slots);
+    for (size_t slot = 0; slot < listed_slots; ++slot) {
         const uint32_t seg = slot_seg[slot];
         counts[seg] = slot_count[slot];
         const size_t keep = std::min<size_t>(slot_count[slot], B);
@@ -742,6 +772,125 @@ int tgh_world_timing(tgh_world *world, double *group_ms, uint64_t *group_launche
             world->group_launches[g] = 0;
         }
     }
+    return TGH_OK;
+}
+
+// Captured batch graphs hold the formation launches of their time.
+static void drop_graphs(tgh_world *w) {
+    (void)hipStreamSynchronize(w->stream);
+    for (auto &kv : w->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (hipGraph_t g : w->graph_templates) (void)hipGraphDestroy(g);
+    w->graphs.clear();
+    w->graph_templates.clear();
+}
+
+int tgh_world_activate_formation_step(tgh_world *world, uint32_t maximum_step, uint64_t *active_segments,
+                                      uint64_t *active_vertices) {
+    if (!world) return set_error(TGH_ERR_INVALID, "world is null");
+    HIP_TRY(tgh::launch_activate(world->d, maximum_step, world->stream));
+    // newly active fibers are missing from the lists (world.rs:2176-2178)
+    HIP_TRY(tgh::launch_request_rebuild(world->d, world->stream));
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    uint64_t segs = 0, verts = 0;
+    for (const Packed &p : world->specimens)
+        for (uint32_t f = 0; f < p.F; ++f)
+            if (p.fstep[f] <= maximum_step) {
+                segs += p.fcount[f] - 1;
+                verts += p.fcount[f];
+            }
+    if (active_segments) *active_segments = segs;
+    if (active_vertices) *active_vertices = verts;
+    return TGH_OK;
+}
+
+int tgh_world_set_layer_targets(tgh_world *world, uint32_t axis, const float *targets, size_t count,
+                                uint32_t first_layer, uint32_t last_layer, float stiffness, float max_translation) {
+    if (!world || !targets) return set_error(TGH_ERR_INVALID, "null argument");
+    if (axis > 2 || count == 0 || count > 0xffffffffu) return set_error(TGH_ERR_INVALID, "axis must be 0-2 and targets non-empty");
+    if (!(stiffness > 0.f && stiffness <= 1.f) || !(max_translation > 0.f))
+        return set_error(TGH_ERR_INVALID, "stiffness must be in (0, 1] and max_translation positive");
+    if (first_layer > last_layer) return set_error(TGH_ERR_INVALID, "first_layer after last_layer");
+    for (size_t i = 0; i < count; ++i)
+        if (!std::isfinite(targets[i])) return set_error(TGH_ERR_INVALID, "layer targets must be finite");
+    tgh::FormationDev &f = world->formation;
+    if (count > world->formation_layer_capacity) {
+        HIP_TRY(world->alloc(&f.layer_targets, count));
+        world->formation_layer_capacity = (uint32_t)count;
+    }
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    HIP_TRY(hipMemcpy(f.layer_targets, targets, count * sizeof(float), hipMemcpyHostToDevice));
+    f.layer_count = (uint32_t)count;
+    f.layer_axis = axis;
+    f.first_layer = first_layer;
+    f.last_layer = last_layer;
+    f.layer_stiffness = stiffness;
+    f.layer_max_translation = max_translation;
+    // the first correction applies at once (world.rs:1766-1795)
+    HIP_TRY(tgh::launch_layer_targets(world->d, f, false, world->stream));
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    drop_graphs(world);
+    return TGH_OK;
+}
+
+int tgh_world_clear_layer_targets(tgh_world *world) {
+    if (!world) return set_error(TGH_ERR_INVALID, "world is null");
+    world->formation.layer_count = 0;
+    drop_graphs(world);
+    return TGH_OK;
+}
+
+int tgh_world_set_vertex_targets(tgh_world *world, uint32_t axis, const uint32_t *vertices, size_t count,
+                                 float displacement, float stiffness, float max_translation) {
+    if (!world || !vertices) return set_error(TGH_ERR_INVALID, "null argument");
+    if (axis > 2 || count == 0 || count > 0xffffffffu) return set_error(TGH_ERR_INVALID, "axis must be 0-2 and vertices non-empty");
+    for (size_t i = 0; i < count; ++i)
+        if (vertices[i] >= world->d.V) return set_error(TGH_ERR_INVALID, "vertex index out of range");
+    if (!std::isfinite(displacement) || displacement == 0.f)
+        return set_error(TGH_ERR_INVALID, "displacement must be finite and nonzero");
+    if (!(stiffness > 0.f && stiffness <= 1.f) || !(max_translation > 0.f))
+        return set_error(TGH_ERR_INVALID, "stiffness must be in (0, 1] and max_translation positive");
+    tgh::FormationDev &f = world->formation;
+    if (count > world->formation_vertex_capacity) {
+        HIP_TRY(world->alloc(&f.vertex_index, count));
+        HIP_TRY(world->alloc(&f.vertex_target, count));
+        HIP_TRY(world->alloc(&f.vertex_command, count));
+        world->formation_vertex_capacity = (uint32_t)count;
+    }
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    HIP_TRY(hipMemcpy(f.vertex_index, vertices, count * sizeof(uint32_t), hipMemcpyHostToDevice));
+    f.vertex_count = (uint32_t)count;
+    f.vertex_axis = axis;
+    f.vertex_stiffness = stiffness;
+    f.vertex_max_translation = max_translation;
+    HIP_TRY(tgh::launch_init_vertex_targets(world->d, f, displacement, world->stream));
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    drop_graphs(world);
+    return TGH_OK;
+}
+
+int tgh_world_clear_vertex_targets(tgh_world *world) {
+    if (!world) return set_error(TGH_ERR_INVALID, "world is null");
+    world->formation.vertex_count = 0;
+    drop_graphs(world);
+    return TGH_OK;
+}
+
+int tgh_world_formation_target_error(tgh_world *world, float *layer_error, float *vertex_error,
+                                     uint32_t *has_layer, uint32_t *has_vertex) {
+    if (!world || !layer_error || !vertex_error || !has_layer || !has_vertex)
+        return set_error(TGH_ERR_INVALID, "null argument");
+    const tgh::FormationDev &f = world->formation;
+    *has_layer = f.layer_count ? 1u : 0u;
+    *has_vertex = f.vertex_count ? 1u : 0u;
+    *layer_error = 0.f;
+    *vertex_error = 0.f;
+    if (!f.layer_count && !f.vertex_count) return TGH_OK;
+    float err[2];
+    HIP_TRY(tgh::launch_target_error(world->d, f, world->stream));
+    HIP_TRY(hipMemcpyAsync(err, f.error, sizeof(err), hipMemcpyDeviceToHost, world->stream));
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    *layer_error = err[0];
+    *vertex_error = err[1];
     return TGH_OK;
 }
 

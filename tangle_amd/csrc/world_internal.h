@@ -26,6 +26,8 @@ struct Packed {
     std::vector<float> kmax;       // V
     std::vector<uint8_t> pinned;   // V
     std::vector<float> arc;        // V
+    std::vector<uint32_t> fstep;   // F: formation step
+    std::vector<uint32_t> flayer;  // F: formation layer (0xffffffff: none)
     float lower[3], upper[3];
     uint32_t periodic[3];
 };
@@ -53,6 +55,9 @@ struct tgh_world {
     tgh::SpecState *host_spec = nullptr;  // pinned [nspec], read after every batch
     tgh_config last_config{};
     bool ran = false;
+    // formation targets (installed on the device, applied every iteration)
+    tgh::FormationDev formation{};
+    uint32_t formation_layer_capacity = 0, formation_vertex_capacity = 0;
     uint64_t device_bytes = 0;
     std::vector<void *> allocations;
     std::map<std::string, hipGraphExec_t> graphs;
