@@ -273,6 +273,14 @@ int tgh_mfma_probe(const uint16_t *a, const uint16_t *b, float *d);
                                     (no sampler in the reference; its density and
                                     quantile: tangle_characterize/src/shape.rs:26-30,
                                     552-556)                                       */
+#define TGH_ORIENT_LAYERED_BIAXIAL 4 /* ::LayeredBiaxial { normal, primary_fraction,
+                                    cross_fraction, maximum_in_plane_deviation,
+                                    maximum_tilt, layer_seed } (:599-628); needs
+                                    TGH_POSITION_LAYERED                           */
+
+#define TGH_POSITION_UNIFORM 0   /* PositionDistribution::Uniform (population.rs:80-100) */
+#define TGH_POSITION_LAYERED 1   /* ::Layered { axis, layers, jitter_fraction }    */
+#define TGH_POSITION_GRADIENT 2  /* ::DensityGradient { axis, exponent, toward_high } */
 
 typedef struct tgh_population_spec {
     uint32_t count;
@@ -290,14 +298,27 @@ typedef struct tgh_population_spec {
     double cell_origin[3];
     double cell_lengths[3];
     uint32_t cell_periodic[3];
+    uint32_t position;                    /* TGH_POSITION_* */
+    uint32_t position_axis;               /* layered / gradient axis */
+    uint32_t position_layers;             /* layered: layer count */
+    uint32_t position_toward_high;        /* gradient: concentrate toward the high side */
+    double position_jitter_fraction;      /* layered: jitter in layer spacings, [0, 1] */
+    double position_exponent;             /* gradient: positive power */
+    /* layered biaxial: orientation_vector is the normal, orientation_angle the maximum tilt */
+    double orientation_primary_fraction;
+    double orientation_cross_fraction;
+    double orientation_in_plane_deviation;
+    uint64_t orientation_layer_seed;
 } tgh_population_spec;
 
 /* Writes spec->count fibers of (segments_per_fiber+1) vertices each:
- * intrinsic/placed [3 * count * (segments_per_fiber+1)], radius [count].
+ * intrinsic/placed [3 * count * (segments_per_fiber+1)], radius [count] and,
+ * when `formation_layer` is not NULL, each fiber's layer [count]
+ * (0xffffffff: none; set by TGH_POSITION_LAYERED).
  * Returns TGH_ERR_INVALID with "could not place fiber i" when the placement
  * search is exhausted (PopulationGenerationError::PlacementExhausted). */
 int tgh_generate_population(const tgh_population_spec *spec, double *intrinsic, double *placed,
-                            double *radius, size_t vertex_capacity);
+                            double *radius, uint32_t *formation_layer, size_t vertex_capacity);
 
 #ifdef __cplusplus
 }

@@ -3,8 +3,9 @@
 Pure-Python restatement of the reference's seeded population generator,
 generate_biased_fiber_population (crates/tangle_generate/src/population.rs:246-357)
 with its helpers local_centerline (:554-565), sample_orientation (:567-655),
-sample_contained_center (:657-726), perpendicular/normalize (:754-790) and
-SplitMix64 (:795-813), plus maximum_polyline_curvature
+sample_contained_center (:657-726), sample_formation_layer (:728-740),
+layer_reference_angle and splitmix64_unit (:742-752), perpendicular/normalize
+(:754-790) and SplitMix64 (:795-813), plus maximum_polyline_curvature
 (crates/tangle_core/src/curvature.rs:20-67). Python floats are IEEE doubles
 and math.sin/cos/acos are the C library's, which is what Rust's f64 methods
 call on Linux, so this restatement and the product's C++ generator must agree
@@ -85,8 +86,32 @@ def max_polyline_curvature(points) -> float:
     return best
 
 
-def _direction(spec, rng):
+def _splitmix64_unit(value: int) -> float:
+    return SplitMix64(value).unit()
+
+
+def _layer_reference_angle(seed: int, layer: int) -> float:
+    return math.pi * _splitmix64_unit(seed ^ ((layer * 0x9E3779B97F4A7C15) & MASK))
+
+
+def _direction(spec, rng, layer=None):
     kind = spec["orientation"]
+    if kind == "layered_biaxial":
+        n = _normalize(spec["vector"])
+        a = _perpendicular(n)
+        b = _normalize(_cross(n, a))
+        reference = _layer_reference_angle(spec["layer_seed"], layer)
+        selection = rng.unit()
+        dev = spec["deviation"]
+        if selection < spec["primary"]:
+            azimuth = reference + dev * (2.0 * rng.unit() - 1.0)
+        elif selection < spec["primary"] + spec["cross"]:
+            azimuth = reference + 0.5 * math.pi + dev * (2.0 * rng.unit() - 1.0)
+        else:
+            azimuth = 2.0 * math.pi * rng.unit()
+        tilt = spec["angle"] * (2.0 * rng.unit() - 1.0)
+        return _add(_scale(_add(_scale(a, math.cos(azimuth)), _scale(b, math.sin(azimuth))), math.cos(tilt)),
+                    _scale(n, math.sin(tilt)))
     if kind == "planar":
         n = _normalize(spec["vector"])
         a = _perpendicular(n)
@@ -126,13 +151,22 @@ def generate(spec: dict):
     """spec keys: count, segments, seed, length (lo, hi), radius (lo, hi),
     amplitude (lo, hi), orientation ('isotropic'|'planar'|'aligned'|'schladitz'),
     vector, angle, beta, min_bend_radius (None or r), max_attempts, origin,
-    lengths, periodic. Returns (intrinsic, placed, radius) as nested lists."""
+    lengths, periodic; for 'layered_biaxial' also primary, cross, deviation,
+    layer_seed; optional position ('uniform'|'layered'|'gradient') with
+    position_axis, layers, jitter, exponent, toward_high. Returns
+    (intrinsic, placed, radius, layers) as lists; layers hold None for
+    unlayered fibers."""
     rng = SplitMix64(spec["seed"])
     n = spec["segments"]
     lo_box = list(spec["origin"])
     hi_box = [spec["origin"][k] + spec["lengths"][k] for k in range(3)]
-    intrinsic_all, placed_all, radii = [], [], []
+    intrinsic_all, placed_all, radii, layers = [], [], [], []
+    position = spec.get("position", "uniform")
     for fiber in range(spec["count"]):
+        layer = None
+        if position == "layered":
+            count = spec["layers"]
+            layer = int(min(math.floor(rng.unit() * count), count - 1.0))
         done = False
         for _ in range(spec.get("max_attempts", 128)):
             length = _sample(*spec["length"], rng)
@@ -147,7 +181,7 @@ def generate(spec: dict):
             mbr = spec.get("min_bend_radius")
             if mbr and max_polyline_curvature(local) > (1.0 / mbr) * (1.0 + 1.0e-10):
                 continue
-            d = _direction(spec, rng)
+            d = _direction(spec, rng, layer)
             ta = _perpendicular(d)
             tb = _normalize(_cross(d, ta))
             offsets = [_add(_add(_scale(d, p[0]), _scale(ta, p[1])), _scale(tb, p[2])) for p in local]
@@ -170,11 +204,23 @@ def generate(spec: dict):
             if not feasible:
                 continue
             center = [lower[k] + (upper[k] - lower[k]) * rng.unit() for k in range(3)]
+            if position == "layered":
+                k = spec["position_axis"]
+                spacing = (upper[k] - lower[k]) / spec["layers"]
+                jitter = spec["jitter"] * (rng.unit() - 0.5) * spacing
+                center[k] = min(max(lower[k] + (layer + 0.5) * spacing + jitter, lower[k]), upper[k])
+            elif position == "gradient":
+                k = spec["position_axis"]
+                u = rng.unit()
+                biased = 1.0 - math.pow(1.0 - u, spec["exponent"]) if spec["toward_high"] else \
+                    math.pow(u, spec["exponent"])
+                center[k] = lower[k] + (upper[k] - lower[k]) * biased
             intrinsic_all.extend(local)
             placed_all.extend(_add(center, o) for o in offsets)
             radii.append(radius)
+            layers.append(layer)
             done = True
             break
         if not done:
             raise ValueError(f"could not place fiber {fiber} within max_attempts_per_fiber")
-    return intrinsic_all, placed_all, radii
+    return intrinsic_all, placed_all, radii, layers

@@ -23,6 +23,7 @@ from typing import Iterable, Sequence
 import numpy as np
 
 from . import _native
+from .formation import CircularFootprint, RandomFiberFraction
 from .units import um
 
 __all__ = [
@@ -34,7 +35,10 @@ __all__ = [
     "PlanarOrientation",
     "AlignedOrientation",
     "SchladitzOrientation",
+    "LayeredBiaxialOrientation",
     "UniformPosition",
+    "LayeredPosition",
+    "DensityGradientPosition",
     "generate_fiber_population",
     "RelaxationSettings",
     "RelaxationOverrides",
@@ -42,6 +46,10 @@ __all__ = [
     "Recipe",
     "RecipeError",
     "RunResult",
+    "CircularFootprint",
+    "RandomFiberFraction",
+    "HeldTargets",
+    "relax_batch",
     "um",
 ]
 
@@ -160,8 +168,46 @@ class SchladitzOrientation:
 
 
 @dataclass(frozen=True)
+class LayeredBiaxialOrientation:
+    """Per-layer biaxial directions in the plane normal to ``normal``
+    (OrientationDistribution::LayeredBiaxial, population.rs:599-628): a
+    ``primary_fraction`` within ``max_in_plane_deviation`` of the layer's
+    seeded reference angle, a ``cross_fraction`` about the angle a quarter
+    turn away, the rest uniform; tilted by at most ``max_tilt``. ``normal``
+    None is the stack axis. Needs ``LayeredPosition`` (generators.rs:96-125)."""
+
+    normal: object = None
+    primary_fraction: float = 0.4
+    cross_fraction: float = 0.4
+    max_in_plane_deviation: float = math.radians(10.0)
+    max_tilt: float = math.radians(10.0)
+    seed: int = 1
+
+
+@dataclass(frozen=True)
 class UniformPosition:
     """Uniform centers over each fiber's feasible region."""
+
+
+@dataclass(frozen=True)
+class LayeredPosition:
+    """Centers on ``layer_count`` evenly spaced layers with a uniform jitter
+    of ``jitter_fraction`` layer spacings; each fiber records its layer as
+    ``formation_layer``. ``axis`` None is the stack axis (generators.rs:263-300)."""
+
+    layer_count: int
+    axis: object = None
+    jitter_fraction: float = 0.25
+
+
+@dataclass(frozen=True)
+class DensityGradientPosition:
+    """Centers concentrated by a power law toward one side of ``axis``
+    (generators.rs:302-330); ``axis`` None is the stack axis."""
+
+    axis: object = None
+    exponent: float = 1.0
+    toward_high: bool = True
 
 
 _DEFAULT_MATERIAL = Material("fiber", 0.025, 0.08)  # FiberPopulationSpec::default (population.rs:146-175)
@@ -204,7 +250,11 @@ class FiberPopulation:
     def replace(self, **changes) -> "FiberPopulation":
         return _dc_replace(self, **changes)
 
-    def _spec(self, cell: Cell) -> _native.PopulationSpec:
+    def _spec(self, cell: Cell, stack_axis: int | None = None) -> _native.PopulationSpec:
+        """FiberPopulation.to_rust (generators.rs:570-600)."""
+        stack = cell.stack_axis if stack_axis is None else stack_axis
+        unit = [0.0, 0.0, 0.0]
+        unit[stack] = 1.0
         spec = _native.PopulationSpec()
         diameter = _range(self.diameter, "diameter") or (self.material.diameter, self.material.diameter)
         mean_d = diameter[0] if diameter[0] == diameter[1] else 0.5 * (diameter[0] + diameter[1])
@@ -212,8 +262,9 @@ class FiberPopulation:
         amplitude = _range(self.curvature_amplitude, "curvature_amplitude") or (0.0, 0.8 * mean_d)
         if self.material.is_oval:
             raise ValueError("oval sections are not supported by the hip backend")
-        if not isinstance(self.position, UniformPosition):
-            raise ValueError("only UniformPosition is supported")
+        if isinstance(self.orientation, LayeredBiaxialOrientation) and not isinstance(self.position, LayeredPosition):
+            raise ValueError("LayeredBiaxialOrientation picks a direction per layer, so it needs "
+                             "position=LayeredPosition(...)")
         spec.count = int(self.count)
         spec.segments_per_fiber = int(self.segments_per_fiber)
         spec.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
@@ -235,8 +286,33 @@ class FiberPopulation:
             spec.orientation = _native.ORIENT_SCHLADITZ
             spec.orientation_vector[:] = _direction(o.axis)
             spec.orientation_beta = float(o.beta)
+        elif isinstance(o, LayeredBiaxialOrientation):
+            spec.orientation = _native.ORIENT_LAYERED_BIAXIAL
+            spec.orientation_vector[:] = unit if o.normal is None else _direction(o.normal)
+            spec.orientation_angle = float(o.max_tilt)
+            spec.orientation_primary_fraction = float(o.primary_fraction)
+            spec.orientation_cross_fraction = float(o.cross_fraction)
+            spec.orientation_in_plane_deviation = float(o.max_in_plane_deviation)
+            spec.orientation_layer_seed = int(o.seed) & 0xFFFFFFFFFFFFFFFF
         else:
             raise TypeError(f"unsupported orientation {o!r}")
+        p = self.position
+        if isinstance(p, UniformPosition):
+            spec.position = _native.POSITION_UNIFORM
+        elif isinstance(p, LayeredPosition):
+            spec.position = _native.POSITION_LAYERED
+            spec.position_axis = stack if p.axis is None else _AXES[p.axis]
+            if int(p.layer_count) <= 0:
+                raise ValueError("layer_count must be positive")
+            spec.position_layers = int(p.layer_count)
+            spec.position_jitter_fraction = float(p.jitter_fraction)
+        elif isinstance(p, DensityGradientPosition):
+            spec.position = _native.POSITION_GRADIENT
+            spec.position_axis = stack if p.axis is None else _AXES[p.axis]
+            spec.position_exponent = float(p.exponent)
+            spec.position_toward_high = 1 if p.toward_high else 0
+        else:
+            raise TypeError("position must be UniformPosition, LayeredPosition, or DensityGradientPosition")
         spec.min_bend_radius = self.material.min_bend_radius or 0.0
         spec.max_attempts_per_fiber = int(self.max_attempts_per_fiber)
         spec.cell_origin[:] = cell.origin
@@ -251,6 +327,8 @@ class _Fiber:
     radius: float
     placed: np.ndarray      # (n, 3) float64
     intrinsic: np.ndarray   # (n, 3) float64
+    formation_layer: int | None = None
+    step: int = 0           # formation step, set by Recipe.insert
 
 
 class FiberCollection:
@@ -260,13 +338,42 @@ class FiberCollection:
         self.name = name
         self._fibers: list[_Fiber] = []
 
-    def add_fiber(self, points, material: Material, rest_points=None) -> int:
+    def add_fiber(self, points, material: Material, rest_points=None, *, formation_layer: int | None = None) -> int:
+        """collection.rs:276-307; `formation_layer` labels the fiber's ply."""
         placed = np.asarray(points, dtype=np.float64).reshape(-1, 3)
         intrinsic = placed.copy() if rest_points is None else np.asarray(rest_points, dtype=np.float64).reshape(-1, 3)
         if placed.shape[0] < 2 or intrinsic.shape != placed.shape:
             raise ValueError("a fiber needs at least two points and matching rest points")
-        self._fibers.append(_Fiber(material, material.radius, placed, intrinsic))
+        if not (np.all(np.isfinite(placed)) and np.all(np.isfinite(intrinsic))):
+            raise ValueError("centerline coordinates must be finite")
+        layer = None if formation_layer is None else int(formation_layer)
+        if layer is not None and not 0 <= layer < 0xFFFFFFFF:
+            raise ValueError("formation_layer must be a u32 below 0xffffffff")
+        self._fibers.append(_Fiber(material, material.radius, placed, intrinsic, layer))
         return len(self._fibers) - 1
+
+    @classmethod
+    def from_centerlines(cls, centerlines, material: Material, *, name: str = "collection",
+                         formation_layer: int | None = None) -> "FiberCollection":
+        """collection.rs:310-331."""
+        out = cls(name)
+        for line in centerlines:
+            out.add_fiber(line, material, formation_layer=formation_layer)
+        return out
+
+    def layer_ids(self) -> list[int]:
+        """The sorted formation-layer labels present (collection.rs:366-376)."""
+        return sorted({f.formation_layer for f in self._fibers if f.formation_layer is not None})
+
+    formation_layers = layer_ids
+
+    def select_layer(self, layer: int, *, name: str | None = None, allow_empty: bool = False) -> "FiberCollection":
+        """A copy of the fibers labelled `layer` (collection.rs:378-396)."""
+        out = FiberCollection(name or f"{self.name} layer {layer}")
+        out._fibers = [f for f in self._fibers if f.formation_layer == layer]
+        if not out._fibers and not allow_empty:
+            raise ValueError(f"collection contains no fibers in layer {layer}")
+        return out
 
     def centerlines(self):
         return [f.placed.tolist() for f in self._fibers]
@@ -288,13 +395,18 @@ class FiberCollection:
 
 def generate_fiber_population(cell: Cell, population: FiberPopulation, *, name: str = "population") -> FiberCollection:
     """Places ``population``'s seeded random fibers in ``cell`` (population.rs:246-357)."""
-    spec = population._spec(cell)
-    intrinsic, placed, radius = _native.generate_population(spec)
+    return _generate(cell, population, cell.stack_axis, name)
+
+
+def _generate(cell: Cell, population: FiberPopulation, stack_axis: int, name: str) -> FiberCollection:
+    spec = population._spec(cell, stack_axis)
+    intrinsic, placed, radius, layers = _native.generate_population(spec)
     nv = population.segments_per_fiber + 1
     out = FiberCollection(name)
     for f in range(population.count):
         sl = slice(f * nv, (f + 1) * nv)
-        out._fibers.append(_Fiber(population.material, float(radius[f]), placed[sl], intrinsic[sl]))
+        layer = None if layers[f] == 0xFFFFFFFF else int(layers[f])
+        out._fibers.append(_Fiber(population.material, float(radius[f]), placed[sl], intrinsic[sl], layer))
     return out
 
 
@@ -555,6 +667,8 @@ class RunResult:
     wall_seconds: float
     positions: np.ndarray
     vertex_counts: np.ndarray
+    post_recipe_iterations: int = 0
+    needling: list = field(default_factory=list)  # (layer, selected vertices, depth) per needle command
 
     def centerlines(self):
         out, start = [], 0
@@ -589,6 +703,8 @@ class _RunState:
         self.history_marked = False
         self.proposals_kept = 0
         self.proposals_reverted = 0
+        self.post_recipe_iterations = 0
+        self.needling: list = []
 
     def _propose(self, world, config):
         """Learned warm start: every `interval` iterations of the first
@@ -686,6 +802,52 @@ class _RunState:
             if status.cell_list_overflow:
                 raise RecipeError(index, description, self.total, "cell list overflow")
 
+    def settle(self, world, index, description, tolerance: float, maximum: int):
+        """RelaxUntilTargetsReached (recipe.rs:1521-1596): full batches until
+        the largest layer-centre or needle residual is within `tolerance`."""
+        begin = self.total
+        config = _native.Config.from_buffer_copy(self.config)
+        config.force_full_iterations = 1
+        tol = np.float32(tolerance)
+        while True:
+            completed = self.total - begin
+            residuals = [e for e in world.formation_target_error() if e is not None]
+            if not residuals:
+                self.events.append("no active formation target to advance")
+                return
+            worst = max(residuals)
+            if np.float32(worst) <= tol:
+                self.events.append(f"advance formation targets in {completed} iterations (residual {worst:.3e})")
+                return
+            if completed >= maximum:
+                raise RecipeError(index, description, self.total,
+                                  f"formation targets remained at residual {worst:.3e} after {maximum} iterations "
+                                  f"(tolerance {tolerance:.3e})")
+            remaining = self._budget(index, description)
+            batch = min(remaining, self.settings.iterations_per_batch, maximum - completed)
+            status = self._batch(world, config, batch)
+            if status.cell_list_overflow:
+                raise RecipeError(index, description, self.total, "cell list overflow")
+
+    def finish(self, world):
+        """After the last operation the targets are released and relaxation
+        continues, unconstrained, until the run's tolerances hold or its
+        iteration budget is spent (recipe.rs:893-915, plugin.rs:1041-1181);
+        not converging here is not an error."""
+        world.clear_layer_targets()
+        world.clear_vertex_targets()
+        begin = self.total
+        config = _native.Config.from_buffer_copy(self.config)
+        config.force_full_iterations = 1 if self.settings.force_full_iterations else 0
+        self.converged = False
+        while self.total < self.settings.max_iterations:
+            batch = min(self.settings.max_iterations - self.total, self.settings.iterations_per_batch)
+            batch = self._proposal_window(world, config, batch)
+            status = self._batch(world, config, batch)
+            if status.cell_list_overflow or self.converged:
+                break
+        self.post_recipe_iterations = self.total - begin
+
     def solve(self, world, index, description, policy: "SolvePolicy", overrides):
         begin = self.total
         config = _native.Config.from_buffer_copy(self.config)
@@ -724,8 +886,50 @@ class _RunState:
                 raise RecipeError(index, description, self.total, "cell list overflow")
 
 
+class HeldTargets:
+    """Returned by operations that hold fibers on targets
+    (crates/tangle_python/src/recipe.rs:148-195). As a context manager it
+    records the matching release when the block ends; releasing layer
+    placement releases every held layer."""
+
+    def __init__(self, recipe: "Recipe", kind: str):
+        self._recipe = recipe
+        self._kind = kind
+
+    def __enter__(self) -> "Recipe":
+        return self._recipe
+
+    def __exit__(self, *exc) -> bool:
+        if self._kind == "layers":
+            self._recipe.release_layer_placement()
+        else:
+            self._recipe.release_needles()
+        return False
+
+    def __repr__(self) -> str:
+        return f"HeldTargets({'layer placement' if self._kind == 'layers' else 'needles'})"
+
+
+def _positive(value, name):
+    value = float(value)
+    if not (math.isfinite(value) and value > 0.0):
+        raise ValueError(f"{name} must be positive and finite")
+    return value
+
+
+def _unit_fraction(value, name):
+    value = float(value)
+    if not (math.isfinite(value) and 0.0 < value <= 1.0):
+        raise ValueError(f"{name} must lie in (0, 1]")
+    return value
+
+
 class Recipe:
-    """An ordered list of steps on one cell; nothing runs until ``run``."""
+    """An ordered list of formation operations on one cell; ``insert`` adds
+    fibers at once, everything else runs in order inside ``run``
+    (crates/tangle_python/src/recipe.rs:197-790). Every ``insert`` is a new
+    formation step: its fibers join the relaxation when the recipe reaches
+    it."""
 
     def __init__(self, cell: Cell, *, stack_axis=None):
         if not isinstance(cell, Cell):
@@ -734,33 +938,55 @@ class Recipe:
         self._fibers: list[_Fiber] = []
         self._operations: list[tuple] = []
         self._descriptions: list[str] = []
+        self._next_step = 0
         self._stack_axis = cell.stack_axis if stack_axis is None else _AXES[stack_axis]
 
     @property
     def stack_axis(self) -> int:
         return self._stack_axis
 
+    def _push(self, operation: tuple, description: str) -> None:
+        self._operations.append(operation)
+        self._descriptions.append(description)
+
     def insert(self, collection, *, name: str | None = None):
+        """Adds the fibers as the next formation step (recipe.rs:250-298); a
+        FiberPopulation is generated in this recipe's cell first, stacked
+        along the recipe's axis."""
         if isinstance(collection, FiberPopulation):
-            collection = generate_fiber_population(self._cell, collection)
+            collection = _generate(self._cell, collection, self._stack_axis, "fiber population")
         if not isinstance(collection, FiberCollection):
             raise TypeError("insert takes a FiberCollection or a FiberPopulation")
-        self._fibers.extend(collection._fibers)
-        self._operations.append(("insert", len(collection)))
-        self._descriptions.append(f"insert {name or collection.name} ({len(collection)} fibers)")
+        if len(collection) == 0:
+            raise ValueError("cannot insert an empty fiber collection")
+        step = self._next_step
+        self._next_step += 1
+        self._fibers.extend(_dc_replace(f, step=step) for f in collection._fibers)
+        label = name or collection.name
+        self._push(("activate", step), f"insert {label!r} ({len(collection)} fibers) as formation step {step}")
         return len(collection)
 
     def relax_until_converged(self, *, max_iterations: int = 2000) -> None:
         if max_iterations <= 0:
             raise ValueError("max_iterations must be positive")
-        self._operations.append(("relax_until_converged", int(max_iterations)))
-        self._descriptions.append(f"relax until converged (at most {max_iterations} iterations)")
+        self._push(("relax_until_converged", int(max_iterations)),
+                   f"relax until converged (up to {max_iterations} iterations)")
 
     def relax_for(self, iterations: int) -> None:
         if iterations <= 0:
             raise ValueError("iterations must be positive")
-        self._operations.append(("relax_for", int(iterations)))
-        self._descriptions.append(f"relax for {iterations} iterations")
+        self._push(("relax_for", int(iterations)), f"relax for exactly {iterations} iterations")
+
+    def settle_targets(self, *, tolerance: float, max_iterations: int) -> None:
+        """Relaxes until held layers and needles are within `tolerance` of
+        their targets (recipe.rs:327-343)."""
+        tolerance = float(tolerance)
+        if not (math.isfinite(tolerance) and tolerance >= 0.0):
+            raise ValueError("tolerance must be non-negative and finite")
+        if max_iterations <= 0:
+            raise ValueError("max_iterations must be positive")
+        self._push(("settle_targets", (tolerance, int(max_iterations))),
+                   f"settle held targets to tolerance {tolerance}")
 
     def solve(self, policy: SolvePolicy, overrides: RelaxationOverrides | None = None) -> None:
         """Relax under `policy`'s targets and gate, with `overrides` layered
@@ -769,8 +995,71 @@ class Recipe:
             raise TypeError("solve takes a SolvePolicy")
         if overrides is not None and not isinstance(overrides, RelaxationOverrides):
             raise TypeError("overrides must be RelaxationOverrides")
-        self._operations.append(("solve", (policy, overrides)))
-        self._descriptions.append(f"solve {policy.name}")
+        self._push(("solve", (policy, overrides)), f"solve {policy.name!r}")
+
+    def scale_layer_spacing(self, factor: float, *, stiffness: float = 0.25,
+                            max_translation: float = 1.0e-5) -> HeldTargets:
+        """Moves every layer toward `factor` times its initial spacing from the
+        cell centre and holds it there (recipe.rs:395-416, 1027-1053)."""
+        factor = _positive(factor, "factor")
+        stiffness = _unit_fraction(stiffness, "stiffness")
+        max_translation = _positive(max_translation, "max_translation")
+        self._push(("scale_layers", (factor, stiffness, max_translation)), f"scale layer spacing by {factor}")
+        return HeldTargets(self, "layers")
+
+    def place_layer_above(self, layer: int, *, gap: float, stiffness: float = 0.25,
+                          max_translation: float = 1.0e-5) -> HeldTargets:
+        """Holds the fibers of formation layer `layer` at `gap` above layer
+        `layer - 1`, leaving the layers below free (recipe.rs:418-444,
+        1055-1089)."""
+        layer = int(layer)
+        if layer <= 0:
+            raise ValueError("layer must be greater than zero")
+        gap = _positive(gap, "gap")
+        stiffness = _unit_fraction(stiffness, "stiffness")
+        max_translation = _positive(max_translation, "max_translation")
+        self._push(("place_layer", (layer, gap, stiffness, max_translation)),
+                   f"place layer {layer} above layer {layer - 1}")
+        return HeldTargets(self, "layers")
+
+    def release_layer_placement(self) -> None:
+        self._push(("release_layers", None), "release layer placement")
+
+    def needle_layer(self, layer: int, *, footprint, depth: float, min_fiber_diameter: float | None = None,
+                     stiffness: float = 0.25, max_translation: float = 1.0e-5,
+                     max_translation_over_diameter: float = 0.25) -> HeldTargets:
+        """Pulls one internal vertex of the selected fibers of formation layer
+        `layer` by `depth` against the stack axis (recipe.rs:451-504,
+        1091-1177)."""
+        from . import formation
+        layer = int(layer)
+        depth = _positive(depth, "depth")
+        if min_fiber_diameter is not None:
+            min_fiber_diameter = _positive(min_fiber_diameter, "min_fiber_diameter")
+        stiffness = _unit_fraction(stiffness, "stiffness")
+        max_translation = _positive(max_translation, "max_translation")
+        ratio = _positive(max_translation_over_diameter, "max_translation_over_diameter")
+        if isinstance(footprint, formation.RandomFiberFraction):
+            selection = footprint
+            text = f"a random {footprint.fraction} of its fibers"
+        elif isinstance(footprint, formation.CircularFootprint):
+            if footprint.center is None:
+                plane = [k for k in range(3) if k != self._stack_axis]
+                origin = [np.float32(self._cell.origin[k]) for k in plane]
+                extent = [np.float32(self._cell.lengths[k]) for k in plane]
+                center = formation.random_footprint_center(footprint.seed or 0, layer, origin, extent)
+            else:
+                center = tuple(np.float32(c) for c in footprint.center)
+            selection = (center, np.float32(footprint.diameter))
+            text = f"a {footprint.diameter:.3e} circle at [{float(center[0]):.3e}, {float(center[1]):.3e}]"
+        else:
+            raise TypeError("footprint must be a CircularFootprint or RandomFiberFraction")
+        needling = formation.Needling(layer, selection, min_fiber_diameter, depth, stiffness, max_translation, ratio)
+        self._push(("needle", needling), f"needle layer {layer} through {text}")
+        return HeldTargets(self, "needles")
+
+    def release_needles(self) -> None:
+        self._push(("release_needles", None), "release needles")
 
     def operations(self) -> list[str]:
         return list(self._descriptions)
@@ -786,14 +1075,24 @@ class Recipe:
         bend = np.array([f.material.min_bend_radius or 0.0 for f in self._fibers])
         if any(f.material.is_oval for f in self._fibers):
             raise ValueError("oval sections are not supported by the hip backend")
+        steps = layers = None
+        if any(f.step for f in self._fibers):
+            steps = np.array([f.step for f in self._fibers], dtype=np.uint32)
+        if any(f.formation_layer is not None for f in self._fibers):
+            layers = np.array([0xFFFFFFFF if f.formation_layer is None else f.formation_layer
+                               for f in self._fibers], dtype=np.uint32)
         return _native.PackedAssembly(counts, placed, intrinsic, radius, bend, self._cell.origin,
-                                      self._cell.lengths, self._cell.periodic, pin_fiber_ends=pin_fiber_ends)
+                                      self._cell.lengths, self._cell.periodic, pin_fiber_ends=pin_fiber_ends,
+                                      formation_steps=steps, formation_layers=layers)
 
     def run(self, settings: RelaxationSettings | None = None, *, warm_start=None) -> RunResult:
-        """Simulates every recorded step (recipe.rs:618-723 and the batch
-        controller plugin.rs:1041-1181, tangle_generate/src/recipe.rs:1311-1397).
-        `warm_start` (a tangle_amd.learned.WarmStart) adds learned proposals,
-        which have no counterpart in the reference."""
+        """Runs every recorded operation on one resident device world, then
+        relaxes on until the run's tolerances hold (recipe.rs:618-723,
+        1123-1239; the controller tangle_generate/src/recipe.rs:857-1652 and
+        the batch loop plugin.rs:1041-1181). `warm_start` (a
+        tangle_amd.learned.WarmStart) adds learned proposals, which have no
+        counterpart in the reference."""
+        from . import formation
         if not self._fibers:
             raise ValueError("recipe contains no fibers")
         if not self._operations:
@@ -804,21 +1103,84 @@ class Recipe:
         config = settings.to_config(d_min)
         operations = list(self._operations)
         descriptions = list(self._descriptions)
-        if all(op[0] == "insert" for op in operations):
-            operations.append(("relax_until_converged", settings.max_iterations))
-            descriptions.append("relax until converged")
+        axis = self._stack_axis
+        current_targets = None
+        if any(op[0] in ("scale_layers", "place_layer") for op in operations):
+            initial_targets = formation.layer_targets([f.placed for f in self._fibers],
+                                                      [f.formation_layer for f in self._fibers], axis)
+            current_targets = list(initial_targets)
         packed = self._packed(settings.pin_fiber_ends)
+        starts = np.concatenate([[0], np.cumsum(packed.counts.astype(np.int64))[:-1]])
         run = _RunState(settings, config, warm_start)
+        active_step = None
         with _native.World(packed, config) as world:
             info = world.info()
             run.uploaded = int(info.device_bytes)
-            for index, (kind, amount) in enumerate(operations):
-                if kind == "insert":
-                    continue
-                if kind == "solve":
-                    run.solve(world, index, descriptions[index], *amount)
+            for index, (kind, arg) in enumerate(operations):
+                desc = descriptions[index]
+                if kind == "activate":
+                    segments, vertices = world.activate_formation_step(arg)
+                    active_step = arg
+                    run.converged = False
+                    run.events.append(f"insert fiber groups through step {arg} "
+                                      f"({segments} segments, {vertices} vertices active)")
+                elif kind == "solve":
+                    run.solve(world, index, desc, *arg)
+                elif kind in ("relax_for", "relax_until_converged"):
+                    run.relax(world, index, desc, kind, arg)
+                elif kind == "settle_targets":
+                    run.settle(world, index, desc, *arg)
+                elif kind == "scale_layers":
+                    factor, stiffness, max_translation = arg
+                    center = np.float32(self._cell.origin[axis] + 0.5 * self._cell.lengths[axis])
+                    current_targets = [np.float32(center + np.float32(np.float32(t - center) * np.float32(factor)))
+                                       for t in initial_targets]
+                    world.set_layer_targets(axis, current_targets, 0, len(current_targets) - 1, stiffness,
+                                            max_translation)
+                    run.converged = False
+                    run.events.append(f"move layers to spacing scale {factor:.3f}")
+                elif kind == "place_layer":
+                    layer, gap, stiffness, max_translation = arg
+                    if layer >= len(current_targets):
+                        raise RecipeError(index, desc, run.total,
+                                          f"cannot place layer {layer}: only {len(current_targets)} layers exist")
+                    current_targets[layer] = np.float32(current_targets[layer - 1] + np.float32(gap))
+                    world.set_layer_targets(axis, current_targets, layer, layer, stiffness, max_translation)
+                    run.converged = False
+                    run.events.append(f"place layer {layer} at gap {gap:.3e} above layer {layer - 1}")
+                elif kind == "release_layers":
+                    world.clear_layer_targets()
+                    run.converged = False
+                    run.events.append("release manufacturing-layer targets")
+                elif kind == "needle":
+                    n = arg
+                    if active_step is None or active_step < n.layer:
+                        raise RecipeError(index, desc, run.total,
+                                          f"cannot needle layer {n.layer}: it has not been inserted yet")
+                    positions = world.download_positions().reshape(-1, 3)
+                    run.downloaded += positions.nbytes
+                    lower = [np.float32(o) for o in self._cell.origin]
+                    upper = [np.float32(o + l) for o, l in zip(self._cell.origin, self._cell.lengths)]
+                    vertices = formation.select_needle_vertices(self._fibers, positions, starts, active_step, axis,
+                                                                self._cell.periodic, lower, upper, n)
+                    world.clear_vertex_targets()
+                    if vertices:
+                        owner = np.searchsorted(starts, np.asarray(vertices), side="right") - 1
+                        diameter = min(2.0 * self._fibers[int(f)].radius for f in owner)
+                        limit = np.float32(np.float32(diameter) * np.float32(n.max_translation_over_diameter))
+                        world.set_vertex_targets(axis, vertices, -n.depth, n.stiffness,
+                                                 min(np.float32(n.max_translation), limit))
+                    run.converged = False
+                    run.needling.append((n.layer, len(vertices), n.depth))
+                    run.events.append(f"needle {len(vertices)} vertices in layer {n.layer} through depth "
+                                      f"{n.depth:.3e}")
+                elif kind == "release_needles":
+                    world.clear_vertex_targets()
+                    run.converged = False
+                    run.events.append("release needle targets")
                 else:
-                    run.relax(world, index, descriptions[index], kind, amount)
+                    raise AssertionError(kind)
+            run.finish(world)
             positions = world.download_positions()
             run.downloaded += positions.nbytes
             rebuilds = int(world.info().neighbor_list_rebuilds)
@@ -835,6 +1197,7 @@ class Recipe:
                                   f"{run.proposals_reverted} reverted"] if warm_start else []),
             warnings=run.warnings, neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
             positions=positions, vertex_counts=packed.counts.copy(),
+            post_recipe_iterations=run.post_recipe_iterations, needling=list(run.needling),
         )
 
 

@@ -24,6 +24,10 @@ ORIENT_ISOTROPIC = 0
 ORIENT_PLANAR = 1
 ORIENT_ALIGNED = 2
 ORIENT_SCHLADITZ = 3
+ORIENT_LAYERED_BIAXIAL = 4
+POSITION_UNIFORM = 0
+POSITION_LAYERED = 1
+POSITION_GRADIENT = 2
 
 _u32p = C.POINTER(C.c_uint32)
 _f64p = C.POINTER(C.c_double)
@@ -131,6 +135,16 @@ class PopulationSpec(C.Structure):
         ("cell_origin", C.c_double * 3),
         ("cell_lengths", C.c_double * 3),
         ("cell_periodic", C.c_uint32 * 3),
+        ("position", C.c_uint32),
+        ("position_axis", C.c_uint32),
+        ("position_layers", C.c_uint32),
+        ("position_toward_high", C.c_uint32),
+        ("position_jitter_fraction", C.c_double),
+        ("position_exponent", C.c_double),
+        ("orientation_primary_fraction", C.c_double),
+        ("orientation_cross_fraction", C.c_double),
+        ("orientation_in_plane_deviation", C.c_double),
+        ("orientation_layer_seed", C.c_uint64),
     ]
 
 
@@ -163,7 +177,8 @@ SIGNATURES = {
     "tgh_world_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32]),
     "tgh_world_destroy": (None, [C.c_void_p]),
     "tgh_last_error": (C.c_char_p, []),
-    "tgh_generate_population": (C.c_int, [C.POINTER(PopulationSpec), _f64p, _f64p, _f64p, C.c_size_t]),
+    "tgh_generate_population": (C.c_int, [C.POINTER(PopulationSpec), _f64p, _f64p, _f64p, C.POINTER(C.c_uint32),
+                                          C.c_size_t]),
     "tgh_mlp_parameter_count": (C.c_size_t, []),
     "tgh_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "tgh_mlp_destroy": (None, [C.c_void_p]),
@@ -444,15 +459,18 @@ class World:
 
 
 def generate_population(spec: PopulationSpec):
-    """Runs tgh_generate_population; returns (intrinsic, placed, radius)."""
+    """Runs tgh_generate_population; returns (intrinsic, placed, radius,
+    formation_layer), the last 0xffffffff for unlayered fibers."""
     nv = int(spec.segments_per_fiber) + 1
     n = int(spec.count)
     intrinsic = np.empty((n * nv, 3), dtype=np.float64)
     placed = np.empty((n * nv, 3), dtype=np.float64)
     radius = np.empty(n, dtype=np.float64)
+    layers = np.empty(n, dtype=np.uint32)
     check(
         library().tgh_generate_population(
-            C.byref(spec), ptr(intrinsic, C.c_double), ptr(placed, C.c_double), ptr(radius, C.c_double), n * nv
+            C.byref(spec), ptr(intrinsic, C.c_double), ptr(placed, C.c_double), ptr(radius, C.c_double),
+            ptr(layers, C.c_uint32), n * nv
         )
     )
-    return intrinsic, placed, radius
+    return intrinsic, placed, radius, layers

@@ -1,14 +1,16 @@
 // Seeded biased fiber population generator.
 //
 // Follows generate_biased_fiber_population
-// (crates/tangle_generate/src/population.rs:246-357) for uniform center
-// positions: per fiber, draws (length, radius, amplitude, handedness) with
+// (crates/tangle_generate/src/population.rs:246-357): per fiber, draws the
+// formation layer of a layered position (sample_formation_layer :728-740),
+// then per attempt (length, radius, amplitude, handedness) with
 // ScalarDistribution::sample (constants draw nothing, :27-32), builds the
 // local sinusoidal centreline (local_centerline :554-565), rejects it when its
 // discrete curvature exceeds the bend limit (maximum_polyline_curvature,
 // crates/tangle_core/src/curvature.rs:20-67), samples a direction
 // (sample_orientation :567-655), rotates the centreline onto it and samples
-// a contained center (sample_contained_center :657-726). SplitMix64 is the
+// a contained center (sample_contained_center :657-726) with its uniform,
+// layered or density-gradient coordinate along the position axis. SplitMix64 is the
 // published generator (:795-813). f64 throughout; sin/cos/acos/pow are the C
 // library's, the functions Rust's f64 methods call on Linux.
 //
@@ -42,6 +44,17 @@ struct SplitMix64 {
         return (double)(v >> 11) * (1.0 / (double)(uint64_t(1) << 53));
     }
 };
+
+// splitmix64_unit (population.rs:746-752): one SplitMix64 step from `value`.
+inline double splitmix64_unit(uint64_t value) {
+    SplitMix64 r{value};
+    return r.unit();
+}
+
+// layer_reference_angle (population.rs:742-744)
+inline double layer_reference_angle(uint64_t seed, uint32_t layer) {
+    return M_PI * splitmix64_unit(seed ^ ((uint64_t)layer * 0x9E3779B97F4A7C15ull));
+}
 
 inline double sample_scalar(double lo, double hi, SplitMix64 &r) {
     if (lo == hi) return lo;  // ScalarDistribution::Constant draws nothing
@@ -91,9 +104,26 @@ double max_polyline_curvature(const std::vector<std::array<double, 3>> &pts) {
     return best;
 }
 
-std::array<double, 3> sample_direction(const tgh_population_spec &s, SplitMix64 &r) {
+std::array<double, 3> sample_direction(const tgh_population_spec &s, uint32_t layer, SplitMix64 &r) {
     const double pi = M_PI;
     switch (s.orientation) {
+        case TGH_ORIENT_LAYERED_BIAXIAL: {
+            auto n = normalize3({s.orientation_vector[0], s.orientation_vector[1], s.orientation_vector[2]});
+            auto a = perpendicular(n);
+            auto b = normalize3(cross3(n, a));
+            const double reference = layer_reference_angle(s.orientation_layer_seed, layer);
+            const double selection = r.unit();
+            double azimuth;
+            if (selection < s.orientation_primary_fraction)
+                azimuth = reference + s.orientation_in_plane_deviation * (2.0 * r.unit() - 1.0);
+            else if (selection < s.orientation_primary_fraction + s.orientation_cross_fraction)
+                azimuth = reference + 0.5 * pi + s.orientation_in_plane_deviation * (2.0 * r.unit() - 1.0);
+            else
+                azimuth = 2.0 * pi * r.unit();
+            double tilt = s.orientation_angle * (2.0 * r.unit() - 1.0);
+            return add3(scale3(add3(scale3(a, std::cos(azimuth)), scale3(b, std::sin(azimuth))), std::cos(tilt)),
+                        scale3(n, std::sin(tilt)));
+        }
         case TGH_ORIENT_PLANAR: {
             auto n = normalize3({s.orientation_vector[0], s.orientation_vector[1], s.orientation_vector[2]});
             auto a = perpendicular(n);
@@ -135,8 +165,11 @@ std::array<double, 3> sample_direction(const tgh_population_spec &s, SplitMix64 
     }
 }
 
+bool unit_interval(double v) { return std::isfinite(v) && v >= 0.0 && v <= 1.0; }
+bool quarter_turn(double v) { return std::isfinite(v) && v >= 0.0 && v <= 0.5 * M_PI; }
+
 int generate(const tgh_population_spec *spec, double *intrinsic, double *placed, double *radius_out,
-             size_t vertex_capacity) {
+             uint32_t *layer_out, size_t vertex_capacity) {
     if (!spec || !intrinsic || !placed || !radius_out) {
         return tgh::fail(TGH_ERR_INVALID, "null argument");
     }
@@ -153,6 +186,28 @@ int generate(const tgh_population_spec *spec, double *intrinsic, double *placed,
     }
     if (s.orientation == TGH_ORIENT_SCHLADITZ && !(s.orientation_beta > 0.0 && std::isfinite(s.orientation_beta))) {
         return tgh::fail(TGH_ERR_INVALID, "Schladitz beta must be positive and finite");
+    }
+    // validate_spec (population.rs:392-492)
+    if (s.orientation == TGH_ORIENT_LAYERED_BIAXIAL) {
+        if (!(s.orientation_primary_fraction >= 0.0) || !(s.orientation_cross_fraction >= 0.0) ||
+            !std::isfinite(s.orientation_primary_fraction) || !std::isfinite(s.orientation_cross_fraction) ||
+            s.orientation_primary_fraction + s.orientation_cross_fraction > 1.0)
+            return tgh::fail(TGH_ERR_INVALID, "layered biaxial fractions must be non-negative with sum <= 1");
+        if (!quarter_turn(s.orientation_in_plane_deviation) || !quarter_turn(s.orientation_angle))
+            return tgh::fail(TGH_ERR_INVALID, "layered biaxial angles must lie in [0, pi/2]");
+        if (s.position != TGH_POSITION_LAYERED)
+            return tgh::fail(TGH_ERR_INVALID, "layered biaxial orientation requires a layered position");
+    } else if (s.orientation > TGH_ORIENT_LAYERED_BIAXIAL) {
+        return tgh::fail(TGH_ERR_INVALID, "unknown orientation distribution");
+    }
+    if (s.position == TGH_POSITION_LAYERED) {
+        if (s.position_axis > 2 || s.position_layers == 0 || !unit_interval(s.position_jitter_fraction))
+            return tgh::fail(TGH_ERR_INVALID, "invalid layered position distribution");
+    } else if (s.position == TGH_POSITION_GRADIENT) {
+        if (s.position_axis > 2 || !(s.position_exponent > 0.0) || !std::isfinite(s.position_exponent))
+            return tgh::fail(TGH_ERR_INVALID, "invalid density-gradient position distribution");
+    } else if (s.position != TGH_POSITION_UNIFORM) {
+        return tgh::fail(TGH_ERR_INVALID, "unknown position distribution");
     }
     for (int k = 0; k < 3; ++k)
         if (!(s.cell_lengths[k] > 0.0)) {
@@ -172,6 +227,10 @@ int generate(const tgh_population_spec *spec, double *intrinsic, double *placed,
     std::vector<std::array<double, 3>> local(nv), offsets(nv);
     const uint32_t attempts = s.max_attempts_per_fiber ? s.max_attempts_per_fiber : 128u;
     for (uint32_t fiber = 0; fiber < s.count; ++fiber) {
+        uint32_t layer = 0xffffffffu;  // sample_formation_layer (population.rs:728-740)
+        if (s.position == TGH_POSITION_LAYERED)
+            layer = (uint32_t)std::fmin(std::floor(rng.unit() * (double)s.position_layers),
+                                        (double)s.position_layers - 1.0);
         bool placed_ok = false;
         for (uint32_t attempt = 0; attempt < attempts && !placed_ok; ++attempt) {
             double length = sample_scalar(s.length_min, s.length_max, rng);
@@ -186,7 +245,7 @@ int generate(const tgh_population_spec *spec, double *intrinsic, double *placed,
             if (s.min_bend_radius > 0.0 &&
                 max_polyline_curvature(local) > (1.0 / s.min_bend_radius) * (1.0 + 1.0e-10))
                 continue;
-            auto dir = sample_direction(s, rng);
+            auto dir = sample_direction(s, layer, rng);
             auto ta = perpendicular(dir);
             auto tb = normalize3(cross3(dir, ta));
             for (uint32_t i = 0; i < nv; ++i)
@@ -211,6 +270,19 @@ int generate(const tgh_population_spec *spec, double *intrinsic, double *placed,
             if (!feasible) continue;
             double center[3];
             for (int k = 0; k < 3; ++k) center[k] = lower[k] + (upper[k] - lower[k]) * rng.unit();
+            if (s.position == TGH_POSITION_LAYERED) {
+                const uint32_t k = s.position_axis;
+                const double spacing = (upper[k] - lower[k]) / (double)s.position_layers;
+                const double jitter = s.position_jitter_fraction * (rng.unit() - 0.5) * spacing;
+                center[k] = std::fmin(std::fmax(lower[k] + ((double)layer + 0.5) * spacing + jitter, lower[k]),
+                                      upper[k]);
+            } else if (s.position == TGH_POSITION_GRADIENT) {
+                const uint32_t k = s.position_axis;
+                const double u = rng.unit();
+                const double biased = s.position_toward_high ? 1.0 - std::pow(1.0 - u, s.position_exponent)
+                                                             : std::pow(u, s.position_exponent);
+                center[k] = lower[k] + (upper[k] - lower[k]) * biased;
+            }
             const size_t base = (size_t)fiber * nv;
             for (uint32_t i = 0; i < nv; ++i)
                 for (int k = 0; k < 3; ++k) {
@@ -218,6 +290,7 @@ int generate(const tgh_population_spec *spec, double *intrinsic, double *placed,
                     placed[3 * (base + i) + k] = center[k] + offsets[i][k];
                 }
             radius_out[fiber] = radius;
+            if (layer_out) layer_out[fiber] = layer;
             placed_ok = true;
         }
         if (!placed_ok) {
@@ -230,6 +303,6 @@ int generate(const tgh_population_spec *spec, double *intrinsic, double *placed,
 }  // namespace
 
 extern "C" int tgh_generate_population(const tgh_population_spec *spec, double *intrinsic, double *placed,
-                                       double *radius, size_t vertex_capacity) {
-    return generate(spec, intrinsic, placed, radius, vertex_capacity);
+                                       double *radius, uint32_t *formation_layer, size_t vertex_capacity) {
+    return generate(spec, intrinsic, placed, radius, formation_layer, vertex_capacity);
 }

@@ -153,3 +153,79 @@ def test_formation_rejects_bad_targets(hip_device):
             w.set_vertex_targets(2, [10_000], 1e-6)
         with pytest.raises(ValueError):
             w.set_vertex_targets(0, [0], 1e-6, max_translation=-1.0)
+
+
+def needled_felt_toy(layer_count=6):
+    """The reference's six-layer needled felt toy
+    (crates/tangle_python/python/examples/native/needled_felt_toy.py:14-84)
+    up to its final compaction, which is out of this scope: plies placed a
+    gap above the last one and needled through the stack."""
+    import math
+    import tangle_amd as tg
+    gap = 0.040
+    fiber = tg.Material("needled felt fiber", diameter=0.037, min_bend_radius=0.050)
+    pop = tg.FiberPopulation(material=fiber, count=180, segments_per_fiber=10, seed=20_260_929, length=(0.36, 0.48),
+                             diameter=(0.032, 0.042), curvature_amplitude=(0.0, 0.006),
+                             orientation=tg.PlanarOrientation(max_tilt=math.radians(6.0)),
+                             position=tg.LayeredPosition(layer_count, jitter_fraction=0.12),
+                             max_attempts_per_fiber=1_024)
+    cell = tg.Cell([1.0, 1.0, 0.32])
+    population = tg.generate_fiber_population(cell, pop, name="batting")
+    recipe = tg.Recipe(cell)
+    recipe.insert(population.select_layer(0))
+    recipe.insert(population.select_layer(1))
+    recipe.relax_for(80)
+    recipe.place_layer_above(1, gap=gap, stiffness=1.0, max_translation=0.08)
+    recipe.relax_for(120)
+    for layer in range(2, layer_count):
+        recipe.insert(population.select_layer(layer))
+        with recipe.place_layer_above(layer, gap=gap, stiffness=1.0, max_translation=0.08):
+            recipe.relax_for(100)
+        with recipe.needle_layer(layer, footprint=tg.RandomFiberFraction(0.30, seed=20_260_930 + layer),
+                                 depth=2.0 * gap, stiffness=1.0, max_translation=0.012,
+                                 max_translation_over_diameter=0.25):
+            recipe.relax_for(160)
+        recipe.relax_for(40)
+    settings = tg.RelaxationSettings(constraint_iterations=4, max_iterations=24_000, iterations_per_batch=20,
+                                     penetration_tolerance=5.0e-4, max_step=0.005)
+    return recipe, settings
+
+
+def test_needled_felt_toy_recipe(hip_device):
+    recipe, settings = needled_felt_toy()
+    result = recipe.run(settings)
+    print(f"[felt toy] iterations={result.iterations} post={result.post_recipe_iterations} "
+          f"pen={result.max_penetration:.3e} curv={result.max_curvature_ratio:.5f} needling={result.needling} "
+          f"wall={result.wall_seconds:.2f}s", flush=True)
+    fixed = 80 + 120 + 4 * (100 + 160 + 40)
+    assert result.iterations - result.post_recipe_iterations == fixed
+    assert len(result.needling) == 4 and all(n[1] > 0 for n in result.needling)
+    assert result.converged and result.max_penetration <= 5.0e-4
+    assert sum(e.startswith("needle ") for e in result.events) == 4
+    # the plies end stacked: each layer's mean height above the last
+    z = [np.mean([p[2] for p in line]) for line in result.centerlines()]
+    layers = [f.formation_layer for f in recipe._fibers]
+    means = [np.mean([zz for zz, l in zip(z, layers) if l == k]) for k in range(6)]
+    assert all(b > a for a, b in zip(means, means[1:]))
+
+
+def test_needles_pull_their_vertices_by_depth(hip_device):
+    """One needle command held until settled moves every selected vertex by
+    the commanded depth against the stack axis (recipe.rs:1091-1177)."""
+    import tangle_amd as tg
+    settings = tg.RelaxationSettings(constraint_iterations=4, max_iterations=4_000, iterations_per_batch=20,
+                                     penetration_tolerance=5.0e-4, max_step=0.005)
+    recipe = tg.Recipe(tg.Cell([1.0, 1.0, 0.32]))
+    fiber = tg.Material("fiber", diameter=0.037, min_bend_radius=0.050)
+    lines = [[[0.1 + 0.08 * i, 0.2 + 0.15 * j, 0.25] for i in range(11)] for j in range(4)]
+    recipe.insert(tg.FiberCollection.from_centerlines(lines, fiber, formation_layer=0))
+    recipe.needle_layer(0, footprint=tg.RandomFiberFraction(1.0, seed=5), depth=0.05, stiffness=1.0,
+                        max_translation=0.01)
+    recipe.settle_targets(tolerance=1e-6, max_iterations=200)
+    recipe.release_needles()
+    result = recipe.run(settings)
+    assert result.needling == [(0, 4, 0.05)]
+    settled = [e for e in result.events if e.startswith("advance formation targets")]
+    assert len(settled) == 1
+    # the command walks at most min(0.01, 0.25 d) = 0.00925 per iteration
+    assert int(settled[0].split(" in ")[1].split()[0]) >= 6
