@@ -34,11 +34,14 @@
 //
 // All float arithmetic must be compiled without contraction
 // (-ffp-contract=off) and with IEEE division and square root, as the Rust
-// reference; cell lists and neighbour pair sets are then bit-exact.
+// reference; cell lists and neighbour pair sets are then bit-exact. The
+// curvature sweep's arc cosine is the host libm's algorithm (acosf_f32.h),
+// so its projections reproduce a host evaluation bit for bit as well.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "acosf_f32.h"
 #include "relax_device.h"
 
 namespace tgh {
@@ -954,7 +957,7 @@ __device__ __forceinline__ bool triplet_projection(float3 pp, float3 pc, float3 
     float violation = min_cos - cosine;
     if (!(violation > 0.f)) return false;
     float sine = sqrtf(fmaxf(1.0f - cosine * cosine, 0.f));
-    if (sine > 1.0e-3f) violation = (acosf(cosine) - acosf(min_cos)) * sine;
+    if (sine > 1.0e-3f) violation = (acosf_f32(cosine) - acosf_f32(min_cos)) * sine;
     if (!(violation > 0.f)) return false;
     float gux = (vnx - cosine * unx) / l1, guy = (vny - cosine * uny) / l1, guz = (vnz - cosine * unz) / l1;
     float gvx = (unx - cosine * vnx) / l2, gvy = (uny - cosine * vny) / l2, gvz = (unz - cosine * vnz) / l2;
@@ -1024,6 +1027,14 @@ __device__ __forceinline__ float quad_bcast(float x, int k) {
 }
 __device__ __forceinline__ float axis_sum(float x) {  // (x + y) + z over lanes 0, 1, 2 of the quad
     return (quad_bcast(x, 0) + quad_bcast(x, 1)) + quad_bcast(x, 2);
+}
+// Exchange between the two quads of an 8-lane group (lanes 0..3 <-> 4..7),
+// with DPP row shifts by 4 inside a 16-lane row: row_shl:4 gives lane i the
+// value of lane i + 4, row_shr:4 that of lane i - 4.
+__device__ __forceinline__ float swap_quads(float x, bool second) {
+    const int from_next = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x104, 0xF, 0xF, false);
+    const int from_prev = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, false);
+    return __int_as_float(second ? from_prev : from_next);
 }
 
 __device__ __forceinline__ void lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
@@ -1105,17 +1116,89 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, const S
             const float cos_est = uv * ru * rv;
             const float half_est = 0.25f * kmax * (uu * ru + vv * rv);
             const float min_cos_est = 1.0f - 2.0f * half_est * half_est;
-            maybe = kmax > 0.f && !(cos_est - min_cos_est > 1.0e-5f);
+            maybe = (kmax > 0.f) & !(cos_est - min_cos_est > 1.0e-5f);
         }
         const uint64_t ballot = (uint64_t)__ballot(maybe);
         const uint64_t mine = (ballot >> gbase) & gmask;
         if (!done && mine == 0) cursor += L;
         if (ballot == 0) continue;
-        // exact evaluation of each group's first candidate on lanes 0..2
+        // exact evaluation of each group's first candidate
         const uint32_t j = mine ? (uint32_t)__builtin_ctzll(mine) : 0u;
         const uint32_t t = dir == 0 ? cursor + j : trip - 1 - (cursor + j);
         bool hit = false;
-        if (mine != 0 && gl < 3) {
+        if constexpr (L >= 8) {
+            // Two quads per group. Lanes 0..2 hold the axes of the first
+            // segment (previous -> center), lanes 4..6 those of the second
+            // (center -> next): one square root, two divisions and one arc
+            // cosine serve both segments, and the values the formula pairs
+            // are swapped between the quads. Every operation and its operand
+            // order are the sequential projection's.
+            const bool second = (gl & 4u) != 0u;
+            const uint32_t ax = gl & 3u;
+            if (mine != 0 && ax < 3u && gl < 8u) {
+                const uint32_t o = second ? 1u : 0u;
+                const float a0 = Pf[4 * (t + o) + ax], a1 = Pf[4 * (t + o + 1) + ax];
+                const float e0 = P[t + o].w, e1 = P[t + o + 1].w;
+                const float2 kwp = KW[t], kwc = KW[t + 1], kwn = KW[t + 2];
+                const float kmax = kwc.x * (1.0f - margin);
+                const float d = a1 - a0;
+                const float len = sqrtf(axis_sum(d * d));
+                const float dn = d / len;
+                const float dn_o = swap_quads(dn, second), len_o = swap_quads(len, second);
+                const float l1 = second ? len_o : len, l2 = second ? len : len_o;
+                const float un = second ? dn_o : dn, vn = second ? dn : dn_o;
+                const float cosine = rclamp(axis_sum(un * vn), -1.f, 1.f);
+                const float sine_half = 0.25f * kmax * (l1 + l2);
+                const float min_cos = 1.0f - 2.0f * sine_half * sine_half;
+                float violation = min_cos - cosine;
+                bool ok = kmax > 0.f && l1 > 1.0e-7f && l2 > 1.0e-7f && sine_half < 1.0f && violation > 0.f;
+                const float sine = sqrtf(fmaxf(1.0f - cosine * cosine, 0.f));
+                // the first quad takes the turning angle, the second the limit
+                const float angle = acosf_f32(second ? min_cos : cosine);
+                const float angle_o = swap_quads(angle, second);
+                const float turn = second ? angle_o : angle, limit_angle = second ? angle : angle_o;
+                if (sine > 1.0e-3f) violation = (turn - limit_angle) * sine;
+                ok = ok && violation > 0.f;
+                // (vn - cos un) / l1 in the first quad, (un - cos vn) / l2 in the second
+                const float g = (dn_o - cosine * dn) / len;
+                const float g_o = swap_quads(g, second);
+                const float gu = second ? g_o : g, gv = second ? g : g_o;
+                const float gc = gu - gv;
+                const float fwp = kwp.y, fwc = kwc.y, fwn = kwn.y;
+                const float sg = axis_sum(g * g), sg_o = swap_quads(sg, second);
+                const float suu = second ? sg_o : sg, svv = second ? sg : sg_o;
+                const float den = fwp * suu + fwc * axis_sum(gc * gc) + fwn * svv;
+                ok = ok && den > 1.0e-12f;
+                if (ok) {
+                    const float scale = stiffness * violation / den;
+                    float *W = (float *)WR;
+                    if (!second) {
+                        const float m_p = a0 - fwp * scale * gu;
+                        const float m_c = a1 + fwc * scale * gc;
+                        float v_p = m_p, v_c = m_c;
+                        if (walled) {
+                            v_p = rclamp(m_p, lo + e0, hi - e0);
+                            v_c = rclamp(m_c, lo + e1, hi - e1);
+                            if (track_walls) {
+                                if (fwp > 0.f) W[4 * t + ax] += m_p - v_p;
+                                if (fwc > 0.f) W[4 * (t + 1) + ax] += m_c - v_c;
+                            }
+                        }
+                        if (fwp > 0.f) Pf[4 * t + ax] = v_p;
+                        if (fwc > 0.f) Pf[4 * (t + 1) + ax] = v_c;
+                    } else {
+                        const float m_n = a1 + fwn * scale * gv;
+                        float v_n = m_n;
+                        if (walled) {
+                            v_n = rclamp(m_n, lo + e1, hi - e1);
+                            if (track_walls && fwn > 0.f) W[4 * (t + 2) + ax] += m_n - v_n;
+                        }
+                        if (fwn > 0.f) Pf[4 * (t + 2) + ax] = v_n;
+                    }
+                }
+                hit = ok;
+            }
+        } else if (mine != 0 && gl < 3) {
             float pp = Pf[4 * t + k], pc = Pf[4 * (t + 1) + k], pn = Pf[4 * (t + 2) + k];
             const float ep = P[t].w, ec = P[t + 1].w, en = P[t + 2].w;
             const float2 kwp = KW[t], kwc = KW[t + 1], kwn = KW[t + 2];
@@ -1130,7 +1213,10 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, const S
             float violation = min_cos - cosine;
             bool ok = kmax > 0.f && l1 > 1.0e-7f && l2 > 1.0e-7f && sine_half < 1.0f && violation > 0.f;
             const float sine = sqrtf(fmaxf(1.0f - cosine * cosine, 0.f));
-            if (sine > 1.0e-3f) violation = (acosf(cosine) - acosf(min_cos)) * sine;
+            // one arc cosine per lane: lane 0 the turning angle, lane 1 the limit
+            const float angle = acosf_f32(gl == 1 ? min_cos : cosine);
+            const float turn = quad_bcast(angle, 0), limit_angle = quad_bcast(angle, 1);
+            if (sine > 1.0e-3f) violation = (turn - limit_angle) * sine;
             ok = ok && violation > 0.f;
             const float gu = (vn - cosine * un) / l1;
             const float gv = (un - cosine * vn) / l2;
@@ -1245,7 +1331,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, const 
                     const float cos_est = uv * ru * rv;
                     const float half_est = 0.25f * kmax * (uu * ru + vv * rv);
                     const float min_cos_est = 1.0f - 2.0f * half_est * half_est;
-                    maybe = kmax > 0.f && !(cos_est - min_cos_est > 1.0e-5f);
+                    maybe = (kmax > 0.f) & !(cos_est - min_cos_est > 1.0e-5f);
                 }
                 const uint64_t cand = (uint64_t)__ballot(maybe);
                 if (cand == 0) {
@@ -1281,7 +1367,9 @@ __global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, const 
                         float violation = min_cos - cosine;
                         bool ok = kmax > 0.f && l1 > 1.0e-7f && l2 > 1.0e-7f && sine_half < 1.0f && violation > 0.f;
                         const float sine = sqrtf(fmaxf(1.0f - cosine * cosine, 0.f));
-                        if (sine > 1.0e-3f) violation = (acosf(cosine) - acosf(min_cos)) * sine;
+                        const float angle = acosf_f32(lane == 1 ? min_cos : cosine);
+                        const float turn = quad_bcast(angle, 0), limit_angle = quad_bcast(angle, 1);
+                        if (sine > 1.0e-3f) violation = (turn - limit_angle) * sine;
                         ok = ok && violation > 0.f;
                         const float gu = (vn - cosine * un) / l1;
                         const float gv = (un - cosine * vn) / l2;

@@ -138,8 +138,8 @@ def test_bimodal_formation_positions_match(hip_device):
         assert sg.total_iterations == so.total_iterations
         pg, po = gpu.download_positions(), ref.download_positions()
         assert float(np.max(np.abs(pg.astype(np.float64) - po))) <= 1e-4 * 7e-6
-        (lg, _), (lr, _) = gpu.formation_target_error(), ref.formation_target_error()
-        assert abs(lg - lr) <= 1e-4 * 7e-6
+        assert np.array_equal(pg.view(np.uint32), po.view(np.uint32))
+        assert gpu.formation_target_error() == ref.formation_target_error()
 
 
 def test_formation_rejects_bad_targets(hip_device):

@@ -250,7 +250,10 @@ def test_bimodal_positions_match(hip_device, scale):
         sg = gpu.run_batch(cfg, 50)
         so = ref.run_batch(cfg, 50)
         assert sg.total_iterations == so.total_iterations == 50
-        assert max_abs(gpu.download_positions(), ref.download_positions()) <= 1e-4 * 7e-6
+        pg, po = gpu.download_positions(), ref.download_positions()
+        assert max_abs(pg, po) <= 1e-4 * 7e-6
+        # the curvature sweep's acosf is the host libm's algorithm: bit for bit
+        assert np.array_equal(pg.view(np.uint32), po.view(np.uint32))
 
 
 @pytest.mark.parametrize("scale", [0.05])
