@@ -193,6 +193,41 @@ int tgh_world_clear_vertex_targets(tgh_world *world);
 int tgh_world_formation_target_error(tgh_world *world, float *layer_error, float *vertex_error,
                                      uint32_t *has_layer, uint32_t *has_vertex);
 
+/* ---- checkpoint and restore (DeviceWorldCheckpoint, world.rs:281-304, 692-823) ----
+ * The persistent state of a one-specimen world: positions, fiber activity,
+ * wall reactions, the lifetime iteration count and the held formation
+ * targets. Broad-phase and correction scratch is left out; the first batch
+ * after a restore rebuilds it, as the reference's restore does.
+ * tgh_world_checkpoint_sizes fills the counts (fiber_count, vertex_count,
+ * layer_target_count, vertex_target_count) and leaves the pointers alone;
+ * tgh_world_checkpoint then fills a checkpoint whose counts are those and
+ * whose arrays hold them. tgh_world_restore creates a world from the
+ * assembly the checkpointed world was created from and puts it in the
+ * checkpoint's state (DeviceFiberWorld::restore). */
+typedef struct tgh_checkpoint {
+    uint64_t total_iterations;
+    uint32_t fiber_count;
+    uint32_t vertex_count;
+    float *positions;             /* [3 * vertex_count] */
+    uint32_t *fiber_active;       /* [fiber_count]: 1 active, 0 not yet inserted */
+    float *wall_reactions;        /* [3 * vertex_count]: zero on periodic axes */
+    uint32_t layer_target_count;  /* 0: no layer target held */
+    uint32_t layer_axis, layer_first, layer_last;
+    float layer_stiffness, layer_max_translation;
+    float *layer_targets;         /* [layer_target_count] */
+    uint32_t vertex_target_count; /* 0: no vertex target held */
+    uint32_t vertex_axis;
+    float vertex_stiffness, vertex_max_translation;
+    uint32_t *vertex_indices;     /* [vertex_target_count] */
+    float *vertex_targets;        /* [vertex_target_count] target coordinates */
+    float *vertex_commands;       /* [vertex_target_count] commanded coordinates */
+} tgh_checkpoint;
+
+int tgh_world_checkpoint_sizes(tgh_world *world, tgh_checkpoint *out);
+int tgh_world_checkpoint(tgh_world *world, tgh_checkpoint *out);
+int tgh_world_restore(const tgh_assembly *assembly, const tgh_config *config, const tgh_checkpoint *checkpoint,
+                      tgh_world **out);
+
 /* Launch shape of the relaxation kernels (no counterpart in the reference,
  * whose launch shapes are fixed). Any setting gives the same positions bit
  * for bit; tgh_world_set_tuning validates the settings and drops captured

@@ -23,6 +23,7 @@ from typing import Iterable, Sequence
 import numpy as np
 
 from . import _native
+from .checkpoint import CheckpointError, CheckpointSettings
 from .formation import CircularFootprint, RandomFiberFraction
 from .units import um
 
@@ -49,6 +50,8 @@ __all__ = [
     "CircularFootprint",
     "RandomFiberFraction",
     "HeldTargets",
+    "CheckpointSettings",
+    "CheckpointError",
     "relax_batch",
     "um",
 ]
@@ -669,6 +672,11 @@ class RunResult:
     vertex_counts: np.ndarray
     post_recipe_iterations: int = 0
     needling: list = field(default_factory=list)  # (layer, selected vertices, depth) per needle command
+    resumed: bool = False
+    resumed_iteration: int | None = None
+    checkpoint_saves: int = 0
+    last_checkpoint_iteration: int | None = None
+    last_checkpoint_bytes: int | None = None
 
     def centerlines(self):
         out, start = [], 0
@@ -705,6 +713,12 @@ class _RunState:
         self.proposals_reverted = 0
         self.post_recipe_iterations = 0
         self.needling: list = []
+        # operation in progress and the iteration it started at (checkpoints)
+        self.op_index = 0
+        self.op_begin = 0
+        self.checkpoint_interval = 0
+        self.checkpoint_hook = None
+        self.last_saved = None
 
     def _propose(self, world, config):
         """Learned warm start: every `interval` iterations of the first
@@ -756,7 +770,21 @@ class _RunState:
         self.total = int(status.total_iterations)
         self.status = status
         self.converged = bool(status.converged)
+        if self.checkpoint_hook is not None and self.total > 0 and self.last_saved != self.total:
+            n = self.checkpoint_interval
+            crossed = self.total >= n if self.last_saved is None else self.total // n > self.last_saved // n
+            if crossed:
+                self.save_checkpoint()
         return status
+
+    def save_checkpoint(self, failure: str | None = None):
+        """Saves the world and the recipe's progress (save_checkpoint,
+        tangle_checkpoint/src/lib.rs:318-375): when an interval boundary is
+        crossed, when the run ends and when it fails."""
+        if self.checkpoint_hook is None or (self.last_saved == self.total and failure is None):
+            return
+        self.checkpoint_hook(failure)
+        self.last_saved = self.total
 
     def _proposal_window(self, world, config, batch):
         """Proposes when one is due and shortens the batch so that the next
@@ -778,8 +806,8 @@ class _RunState:
         return (f"penetration {s.max_penetration:.3e}, curvature ratio {s.max_curvature_ratio:.6f}"
                 if s is not None else "no measurement")
 
-    def relax(self, world, index, description, kind, amount):
-        begin = self.total
+    def relax(self, world, index, description, kind, amount, begin=None):
+        begin = self.total if begin is None else begin
         config = _native.Config.from_buffer_copy(self.config)
         force_full = kind == "relax_for"
         config.force_full_iterations = 1 if (force_full or self.settings.force_full_iterations) else 0
@@ -802,10 +830,10 @@ class _RunState:
             if status.cell_list_overflow:
                 raise RecipeError(index, description, self.total, "cell list overflow")
 
-    def settle(self, world, index, description, tolerance: float, maximum: int):
+    def settle(self, world, index, description, tolerance: float, maximum: int, begin=None):
         """RelaxUntilTargetsReached (recipe.rs:1521-1596): full batches until
         the largest layer-centre or needle residual is within `tolerance`."""
-        begin = self.total
+        begin = self.total if begin is None else begin
         config = _native.Config.from_buffer_copy(self.config)
         config.force_full_iterations = 1
         tol = np.float32(tolerance)
@@ -829,14 +857,14 @@ class _RunState:
             if status.cell_list_overflow:
                 raise RecipeError(index, description, self.total, "cell list overflow")
 
-    def finish(self, world):
+    def finish(self, world, begin=None):
         """After the last operation the targets are released and relaxation
         continues, unconstrained, until the run's tolerances hold or its
         iteration budget is spent (recipe.rs:893-915, plugin.rs:1041-1181);
         not converging here is not an error."""
         world.clear_layer_targets()
         world.clear_vertex_targets()
-        begin = self.total
+        begin = self.total if begin is None else begin
         config = _native.Config.from_buffer_copy(self.config)
         config.force_full_iterations = 1 if self.settings.force_full_iterations else 0
         self.converged = False
@@ -848,8 +876,8 @@ class _RunState:
                 break
         self.post_recipe_iterations = self.total - begin
 
-    def solve(self, world, index, description, policy: "SolvePolicy", overrides):
-        begin = self.total
+    def solve(self, world, index, description, policy: "SolvePolicy", overrides, begin=None):
+        begin = self.total if begin is None else begin
         config = _native.Config.from_buffer_copy(self.config)
         if overrides is not None:
             config = overrides.apply(config)
@@ -1085,102 +1113,113 @@ class Recipe:
                                       self._cell.lengths, self._cell.periodic, pin_fiber_ends=pin_fiber_ends,
                                       formation_steps=steps, formation_layers=layers)
 
-    def run(self, settings: RelaxationSettings | None = None, *, warm_start=None) -> RunResult:
+    def _fingerprint(self, packed: _native.PackedAssembly) -> str:
+        """Identifies the fibers a checkpoint belongs to."""
+        import hashlib
+        h = hashlib.sha256()
+        for arr in (packed.counts, packed.intrinsic, packed.placed, packed.radius,
+                    packed.min_bend_radius if packed.min_bend_radius is not None else np.zeros(0),
+                    packed.formation_steps if packed.formation_steps is not None else np.zeros(0),
+                    packed.formation_layers if packed.formation_layers is not None else np.zeros(0)):
+            h.update(np.ascontiguousarray(arr).tobytes())
+        return h.hexdigest()
+
+    def run(self, settings: RelaxationSettings | None = None, *, warm_start=None,
+            checkpoint: "CheckpointSettings | None" = None) -> RunResult:
         """Runs every recorded operation on one resident device world, then
         relaxes on until the run's tolerances hold (recipe.rs:618-723,
         1123-1239; the controller tangle_generate/src/recipe.rs:857-1652 and
-        the batch loop plugin.rs:1041-1181). `warm_start` (a
-        tangle_amd.learned.WarmStart) adds learned proposals, which have no
-        counterpart in the reference."""
+        the batch loop plugin.rs:1041-1181). With `checkpoint`, the world and
+        the recipe's progress are saved every `interval_iterations`, at the
+        end and on failure, and a run with `checkpoint.resume` continues the
+        saved one (tangle_checkpoint/src/lib.rs:196-375); the recipe must be
+        the one that wrote it. `warm_start` (a tangle_amd.learned.WarmStart)
+        adds learned proposals, which have no counterpart in the reference."""
+        from . import checkpoint as ckpt
         from . import formation
         if not self._fibers:
             raise ValueError("recipe contains no fibers")
         if not self._operations:
             raise ValueError("recipe contains no operations")
+        if checkpoint is not None and not isinstance(checkpoint, ckpt.CheckpointSettings):
+            raise TypeError("checkpoint must be CheckpointSettings")
         settings = settings or RelaxationSettings()
         started = time.perf_counter()
         d_min = min(2.0 * f.radius for f in self._fibers)
         config = settings.to_config(d_min)
         operations = list(self._operations)
         descriptions = list(self._descriptions)
+        if all(op[0] == "activate" for op in operations):
+            # a recipe that only inserts fibers relaxes them (recipe.rs:681-690)
+            operations.append(("relax_until_converged", settings.max_iterations))
+            descriptions.append(f"relax until converged (up to {settings.max_iterations} iterations)")
         axis = self._stack_axis
-        current_targets = None
+        initial_targets = None
         if any(op[0] in ("scale_layers", "place_layer") for op in operations):
             initial_targets = formation.layer_targets([f.placed for f in self._fibers],
                                                       [f.formation_layer for f in self._fibers], axis)
-            current_targets = list(initial_targets)
         packed = self._packed(settings.pin_fiber_ends)
         starts = np.concatenate([[0], np.cumsum(packed.counts.astype(np.int64))[:-1]])
+        fingerprint = self._fingerprint(packed)
         run = _RunState(settings, config, warm_start)
-        active_step = None
-        with _native.World(packed, config) as world:
+        state = {"active_step": None, "targets": None if initial_targets is None else list(initial_targets)}
+        report = {"resumed": False, "resumed_iteration": None, "saves": 0, "last_iteration": None, "last_bytes": None}
+        start_index, first_begin, restored = 0, None, None
+        if checkpoint is not None and checkpoint.resume:
+            path, case_id = checkpoint.source
+            restored, progress = ckpt.load(path, case_id)
+            if progress.get("fingerprint") != fingerprint:
+                raise ckpt.CheckpointError("the checkpoint was written by another recipe (its fibers differ)")
+            run.total = restored.total_iterations
+            run.last_saved = run.total
+            report.update(resumed=True, resumed_iteration=run.total, last_iteration=run.total)
+            if checkpoint.fresh_formation_on_resume:
+                restored.layer_target = restored.vertex_target = None
+            else:
+                if progress.get("failure"):
+                    raise RecipeError(progress["operation"], descriptions[progress["operation"]]
+                                      if progress["operation"] < len(descriptions) else "run", run.total,
+                                      progress["failure"])
+                start_index, first_begin = int(progress["operation"]), int(progress["operation_begin"])
+                run.events = list(progress["events"])
+                run.warnings = list(progress["warnings"])
+                run.needling = [tuple(n) for n in progress["needling"]]
+                run.post_recipe_iterations = int(progress.get("post_recipe_iterations", 0))
+                state["active_step"] = progress["active_step"]
+                if progress["layer_targets"] is not None:
+                    state["targets"] = [np.float32(t) for t in progress["layer_targets"]]
+        world = (_native.World(packed, config) if restored is None
+                 else _native.World.restore(packed, config, restored))
+
+        def save(failure=None):
+            progress = {"fingerprint": fingerprint, "operation": run.op_index, "operation_begin": run.op_begin,
+                        "events": run.events, "warnings": run.warnings, "needling": run.needling,
+                        "active_step": state["active_step"], "post_recipe_iterations": run.post_recipe_iterations,
+                        "layer_targets": None if state["targets"] is None else [float(t) for t in state["targets"]],
+                        "failure": failure}
+            nbytes = ckpt.save(checkpoint.path, checkpoint.case_id, world.checkpoint(), progress)
+            report.update(saves=report["saves"] + 1, last_iteration=run.total, last_bytes=nbytes)
+
+        if checkpoint is not None:
+            run.checkpoint_interval = int(checkpoint.interval_iterations)
+            run.checkpoint_hook = save
+        with world:
             info = world.info()
             run.uploaded = int(info.device_bytes)
-            for index, (kind, arg) in enumerate(operations):
-                desc = descriptions[index]
-                if kind == "activate":
-                    segments, vertices = world.activate_formation_step(arg)
-                    active_step = arg
-                    run.converged = False
-                    run.events.append(f"insert fiber groups through step {arg} "
-                                      f"({segments} segments, {vertices} vertices active)")
-                elif kind == "solve":
-                    run.solve(world, index, desc, *arg)
-                elif kind in ("relax_for", "relax_until_converged"):
-                    run.relax(world, index, desc, kind, arg)
-                elif kind == "settle_targets":
-                    run.settle(world, index, desc, *arg)
-                elif kind == "scale_layers":
-                    factor, stiffness, max_translation = arg
-                    center = np.float32(self._cell.origin[axis] + 0.5 * self._cell.lengths[axis])
-                    current_targets = [np.float32(center + np.float32(np.float32(t - center) * np.float32(factor)))
-                                       for t in initial_targets]
-                    world.set_layer_targets(axis, current_targets, 0, len(current_targets) - 1, stiffness,
-                                            max_translation)
-                    run.converged = False
-                    run.events.append(f"move layers to spacing scale {factor:.3f}")
-                elif kind == "place_layer":
-                    layer, gap, stiffness, max_translation = arg
-                    if layer >= len(current_targets):
-                        raise RecipeError(index, desc, run.total,
-                                          f"cannot place layer {layer}: only {len(current_targets)} layers exist")
-                    current_targets[layer] = np.float32(current_targets[layer - 1] + np.float32(gap))
-                    world.set_layer_targets(axis, current_targets, layer, layer, stiffness, max_translation)
-                    run.converged = False
-                    run.events.append(f"place layer {layer} at gap {gap:.3e} above layer {layer - 1}")
-                elif kind == "release_layers":
-                    world.clear_layer_targets()
-                    run.converged = False
-                    run.events.append("release manufacturing-layer targets")
-                elif kind == "needle":
-                    n = arg
-                    if active_step is None or active_step < n.layer:
-                        raise RecipeError(index, desc, run.total,
-                                          f"cannot needle layer {n.layer}: it has not been inserted yet")
-                    positions = world.download_positions().reshape(-1, 3)
-                    run.downloaded += positions.nbytes
-                    lower = [np.float32(o) for o in self._cell.origin]
-                    upper = [np.float32(o + l) for o, l in zip(self._cell.origin, self._cell.lengths)]
-                    vertices = formation.select_needle_vertices(self._fibers, positions, starts, active_step, axis,
-                                                                self._cell.periodic, lower, upper, n)
-                    world.clear_vertex_targets()
-                    if vertices:
-                        owner = np.searchsorted(starts, np.asarray(vertices), side="right") - 1
-                        diameter = min(2.0 * self._fibers[int(f)].radius for f in owner)
-                        limit = np.float32(np.float32(diameter) * np.float32(n.max_translation_over_diameter))
-                        world.set_vertex_targets(axis, vertices, -n.depth, n.stiffness,
-                                                 min(np.float32(n.max_translation), limit))
-                    run.converged = False
-                    run.needling.append((n.layer, len(vertices), n.depth))
-                    run.events.append(f"needle {len(vertices)} vertices in layer {n.layer} through depth "
-                                      f"{n.depth:.3e}")
-                elif kind == "release_needles":
-                    world.clear_vertex_targets()
-                    run.converged = False
-                    run.events.append("release needle targets")
-                else:
-                    raise AssertionError(kind)
-            run.finish(world)
+            try:
+                for index in range(start_index, len(operations)):
+                    kind, arg = operations[index]
+                    begin = first_begin if index == start_index and first_begin is not None else run.total
+                    run.op_index, run.op_begin = index, begin
+                    self._run_operation(world, run, state, index, descriptions[index], kind, arg, begin,
+                                        initial_targets, starts)
+                resumed_in_finish = start_index == len(operations) and first_begin is not None
+                run.op_index, run.op_begin = len(operations), first_begin if resumed_in_finish else run.total
+                run.finish(world, begin=run.op_begin)
+            except RecipeError as err:
+                run.save_checkpoint(failure=err.reason)
+                raise
+            run.save_checkpoint()
             positions = world.download_positions()
             run.downloaded += positions.nbytes
             rebuilds = int(world.info().neighbor_list_rebuilds)
@@ -1198,7 +1237,80 @@ class Recipe:
             warnings=run.warnings, neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
             positions=positions, vertex_counts=packed.counts.copy(),
             post_recipe_iterations=run.post_recipe_iterations, needling=list(run.needling),
+            resumed=report["resumed"], resumed_iteration=report["resumed_iteration"],
+            checkpoint_saves=report["saves"], last_checkpoint_iteration=report["last_iteration"],
+            last_checkpoint_bytes=report["last_bytes"],
         )
+
+    def _run_operation(self, world, run, state, index, desc, kind, arg, begin, initial_targets, starts):
+        """One recorded operation (control_formation_recipe,
+        tangle_generate/src/recipe.rs:857-1652)."""
+        from . import formation
+        axis = self._stack_axis
+        if kind == "activate":
+            segments, vertices = world.activate_formation_step(arg)
+            state["active_step"] = arg
+            run.converged = False
+            run.events.append(f"insert fiber groups through step {arg} "
+                              f"({segments} segments, {vertices} vertices active)")
+        elif kind == "solve":
+            run.solve(world, index, desc, *arg, begin=begin)
+        elif kind in ("relax_for", "relax_until_converged"):
+            run.relax(world, index, desc, kind, arg, begin=begin)
+        elif kind == "settle_targets":
+            run.settle(world, index, desc, *arg, begin=begin)
+        elif kind == "scale_layers":
+            factor, stiffness, max_translation = arg
+            center = np.float32(self._cell.origin[axis] + 0.5 * self._cell.lengths[axis])
+            state["targets"] = [np.float32(center + np.float32(np.float32(t - center) * np.float32(factor)))
+                                for t in initial_targets]
+            world.set_layer_targets(axis, state["targets"], 0, len(state["targets"]) - 1, stiffness,
+                                    max_translation)
+            run.converged = False
+            run.events.append(f"move layers to spacing scale {factor:.3f}")
+        elif kind == "place_layer":
+            layer, gap, stiffness, max_translation = arg
+            targets = state["targets"]
+            if layer >= len(targets):
+                raise RecipeError(index, desc, run.total,
+                                  f"cannot place layer {layer}: only {len(targets)} layers exist")
+            targets[layer] = np.float32(targets[layer - 1] + np.float32(gap))
+            world.set_layer_targets(axis, targets, layer, layer, stiffness, max_translation)
+            run.converged = False
+            run.events.append(f"place layer {layer} at gap {gap:.3e} above layer {layer - 1}")
+        elif kind == "release_layers":
+            world.clear_layer_targets()
+            run.converged = False
+            run.events.append("release manufacturing-layer targets")
+        elif kind == "needle":
+            n = arg
+            active_step = state["active_step"]
+            if active_step is None or active_step < n.layer:
+                raise RecipeError(index, desc, run.total,
+                                  f"cannot needle layer {n.layer}: it has not been inserted yet")
+            positions = world.download_positions().reshape(-1, 3)
+            run.downloaded += positions.nbytes
+            lower = [np.float32(o) for o in self._cell.origin]
+            upper = [np.float32(o + l) for o, l in zip(self._cell.origin, self._cell.lengths)]
+            vertices = formation.select_needle_vertices(self._fibers, positions, starts, active_step, axis,
+                                                        self._cell.periodic, lower, upper, n)
+            world.clear_vertex_targets()
+            if vertices:
+                owner = np.searchsorted(starts, np.asarray(vertices), side="right") - 1
+                diameter = min(2.0 * self._fibers[int(f)].radius for f in owner)
+                limit = np.float32(np.float32(diameter) * np.float32(n.max_translation_over_diameter))
+                world.set_vertex_targets(axis, vertices, -n.depth, n.stiffness,
+                                         min(np.float32(n.max_translation), limit))
+            run.converged = False
+            run.needling.append((n.layer, len(vertices), n.depth))
+            run.events.append(f"needle {len(vertices)} vertices in layer {n.layer} through depth "
+                              f"{n.depth:.3e}")
+        elif kind == "release_needles":
+            world.clear_vertex_targets()
+            run.converged = False
+            run.events.append("release needle targets")
+        else:
+            raise AssertionError(kind)
 
 
 def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iterations: int | None = None,

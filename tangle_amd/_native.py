@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import threading
+from dataclasses import dataclass
 from pathlib import Path
 
 import numpy as np
@@ -115,6 +116,77 @@ class Tuning(C.Structure):
     ]
 
 
+class CheckpointStruct(C.Structure):
+    _fields_ = [
+        ("total_iterations", C.c_uint64),
+        ("fiber_count", C.c_uint32),
+        ("vertex_count", C.c_uint32),
+        ("positions", C.POINTER(C.c_float)),
+        ("fiber_active", C.POINTER(C.c_uint32)),
+        ("wall_reactions", C.POINTER(C.c_float)),
+        ("layer_target_count", C.c_uint32),
+        ("layer_axis", C.c_uint32),
+        ("layer_first", C.c_uint32),
+        ("layer_last", C.c_uint32),
+        ("layer_stiffness", C.c_float),
+        ("layer_max_translation", C.c_float),
+        ("layer_targets", C.POINTER(C.c_float)),
+        ("vertex_target_count", C.c_uint32),
+        ("vertex_axis", C.c_uint32),
+        ("vertex_stiffness", C.c_float),
+        ("vertex_max_translation", C.c_float),
+        ("vertex_indices", C.POINTER(C.c_uint32)),
+        ("vertex_targets", C.POINTER(C.c_float)),
+        ("vertex_commands", C.POINTER(C.c_float)),
+    ]
+
+
+@dataclass
+class DeviceCheckpoint:
+    """The persistent state of a one-specimen world (DeviceWorldCheckpoint,
+    crates/tangle_relax/src/device/world.rs:281-304): positions, fiber
+    activity, wall reactions, the lifetime iteration count and the held
+    formation targets (None when not held)."""
+
+    total_iterations: int
+    positions: np.ndarray        # (3V,) float32
+    fiber_active: np.ndarray     # (F,) uint32
+    wall_reactions: np.ndarray   # (3V,) float32
+    layer_target: dict | None = None   # axis, first, last, stiffness, max_translation, targets
+    vertex_target: dict | None = None  # axis, stiffness, max_translation, indices, targets, commands
+
+    def struct(self) -> CheckpointStruct:
+        """A tgh_checkpoint view of these arrays (keep self alive while used)."""
+        c = CheckpointStruct()
+        c.total_iterations = int(self.total_iterations)
+        c.fiber_count = int(self.fiber_active.size)
+        c.vertex_count = int(self.positions.size // 3)
+        self.positions = np.ascontiguousarray(self.positions, np.float32)
+        self.fiber_active = np.ascontiguousarray(self.fiber_active, np.uint32)
+        self.wall_reactions = np.ascontiguousarray(self.wall_reactions, np.float32)
+        c.positions = ptr(self.positions, C.c_float)
+        c.fiber_active = ptr(self.fiber_active, C.c_uint32)
+        c.wall_reactions = ptr(self.wall_reactions, C.c_float)
+        if self.layer_target is not None:
+            lt = self.layer_target
+            lt["targets"] = np.ascontiguousarray(lt["targets"], np.float32)
+            c.layer_target_count = lt["targets"].size
+            c.layer_axis, c.layer_first, c.layer_last = lt["axis"], lt["first"], lt["last"]
+            c.layer_stiffness, c.layer_max_translation = lt["stiffness"], lt["max_translation"]
+            c.layer_targets = ptr(lt["targets"], C.c_float)
+        if self.vertex_target is not None:
+            vt = self.vertex_target
+            for key, dt in (("indices", np.uint32), ("targets", np.float32), ("commands", np.float32)):
+                vt[key] = np.ascontiguousarray(vt[key], dt)
+            c.vertex_target_count = vt["indices"].size
+            c.vertex_axis = vt["axis"]
+            c.vertex_stiffness, c.vertex_max_translation = vt["stiffness"], vt["max_translation"]
+            c.vertex_indices = ptr(vt["indices"], C.c_uint32)
+            c.vertex_targets = ptr(vt["targets"], C.c_float)
+            c.vertex_commands = ptr(vt["commands"], C.c_float)
+        return c
+
+
 class PopulationSpec(C.Structure):
     _fields_ = [
         ("count", C.c_uint32),
@@ -174,6 +246,10 @@ SIGNATURES = {
     "tgh_world_formation_target_error": (C.c_int, [C.c_void_p, _f32p, _f32p, C.POINTER(C.c_uint32),
                                                    C.POINTER(C.c_uint32)]),
     "tgh_world_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "tgh_world_checkpoint_sizes": (C.c_int, [C.c_void_p, C.POINTER(CheckpointStruct)]),
+    "tgh_world_checkpoint": (C.c_int, [C.c_void_p, C.POINTER(CheckpointStruct)]),
+    "tgh_world_restore": (C.c_int, [C.POINTER(Assembly), C.POINTER(Config), C.POINTER(CheckpointStruct),
+                                    C.POINTER(C.c_void_p)]),
     "tgh_world_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32]),
     "tgh_world_destroy": (None, [C.c_void_p]),
     "tgh_last_error": (C.c_char_p, []),
@@ -329,6 +405,49 @@ class World:
         check(lib.tgh_world_create_batch(arr, len(specimens), C.byref(config), C.byref(handle)))
         self._handle = handle
         return self
+
+    @classmethod
+    def restore(cls, packed: PackedAssembly, config: Config, checkpoint: "DeviceCheckpoint") -> "World":
+        """A world for `packed` (the assembly the checkpointed world was made
+        from) in the checkpoint's state (DeviceFiberWorld::restore,
+        world.rs:692-760)."""
+        lib = library()
+        self = cls.__new__(cls)
+        self._packed = packed
+        self._specimens = [packed]
+        self._vertex_count = packed.vertex_count
+        c = checkpoint.struct()
+        handle = C.c_void_p()
+        check(lib.tgh_world_restore(C.byref(packed.struct), C.byref(config), C.byref(c), C.byref(handle)))
+        self._handle = handle
+        return self
+
+    def checkpoint(self) -> "DeviceCheckpoint":
+        """Downloads the persistent state (DeviceFiberWorld::checkpoint,
+        world.rs:762-823)."""
+        lib = library()
+        sizes = CheckpointStruct()
+        check(lib.tgh_world_checkpoint_sizes(self._handle, C.byref(sizes)))
+        V, F = int(sizes.vertex_count), int(sizes.fiber_count)
+        ck = DeviceCheckpoint(0, np.empty(3 * V, np.float32), np.empty(F, np.uint32), np.empty(3 * V, np.float32))
+        if sizes.layer_target_count:
+            ck.layer_target = dict(axis=0, first=0, last=0, stiffness=1.0, max_translation=1.0,
+                                   targets=np.empty(sizes.layer_target_count, np.float32))
+        if sizes.vertex_target_count:
+            n = sizes.vertex_target_count
+            ck.vertex_target = dict(axis=0, stiffness=1.0, max_translation=1.0, indices=np.empty(n, np.uint32),
+                                    targets=np.empty(n, np.float32), commands=np.empty(n, np.float32))
+        c = ck.struct()
+        check(lib.tgh_world_checkpoint(self._handle, C.byref(c)))
+        ck.total_iterations = int(c.total_iterations)
+        if ck.layer_target is not None:
+            ck.layer_target.update(axis=int(c.layer_axis), first=int(c.layer_first), last=int(c.layer_last),
+                                   stiffness=float(c.layer_stiffness),
+                                   max_translation=float(c.layer_max_translation))
+        if ck.vertex_target is not None:
+            ck.vertex_target.update(axis=int(c.vertex_axis), stiffness=float(c.vertex_stiffness),
+                                    max_translation=float(c.vertex_max_translation))
+        return ck
 
     @property
     def specimen_count(self) -> int:

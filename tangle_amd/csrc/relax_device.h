@@ -148,6 +148,8 @@ hipError_t launch_vertex_passes(const WorldDev &w, const IterParams &p, hipStrea
 hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t stream);
 // Formation: activation of fibers through a step; layer and vertex targets.
 hipError_t launch_activate(const WorldDev &w, uint32_t maximum_step, hipStream_t stream);
+// mirrors w.factive (as uploaded) into every vertex's inactive flag
+hipError_t launch_apply_activity(const WorldDev &w, hipStream_t stream);
 hipError_t launch_layer_targets(const WorldDev &w, const FormationDev &f, bool gate_on_running, hipStream_t stream);
 hipError_t launch_init_vertex_targets(const WorldDev &w, const FormationDev &f, float displacement,
                                       hipStream_t stream);

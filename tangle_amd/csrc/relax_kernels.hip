@@ -1554,6 +1554,21 @@ __global__ void k_activate(const uint4 *fspan, const uint32_t *fstep, uint32_t *
     }
 }
 
+// The vertex flags of every fiber set from its activity flag (restore).
+__global__ void k_apply_activity(const uint4 *fspan, const uint32_t *factive, float4 *vB, uint32_t F) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const uint32_t on = factive[f];
+    const uint4 span = fspan[f];
+    for (uint32_t i = 0; i < span.y; ++i) {
+        float4 b = vB[span.x + i];
+        uint32_t flags = __float_as_uint(b.z);
+        flags = on ? (flags & ~kInactive) : (flags | kInactive);
+        b.z = __uint_as_float(flags);
+        vB[span.x + i] = b;
+    }
+}
+
 __device__ __forceinline__ float axis_of(const float4 &p, uint32_t axis) {
     return axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
 }
@@ -1799,6 +1814,11 @@ hipError_t launch_layer_targets(const WorldDev &w, const FormationDev &f, bool g
     hipLaunchKernelGGL(k_layer_targets, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.ss, w.pos, w.fspan,
                        w.flayer, w.factive, f.layer_targets, f.layer_count, f.layer_axis, f.first_layer,
                        f.last_layer, f.layer_stiffness, f.layer_max_translation, w.F, gate_on_running ? 1u : 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_activity(const WorldDev &w, hipStream_t stream) {
+    hipLaunchKernelGGL(k_apply_activity, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.fspan, w.factive, w.vB, w.F);
     return hipGetLastError();
 }
 

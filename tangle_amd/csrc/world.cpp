@@ -803,8 +803,12 @@ int tgh_world_activate_formation_step(tgh_world *world, uint32_t maximum_step, u
     return TGH_OK;
 }
 
-int tgh_world_set_layer_targets(tgh_world *world, uint32_t axis, const float *targets, size_t count,
-                                uint32_t first_layer, uint32_t last_layer, float stiffness, float max_translation) {
+// Holds `targets` as the world's layer target; with apply_now the first
+// correction is applied at once (apply_layer_target_range, world.rs:1752-1803),
+// without it the target is only reinstated (restore, world.rs:721-733).
+static int install_layer_targets(tgh_world *world, uint32_t axis, const float *targets, size_t count,
+                                 uint32_t first_layer, uint32_t last_layer, float stiffness, float max_translation,
+                                 bool apply_now) {
     if (!world || !targets) return set_error(TGH_ERR_INVALID, "null argument");
     if (axis > 2 || count == 0 || count > 0xffffffffu) return set_error(TGH_ERR_INVALID, "axis must be 0-2 and targets non-empty");
     if (!(stiffness > 0.f && stiffness <= 1.f) || !(max_translation > 0.f))
@@ -825,27 +829,26 @@ int tgh_world_set_layer_targets(tgh_world *world, uint32_t axis, const float *ta
     f.last_layer = last_layer;
     f.layer_stiffness = stiffness;
     f.layer_max_translation = max_translation;
-    // the first correction applies at once (world.rs:1766-1795)
-    HIP_TRY(tgh::launch_layer_targets(world->d, f, false, world->stream));
-    HIP_TRY(hipStreamSynchronize(world->stream));
+    if (apply_now) {
+        HIP_TRY(tgh::launch_layer_targets(world->d, f, false, world->stream));
+        HIP_TRY(hipStreamSynchronize(world->stream));
+    }
     drop_graphs(world);
     return TGH_OK;
 }
 
-int tgh_world_clear_layer_targets(tgh_world *world) {
-    if (!world) return set_error(TGH_ERR_INVALID, "world is null");
-    world->formation.layer_count = 0;
-    drop_graphs(world);
-    return TGH_OK;
-}
-
-int tgh_world_set_vertex_targets(tgh_world *world, uint32_t axis, const uint32_t *vertices, size_t count,
-                                 float displacement, float stiffness, float max_translation) {
+// Holds vertex targets; `commands` null: each command starts at the
+// vertex's coordinate and the target lies `displacement` from it
+// (initialize_vertex_displacement_targets); otherwise targets and commands
+// are reinstated as given (restore).
+static int install_vertex_targets(tgh_world *world, uint32_t axis, const uint32_t *vertices, size_t count,
+                                  float displacement, const float *targets, const float *commands, float stiffness,
+                                  float max_translation) {
     if (!world || !vertices) return set_error(TGH_ERR_INVALID, "null argument");
     if (axis > 2 || count == 0 || count > 0xffffffffu) return set_error(TGH_ERR_INVALID, "axis must be 0-2 and vertices non-empty");
     for (size_t i = 0; i < count; ++i)
         if (vertices[i] >= world->d.V) return set_error(TGH_ERR_INVALID, "vertex index out of range");
-    if (!std::isfinite(displacement) || displacement == 0.f)
+    if (!commands && (!std::isfinite(displacement) || displacement == 0.f))
         return set_error(TGH_ERR_INVALID, "displacement must be finite and nonzero");
     if (!(stiffness > 0.f && stiffness <= 1.f) || !(max_translation > 0.f))
         return set_error(TGH_ERR_INVALID, "stiffness must be in (0, 1] and max_translation positive");
@@ -862,10 +865,34 @@ int tgh_world_set_vertex_targets(tgh_world *world, uint32_t axis, const uint32_t
     f.vertex_axis = axis;
     f.vertex_stiffness = stiffness;
     f.vertex_max_translation = max_translation;
-    HIP_TRY(tgh::launch_init_vertex_targets(world->d, f, displacement, world->stream));
-    HIP_TRY(hipStreamSynchronize(world->stream));
+    if (commands) {
+        HIP_TRY(hipMemcpy(f.vertex_target, targets, count * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(f.vertex_command, commands, count * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(tgh::launch_init_vertex_targets(world->d, f, displacement, world->stream));
+        HIP_TRY(hipStreamSynchronize(world->stream));
+    }
     drop_graphs(world);
     return TGH_OK;
+}
+
+int tgh_world_set_layer_targets(tgh_world *world, uint32_t axis, const float *targets, size_t count,
+                                uint32_t first_layer, uint32_t last_layer, float stiffness, float max_translation) {
+    return install_layer_targets(world, axis, targets, count, first_layer, last_layer, stiffness, max_translation,
+                                 true);
+}
+
+int tgh_world_clear_layer_targets(tgh_world *world) {
+    if (!world) return set_error(TGH_ERR_INVALID, "world is null");
+    world->formation.layer_count = 0;
+    drop_graphs(world);
+    return TGH_OK;
+}
+
+int tgh_world_set_vertex_targets(tgh_world *world, uint32_t axis, const uint32_t *vertices, size_t count,
+                                 float displacement, float stiffness, float max_translation) {
+    return install_vertex_targets(world, axis, vertices, count, displacement, nullptr, nullptr, stiffness,
+                                  max_translation);
 }
 
 int tgh_world_clear_vertex_targets(tgh_world *world) {
@@ -891,6 +918,124 @@ int tgh_world_formation_target_error(tgh_world *world, float *layer_error, float
     HIP_TRY(hipStreamSynchronize(world->stream));
     *layer_error = err[0];
     *vertex_error = err[1];
+    return TGH_OK;
+}
+
+int tgh_world_checkpoint_sizes(tgh_world *world, tgh_checkpoint *out) {
+    if (!world || !out) return set_error(TGH_ERR_INVALID, "null argument");
+    if (world->d.nspec != 1) return set_error(TGH_ERR_INVALID, "checkpoints hold one-specimen worlds");
+    out->fiber_count = world->d.F;
+    out->vertex_count = world->d.V;
+    out->layer_target_count = world->formation.layer_count;
+    out->vertex_target_count = world->formation.vertex_count;
+    return TGH_OK;
+}
+
+int tgh_world_checkpoint(tgh_world *world, tgh_checkpoint *out) {
+    if (!world || !out) return set_error(TGH_ERR_INVALID, "null argument");
+    if (world->d.nspec != 1) return set_error(TGH_ERR_INVALID, "checkpoints hold one-specimen worlds");
+    const tgh::FormationDev &f = world->formation;
+    if (out->fiber_count != world->d.F || out->vertex_count != world->d.V ||
+        out->layer_target_count != f.layer_count || out->vertex_target_count != f.vertex_count)
+        return set_error(TGH_ERR_SIZE, "checkpoint counts differ from tgh_world_checkpoint_sizes");
+    if (!out->positions || !out->fiber_active || !out->wall_reactions ||
+        (f.layer_count && !out->layer_targets) ||
+        (f.vertex_count && (!out->vertex_indices || !out->vertex_targets || !out->vertex_commands)))
+        return set_error(TGH_ERR_INVALID, "checkpoint array is null");
+    int rc = tgh_world_download_positions(world, out->positions, 3 * (size_t)world->d.V);
+    if (rc) return rc;
+    const size_t V = world->d.V;
+    HIP_TRY(hipMemcpy(out->fiber_active, world->d.factive, world->d.F * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (world->d.track_walls) {
+        std::vector<float4> w(V);
+        HIP_TRY(hipMemcpy(w.data(), world->d.wall, V * sizeof(float4), hipMemcpyDeviceToHost));
+        for (size_t v = 0; v < V; ++v) {
+            out->wall_reactions[3 * v] = w[v].x;
+            out->wall_reactions[3 * v + 1] = w[v].y;
+            out->wall_reactions[3 * v + 2] = w[v].z;
+        }
+    } else {
+        std::fill(out->wall_reactions, out->wall_reactions + 3 * V, 0.f);
+    }
+    tgh::SpecState spec;
+    HIP_TRY(hipMemcpy(&spec, world->d.ss, sizeof(spec), hipMemcpyDeviceToHost));
+    out->total_iterations = spec.control[1];
+    out->layer_axis = f.layer_axis;
+    out->layer_first = f.first_layer;
+    out->layer_last = f.last_layer;
+    out->layer_stiffness = f.layer_stiffness;
+    out->layer_max_translation = f.layer_max_translation;
+    if (f.layer_count)
+        HIP_TRY(hipMemcpy(out->layer_targets, f.layer_targets, f.layer_count * sizeof(float), hipMemcpyDeviceToHost));
+    out->vertex_axis = f.vertex_axis;
+    out->vertex_stiffness = f.vertex_stiffness;
+    out->vertex_max_translation = f.vertex_max_translation;
+    if (f.vertex_count) {
+        HIP_TRY(hipMemcpy(out->vertex_indices, f.vertex_index, f.vertex_count * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(out->vertex_targets, f.vertex_target, f.vertex_count * sizeof(float),
+                          hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(out->vertex_commands, f.vertex_command, f.vertex_count * sizeof(float),
+                          hipMemcpyDeviceToHost));
+    }
+    return TGH_OK;
+}
+
+int tgh_world_restore(const tgh_assembly *assembly, const tgh_config *config, const tgh_checkpoint *ck,
+                      tgh_world **out) {
+    if (!out || !ck) return set_error(TGH_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (!ck->positions || !ck->fiber_active || !ck->wall_reactions)
+        return set_error(TGH_ERR_INVALID, "checkpoint array is null");
+    if (ck->total_iterations > 0xffffffffull)
+        return set_error(TGH_ERR_INVALID, "checkpoint iteration count exceeds u32");
+    tgh_world *w = nullptr;
+    int rc = create_world(assembly, 1, config, &w);
+    if (rc) return rc;
+    auto fail = [&](int code) {
+        delete w;
+        return code;
+    };
+    if (ck->fiber_count != w->d.F || ck->vertex_count != w->d.V)
+        return fail(set_error(TGH_ERR_SIZE, "the checkpoint was taken from another assembly"));
+    rc = tgh_world_upload_positions(w, ck->positions, 3 * (size_t)w->d.V);
+    if (rc) return fail(rc);
+    for (uint32_t fi = 0; fi < w->d.F; ++fi)
+        if (ck->fiber_active[fi] > 1u) return fail(set_error(TGH_ERR_INVALID, "fiber_active entries must be 0 or 1"));
+    if (hipMemcpy(w->d.factive, ck->fiber_active, w->d.F * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        tgh::launch_apply_activity(w->d, w->stream) != hipSuccess || hipStreamSynchronize(w->stream) != hipSuccess)
+        return fail(set_error(TGH_ERR_DEVICE, "restoring fiber activity failed"));
+    if (w->d.track_walls) {
+        const size_t V = w->d.V;
+        std::vector<float4> wr(V);
+        for (size_t v = 0; v < V; ++v)
+            wr[v] = make_float4(ck->wall_reactions[3 * v], ck->wall_reactions[3 * v + 1],
+                                ck->wall_reactions[3 * v + 2], 0.f);
+        if (hipMemcpy(w->d.wall, wr.data(), V * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_error(TGH_ERR_DEVICE, "restoring wall reactions failed"));
+    }
+    tgh::SpecState spec;
+    if (hipMemcpy(&spec, w->d.ss, sizeof(spec), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(set_error(TGH_ERR_DEVICE, "reading the specimen state failed"));
+    spec.control[0] = 0;
+    spec.control[1] = (uint32_t)ck->total_iterations;
+    spec.control[2] = 0;
+    spec.control[3] = 0;
+    if (hipMemcpy(w->d.ss, &spec, sizeof(spec), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_error(TGH_ERR_DEVICE, "writing the specimen state failed"));
+    if (ck->layer_target_count) {
+        rc = install_layer_targets(w, ck->layer_axis, ck->layer_targets, ck->layer_target_count, ck->layer_first,
+                                   ck->layer_last, ck->layer_stiffness, ck->layer_max_translation, false);
+        if (rc) return fail(rc);
+    }
+    if (ck->vertex_target_count) {
+        if (!ck->vertex_targets || !ck->vertex_commands) return fail(set_error(TGH_ERR_INVALID, "checkpoint array is null"));
+        rc = install_vertex_targets(w, ck->vertex_axis, ck->vertex_indices, ck->vertex_target_count, 0.f,
+                                    ck->vertex_targets, ck->vertex_commands, ck->vertex_stiffness,
+                                    ck->vertex_max_translation);
+        if (rc) return fail(rc);
+    }
+    *out = w;
     return TGH_OK;
 }
 
