@@ -677,6 +677,35 @@ class RunResult:
     checkpoint_saves: int = 0
     last_checkpoint_iteration: int | None = None
     last_checkpoint_bytes: int | None = None
+    _fibers: list = field(default=None, repr=False)  # the recipe's fibers, in packed order
+    _cell: object = field(default=None, repr=False)
+
+    def write_ovito(self, path, *, view_script_path=None, session_path=None, coloring: str = "fiber",
+                    bond_radius_ratio: float | None = 1.0) -> None:
+        """Writes the relaxed fibers as one OVITO frame, one spherocylinder per
+        segment, and optionally the script that styles it and saves a session
+        (RunResult.write_ovito, crates/tangle_python/src/recipe.rs:1005-1030;
+        tangle_export/src/ovito.rs). Junction bonds are not captured by this
+        backend, so `bond_radius_ratio` (None: no bonds) only adds their
+        styling to the script."""
+        from . import export
+        if coloring not in export.COLORINGS:
+            raise ValueError(f"coloring must be one of {', '.join(export.COLORINGS)}")
+        if self._fibers is None:
+            raise RuntimeError("this result does not carry its fibers")
+        xyz = self.positions.reshape(-1, 3).astype(np.float64)
+        rows, start = [], 0
+        for index, (fiber, n) in enumerate(zip(self._fibers, self.vertex_counts)):
+            rows.append((index + 1, fiber.radius, xyz[start:start + n], fiber.intrinsic,
+                         fiber.material.min_bend_radius))
+            start += n
+        cell = self._cell
+        lower = [float(o) for o in cell.origin]
+        upper = [float(o) + float(l) for o, l in zip(cell.origin, cell.lengths)]
+        export.write_ovito_frame(path, rows, lower, upper, cell.periodic, self.iterations)
+        if view_script_path is not None:
+            export.write_ovito_view_script(view_script_path, path, session_path=session_path, coloring=coloring,
+                                           bond_radius_ratio=bond_radius_ratio)
 
     def centerlines(self):
         out, start = [], 0
@@ -1239,7 +1268,7 @@ class Recipe:
             post_recipe_iterations=run.post_recipe_iterations, needling=list(run.needling),
             resumed=report["resumed"], resumed_iteration=report["resumed_iteration"],
             checkpoint_saves=report["saves"], last_checkpoint_iteration=report["last_iteration"],
-            last_checkpoint_bytes=report["last_bytes"],
+            last_checkpoint_bytes=report["last_bytes"], _fibers=list(self._fibers), _cell=self._cell,
         )
 
     def _run_operation(self, world, run, state, index, desc, kind, arg, begin, initial_targets, starts):
@@ -1374,5 +1403,6 @@ def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iter
             max_displacement=float(st.max_displacement), active_segments=int(p.vertex_count - p.counts.size),
             active_vertices=int(p.vertex_count), cell_count=int(info.cell_count), uploaded_bytes=0,
             downloaded_bytes=int(pos.nbytes), events=[f"batched with {len(recipes)} specimens"], warnings=[],
-            neighbor_list_rebuilds=rebuilds, wall_seconds=wall, positions=pos, vertex_counts=p.counts.copy()))
+            neighbor_list_rebuilds=rebuilds, wall_seconds=wall, positions=pos, vertex_counts=p.counts.copy(),
+            _fibers=list(r._fibers), _cell=r._cell))
     return out

@@ -191,9 +191,16 @@ def needled_felt_toy(layer_count=6):
     return recipe, settings
 
 
-def test_needled_felt_toy_recipe(hip_device):
+def test_needled_felt_toy_recipe(hip_device, tmp_path):
     recipe, settings = needled_felt_toy()
     result = recipe.run(settings)
+    # the reference example writes its result for OVITO
+    result.write_ovito(tmp_path / "final.dump", view_script_path=tmp_path / "view.py", coloring="curvature_ratio")
+    dump = (tmp_path / "final.dump").read_text().splitlines()
+    assert dump[3] == str(int(result.vertex_counts.sum()) - result.fiber_count)
+    first = dump[9].split()
+    p = result.positions.reshape(-1, 3).astype(np.float64)
+    assert np.allclose([float(v) for v in first[10:13]], 0.5 * (p[0] + p[1]), rtol=0, atol=1e-15)
     print(f"[felt toy] iterations={result.iterations} post={result.post_recipe_iterations} "
           f"pen={result.max_penetration:.3e} curv={result.max_curvature_ratio:.5f} needling={result.needling} "
           f"wall={result.wall_seconds:.2f}s", flush=True)
