@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (reference kernel for the HBM counters)")
     ap.add_argument("--skip-batched", action="store_true", help="skip the batched-specimens throughput run")
+    ap.add_argument("--skip-formation", action="store_true", help="skip the layered, needled felt workload")
+    ap.add_argument("--formation-plies", type=int, default=20, help="plies of the layered felt workload")
     ap.add_argument("--batched-specimens", type=int, default=256, help="specimens in the batched run")
     ap.add_argument("--batched-steps", type=int, default=2, help="timed 128-iteration batches of the batched run")
     ap.add_argument("--curvature-lanes", type=int, default=-1,
@@ -262,6 +264,9 @@ def main():
     if not args.skip_batched:
         result["batched"] = batched_throughput(args, fx, tg, _native)
 
+    if not args.skip_formation:
+        result["formation"] = formation_workload(args, fx, tg)
+
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(packed, cfg, S, args.cpu_iterations)
 
@@ -398,6 +403,49 @@ def copy_calibration(mib: int) -> None:
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "dispatches": 5}), flush=True)
+
+
+def formation_workload(args, fx, tg):
+    """The layered, needled felt: plies of 2,201 x 7 um + 299 x 19 um fibers
+    on a 2 x 2 mm periodic footprint, inserted one formation step at a time
+    and relaxed, 5 % of every ply's 19 um fibers needled 30 um down and
+    released, then a contact and curvature cleanup (tests/fixtures.py
+    layered_felt_recipe), through Recipe.run on one resident world. Reports
+    active-segment iterations per second over the whole run, and the oracle
+    on the fully stacked felt for a few iterations beside it."""
+    import oracle
+
+    recipe, settings = fx.layered_felt_recipe(plies=args.formation_plies)
+    packed = recipe._packed(False)
+    S = packed.vertex_count - packed.counts.size
+    t0 = time.perf_counter()
+    try:
+        res = recipe.run(settings)
+    except tg.RecipeError as err:
+        return {"error": str(err), "seconds": round(time.perf_counter() - t0, 3)}
+    wall = time.perf_counter() - t0
+    out = {
+        "workload": f"layered needled felt: {args.formation_plies} plies x (2,201 x 7 um + 299 x 19 um), "
+                    f"1 mm fibers, 2 x 2 mm periodic footprint, bounded z",
+        "fibers": len(recipe._fibers), "segments": int(S), "operations": len(recipe.operations()),
+        "iterations": res.iterations, "post_recipe_iterations": res.post_recipe_iterations,
+        "seconds": round(wall, 3), "value": round(res.segment_iterations / wall, 1), "unit": "segment-iters/s",
+        "needled_vertices": int(sum(n[1] for n in res.needling)), "converged": res.converged,
+        "max_penetration": res.max_penetration, "max_curvature_ratio": res.max_curvature_ratio,
+        "warnings": res.warnings[:2],
+    }
+    if not args.no_cpu:
+        cfg = settings.to_config(7e-6)
+        with oracle.OracleWorld(packed, cfg) as w:
+            t1 = time.perf_counter()
+            status = w.run_batch(cfg, 4)
+            t2 = time.perf_counter()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        out["cpu_baseline"] = {"value": round(S * int(status.batch_iterations) / (t2 - t1), 1),
+                               "unit": "segment-iters/s", "cores": threads, "kind": "port",
+                               "sample": f"{int(status.batch_iterations)} iterations of the fully stacked felt "
+                                         f"from the generated positions ({t2 - t1:.1f} s)"}
+    return out
 
 
 def cpu_baseline(packed, cfg, S, iterations):

@@ -671,6 +671,7 @@ class RunResult:
     positions: np.ndarray
     vertex_counts: np.ndarray
     post_recipe_iterations: int = 0
+    segment_iterations: int = 0  # active segments x iterations over the run
     needling: list = field(default_factory=list)  # (layer, selected vertices, depth) per needle command
     resumed: bool = False
     resumed_iteration: int | None = None
@@ -742,6 +743,8 @@ class _RunState:
         self.proposals_reverted = 0
         self.post_recipe_iterations = 0
         self.needling: list = []
+        self.active_segments = 0
+        self.segment_iterations = 0  # active segments x iterations, summed over batches
         # operation in progress and the iteration it started at (checkpoints)
         self.op_index = 0
         self.op_begin = 0
@@ -795,6 +798,7 @@ class _RunState:
 
     def _batch(self, world, config, iterations):
         status = world.run_batch(config, iterations)
+        self.segment_iterations += self.active_segments * int(status.batch_iterations)
         self.downloaded += 11 * 4 + 3 * 4
         self.total = int(status.total_iterations)
         self.status = status
@@ -1219,6 +1223,8 @@ class Recipe:
                     state["targets"] = [np.float32(t) for t in progress["layer_targets"]]
         world = (_native.World(packed, config) if restored is None
                  else _native.World.restore(packed, config, restored))
+        if restored is not None:
+            run.active_segments = int(((packed.counts.astype(np.int64) - 1) * (restored.fiber_active != 0)).sum())
 
         def save(failure=None):
             progress = {"fingerprint": fingerprint, "operation": run.op_index, "operation_begin": run.op_begin,
@@ -1265,8 +1271,8 @@ class Recipe:
                                   f"{run.proposals_reverted} reverted"] if warm_start else []),
             warnings=run.warnings, neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
             positions=positions, vertex_counts=packed.counts.copy(),
-            post_recipe_iterations=run.post_recipe_iterations, needling=list(run.needling),
-            resumed=report["resumed"], resumed_iteration=report["resumed_iteration"],
+            post_recipe_iterations=run.post_recipe_iterations, segment_iterations=run.segment_iterations,
+            needling=list(run.needling), resumed=report["resumed"], resumed_iteration=report["resumed_iteration"],
             checkpoint_saves=report["saves"], last_checkpoint_iteration=report["last_iteration"],
             last_checkpoint_bytes=report["last_bytes"], _fibers=list(self._fibers), _cell=self._cell,
         )
@@ -1278,6 +1284,7 @@ class Recipe:
         axis = self._stack_axis
         if kind == "activate":
             segments, vertices = world.activate_formation_step(arg)
+            run.active_segments = segments
             state["active_step"] = arg
             run.converged = False
             run.events.append(f"insert fiber groups through step {arg} "

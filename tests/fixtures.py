@@ -235,3 +235,58 @@ def slab_recipe(fibers=20_000, beta=1.0 / 0.3, seed_offset=0):
                                  orientation=tg.SchladitzOrientation(beta=beta, axis="z"))
         recipe.insert(tg.generate_fiber_population(cell, pop))
     return recipe
+
+
+# The layered, needled felt: plies of the bimodal mix (2,201 x 7 um + 299 x
+# 19 um each, 1 mm long, segment ~2 d, bend limits as above) on a 2 x 2 mm
+# periodic footprint, bounded along z, each ply generated in its own band
+# `gap` thick. Each ply is inserted as a formation step and relaxed; then
+# every ply has 5 % of its 19 um fibers pulled 30 um down through the stack,
+# held until the needles reach their targets, and released; a final cleanup
+# (contact_cleanup preset, fixed budget, soft gates) and the post-recipe
+# relaxation to the run's tolerances follow. With `place`, each new ply is
+# also held `gap` above the last while it relaxes (place_layer_above). The
+# reference's analogue is its 20-ply felt example
+# (examples/felt_20ply_control_vs_needled/src/main.rs:42-58, recipe.rs:9-66),
+# whose plies hold about 27 fibers each; a layer target pulls every fiber's
+# centre to its ply's plane, which for plies of 2,500 fibers (a 42 um solid
+# sheet) crushes the ply into deep crossings that do not come apart, so the
+# benchmark deposits plies in their bands instead. Compaction is left out.
+def layered_felt_recipe(plies=20, fine=2201, coarse=299, footprint=2e-3, gap=300e-6, relax_per_ply=64,
+                        needle_fraction=0.05, needle_depth=30e-6, settle_iterations=64, final_iterations=512,
+                        seed=20_260_921, place=False, needle=True):
+    import math
+    height = plies * gap
+    cell = tg.Cell([footprint, footprint, height], periodic="xy")
+    population = None
+    for name, d, count, bend, offset in (("fine", 7e-6, fine, 200e-6, 0), ("coarse", 19e-6, coarse, 60e-6, 1)):
+        material = tg.Material(name, d, bend)
+        pop = tg.FiberPopulation(material=material, count=count * plies, length=1e-3,
+                                 segments_per_fiber=max(2, round(1e-3 / (2 * d))), seed=seed + offset,
+                                 curvature_amplitude=(0.0, 0.4 * d),
+                                 orientation=tg.PlanarOrientation(normal="z", max_tilt=math.radians(3.0)),
+                                 position=tg.LayeredPosition(plies, jitter_fraction=1.0))
+        col = tg.generate_fiber_population(cell, pop, name=name)
+        population = col if population is None else population + col
+    recipe = tg.Recipe(cell)
+    for ply in range(plies):
+        recipe.insert(population.select_layer(ply), name=f"ply {ply}")
+        if ply == 0 or not place:
+            recipe.relax_for(relax_per_ply)
+        else:
+            with recipe.place_layer_above(ply, gap=gap, stiffness=1.0, max_translation=2e-6):
+                recipe.relax_for(relax_per_ply)
+    for ply in range(plies if needle else 0):
+        with recipe.needle_layer(ply, footprint=tg.RandomFiberFraction(needle_fraction, seed=seed + 7 + ply),
+                                 depth=needle_depth, min_fiber_diameter=19e-6, stiffness=1.0, max_translation=2e-6):
+            recipe.settle_targets(tolerance=0.05 * needle_depth, max_iterations=settle_iterations)
+    # final contact and curvature cleanup toward the hard gates, for a fixed
+    # budget: its limits are soft, so an unfinished cleanup is reported, not raised
+    policy = tg.SolvePolicy("final cleanup", target_penetration=HARD_PENETRATION,
+                            target_curvature_ratio=HARD_CURVATURE_RATIO, max_iterations=final_iterations,
+                            max_extra_iterations=0, hard_penetration=False, hard_curvature=False,
+                            on_budget_exhausted="continue_if_hard_ok")
+    recipe.solve(policy, tg.RelaxationOverrides.preset("contact_cleanup"))
+    settings = tg.RelaxationSettings(max_iterations=plies * (relax_per_ply + settle_iterations) + final_iterations,
+                                     iterations_per_batch=64)
+    return recipe, settings
