@@ -203,7 +203,8 @@ int tgh_world_formation_target_error(tgh_world *world, float *layer_error, float
  * tgh_world_checkpoint then fills a checkpoint whose counts are those and
  * whose arrays hold them. tgh_world_restore creates a world from the
  * assembly the checkpointed world was created from and puts it in the
- * checkpoint's state (DeviceFiberWorld::restore). */
+ * checkpoint's state (DeviceFiberWorld::restore), including a cell compacted
+ * since creation (it must lie inside the assembly's cell). */
 typedef struct tgh_checkpoint {
     uint64_t total_iterations;
     uint32_t fiber_count;
@@ -221,12 +222,67 @@ typedef struct tgh_checkpoint {
     uint32_t *vertex_indices;     /* [vertex_target_count] */
     float *vertex_targets;        /* [vertex_target_count] target coordinates */
     float *vertex_commands;       /* [vertex_target_count] commanded coordinates */
+    float cell_lower[3];          /* the world's cell at the checkpoint (it may */
+    float cell_upper[3];          /* have been compacted since creation) */
 } tgh_checkpoint;
 
 int tgh_world_checkpoint_sizes(tgh_world *world, tgh_checkpoint *out);
 int tgh_world_checkpoint(tgh_world *world, tgh_checkpoint *out);
 int tgh_world_restore(const tgh_assembly *assembly, const tgh_config *config, const tgh_checkpoint *checkpoint,
                       tgh_world **out);
+
+/* ---- compaction and contact capture (world.rs:1931-2334, kernels.rs:712-923,
+ * crates/tangle_contact/src/device/contact.rs:9-230); one-specimen worlds ----
+ * tgh_world_cell_bounds: the resident cell (cell_bounds).
+ * tgh_world_compact_cell: the cell becomes [new_lower, new_upper], inside the
+ * current one on every axis and wider than two fiber radii, and the active
+ * material follows it by `kinematics` (compact_cell):
+ *   TGH_COMPACT_RIGID_FIBER_CENTERS: each fiber translates so that its
+ *     vertex-mean centre keeps its fraction of the cell; reactions reset;
+ *   TGH_COMPACT_MOVING_WALLS: vertices are projected inside the new walls of
+ *     the bounded axes; the wall reaction is the projection;
+ *   TGH_COMPACT_AFFINE_VERTICES: every vertex keeps its fractional
+ *     coordinates; reactions reset.
+ * The grid keeps its cell size (fewer cells) and the lists are rebuilt by the
+ * next batch.
+ * tgh_world_compaction_metrics: directional pressures and penalty energies of
+ * the current state from the last iteration's contact corrections, curvature
+ * ratios and wall reactions (compaction_metrics, measure_compaction_metrics);
+ * the reductions run in index order, as the reference's single thread does.
+ * tgh_world_capture_contacts: every pair of active segments of different
+ * fibers whose capsule surfaces are at most `maximum_surface_gap` apart, with
+ * the closest-point coordinates and the crossing angle, sorted by (first,
+ * second, first coordinate, second coordinate) (capture_contacts). At most
+ * `capacity` are returned; *overflow is set when more were found. The
+ * capture uses the shared cell buffers, so the next batch rebuilds its lists.
+ * tgh_world_set_fiber_maximum_curvatures: per-fiber curvature limits
+ * (inverse length; 0 disables) replace the assembly's bend radii
+ * (set_fiber_maximum_curvatures). */
+#define TGH_COMPACT_RIGID_FIBER_CENTERS 0u
+#define TGH_COMPACT_MOVING_WALLS 1u
+#define TGH_COMPACT_AFFINE_VERTICES 2u
+typedef struct tgh_compaction_metrics {
+    float pressure[3];        /* per axis: contact stiffness x load / (projection x face area) */
+    float contact_energy;     /* sum of k_c p^2 / 2 over segment penetrations */
+    float stretch_energy;     /* sum of k_s (l - l0)^2 / (2 l0) */
+    float bending_energy;     /* sum of k_b max(ratio - 1, 0)^2 / 2 */
+    float total_energy;
+    float face_pressure[3][2]; /* per axis: [lower face, upper face] from wall reactions */
+} tgh_compaction_metrics;
+typedef struct tgh_contact_candidate {
+    uint32_t first_segment, second_segment; /* first < second, different fibers */
+    float first_coordinate, second_coordinate; /* closest points, in [0, 1] */
+    float surface_gap;                      /* centre distance minus both radii */
+    float crossing_angle;                   /* acos |cos| of the axes, radians */
+} tgh_contact_candidate;
+int tgh_world_cell_bounds(const tgh_world *world, float lower[3], float upper[3]);
+int tgh_world_compact_cell(tgh_world *world, const float new_lower[3], const float new_upper[3],
+                           uint32_t kinematics);
+int tgh_world_compaction_metrics(tgh_world *world, float correction_fraction, float contact_stiffness,
+                                 float stretch_stiffness, float bending_stiffness, tgh_compaction_metrics *out);
+int tgh_world_capture_contacts(tgh_world *world, float maximum_surface_gap, size_t capacity,
+                               tgh_contact_candidate *out, size_t *count, uint32_t *overflow);
+int tgh_world_set_fiber_maximum_curvatures(tgh_world *world, const float *maximum_curvatures, size_t count);
 
 /* Launch shape of the relaxation kernels (no counterpart in the reference,
  * whose launch shapes are fixed). Any setting gives the same positions bit

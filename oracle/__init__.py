@@ -57,6 +57,14 @@ def library() -> C.CDLL:
             "orc_world_clear_vertex_targets": (None, [C.c_void_p]),
             "orc_world_formation_target_error": (None, [C.c_void_p, f32p, f32p, C.POINTER(C.c_uint32),
                                                   C.POINTER(C.c_uint32)]),
+            "orc_world_compact_cell": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                                 C.c_uint32]),
+            "orc_world_cell_bounds": (None, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+            "orc_world_compaction_metrics": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float,
+                                                    C.POINTER(C.c_float)]),
+            "orc_world_capture_contacts": (C.c_int, [C.c_void_p, C.c_float, C.c_size_t, C.POINTER(C.c_float),
+                                                     C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)]),
+            "orc_world_set_fiber_maximum_curvatures": (None, [C.c_void_p, C.POINTER(C.c_float)]),
             "orc_world_destroy": (None, [C.c_void_p]),
             "orc_last_error": (C.c_char_p, []),
         }
@@ -180,3 +188,36 @@ class OracleWorld:
         _check(library().orc_world_contact_state(self._h, _p(corr, C.c_float), _p(smax, C.c_float),
                                                  _p(curv, C.c_float)))
         return corr.reshape(-1, 6), smax, curv
+
+    # compaction and contact capture, as _native.World
+    def cell_bounds(self):
+        lo, hi = (C.c_float * 3)(), (C.c_float * 3)()
+        library().orc_world_cell_bounds(self._h, lo, hi)
+        return tuple(float(v) for v in lo), tuple(float(v) for v in hi)
+
+    def compact_cell(self, new_lower, new_upper, kinematics: int) -> None:
+        lo = (C.c_float * 3)(*[float(v) for v in new_lower])
+        hi = (C.c_float * 3)(*[float(v) for v in new_upper])
+        _check(library().orc_world_compact_cell(self._h, lo, hi, int(kinematics)))
+
+    def compaction_metrics(self, correction_fraction, contact_stiffness=1.0, stretch_stiffness=1.0,
+                           bending_stiffness=1.0) -> np.ndarray:
+        """The 13 outputs of measure_compaction_metrics (kernels.rs:811-923)."""
+        out = np.empty(13, dtype=np.float32)
+        library().orc_world_compaction_metrics(self._h, float(correction_fraction), float(contact_stiffness),
+                                               float(stretch_stiffness), float(bending_stiffness),
+                                               _p(out, C.c_float))
+        return out
+
+    def capture_contact_arrays(self, maximum_surface_gap, capacity):
+        out = np.zeros((max(1, int(capacity)), 6), dtype=np.float32)
+        count, overflow = C.c_size_t(), C.c_uint32()
+        _check(library().orc_world_capture_contacts(self._h, float(maximum_surface_gap), int(capacity),
+                                                    _p(out, C.c_float), C.byref(count), C.byref(overflow)))
+        n = int(count.value)
+        return (out[:n, :2].view(np.uint32).copy(), out[:n, 2:4].copy(), out[:n, 4].copy(), out[:n, 5].copy(),
+                bool(overflow.value))
+
+    def set_fiber_maximum_curvatures(self, values) -> None:
+        arr = np.ascontiguousarray(values, dtype=np.float32)
+        library().orc_world_set_fiber_maximum_curvatures(self._h, _p(arr, C.c_float))

@@ -13,6 +13,8 @@
 //   contact             crates/tangle_contact/src/device/contact.rs:232-902
 //   relax kernels       crates/tangle_relax/src/device/kernels.rs:3-162, 925-1537
 //   batch loop/status   crates/tangle_relax/src/device/world.rs:1020-1505, 2337-2369
+//   compaction          crates/tangle_relax/src/device/kernels.rs:712-923, world.rs:1931-2100
+//   contact capture     crates/tangle_contact/src/device/contact.rs:9-230, world.rs:2193-2334
 //
 // Arithmetic follows Rust f32 semantics: no FMA contraction (compile with
 // -ffp-contract=off), IEEE division and square root, f32::round = half away
@@ -107,7 +109,7 @@ struct World {
     float vt_stiffness = 0, vt_max = 0;
 
     // sizing (world.rs:322-417)
-    float skin = 0, cell_size = 0;
+    float skin = 0, cell_size = 0, segment_cell_size = 0, max_radius = 0;
     uint32_t cx = 1, cy = 1, cz = 1, C = 1;
     size_t slots = 1;
     uint32_t capacity = 48, block = 48;
@@ -277,6 +279,8 @@ int size_world(World &w, const tgh_config *c) {
     }
     float piece = block_len < max_len ? 1.05f * block_len : block_len;
     w.cell_size = (piece + margin) * c->cell_size_scale;
+    w.segment_cell_size = (max_len + margin) * c->cell_size_scale;  // world.rs:405
+    w.max_radius = max_radius;
     float ext[3];
     for (int k = 0; k < 3; ++k) ext[k] = w.upper[k] - w.lower[k];
     w.cx = sat_u32(rmax(std::floor(ext[0] / w.cell_size), 1.0f));
@@ -1031,6 +1035,265 @@ int run_batch(World &w, const tgh_config *c, uint32_t iterations, tgh_batch_stat
     return TGH_OK;
 }
 
+// ---- compaction: kernels.rs:712-809, world.rs:1931-2053 ---------------------
+
+int compact_cell(World &w, const float *nl, const float *nu, uint32_t kinematics) {
+    for (int k = 0; k < 3; ++k) {
+        if (!std::isfinite(nl[k]) || !std::isfinite(nu[k])) return fail(TGH_ERR_INVALID, "bounds must be finite");
+        if (!(nu[k] - nl[k] > 2.0f * w.max_radius)) return fail(TGH_ERR_INVALID, "cell narrower than two radii");
+        if (!(nl[k] >= w.lower[k]) || !(nu[k] <= w.upper[k])) return fail(TGH_ERR_INVALID, "cell outside the current");
+    }
+    uint32_t cells[3];
+    for (int k = 0; k < 3; ++k) cells[k] = sat_u32(rmax(std::floor((nu[k] - nl[k]) / w.cell_size), 1.0f));
+    if ((uint64_t)cells[0] * cells[1] * cells[2] > w.C) return fail(TGH_ERR_SIZE, "more cells than allocated");
+    if (kinematics == 0) {  // compact_rigid_fiber_centers
+        for (uint32_t f = 0; f < w.F; ++f) {
+            const uint32_t start = w.fiber_span[2 * f], count = w.fiber_span[2 * f + 1];
+            if (count == 0 || !vtx_on(w, start)) continue;
+            float c[3] = {0.f, 0.f, 0.f};
+            for (uint32_t i = 0; i < count; ++i)
+                for (int k = 0; k < 3; ++k) c[k] += w.pos[3 * (size_t)(start + i) + k];
+            for (int k = 0; k < 3; ++k) c[k] /= (float)count;
+            float d[3];
+            for (int k = 0; k < 3; ++k) {
+                float fraction = (c[k] - w.lower[k]) / (w.upper[k] - w.lower[k]);
+                d[k] = nl[k] + fraction * (nu[k] - nl[k]) - c[k];
+            }
+            for (uint32_t i = 0; i < count; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    w.pos[3 * (size_t)(start + i) + k] += d[k];
+                    w.wall[3 * (size_t)(start + i) + k] = 0.f;
+                }
+        }
+    } else if (kinematics == 1) {  // compact_moving_walls
+        for (uint32_t v = 0; v < w.V; ++v) {
+            if (!vtx_on(w, v)) continue;
+            for (int k = 0; k < 3; ++k) {
+                const size_t i = 3 * (size_t)v + k;
+                if (!w.periodic[k]) {
+                    float previous = w.pos[i];
+                    float projected = rclamp(previous, nl[k] + w.wall_extent[i], nu[k] - w.wall_extent[i]);
+                    w.pos[i] = projected;
+                    w.wall[i] = previous - projected;
+                } else {
+                    w.wall[i] = 0.f;
+                }
+            }
+        }
+    } else if (kinematics == 2) {  // compact_affine_vertices
+        for (uint32_t v = 0; v < w.V; ++v) {
+            if (!vtx_on(w, v)) continue;
+            for (int k = 0; k < 3; ++k) {
+                const size_t i = 3 * (size_t)v + k;
+                float fraction = (w.pos[i] - w.lower[k]) / (w.upper[k] - w.lower[k]);
+                w.pos[i] = nl[k] + fraction * (nu[k] - nl[k]);
+                w.wall[i] = 0.f;
+            }
+        }
+    } else {
+        return fail(TGH_ERR_INVALID, "unknown kinematics");
+    }
+    for (int k = 0; k < 3; ++k) {
+        w.lower[k] = nl[k];
+        w.upper[k] = nu[k];
+    }
+    w.cx = cells[0];
+    w.cy = cells[1];
+    w.cz = cells[2];
+    w.nstate[0] = 1;  // request_neighbor_list_rebuild
+    return TGH_OK;
+}
+
+// measure_compaction_metrics (kernels.rs:811-923), one thread in index order
+void compaction_metrics(const World &w, float cf, float kc, float ks, float kb, float *out) {
+    float lx = 0.f, ly = 0.f, lz = 0.f, contact = 0.f, stretch = 0.f;
+    for (uint32_t s = 0; s < w.S; ++s) {
+        if (!seg_on(w, s)) continue;
+        const float *c = &w.corr[6 * (size_t)s];
+        lx += std::fabs(c[0]) + std::fabs(c[3]);
+        ly += std::fabs(c[1]) + std::fabs(c[4]);
+        lz += std::fabs(c[2]) + std::fabs(c[5]);
+        float penetration = w.seg_max[s];
+        contact += 0.5f * kc * penetration * penetration;
+        const uint32_t a = w.seg_vtx[2 * s], b = w.seg_vtx[2 * s + 1];
+        float dx = w.pos[3 * (size_t)b] - w.pos[3 * (size_t)a];
+        float dy = w.pos[3 * (size_t)b + 1] - w.pos[3 * (size_t)a + 1];
+        float dz = w.pos[3 * (size_t)b + 2] - w.pos[3 * (size_t)a + 2];
+        float length = std::sqrt(dx * dx + dy * dy + dz * dz);
+        float rest = w.seg_rest[s];
+        if (rest > 1.0e-7f) {
+            float extension = length - rest;
+            stretch += 0.5f * ks * extension * extension / rest;
+        }
+    }
+    float bending = 0.f;
+    for (uint32_t v = 0; v < w.V; ++v) {
+        float excess = rmax(w.curv[v] - 1.0f, 0.f);
+        bending += 0.5f * kb * excess * excess;
+    }
+    float L[3];
+    for (int k = 0; k < 3; ++k) L[k] = w.upper[k] - w.lower[k];
+    float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
+    for (uint32_t v = 0; v < w.V; ++v)
+        for (int k = 0; k < 3; ++k) {
+            float r = w.wall[3 * (size_t)v + k];
+            lo[k] += rmax(-r, 0.f);
+            hi[k] += rmax(r, 0.f);
+        }
+    float ip = 1.0f / rmax(cf, 1.0e-7f);
+    float load[3] = {lx, ly, lz};
+    float area[3] = {L[1] * L[2], L[0] * L[2], L[0] * L[1]};
+    for (int k = 0; k < 3; ++k) {
+        float pl = w.periodic[k] ? load[k] : lo[k] + hi[k];
+        out[k] = kc * pl * ip / area[k];
+        out[7 + 2 * k] = kc * lo[k] * ip / area[k];
+        out[8 + 2 * k] = kc * hi[k] * ip / area[k];
+    }
+    out[3] = contact;
+    out[4] = stretch;
+    out[5] = bending;
+    out[6] = contact + stretch + bending;
+}
+
+// capture_segment_contacts (contact.rs:9-230) on the whole-segment grid of
+// capture_contacts (world.rs:2193-2334), then the host sort by (first,
+// second, total order of the coordinates)
+struct Candidate {
+    uint32_t first, second;
+    float s, t, gap, angle;
+};
+
+inline int32_t total_key(float x) {
+    int32_t i;
+    std::memcpy(&i, &x, 4);
+    return i ^ (int32_t)((uint32_t)(i >> 31) >> 1);
+}
+
+int capture_contacts(World &w, float max_gap, size_t capacity, std::vector<Candidate> &out, bool &overflow) {
+    if (!std::isfinite(max_gap) || !(max_gap >= 0.0f)) return fail(TGH_ERR_INVALID, "gap must be finite and >= 0");
+    if (capacity == 0) return fail(TGH_ERR_INVALID, "capacity must be positive");
+    const float size = rmax(w.segment_cell_size, w.cell_size) + max_gap;
+    float L[3];
+    uint32_t n[3];
+    for (int k = 0; k < 3; ++k) {
+        L[k] = w.upper[k] - w.lower[k];
+        n[k] = sat_u32(rmax(std::floor(L[k] / size), 1.0f));
+    }
+    float width[3];
+    for (int k = 0; k < 3; ++k) width[k] = L[k] / (float)n[k];
+    auto home_of = [&](const float *p, const float *q, int32_t *h) {
+        for (int k = 0; k < 3; ++k) {
+            float mid = 0.5f * (p[k] + q[k]);
+            int32_t raw = sat_i32(std::floor((mid - w.lower[k]) / width[k]));
+            int32_t nk = (int32_t)n[k];
+            h[k] = w.periodic[k] ? ((raw % nk) + nk) % nk : std::min(std::max(raw, 0), nk - 1);
+        }
+    };
+    const uint64_t C = (uint64_t)n[0] * n[1] * n[2];
+    std::vector<std::vector<uint32_t>> cells(C);
+    for (uint32_t s = 0; s < w.S; ++s) {  // ascending within each cell (rank_cell_segments)
+        if (!seg_on(w, s)) continue;
+        int32_t h[3];
+        home_of(&w.pos[3 * (size_t)w.seg_vtx[2 * s]], &w.pos[3 * (size_t)w.seg_vtx[2 * s + 1]], h);
+        cells[((uint64_t)h[2] * n[1] + h[1]) * n[0] + h[0]].push_back(s);
+    }
+    std::vector<Candidate> found;
+    const float eps_len = 1.0e-20f, eps_par = 1.0e-6f;
+    for (uint32_t seg = 0; seg < w.S; ++seg) {
+        if (!seg_on(w, seg)) continue;
+        const float *P1 = &w.pos[3 * (size_t)w.seg_vtx[2 * seg]], *Q1 = &w.pos[3 * (size_t)w.seg_vtx[2 * seg + 1]];
+        float d1[3] = {Q1[0] - P1[0], Q1[1] - P1[1], Q1[2] - P1[2]};
+        int32_t h[3];
+        home_of(P1, Q1, h);
+        for (uint32_t nb = 0; nb < 27; ++nb) {
+            int32_t o[3] = {(int32_t)(nb % 3) - 1, (int32_t)((nb / 3) % 3) - 1, (int32_t)(nb / 9) - 1};
+            bool ok = true;
+            int32_t raw[3], cell[3];
+            float shift[3];
+            for (int k = 0; k < 3; ++k) {
+                int32_t nk = (int32_t)n[k];
+                raw[k] = h[k] + o[k];
+                bool valid = w.periodic[k] || (raw[k] >= 0 && raw[k] < nk);
+                bool unique = !w.periodic[k] ||
+                              !((n[k] == 1 && o[k] != 0) || (n[k] == 2 && ((h[k] == 0 && o[k] > 0) || (h[k] == 1 && o[k] < 0))));
+                ok = ok && valid && unique;
+                cell[k] = ((raw[k] % nk) + nk) % nk;
+                shift[k] = raw[k] < 0 ? -L[k] : (raw[k] >= nk ? L[k] : 0.f);
+            }
+            if (!ok) continue;
+            for (uint32_t other : cells[((uint64_t)cell[2] * n[1] + cell[1]) * n[0] + cell[0]]) {
+                if (!(other > seg && w.seg_fiber[other] != w.seg_fiber[seg])) continue;
+                const float *P2 = &w.pos[3 * (size_t)w.seg_vtx[2 * other]];
+                const float *Q2 = &w.pos[3 * (size_t)w.seg_vtx[2 * other + 1]];
+                float p2[3], q2[3];
+                for (int k = 0; k < 3; ++k) {
+                    p2[k] = P2[k] + shift[k];
+                    q2[k] = Q2[k] + shift[k];
+                }
+                for (int k = 0; k < 3; ++k) {
+                    float image = 0.5f * (p2[k] + q2[k] - P1[k] - Q1[k]);
+                    if (w.periodic[k]) {
+                        float sh = std::round(image / L[k]) * L[k];
+                        p2[k] -= sh;
+                        q2[k] -= sh;
+                    }
+                }
+                float d2[3] = {q2[0] - p2[0], q2[1] - p2[1], q2[2] - p2[2]};
+                float r[3] = {P1[0] - p2[0], P1[1] - p2[1], P1[2] - p2[2]};
+                float a = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+                float e = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+                float f = d2[0] * r[0] + d2[1] * r[1] + d2[2] * r[2];
+                float sp = 0.f, tp = 0.f;
+                if (a <= eps_len && e > eps_len) {
+                    tp = rclamp(f / e, 0.f, 1.f);
+                } else if (a > eps_len) {
+                    float c = d1[0] * r[0] + d1[1] * r[1] + d1[2] * r[2];
+                    if (e <= eps_len) {
+                        sp = rclamp(-c / a, 0.f, 1.f);
+                    } else {
+                        float b = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+                        float den = a * e - b * b;
+                        if (std::fabs(den) > eps_par * a * e) sp = rclamp((b * f - c * e) / den, 0.f, 1.f);
+                        float proj = (b * sp + f) / e;
+                        if (proj < 0.f) {
+                            sp = rclamp(-c / a, 0.f, 1.f);
+                        } else if (proj > 1.f) {
+                            tp = 1.f;
+                            sp = rclamp((b - c) / a, 0.f, 1.f);
+                        } else {
+                            tp = proj;
+                        }
+                    }
+                }
+                float dx = (p2[0] + d2[0] * tp) - (P1[0] + d1[0] * sp);
+                float dy = (p2[1] + d2[1] * tp) - (P1[1] + d1[1] * sp);
+                float dz = (p2[2] + d2[2] * tp) - (P1[2] + d1[2] * sp);
+                float center = std::sqrt(dx * dx + dy * dy + dz * dz);
+                float gap = center - w.seg_radius[seg] - w.seg_radius[other];
+                if (!(gap <= max_gap)) continue;
+                float product = std::sqrt(a * e);
+                float angle = 0.f;
+                if (product > eps_len)
+                    angle = std::acos(rclamp(std::fabs((d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) / product), 0.f, 1.f));
+                found.push_back(Candidate{seg, other, sp, tp, gap, angle});
+            }
+        }
+    }
+    overflow = found.size() > capacity;
+    // the device keeps the first `capacity` appends in a run-dependent order;
+    // which ones is unspecified when it overflows, so keep the sorted prefix
+    std::sort(found.begin(), found.end(), [](const Candidate &l, const Candidate &r) {
+        if (l.first != r.first) return l.first < r.first;
+        if (l.second != r.second) return l.second < r.second;
+        if (total_key(l.s) != total_key(r.s)) return total_key(l.s) < total_key(r.s);
+        return total_key(l.t) < total_key(r.t);
+    });
+    if (found.size() > capacity) found.resize(capacity);
+    out.swap(found);
+    w.nstate[0] = 1;  // the shared grid held the capture layout
+    return TGH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1207,6 +1470,52 @@ void orc_world_formation_target_error(void *h, float *layer_error, float *vertex
     *vertex_error = ve;
     *has_layer = w.layer_targets.empty() ? 0u : 1u;
     *has_vertex = w.vt_index.empty() ? 0u : 1u;
+}
+
+int orc_world_compact_cell(void *h, const float *lower, const float *upper, uint32_t kinematics) {
+    return compact_cell(*(World *)h, lower, upper, kinematics);
+}
+
+void orc_world_cell_bounds(void *h, float *lower, float *upper) {
+    World &w = *(World *)h;
+    for (int k = 0; k < 3; ++k) {
+        lower[k] = w.lower[k];
+        upper[k] = w.upper[k];
+    }
+}
+
+void orc_world_compaction_metrics(void *h, float correction_fraction, float contact_stiffness,
+                                  float stretch_stiffness, float bending_stiffness, float *out) {
+    compaction_metrics(*(World *)h, correction_fraction, contact_stiffness, stretch_stiffness, bending_stiffness,
+                       out);
+}
+
+// out: 6 words per candidate (first, second as u32 bits; s, t, gap, angle)
+int orc_world_capture_contacts(void *h, float max_gap, size_t capacity, float *out, size_t *count,
+                               uint32_t *overflow) {
+    std::vector<Candidate> found;
+    bool over = false;
+    int rc = capture_contacts(*(World *)h, max_gap, capacity, found, over);
+    if (rc) return rc;
+    for (size_t i = 0; i < found.size(); ++i) {
+        std::memcpy(&out[6 * i], &found[i].first, 4);
+        std::memcpy(&out[6 * i + 1], &found[i].second, 4);
+        out[6 * i + 2] = found[i].s;
+        out[6 * i + 3] = found[i].t;
+        out[6 * i + 4] = found[i].gap;
+        out[6 * i + 5] = found[i].angle;
+    }
+    *count = found.size();
+    *overflow = over ? 1u : 0u;
+    return TGH_OK;
+}
+
+// set_fiber_maximum_curvatures (world.rs:2182-2191): the limit of every
+// vertex of the fiber (only internal vertices are ever measured or projected)
+void orc_world_set_fiber_maximum_curvatures(void *h, const float *values) {
+    World &w = *(World *)h;
+    for (uint32_t f = 0; f < w.F; ++f)
+        for (uint32_t i = 0; i < w.fiber_span[2 * f + 1]; ++i) w.vtx_kmax[w.fiber_span[2 * f] + i] = values[f];
 }
 
 void orc_world_destroy(void *h) { delete (World *)h; }

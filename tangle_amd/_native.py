@@ -138,7 +138,97 @@ class CheckpointStruct(C.Structure):
         ("vertex_indices", C.POINTER(C.c_uint32)),
         ("vertex_targets", C.POINTER(C.c_float)),
         ("vertex_commands", C.POINTER(C.c_float)),
+        ("cell_lower", C.c_float * 3),
+        ("cell_upper", C.c_float * 3),
     ]
+
+
+class CompactionMetricsStruct(C.Structure):
+    _fields_ = [
+        ("pressure", C.c_float * 3),
+        ("contact_energy", C.c_float),
+        ("stretch_energy", C.c_float),
+        ("bending_energy", C.c_float),
+        ("total_energy", C.c_float),
+        ("face_pressure", (C.c_float * 2) * 3),
+    ]
+
+
+class ContactCandidateStruct(C.Structure):
+    _fields_ = [
+        ("first_segment", C.c_uint32),
+        ("second_segment", C.c_uint32),
+        ("first_coordinate", C.c_float),
+        ("second_coordinate", C.c_float),
+        ("surface_gap", C.c_float),
+        ("crossing_angle", C.c_float),
+    ]
+
+
+# CompactionKinematics (crates/tangle_relax/src/compaction.rs), by the names
+# the Python API uses (crates/tangle_python/src/compaction.rs:15-22)
+COMPACT_RIGID_FIBER_CENTERS = 0
+COMPACT_MOVING_WALLS = 1
+COMPACT_AFFINE_VERTICES = 2
+KINEMATICS = {"rigid_fiber_centers": COMPACT_RIGID_FIBER_CENTERS, "moving_walls": COMPACT_MOVING_WALLS,
+              "affine_vertices": COMPACT_AFFINE_VERTICES}
+
+
+def kinematics_code(kinematics) -> int:
+    if isinstance(kinematics, str):
+        if kinematics not in KINEMATICS:
+            raise ValueError(f"kinematics must be one of {', '.join(KINEMATICS)}")
+        return KINEMATICS[kinematics]
+    code = int(kinematics)
+    if code not in KINEMATICS.values():
+        raise ValueError("unknown compaction kinematics")
+    return code
+
+
+@dataclass(frozen=True)
+class CompactionEnergyModel:
+    """Penalty stiffnesses of the compaction metrics (CompactionEnergyModel,
+    crates/tangle_relax/src/compaction.rs:21-38)."""
+
+    contact_stiffness: float = 1.0
+    stretch_stiffness: float = 1.0
+    bending_stiffness: float = 1.0
+
+
+@dataclass(frozen=True)
+class CompactionMetrics:
+    """Directional pressures and penalty energies of the resident state
+    (CompactionMetrics, compaction.rs:42-53), plus the lower/upper face
+    pressures per axis (wall_face_pressures, world.rs:2113-2127)."""
+
+    pressure: tuple
+    contact_energy: float
+    stretch_energy: float
+    bending_energy: float
+    total_energy: float
+    face_pressures: tuple  # ((lower x, upper x), (lower y, upper y), (lower z, upper z))
+
+
+@dataclass(frozen=True)
+class SegmentContactCandidate:
+    """One captured inter-fiber segment pair (SegmentContactCandidate,
+    world.rs:44-57)."""
+
+    first_segment: int
+    second_segment: int
+    first_coordinate: float
+    second_coordinate: float
+    surface_gap: float
+    crossing_angle: float
+
+
+@dataclass(frozen=True)
+class ContactCapture:
+    """Captured pairs, sorted, and whether the capacity was exceeded
+    (ContactCapture, world.rs:61-66)."""
+
+    candidates: list
+    overflow: bool
 
 
 @dataclass
@@ -154,6 +244,8 @@ class DeviceCheckpoint:
     wall_reactions: np.ndarray   # (3V,) float32
     layer_target: dict | None = None   # axis, first, last, stiffness, max_translation, targets
     vertex_target: dict | None = None  # axis, stiffness, max_translation, indices, targets, commands
+    cell_lower: tuple | None = None    # the world's cell when compacted since creation
+    cell_upper: tuple | None = None
 
     def struct(self) -> CheckpointStruct:
         """A tgh_checkpoint view of these arrays (keep self alive while used)."""
@@ -167,6 +259,9 @@ class DeviceCheckpoint:
         c.positions = ptr(self.positions, C.c_float)
         c.fiber_active = ptr(self.fiber_active, C.c_uint32)
         c.wall_reactions = ptr(self.wall_reactions, C.c_float)
+        for k in range(3):  # NaN: the cell of the assembly the world is made from
+            c.cell_lower[k] = float("nan") if self.cell_lower is None else float(self.cell_lower[k])
+            c.cell_upper[k] = float("nan") if self.cell_upper is None else float(self.cell_upper[k])
         if self.layer_target is not None:
             lt = self.layer_target
             lt["targets"] = np.ascontiguousarray(lt["targets"], np.float32)
@@ -247,6 +342,13 @@ SIGNATURES = {
                                                    C.POINTER(C.c_uint32)]),
     "tgh_world_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "tgh_world_checkpoint_sizes": (C.c_int, [C.c_void_p, C.POINTER(CheckpointStruct)]),
+    "tgh_world_cell_bounds": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "tgh_world_compact_cell": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32]),
+    "tgh_world_compaction_metrics": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float,
+                                               C.POINTER(CompactionMetricsStruct)]),
+    "tgh_world_capture_contacts": (C.c_int, [C.c_void_p, C.c_float, C.c_size_t, C.POINTER(ContactCandidateStruct),
+                                             C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)]),
+    "tgh_world_set_fiber_maximum_curvatures": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t]),
     "tgh_world_checkpoint": (C.c_int, [C.c_void_p, C.POINTER(CheckpointStruct)]),
     "tgh_world_restore": (C.c_int, [C.POINTER(Assembly), C.POINTER(Config), C.POINTER(CheckpointStruct),
                                     C.POINTER(C.c_void_p)]),
@@ -440,6 +542,8 @@ class World:
         c = ck.struct()
         check(lib.tgh_world_checkpoint(self._handle, C.byref(c)))
         ck.total_iterations = int(c.total_iterations)
+        ck.cell_lower = tuple(float(v) for v in c.cell_lower)
+        ck.cell_upper = tuple(float(v) for v in c.cell_upper)
         if ck.layer_target is not None:
             ck.layer_target.update(axis=int(c.layer_axis), first=int(c.layer_first), last=int(c.layer_last),
                                    stiffness=float(c.layer_stiffness),
@@ -553,6 +657,63 @@ class World:
 
     def rebuild_lists(self) -> None:
         check(library().tgh_world_rebuild_lists(self._handle))
+
+    # compaction and contact capture (world.rs:1931-2334)
+    def cell_bounds(self):
+        """(lower, upper) of the resident cell, f32 values (cell_bounds)."""
+        lo, hi = (C.c_float * 3)(), (C.c_float * 3)()
+        check(library().tgh_world_cell_bounds(self._handle, lo, hi))
+        return tuple(float(v) for v in lo), tuple(float(v) for v in hi)
+
+    def compact_cell(self, new_lower, new_upper, kinematics="moving_walls") -> None:
+        """Changes the cell and moves the active material with it
+        (compact_cell); kinematics "rigid_fiber_centers", "moving_walls" or
+        "affine_vertices"."""
+        lo = (C.c_float * 3)(*[float(v) for v in new_lower])
+        hi = (C.c_float * 3)(*[float(v) for v in new_upper])
+        check(library().tgh_world_compact_cell(self._handle, lo, hi, kinematics_code(kinematics)))
+
+    def compaction_metrics(self, correction_fraction: float, model: CompactionEnergyModel | None = None):
+        """Pressures and penalty energies of the current state
+        (compaction_metrics)."""
+        model = model or CompactionEnergyModel()
+        m = CompactionMetricsStruct()
+        check(library().tgh_world_compaction_metrics(self._handle, float(correction_fraction),
+                                                     float(model.contact_stiffness), float(model.stretch_stiffness),
+                                                     float(model.bending_stiffness), C.byref(m)))
+        return CompactionMetrics(tuple(float(v) for v in m.pressure), float(m.contact_energy),
+                                 float(m.stretch_energy), float(m.bending_energy), float(m.total_energy),
+                                 tuple((float(m.face_pressure[k][0]), float(m.face_pressure[k][1])) for k in range(3)))
+
+    def capture_contacts(self, maximum_surface_gap: float, capacity: int) -> ContactCapture:
+        """Inter-fiber segment pairs within the surface gap, sorted
+        (capture_contacts)."""
+        out = (ContactCandidateStruct * max(1, int(capacity)))()
+        count, overflow = C.c_size_t(), C.c_uint32()
+        check(library().tgh_world_capture_contacts(self._handle, float(maximum_surface_gap), int(capacity), out,
+                                                   C.byref(count), C.byref(overflow)))
+        cands = [SegmentContactCandidate(int(c.first_segment), int(c.second_segment), float(c.first_coordinate),
+                                         float(c.second_coordinate), float(c.surface_gap), float(c.crossing_angle))
+                 for c in out[: count.value]]
+        return ContactCapture(cands, bool(overflow.value))
+
+    def capture_contact_arrays(self, maximum_surface_gap: float, capacity: int):
+        """capture_contacts as arrays: (pairs (n, 2) u32, coordinates (n, 2),
+        gaps (n,), angles (n,), overflow)."""
+        out = np.zeros((max(1, int(capacity)), 6), dtype=np.float32)
+        count, overflow = C.c_size_t(), C.c_uint32()
+        check(library().tgh_world_capture_contacts(self._handle, float(maximum_surface_gap), int(capacity),
+                                                   out.ctypes.data_as(C.POINTER(ContactCandidateStruct)),
+                                                   C.byref(count), C.byref(overflow)))
+        n = int(count.value)
+        return (out[:n, :2].view(np.uint32).copy(), out[:n, 2:4].copy(), out[:n, 4].copy(), out[:n, 5].copy(),
+                bool(overflow.value))
+
+    def set_fiber_maximum_curvatures(self, maximum_curvatures) -> None:
+        """Per-fiber curvature limits, inverse length, 0 disables
+        (set_fiber_maximum_curvatures)."""
+        arr = np.ascontiguousarray(maximum_curvatures, dtype=np.float32)
+        check(library().tgh_world_set_fiber_maximum_curvatures(self._handle, ptr(arr, C.c_float), arr.size))
 
     def cell_list(self):
         info = self.info()

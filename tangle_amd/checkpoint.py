@@ -3,7 +3,8 @@ crates/tangle_python/src/checkpoint.rs).
 
 A checkpoint file holds the case identifier, the persistent device state of
 the world (`_native.DeviceCheckpoint`: positions, fiber activity, wall
-reactions, the lifetime iteration count and the held formation targets) and
+reactions, the lifetime iteration count, the held formation targets and the
+cell, which compaction may have changed since creation) and
 the recipe's progress (the operation in progress and where it started, the
 held layer targets, the events so far). Scratch broad-phase state is not
 kept; the first batch after a restore rebuilds it.
@@ -94,6 +95,8 @@ def save(path, case_id: str, device: DeviceCheckpoint, recipe: dict) -> int:
     path = Path(path)
     arrays = _arrays(device)
     scalars = {"total_iterations": int(device.total_iterations)}
+    if device.cell_lower is not None:
+        scalars["cell"] = [[float(v) for v in device.cell_lower], [float(v) for v in device.cell_upper]]
     if device.layer_target is not None:
         scalars["layer_target"] = {k: v for k, v in device.layer_target.items() if k != "targets"}
     if device.vertex_target is not None:
@@ -155,6 +158,8 @@ def load(path, case_id: str) -> tuple[DeviceCheckpoint, dict]:
     dev = header["device"]
     ck = DeviceCheckpoint(int(dev["total_iterations"]), arrays["positions"], arrays["fiber_active"],
                           arrays["wall_reactions"])
+    if "cell" in dev:
+        ck.cell_lower, ck.cell_upper = tuple(dev["cell"][0]), tuple(dev["cell"][1])
     if "layer_target" in dev:
         ck.layer_target = dict(dev["layer_target"], targets=arrays["layer_targets"])
     if "vertex_target" in dev:

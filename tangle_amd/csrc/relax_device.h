@@ -35,6 +35,12 @@ constexpr uint32_t kFlagMask = (1u << kSpecShift) - 1u;
 // once in f32, exactly as the reference's proxy_cell computes them per call.
 // The specimen's cells occupy [cell_base, cell_base + cells product) of the
 // world's cell list; `skin` is its Verlet skin.
+// A change of the cell: old and new bounds and the periodic axes.
+struct CellChange {
+    float old_lower[3], old_upper[3], new_lower[3], new_upper[3];
+    uint32_t periodic[3];
+};
+
 struct GridParams {
     float lower[3];
     float upper[3];
@@ -109,6 +115,7 @@ struct WorldDev {
     uint4 *slotC;
     uint32_t *ncount, *nbr, *ovf;
     float4 *corr0, *corr1;
+    float *ratio;  // per vertex: curvature ratio at the last measure (0 until measured)
 };
 
 // Per-batch scalar settings of the relaxation loop (RelaxationConfig).
@@ -150,6 +157,15 @@ hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t 
 hipError_t launch_activate(const WorldDev &w, uint32_t maximum_step, hipStream_t stream);
 // mirrors w.factive (as uploaded) into every vertex's inactive flag
 hipError_t launch_apply_activity(const WorldDev &w, hipStream_t stream);
+// compaction: kinematics 0 rigid fiber centres, 1 moving walls, 2 affine
+hipError_t launch_compact(const WorldDev &w, const CellChange &box, uint32_t kinematics, hipStream_t stream);
+// the 13 compaction metrics (measure_compaction_metrics order) into out[13]
+hipError_t launch_compaction_metrics(const WorldDev &w, const GridParams *grid, float correction_fraction,
+                                     float contact_stiffness, float stretch_stiffness, float bending_stiffness,
+                                     float *out, hipStream_t stream);
+hipError_t launch_capture(const WorldDev &w, const GridParams *capture_grid, uint32_t capture_cells, float max_gap,
+                          uint32_t capacity, uint32_t *counter, uint2 *pairs, float2 *coordinates, float *gaps,
+                          float *angles, hipStream_t stream);
 hipError_t launch_layer_targets(const WorldDev &w, const FormationDev &f, bool gate_on_running, hipStream_t stream);
 hipError_t launch_init_vertex_targets(const WorldDev &w, const FormationDev &f, float displacement,
                                       hipStream_t stream);

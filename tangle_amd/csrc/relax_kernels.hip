@@ -751,17 +751,20 @@ __device__ __forceinline__ void assess_specimen(SpecState &sp, uint32_t pen_bits
 // Curvature ratio of every vertex of a running specimen, reduced per
 // workgroup into the specimen's slots.
 __global__ __launch_bounds__(256) void k_measure(const SpecState *ss, uint32_t red_slots, const float4 *pos,
-                                                 const float4 *vA, const float4 *vB, uint32_t *red, uint32_t V) {
+                                                 const float4 *vA, const float4 *vB, uint32_t *red, float *ratios,
+                                                 uint32_t V) {
     __shared__ uint32_t scratch[5];
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     float ratio = 0.f;
     uint32_t spec = 0;
+    bool measured = false;
     if (v < V) {
         const float4 b4 = vB[v];
         const uint32_t flags = __float_as_uint(b4.z);
         spec = flags >> kSpecShift;
         const float kmax = vA[v].x;
-        if (running(ss, spec) && !(flags & kInactive) && kmax > 0.f && (flags & kHasPrev) && (flags & kHasNext)) {
+        measured = running(ss, spec) && !(flags & kInactive);
+        if (measured && kmax > 0.f && (flags & kHasPrev) && (flags & kHasNext)) {
             float4 a = pos[v - 1], b = pos[v], c = pos[v + 1];
             float ax = b.x - a.x, ay = b.y - a.y, az = b.z - a.z;
             float bx = c.x - b.x, by = c.y - b.y, bz = c.z - b.z;
@@ -775,6 +778,9 @@ __global__ __launch_bounds__(256) void k_measure(const SpecState *ss, uint32_t r
             }
         }
     }
+    // every measured vertex keeps its ratio (the reference's curvature_ratio
+    // buffer), which the compaction metrics read
+    if (measured) ratios[v] = ratio;
     spec_block_max(__float_as_uint(fmaxf(ratio, 0.f)), spec, red, kMetricCurv, red_slots, scratch);
 }
 
@@ -1075,8 +1081,9 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, const S
         KW[i] = make_float2(a.x, (__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f);
         if (track_walls) WR[i] = wall[start + i];
     }
-    // the axis of an exact lane and that axis's walls, held in registers
-    const uint32_t k = gl < 3 ? gl : 0u;
+    // the axis of an exact lane (lanes 0..2, and 4..6 of the two-quad form)
+    // and that axis's walls, held in registers
+    const uint32_t k = (gl & 3u) < 3u ? (gl & 3u) : 0u;
     const bool walled = !g.periodic[k];
     const float lo = g.lower[k], hi = g.upper[k];
     __syncthreads();
@@ -1669,6 +1676,312 @@ __global__ __launch_bounds__(256) void k_target_error(const float4 *pos, const u
     }
 }
 
+// ---------------------------------------------------------------------------
+// Compaction kinematics (compact_rigid_fiber_centers, compact_affine_vertices,
+// compact_moving_walls, kernels.rs:712-809): the cell changes from `old` to
+// `nw` and the active material follows it; wall reactions are reset (rigid,
+// affine) or set to the projection onto the new walls (moving walls).
+
+// rigid: each active fiber translates so that its vertex-mean centre keeps
+// its fraction of the cell, its shape unchanged
+__global__ void k_compact_rigid(float4 *pos, const uint4 *fspan, const float4 *vB, float4 *wall, uint32_t F,
+                                CellChange box) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const uint4 span = fspan[f];
+    if (span.y == 0 || (__float_as_uint(vB[span.x].z) & kInactive)) return;
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    for (uint32_t i = 0; i < span.y; ++i) {
+        const float4 p = pos[span.x + i];
+        cx += p.x;
+        cy += p.y;
+        cz += p.z;
+    }
+    const float n = (float)span.y;
+    cx /= n;
+    cy /= n;
+    cz /= n;
+    const float c[3] = {cx, cy, cz};
+    float d[3];
+    for (int k = 0; k < 3; ++k) {
+        const float fraction = (c[k] - box.old_lower[k]) / (box.old_upper[k] - box.old_lower[k]);
+        d[k] = box.new_lower[k] + fraction * (box.new_upper[k] - box.new_lower[k]) - c[k];
+    }
+    for (uint32_t i = 0; i < span.y; ++i) {
+        float4 p = pos[span.x + i];
+        p.x += d[0];
+        p.y += d[1];
+        p.z += d[2];
+        pos[span.x + i] = p;
+        if (wall) wall[span.x + i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// affine: every active vertex keeps its fractional coordinates
+__global__ void k_compact_affine(float4 *pos, const float4 *vB, float4 *wall, uint32_t V, CellChange box) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V || (__float_as_uint(vB[v].z) & kInactive)) return;
+    float4 p = pos[v];
+    float q[3] = {p.x, p.y, p.z};
+    for (int k = 0; k < 3; ++k) {
+        const float fraction = (q[k] - box.old_lower[k]) / (box.old_upper[k] - box.old_lower[k]);
+        q[k] = box.new_lower[k] + fraction * (box.new_upper[k] - box.new_lower[k]);
+    }
+    p.x = q[0];
+    p.y = q[1];
+    p.z = q[2];
+    pos[v] = p;
+    if (wall) wall[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// moving walls: active vertices are projected inside the new walls of the
+// non-periodic axes; the reaction is the projection (negative: lower face)
+__global__ void k_compact_walls(float4 *pos, const float4 *vA, const float4 *vB, float4 *wall, uint32_t V,
+                                CellChange box) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V || (__float_as_uint(vB[v].z) & kInactive)) return;
+    float4 p = pos[v];
+    const float extent = vA[v].w;
+    float q[3] = {p.x, p.y, p.z}, r[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < 3; ++k) {
+        if (box.periodic[k]) continue;
+        const float previous = q[k];
+        q[k] = rclamp(previous, box.new_lower[k] + extent, box.new_upper[k] - extent);
+        r[k] = previous - q[k];
+    }
+    p.x = q[0];
+    p.y = q[1];
+    p.z = q[2];
+    pos[v] = p;
+    if (wall) wall[v] = make_float4(r[0], r[1], r[2], 0.f);
+}
+
+// measure_compaction_metrics (kernels.rs:811-923). The reference reduces on
+// one thread in index order; so does lane 0 here, so every f32 sum rounds
+// alike. The other lanes only fetch: each chunk of 64 segments (then
+// vertices) has its per-item terms computed by all lanes into LDS, then
+// lane 0 adds them in order. out[13] = {pressure x, y, z, contact, stretch,
+// bending and total energy, lower x, upper x, lower y, upper y, lower z,
+// upper z face pressures}.
+__global__ __launch_bounds__(64) void k_compaction_metrics(const GridParams *grid, const float4 *pos,
+                                                           const uint4 *sstat, const uint32_t *factive,
+                                                           const float4 *vA, const float4 *corr0,
+                                                           const float4 *corr1, const float *ratios,
+                                                           const float4 *wall, uint32_t S, uint32_t V,
+                                                           float correction_fraction, float contact_stiffness,
+                                                           float stretch_stiffness, float bending_stiffness,
+                                                           float *out) {
+    __shared__ float terms[7][64];
+    const uint32_t lane = threadIdx.x;
+    float load_x = 0.f, load_y = 0.f, load_z = 0.f, contact = 0.f, stretch = 0.f;
+    for (uint32_t base = 0; base < S; base += 64) {
+        const uint32_t s = base + lane;
+        float t[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+        if (s < S) {
+            const uint4 me = sstat[s];
+            if (factive[me.y] != 0u) {
+                const float4 c0 = corr0[s], c1 = corr1[s];
+                t[0] = fabsf(c0.x) + fabsf(c1.x);
+                t[1] = fabsf(c0.y) + fabsf(c1.y);
+                t[2] = fabsf(c0.z) + fabsf(c1.z);
+                const float penetration = c0.w;
+                t[3] = 0.5f * contact_stiffness * penetration * penetration;
+                const float4 a = pos[me.x], b = pos[me.x + 1];
+                const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+                const float length = sqrtf(dx * dx + dy * dy + dz * dz);
+                const float rest = vA[me.x].z;
+                if (rest > 1.0e-7f) {
+                    const float extension = length - rest;
+                    t[4] = 0.5f * stretch_stiffness * extension * extension / rest;
+                }
+            }
+        }
+        // skipped items add +0, which leaves the nonnegative sums unchanged
+        for (int k = 0; k < 5; ++k) terms[k][lane] = t[k];
+        __syncthreads();
+        if (lane == 0) {
+            const uint32_t n = min(64u, S - base);
+            for (uint32_t i = 0; i < n; ++i) {
+                load_x += terms[0][i];
+                load_y += terms[1][i];
+                load_z += terms[2][i];
+                contact += terms[3][i];
+                stretch += terms[4][i];
+            }
+        }
+        __syncthreads();
+    }
+    float bending = 0.f, lower_x = 0.f, upper_x = 0.f, lower_y = 0.f, upper_y = 0.f, lower_z = 0.f, upper_z = 0.f;
+    for (uint32_t base = 0; base < V; base += 64) {
+        const uint32_t v = base + lane;
+        float t[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (v < V) {
+            const float excess = fmaxf(ratios[v] - 1.0f, 0.f);
+            t[0] = 0.5f * bending_stiffness * excess * excess;
+            if (wall) {
+                const float4 r = wall[v];
+                t[1] = fmaxf(-r.x, 0.f);
+                t[2] = fmaxf(r.x, 0.f);
+                t[3] = fmaxf(-r.y, 0.f);
+                t[4] = fmaxf(r.y, 0.f);
+                t[5] = fmaxf(-r.z, 0.f);
+                t[6] = fmaxf(r.z, 0.f);
+            }
+        }
+        for (int k = 0; k < 7; ++k) terms[k][lane] = t[k];
+        __syncthreads();
+        if (lane == 0) {
+            const uint32_t n = min(64u, V - base);
+            for (uint32_t i = 0; i < n; ++i) {
+                bending += terms[0][i];
+                lower_x += terms[1][i];
+                upper_x += terms[2][i];
+                lower_y += terms[3][i];
+                upper_y += terms[4][i];
+                lower_z += terms[5][i];
+                upper_z += terms[6][i];
+            }
+        }
+        __syncthreads();
+    }
+    if (lane != 0) return;
+    const GridParams g = *grid;
+    const float length_x = g.upper[0] - g.lower[0];
+    const float length_y = g.upper[1] - g.lower[1];
+    const float length_z = g.upper[2] - g.lower[2];
+    const float inverse_projection = 1.0f / fmaxf(correction_fraction, 1.0e-7f);
+    const float pressure_x = g.periodic[0] ? load_x : lower_x + upper_x;
+    const float pressure_y = g.periodic[1] ? load_y : lower_y + upper_y;
+    const float pressure_z = g.periodic[2] ? load_z : lower_z + upper_z;
+    const float area_x = length_y * length_z, area_y = length_x * length_z, area_z = length_x * length_y;
+    out[0] = contact_stiffness * pressure_x * inverse_projection / area_x;
+    out[1] = contact_stiffness * pressure_y * inverse_projection / area_y;
+    out[2] = contact_stiffness * pressure_z * inverse_projection / area_z;
+    out[3] = contact;
+    out[4] = stretch;
+    out[5] = bending;
+    out[6] = contact + stretch + bending;
+    out[7] = contact_stiffness * lower_x * inverse_projection / area_x;
+    out[8] = contact_stiffness * upper_x * inverse_projection / area_x;
+    out[9] = contact_stiffness * lower_y * inverse_projection / area_y;
+    out[10] = contact_stiffness * upper_y * inverse_projection / area_y;
+    out[11] = contact_stiffness * lower_z * inverse_projection / area_z;
+    out[12] = contact_stiffness * upper_z * inverse_projection / area_z;
+}
+
+// ---------------------------------------------------------------------------
+// Contact capture (capture_segment_contacts, contact.rs:9-230): every active
+// segment scans the 27 cells around its midpoint on a whole-segment grid and
+// appends each later segment of another fiber whose surface gap is at most
+// `max_gap`, with the closest-point coordinates and the crossing angle. The
+// append order is unordered; the host sorts the pairs.
+__global__ void k_capture(const GridParams *grid, const float4 *pos, const uint4 *sstat, const uint32_t *factive,
+                          const uint32_t *cell_counts, const uint32_t *cell_offsets, const uint32_t *cell_segs,
+                          uint32_t S, float max_gap, uint32_t capacity, uint32_t *counter, uint2 *pairs,
+                          float2 *coordinates, float *gaps, float *angles) {
+    const uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x;
+    if (seg >= S) return;
+    const uint4 me = sstat[seg];
+    if (factive[me.y] == 0u) return;
+    const GridParams g = *grid;
+    const float4 P1 = pos[me.x], Q1 = pos[me.x + 1];
+    const float p1x = P1.x, p1y = P1.y, p1z = P1.z, q1x = Q1.x, q1y = Q1.y, q1z = Q1.z;
+    const float d1x = q1x - p1x, d1y = q1y - p1y, d1z = q1z - p1z;
+    const float radius = __uint_as_float(me.z);
+    const float eps_len = 1.0e-20f, eps_par = 1.0e-6f;
+    uint32_t hx, hy, hz;
+    cell_coords(g, p1x, p1y, p1z, q1x, q1y, q1z, hx, hy, hz);
+    const uint32_t home = (hz * g.cells[1] + hy) * g.cells[0] + hx;
+    for (uint32_t n = 0; n < 27; ++n) {
+        uint32_t cell;
+        if (!stencil_cell(g, home, n, cell)) continue;
+        // the unwrapped neighbour's box shift, then the minimum image
+        const int32_t o[3] = {(int32_t)(n % 3) - 1, (int32_t)((n / 3) % 3) - 1, (int32_t)(n / 9) - 1};
+        const uint32_t h[3] = {hx, hy, hz};
+        float shift[3] = {0.f, 0.f, 0.f};
+        for (int k = 0; k < 3; ++k) {
+            const int32_t raw = (int32_t)h[k] + o[k];
+            if (raw < 0) shift[k] = -(g.upper[k] - g.lower[k]);
+            else if (raw >= (int32_t)g.cells[k]) shift[k] = g.upper[k] - g.lower[k];
+        }
+        const uint32_t start = cell_offsets[cell], count = cell_counts[cell];
+        for (uint32_t j = 0; j < count; ++j) {
+            const uint32_t other = cell_segs[start + j];
+            if (other <= seg) continue;
+            const uint4 ot = sstat[other];
+            if (ot.y == me.y) continue;
+            const float4 P2 = pos[ot.x], Q2 = pos[ot.x + 1];
+            float p2x = P2.x + shift[0], p2y = P2.y + shift[1], p2z = P2.z + shift[2];
+            float q2x = Q2.x + shift[0], q2y = Q2.y + shift[1], q2z = Q2.z + shift[2];
+            const float lx = g.upper[0] - g.lower[0], ly = g.upper[1] - g.lower[1], lz = g.upper[2] - g.lower[2];
+            const float idx = 0.5f * (p2x + q2x - p1x - q1x);
+            const float idy = 0.5f * (p2y + q2y - p1y - q1y);
+            const float idz = 0.5f * (p2z + q2z - p1z - q1z);
+            if (g.periodic[0]) {
+                const float sh = roundf(idx / lx) * lx;
+                p2x -= sh;
+                q2x -= sh;
+            }
+            if (g.periodic[1]) {
+                const float sh = roundf(idy / ly) * ly;
+                p2y -= sh;
+                q2y -= sh;
+            }
+            if (g.periodic[2]) {
+                const float sh = roundf(idz / lz) * lz;
+                p2z -= sh;
+                q2z -= sh;
+            }
+            const float d2x = q2x - p2x, d2y = q2y - p2y, d2z = q2z - p2z;
+            const float rx = p1x - p2x, ry = p1y - p2y, rz = p1z - p2z;
+            const float a = d1x * d1x + d1y * d1y + d1z * d1z;
+            const float e = d2x * d2x + d2y * d2y + d2z * d2z;
+            const float f = d2x * rx + d2y * ry + d2z * rz;
+            float sp = 0.f, tp = 0.f;
+            if (a <= eps_len && e > eps_len) {
+                tp = rclamp(f / e, 0.f, 1.f);
+            } else if (a > eps_len) {
+                const float c = d1x * rx + d1y * ry + d1z * rz;
+                if (e <= eps_len) {
+                    sp = rclamp(-c / a, 0.f, 1.f);
+                } else {
+                    const float b = d1x * d2x + d1y * d2y + d1z * d2z;
+                    const float den = a * e - b * b;
+                    if (fabsf(den) > eps_par * a * e) sp = rclamp((b * f - c * e) / den, 0.f, 1.f);
+                    const float proj = (b * sp + f) / e;
+                    if (proj < 0.f) {
+                        sp = rclamp(-c / a, 0.f, 1.f);
+                    } else if (proj > 1.f) {
+                        tp = 1.f;
+                        sp = rclamp((b - c) / a, 0.f, 1.f);
+                    } else {
+                        tp = proj;
+                    }
+                }
+            }
+            const float dx = (p2x + d2x * tp) - (p1x + d1x * sp);
+            const float dy = (p2y + d2y * tp) - (p1y + d1y * sp);
+            const float dz = (p2z + d2z * tp) - (p1z + d1z * sp);
+            const float center = sqrtf(dx * dx + dy * dy + dz * dz);
+            const float gap = center - radius - __uint_as_float(ot.z);
+            if (!(gap <= max_gap)) continue;
+            const uint32_t k = atomicAdd(&counter[0], 1u);
+            if (k < capacity) {
+                pairs[k] = make_uint2(seg, other);
+                coordinates[k] = make_float2(sp, tp);
+                gaps[k] = gap;
+                const float product = sqrtf(a * e);
+                float angle = 0.f;
+                if (product > eps_len)
+                    angle = acosf_f32(rclamp(fabsf((d1x * d2x + d1y * d2y + d1z * d2z) / product), 0.f, 1.f));
+                angles[k] = angle;
+            } else {
+                atomicOr(&counter[1], 1u);
+            }
+        }
+    }
+}
+
 }  // namespace tgh
 
 // ---------------------------------------------------------------------------
@@ -1751,7 +2064,7 @@ hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t st
 
 hipError_t launch_measure(const WorldDev &w, const IterParams &p, hipStream_t stream) {
     hipLaunchKernelGGL(k_measure, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.ss, w.red_slots, w.pos, w.vA,
-                       w.vB, w.red, w.V);
+                       w.vB, w.red, w.ratio, w.V);
     hipLaunchKernelGGL(k_assess, dim3(1), dim3(256), 0, stream, w.ss, w.nspec, w.red_slots, w.red,
                        p.penetration_tolerance, p.curvature_ratio_tolerance, p.iterations, p.force_full);
     return hipGetLastError();
@@ -1814,6 +2127,57 @@ hipError_t launch_layer_targets(const WorldDev &w, const FormationDev &f, bool g
     hipLaunchKernelGGL(k_layer_targets, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.ss, w.pos, w.fspan,
                        w.flayer, w.factive, f.layer_targets, f.layer_count, f.layer_axis, f.first_layer,
                        f.last_layer, f.layer_stiffness, f.layer_max_translation, w.F, gate_on_running ? 1u : 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(const WorldDev &w, const CellChange &box, uint32_t kinematics, hipStream_t stream) {
+    float4 *wall = w.track_walls ? w.wall : nullptr;
+    if (kinematics == 0u) {
+        hipLaunchKernelGGL(k_compact_rigid, dim3(blocks_for(w.F, 64)), dim3(64), 0, stream, w.pos, w.fspan, w.vB,
+                           wall, w.F, box);
+    } else if (kinematics == 1u) {
+        hipLaunchKernelGGL(k_compact_walls, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.pos, w.vA, w.vB,
+                           wall, w.V, box);
+    } else {
+        hipLaunchKernelGGL(k_compact_affine, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.pos, w.vB, wall,
+                           w.V, box);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_compaction_metrics(const WorldDev &w, const GridParams *grid, float correction_fraction,
+                                     float contact_stiffness, float stretch_stiffness, float bending_stiffness,
+                                     float *out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_compaction_metrics, dim3(1), dim3(64), 0, stream, grid, w.pos, w.sstat, w.factive, w.vA,
+                       w.corr0, w.corr1, w.ratio, w.track_walls ? w.wall : nullptr, w.S, w.V, correction_fraction,
+                       contact_stiffness, stretch_stiffness, bending_stiffness, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_capture(const WorldDev &w, const GridParams *capture_grid, uint32_t capture_cells, float max_gap,
+                          uint32_t capacity, uint32_t *counter, uint2 *pairs, float2 *coordinates, float *gaps,
+                          float *angles, hipStream_t stream) {
+    if (w.S == 0 || capture_cells == 0) return hipErrorInvalidValue;
+    // the whole-segment cell list on the capture grid, in the shared buffers
+    WorldDev c = w;
+    c.grids = const_cast<GridParams *>(capture_grid);
+    c.C = capture_cells;
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(64), 0, stream, c.st);
+    const uint32_t clear_n = c.C > c.V ? c.C : c.V;
+    hipLaunchKernelGGL(k_rebuild_clear, dim3(blocks_for(clear_n, 256)), dim3(256), 0, stream, c.st, c.cell_counts,
+                       c.cell_cursor, c.C, c.pos, c.refpos, c.V);
+    hipLaunchKernelGGL(k_bin_segments, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.grids, c.pos,
+                       c.sstat, c.factive, c.seg_cell, c.cell_counts, c.S);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, c.st, c.cell_counts, c.cell_offsets, c.C);
+    hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.seg_cell,
+                       c.cell_offsets, c.cell_cursor, c.scattered, c.S);
+    hipLaunchKernelGGL(k_rank_cells, dim3(blocks_for(c.C, 4)), dim3(256), 0, stream, c.st, c.cell_counts,
+                       c.cell_offsets, c.scattered, c.cell_segs, c.pos, c.sstat, c.slotA, c.slotB, c.slotC, c.C);
+    hipLaunchKernelGGL(k_capture, dim3(blocks_for(c.S, 64)), dim3(64), 0, stream, capture_grid, c.pos, c.sstat,
+                       c.factive, c.cell_counts, c.cell_offsets, c.cell_segs, c.S, max_gap, capacity, counter, pairs,
+                       coordinates, gaps, angles);
+    // the rebuild request stays pending: the next batch rebuilds the lists
+    // on the relaxation grid
     return hipGetLastError();
 }
 

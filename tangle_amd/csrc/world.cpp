@@ -206,6 +206,8 @@ int size_grid(const Packed &p, const tgh_config *c, Sizing &z) {
         }
     const float piece = block_len < max_len ? 1.05f * block_len : block_len;
     z.cell_size = (piece + margin) * c->cell_size_scale;
+    z.segment_cell_size = (max_len + margin) * c->cell_size_scale;
+    z.max_radius = max_r;
     uint64_t C = 1;
     for (int k = 0; k < 3; ++k) {
         float extent = p.upper[k] - p.lower[k];
@@ -322,6 +324,7 @@ int upload_static(tgh_world *w) {
     HIP_TRY(hipMemset(d.wall, 0, V * sizeof(float4)));
     HIP_TRY(hipMemset(d.corr0, 0, S * sizeof(float4)));
     HIP_TRY(hipMemset(d.corr1, 0, S * sizeof(float4)));
+    HIP_TRY(hipMemset(d.ratio, 0, V * sizeof(float)));
     HIP_TRY(hipMemset(d.ncount, 0, S * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.red, 0, (size_t)d.nspec * 3 * d.red_slots * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_counts, 0, d.C * sizeof(uint32_t)));
@@ -460,6 +463,7 @@ int create_world(const tgh_assembly *a, uint32_t count, const tgh_config *c, tgh
     if (e == hipSuccess) e = w->alloc(&d.nbr, std::max<size_t>(1, (size_t)d.block * S));
     if (e == hipSuccess) e = w->alloc(&d.corr0, S);
     if (e == hipSuccess) e = w->alloc(&d.corr1, S);
+    if (e == hipSuccess) e = w->alloc(&d.ratio, V);
     if (e != hipSuccess) return fail(set_error(TGH_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e)));
     d.grids = grids;
     rc = upload_static(w);
@@ -921,6 +925,51 @@ int tgh_world_formation_target_error(tgh_world *world, float *layer_error, float
     return TGH_OK;
 }
 
+namespace {
+
+// Checks a new cell for a one-specimen world (compact_cell's assertions,
+// world.rs:1937-1948) and moves the grid to it: same cell size, the cell
+// counts of the new extent, every list stale.
+int set_cell(tgh_world *w, const float *lower, const float *upper) {
+    if (w->d.nspec != 1) return set_error(TGH_ERR_INVALID, "cell changes need a one-specimen world");
+    GridParams g = w->grids[0];
+    const tgh::Sizing &z = w->sizes[0];
+    uint64_t C = 1;
+    uint32_t cells[3];
+    for (int k = 0; k < 3; ++k) {
+        if (!std::isfinite(lower[k]) || !std::isfinite(upper[k]))
+            return set_error(TGH_ERR_INVALID, "cell bounds must be finite");
+        if (!(upper[k] - lower[k] > 2.0f * z.max_radius))
+            return set_error(TGH_ERR_INVALID, "the cell must be wider than two fiber radii on every axis");
+        if (!(lower[k] >= g.lower[k]) || !(upper[k] <= g.upper[k]))
+            return set_error(TGH_ERR_INVALID, "the new cell must lie inside the current one");
+        cells[k] = to_u32_saturating(std::fmax(std::floor((upper[k] - lower[k]) / z.cell_size), 1.0f));
+        C *= cells[k];
+    }
+    if (C > w->d.C) return set_error(TGH_ERR_SIZE, "the new cell needs more grid cells than were allocated");
+    for (int k = 0; k < 3; ++k) {
+        g.lower[k] = lower[k];
+        g.upper[k] = upper[k];
+        g.cells[k] = cells[k];
+        g.width[k] = (upper[k] - lower[k]) / (float)cells[k];
+        g.length[k] = upper[k] - lower[k];
+    }
+    HIP_TRY(hipMemcpyAsync((void *)w->d.grids, &g, sizeof(g), hipMemcpyHostToDevice, w->stream));
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    w->grids[0] = g;
+    tgh::Packed &p = w->specimens[0];
+    for (int k = 0; k < 3; ++k) {
+        p.lower[k] = lower[k];
+        p.upper[k] = upper[k];
+        w->sizes[0].cells[k] = cells[k];
+    }
+    HIP_TRY(tgh::launch_request_rebuild(w->d, w->stream));
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    return TGH_OK;
+}
+
+}  // namespace
+
 int tgh_world_checkpoint_sizes(tgh_world *world, tgh_checkpoint *out) {
     if (!world || !out) return set_error(TGH_ERR_INVALID, "null argument");
     if (world->d.nspec != 1) return set_error(TGH_ERR_INVALID, "checkpoints hold one-specimen worlds");
@@ -960,6 +1009,10 @@ int tgh_world_checkpoint(tgh_world *world, tgh_checkpoint *out) {
     tgh::SpecState spec;
     HIP_TRY(hipMemcpy(&spec, world->d.ss, sizeof(spec), hipMemcpyDeviceToHost));
     out->total_iterations = spec.control[1];
+    for (int k = 0; k < 3; ++k) {
+        out->cell_lower[k] = world->grids[0].lower[k];
+        out->cell_upper[k] = world->grids[0].upper[k];
+    }
     out->layer_axis = f.layer_axis;
     out->layer_first = f.first_layer;
     out->layer_last = f.last_layer;
@@ -998,6 +1051,18 @@ int tgh_world_restore(const tgh_assembly *assembly, const tgh_config *config, co
     };
     if (ck->fiber_count != w->d.F || ck->vertex_count != w->d.V)
         return fail(set_error(TGH_ERR_SIZE, "the checkpoint was taken from another assembly"));
+    {
+        // a cell compacted since creation: the grid follows it, the positions
+        // come from the checkpoint
+        const bool keep = std::isnan(ck->cell_lower[0]);  // NaN: the assembly's cell
+        bool same = true;
+        for (int k = 0; k < 3; ++k)
+            same = same && ck->cell_lower[k] == w->grids[0].lower[k] && ck->cell_upper[k] == w->grids[0].upper[k];
+        if (!keep && !same) {
+            rc = set_cell(w, ck->cell_lower, ck->cell_upper);
+            if (rc) return fail(rc);
+        }
+    }
     rc = tgh_world_upload_positions(w, ck->positions, 3 * (size_t)w->d.V);
     if (rc) return fail(rc);
     for (uint32_t fi = 0; fi < w->d.F; ++fi)
@@ -1036,6 +1101,198 @@ int tgh_world_restore(const tgh_assembly *assembly, const tgh_config *config, co
         if (rc) return fail(rc);
     }
     *out = w;
+    return TGH_OK;
+}
+
+int tgh_world_cell_bounds(const tgh_world *world, float lower[3], float upper[3]) {
+    if (!world || !lower || !upper) return set_error(TGH_ERR_INVALID, "null argument");
+    if (world->d.nspec != 1) return set_error(TGH_ERR_INVALID, "cell bounds of a one-specimen world");
+    for (int k = 0; k < 3; ++k) {
+        lower[k] = world->grids[0].lower[k];
+        upper[k] = world->grids[0].upper[k];
+    }
+    return TGH_OK;
+}
+
+// compact_cell (world.rs:1931-2053): the kinematics move the material from
+// the current cell to the new one, then the grid follows.
+int tgh_world_compact_cell(tgh_world *world, const float new_lower[3], const float new_upper[3],
+                           uint32_t kinematics) {
+    if (!world || !new_lower || !new_upper) return set_error(TGH_ERR_INVALID, "null argument");
+    if (kinematics > TGH_COMPACT_AFFINE_VERTICES) return set_error(TGH_ERR_INVALID, "unknown compaction kinematics");
+    if (world->d.nspec != 1) return set_error(TGH_ERR_INVALID, "cell changes need a one-specimen world");
+    const GridParams old = world->grids[0];
+    tgh::CellChange box{};
+    for (int k = 0; k < 3; ++k) {
+        box.old_lower[k] = old.lower[k];
+        box.old_upper[k] = old.upper[k];
+        box.new_lower[k] = new_lower[k];
+        box.new_upper[k] = new_upper[k];
+        box.periodic[k] = old.periodic[k];
+    }
+    // validate before moving anything: set_cell's checks, on a copy
+    {
+        const tgh::Sizing &z = world->sizes[0];
+        uint64_t C = 1;
+        for (int k = 0; k < 3; ++k) {
+            if (!std::isfinite(new_lower[k]) || !std::isfinite(new_upper[k]))
+                return set_error(TGH_ERR_INVALID, "cell bounds must be finite");
+            if (!(new_upper[k] - new_lower[k] > 2.0f * z.max_radius))
+                return set_error(TGH_ERR_INVALID, "the cell must be wider than two fiber radii on every axis");
+            if (!(new_lower[k] >= old.lower[k]) || !(new_upper[k] <= old.upper[k]))
+                return set_error(TGH_ERR_INVALID, "the new cell must lie inside the current one");
+            C *= to_u32_saturating(std::fmax(std::floor((new_upper[k] - new_lower[k]) / z.cell_size), 1.0f));
+        }
+        if (C > world->d.C) return set_error(TGH_ERR_SIZE, "the new cell needs more grid cells than were allocated");
+    }
+    world->history_valid = false;
+    HIP_TRY(tgh::launch_compact(world->d, box, kinematics, world->stream));
+    return set_cell(world, new_lower, new_upper);
+}
+
+int tgh_world_compaction_metrics(tgh_world *world, float correction_fraction, float contact_stiffness,
+                                 float stretch_stiffness, float bending_stiffness, tgh_compaction_metrics *out) {
+    if (!world || !out) return set_error(TGH_ERR_INVALID, "null argument");
+    if (world->d.nspec != 1) return set_error(TGH_ERR_INVALID, "compaction metrics of a one-specimen world");
+    if (!(correction_fraction > 0.0f)) return set_error(TGH_ERR_INVALID, "correction_fraction must be positive");
+    if (!(contact_stiffness >= 0.0f) || !(stretch_stiffness >= 0.0f) || !(bending_stiffness >= 0.0f))
+        return set_error(TGH_ERR_INVALID, "stiffnesses must be nonnegative");
+    float *dev = nullptr;
+    HIP_TRY(hipMalloc((void **)&dev, 13 * sizeof(float)));
+    float values[13];
+    hipError_t e = tgh::launch_compaction_metrics(world->d, world->d.grids, correction_fraction, contact_stiffness,
+                                                  stretch_stiffness, bending_stiffness, dev, world->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(values, dev, sizeof(values), hipMemcpyDeviceToHost, world->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(world->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return set_error(TGH_ERR_DEVICE, std::string("compaction metrics: ") + hipGetErrorString(e));
+    for (int k = 0; k < 3; ++k) {
+        out->pressure[k] = values[k];
+        out->face_pressure[k][0] = values[7 + 2 * k];
+        out->face_pressure[k][1] = values[8 + 2 * k];
+    }
+    out->contact_energy = values[3];
+    out->stretch_energy = values[4];
+    out->bending_energy = values[5];
+    out->total_energy = values[6];
+    return TGH_OK;
+}
+
+namespace {
+
+// f32::total_cmp's key: the bits, with the magnitude bits of negatives flipped
+inline int32_t total_order_key(float x) {
+    int32_t i;
+    std::memcpy(&i, &x, 4);
+    return i ^ (int32_t)((uint32_t)(i >> 31) >> 1);
+}
+
+}  // namespace
+
+// capture_contacts (world.rs:2193-2334): a whole-segment cell list on a grid
+// of cells at least max(segment cell, list cell) + gap wide, one append per
+// accepted pair, then the host sort.
+int tgh_world_capture_contacts(tgh_world *world, float maximum_surface_gap, size_t capacity,
+                               tgh_contact_candidate *out, size_t *count, uint32_t *overflow) {
+    if (!world || !count || !overflow || (capacity && !out)) return set_error(TGH_ERR_INVALID, "null argument");
+    if (world->d.nspec != 1) return set_error(TGH_ERR_INVALID, "contact capture of a one-specimen world");
+    if (!std::isfinite(maximum_surface_gap) || !(maximum_surface_gap >= 0.0f))
+        return set_error(TGH_ERR_INVALID, "maximum_surface_gap must be finite and nonnegative");
+    if (capacity == 0) return set_error(TGH_ERR_INVALID, "capacity must be positive");
+    if (capacity > 0xffffffffull) return set_error(TGH_ERR_INVALID, "capacity exceeds u32");
+    *count = 0;
+    *overflow = 0;
+    const tgh::Sizing &z = world->sizes[0];
+    GridParams g = world->grids[0];
+    const float size = std::fmax(z.segment_cell_size, z.cell_size) + maximum_surface_gap;
+    uint64_t C = 1;
+    for (int k = 0; k < 3; ++k) {
+        g.cells[k] = to_u32_saturating(std::fmax(std::floor((g.upper[k] - g.lower[k]) / size), 1.0f));
+        g.width[k] = (g.upper[k] - g.lower[k]) / (float)g.cells[k];
+        C *= g.cells[k];
+    }
+    g.cell_base = 0;
+    if (C > world->d.C) return set_error(TGH_ERR_SIZE, "the capture grid needs more cells than were allocated");
+    if (world->d.S == 0) return TGH_OK;
+    struct Buffers {
+        GridParams *grid = nullptr;
+        uint32_t *counter = nullptr;
+        uint2 *pairs = nullptr;
+        float2 *coordinates = nullptr;
+        float *gaps = nullptr, *angles = nullptr;
+        ~Buffers() {
+            (void)hipFree(grid);
+            (void)hipFree(counter);
+            (void)hipFree(pairs);
+            (void)hipFree(coordinates);
+            (void)hipFree(gaps);
+            (void)hipFree(angles);
+        }
+    } b;
+    HIP_TRY(hipMalloc((void **)&b.grid, sizeof(GridParams)));
+    HIP_TRY(hipMalloc((void **)&b.counter, 2 * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc((void **)&b.pairs, capacity * sizeof(uint2)));
+    HIP_TRY(hipMalloc((void **)&b.coordinates, capacity * sizeof(float2)));
+    HIP_TRY(hipMalloc((void **)&b.gaps, capacity * sizeof(float)));
+    HIP_TRY(hipMalloc((void **)&b.angles, capacity * sizeof(float)));
+    HIP_TRY(hipMemcpyAsync(b.grid, &g, sizeof(g), hipMemcpyHostToDevice, world->stream));
+    HIP_TRY(hipMemsetAsync(b.counter, 0, 2 * sizeof(uint32_t), world->stream));
+    HIP_TRY(tgh::launch_capture(world->d, b.grid, (uint32_t)C, maximum_surface_gap, (uint32_t)capacity, b.counter,
+                                b.pairs, b.coordinates, b.gaps, b.angles, world->stream));
+    uint32_t counter[2];
+    HIP_TRY(hipMemcpyAsync(counter, b.counter, sizeof(counter), hipMemcpyDeviceToHost, world->stream));
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    const size_t n = std::min<size_t>(counter[0], capacity);
+    std::vector<uint2> pairs(n);
+    std::vector<float2> coordinates(n);
+    std::vector<float> gaps(n), angles(n);
+    if (n) {
+        HIP_TRY(hipMemcpy(pairs.data(), b.pairs, n * sizeof(uint2), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(coordinates.data(), b.coordinates, n * sizeof(float2), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(gaps.data(), b.gaps, n * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(angles.data(), b.angles, n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t l, size_t r) {
+        if (pairs[l].x != pairs[r].x) return pairs[l].x < pairs[r].x;
+        if (pairs[l].y != pairs[r].y) return pairs[l].y < pairs[r].y;
+        const int32_t a = total_order_key(coordinates[l].x), b2 = total_order_key(coordinates[r].x);
+        if (a != b2) return a < b2;
+        return total_order_key(coordinates[l].y) < total_order_key(coordinates[r].y);
+    });
+    for (size_t i = 0; i < n; ++i) {
+        const size_t k = order[i];
+        out[i] = tgh_contact_candidate{pairs[k].x, pairs[k].y, coordinates[k].x, coordinates[k].y, gaps[k], angles[k]};
+    }
+    *count = n;
+    *overflow = counter[1] != 0u ? 1u : 0u;
+    return TGH_OK;
+}
+
+int tgh_world_set_fiber_maximum_curvatures(tgh_world *world, const float *maximum_curvatures, size_t count) {
+    if (!world || (count && !maximum_curvatures)) return set_error(TGH_ERR_INVALID, "null argument");
+    if (count != world->d.F) return set_error(TGH_ERR_SIZE, "one curvature limit per fiber");
+    for (size_t f = 0; f < count; ++f)
+        if (!std::isfinite(maximum_curvatures[f]) || maximum_curvatures[f] < 0.0f)
+            return set_error(TGH_ERR_INVALID, "curvature limits must be finite and nonnegative");
+    const size_t V = world->d.V;
+    std::vector<float4> vA(V);
+    std::vector<uint4> spans(world->d.F);
+    HIP_TRY(hipMemcpyAsync(vA.data(), world->d.vA, V * sizeof(float4), hipMemcpyDeviceToHost, world->stream));
+    HIP_TRY(hipMemcpyAsync(spans.data(), world->d.fspan, spans.size() * sizeof(uint4), hipMemcpyDeviceToHost,
+                           world->stream));
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    // the limit acts on internal vertices only, as at creation
+    for (size_t f = 0; f < count; ++f)
+        for (uint32_t l = 1; l + 1 < spans[f].y; ++l) vA[spans[f].x + l].x = maximum_curvatures[f];
+    HIP_TRY(hipMemcpyAsync(world->d.vA, vA.data(), V * sizeof(float4), hipMemcpyHostToDevice, world->stream));
+    HIP_TRY(hipStreamSynchronize(world->stream));
+    for (uint32_t sp = 0, f = 0; sp < world->specimens.size(); ++sp) {
+        tgh::Packed &p = world->specimens[sp];
+        for (uint32_t lf = 0; lf < p.F; ++lf, ++f)
+            for (uint32_t l = 1; l + 1 < p.fcount[lf]; ++l) p.kmax[p.fstart[lf] + l] = maximum_curvatures[f];
+    }
     return TGH_OK;
 }
 

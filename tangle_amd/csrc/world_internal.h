@@ -34,6 +34,8 @@ struct Packed {
 
 struct Sizing {
     float skin = 0.f, cell_size = 0.f;
+    float segment_cell_size = 0.f;  // whole-segment grid of the contact capture (world.rs:405)
+    float max_radius = 0.f;
     uint32_t cells[3] = {1, 1, 1};
     uint32_t C = 1;
     size_t slots = 1;

@@ -1,0 +1,211 @@
+"""Compaction kinematics, compaction metrics and contact capture on the HIP
+backend against the oracle, through the C ABI (compact_cell,
+compaction_metrics and capture_contacts, crates/tangle_relax/src/device/
+world.rs:1931-2334; kernels.rs:712-923; crates/tangle_contact/src/device/
+contact.rs:9-230), plus checkpoints of a compacted world and per-fiber
+curvature limits (world.rs:2182-2191).
+
+Bars: positions, wall reactions (through the face pressures), the 13 metric
+values and every captured pair with its coordinates, gap and angle equal
+bit for bit; relaxation after a compaction stays bit-exact (the grid follows
+the cell); the reference's known answers hold on the device."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import fixtures as fx
+import oracle
+from tangle_amd import _native
+
+pytestmark = pytest.mark.gpu
+
+KIN = {"rigid_fiber_centers": 0, "moving_walls": 1, "affine_vertices": 2}
+
+
+def metrics_array(m: _native.CompactionMetrics) -> np.ndarray:
+    fp = m.face_pressures
+    return np.array([*m.pressure, m.contact_energy, m.stretch_energy, m.bending_energy, m.total_energy,
+                     fp[0][0], fp[0][1], fp[1][0], fp[1][1], fp[2][0], fp[2][1]], dtype=np.float32)
+
+
+def assert_bits(a, b, what):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    diff = np.nonzero(a.view(np.uint32) != b.view(np.uint32))[0]
+    assert diff.size == 0, f"{what}: {diff.size} values differ, first at {diff[0]}: {a[diff[0]]} vs {b[diff[0]]}"
+
+
+def assert_capture_equal(gpu, ref):
+    gp, gc, gg, ga, go = gpu
+    op, oc, og, oa, oo = ref
+    assert go == oo
+    assert np.array_equal(gp, op), "captured pairs differ"
+    assert_bits(gc, oc, "closest-point coordinates")
+    assert_bits(gg, og, "surface gaps")
+    assert_bits(ga, oa, "crossing angles")
+
+
+def single_fiber():
+    return fx.packed([[[1.0, 1.0, 1.0], [2.0, 1.0, 1.0]]], 0.1, [4.0] * 3, [False] * 3)
+
+
+@pytest.mark.parametrize("kinematics", list(KIN))
+def test_single_fiber_kinematics_known_answers(hip_device, kinematics):
+    """world_tests.rs:876-932: rigid keeps length 1 and moves the centre to
+    0.75; affine halves the length; the moving wall projects the far vertex."""
+    print(f"[case] single fiber {kinematics}", flush=True)
+    cfg = fx.config(max_step=0.1)
+    upper = [2.0, 4.0, 4.0] if kinematics != "moving_walls" else [1.5, 4.0, 4.0]
+    with _native.World(single_fiber(), cfg) as gpu, oracle.OracleWorld(single_fiber(), cfg) as ref:
+        gpu.compact_cell([0.0, 0.0, 0.0], upper, kinematics)
+        ref.compact_cell([0.0, 0.0, 0.0], upper, KIN[kinematics])
+        assert gpu.cell_bounds() == ref.cell_bounds() == ((0.0, 0.0, 0.0), tuple(upper))
+        g, o = gpu.download_positions(), ref.download_positions()
+        assert_bits(g, o, "positions")
+        if kinematics == "rigid_fiber_centers":
+            assert abs(g[3] - g[0] - 1.0) < 1e-6 and abs(0.5 * (g[0] + g[3]) - 0.75) < 1e-6
+        elif kinematics == "affine_vertices":
+            assert abs(g[3] - g[0] - 0.5) < 1e-6
+        else:
+            assert g[3] == np.float32(np.float32(1.5) - np.float32(0.1))
+        m = gpu.compaction_metrics(0.5, _native.CompactionEnergyModel(2.0, 3.0, 1.0))
+        assert_bits(metrics_array(m), ref.compaction_metrics(0.5, 2.0, 3.0, 1.0), "metrics")
+        if kinematics == "moving_walls":
+            assert m.face_pressures[0][1] > 0.0 and m.face_pressures[0][0] == 0.0
+
+
+@pytest.mark.parametrize("builder,gap,tol,step", [(fx.periodic_pair, 0.1, 1e-5, 0.1),
+                                                  (fx.si_crossing, 20e-6, 0.01e-6, 1e-6),
+                                                  (fx.unwrapped_images, 20e-6, 0.01e-6, 1e-6)])
+def test_capture_reference_cases(hip_device, builder, gap, tol, step):
+    """world_tests.rs:934-1026: one candidate each, gap -0.1 or -20 um; after
+    the capture the periodic pair relaxes to the tolerance in 40 iterations
+    (the lists rebuild on the relaxation grid)."""
+    print(f"[case] capture {builder.__name__}", flush=True)
+    cfg = fx.config(max_step=step)
+    with _native.World(builder(), cfg) as gpu, oracle.OracleWorld(builder(), cfg) as ref:
+        cap = gpu.capture_contacts(0.0, 16)
+        assert not cap.overflow and len(cap.candidates) == 1
+        assert abs(cap.candidates[0].surface_gap + gap) < tol
+        assert_capture_equal(gpu.capture_contact_arrays(0.0, 16), ref.capture_contact_arrays(0.0, 16))
+        if builder is fx.periodic_pair:
+            run = fx.config(penetration_tolerance=1e-5, correction_fraction=0.8, max_step=0.05)
+            s, r = gpu.run_batch(run, 40), ref.run_batch(run, 40)
+            assert s.total_iterations == r.total_iterations
+            assert_bits(gpu.download_positions(), ref.download_positions(), "positions after capture")
+            assert s.converged and s.max_penetration <= 1e-5
+
+
+def test_capture_random_mat_and_overflow(hip_device):
+    rng = np.random.default_rng(7)
+    fibers = []
+    for _ in range(24):
+        c = rng.uniform(0.0, 1.0, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        fibers.append([c + (t - 1.5) * 0.12 * d for t in range(4)])
+    pk = fx.packed(fibers, np.full(24, 0.03), [1.0, 1.0, 1.0], [True, True, False])
+    cfg = fx.config(max_step=0.01)
+    print("[case] capture random mat", flush=True)
+    with _native.World(pk, cfg) as gpu, oracle.OracleWorld(pk, cfg) as ref:
+        full = ref.capture_contact_arrays(0.02, 4096)
+        assert_capture_equal(gpu.capture_contact_arrays(0.02, 4096), full)
+        assert len(full[0]) > 10
+        # overflow: the kept pairs are some of the full list, sorted
+        pairs, coords, gaps, angles, overflow = gpu.capture_contact_arrays(0.02, 5)
+        assert overflow and len(pairs) == 5
+        keys = {tuple(p) for p in full[0].tolist()}
+        assert all(tuple(p) in keys for p in pairs.tolist())
+        k = pairs[:, 0].astype(np.int64) * 100000 + pairs[:, 1]
+        assert np.all(np.diff(k) > 0)
+
+
+def bimodal_bounded():
+    """The small bimodal felt with z bounded, so the walls act."""
+    base = fx.bimodal_recipe(scale=0.05)._packed(False)
+    return _native.PackedAssembly(base.counts, base.placed, base.intrinsic, base.radius, base.min_bend_radius,
+                                  list(base.struct.cell_origin), list(base.struct.cell_lengths),
+                                  [True, True, False])
+
+
+@pytest.mark.parametrize("kinematics", list(KIN))
+def test_felt_compaction_then_relaxation_bitwise(hip_device, kinematics):
+    """40 iterations, compaction of x and z by 4 % (y unchanged), metrics,
+    40 more iterations on the compacted grid, capture: all bit-exact."""
+    print(f"[case] bimodal felt compaction {kinematics}", flush=True)
+    pk = bimodal_bounded()
+    cfg = fx.hard_gate_settings(7e-6).to_config(7e-6)
+    with _native.World(pk, cfg) as gpu, oracle.OracleWorld(pk, cfg) as ref:
+        s, r = gpu.run_batch(cfg, 40), ref.run_batch(cfg, 40)
+        assert s.total_iterations == r.total_iterations
+        # bounded z: the curvature sweep clamps to the walls on every lane's axis
+        assert_bits(gpu.download_positions(), ref.download_positions(), "positions before compaction")
+        lo, hi = gpu.cell_bounds()
+        span = [hi[k] - lo[k] for k in range(3)]
+        new_lo = [lo[0] + 0.02 * span[0], lo[1], lo[2] + 0.02 * span[2]]
+        new_hi = [hi[0] - 0.02 * span[0], hi[1], hi[2] - 0.02 * span[2]]
+        gpu.compact_cell(new_lo, new_hi, kinematics)
+        ref.compact_cell(new_lo, new_hi, KIN[kinematics])
+        assert gpu.cell_bounds() == ref.cell_bounds()
+        assert_bits(gpu.download_positions(), ref.download_positions(), "positions after compaction")
+        model = _native.CompactionEnergyModel(1.0e3, 2.0, 0.5)
+        assert_bits(metrics_array(gpu.compaction_metrics(cfg.correction_fraction, model)),
+                    ref.compaction_metrics(cfg.correction_fraction, 1.0e3, 2.0, 0.5), "metrics after compaction")
+        s, r = gpu.run_batch(cfg, 40), ref.run_batch(cfg, 40)
+        assert (s.total_iterations, s.batch_iterations) == (r.total_iterations, r.batch_iterations)
+        assert_bits(gpu.download_positions(), ref.download_positions(), "positions after relaxing")
+        gm = metrics_array(gpu.compaction_metrics(cfg.correction_fraction, model))
+        assert_bits(gm, ref.compaction_metrics(cfg.correction_fraction, 1.0e3, 2.0, 0.5), "metrics after relaxing")
+        assert gm[3] > 0.0 and gm[5] >= 0.0
+        if kinematics == "moving_walls":
+            assert gm[11] > 0.0 or gm[12] > 0.0  # the z walls push back
+        gap = 1e-6
+        assert_capture_equal(gpu.capture_contact_arrays(gap, 1 << 16), ref.capture_contact_arrays(gap, 1 << 16))
+        # the world relaxes on after a capture
+        s, r = gpu.run_batch(cfg, 10), ref.run_batch(cfg, 10)
+        assert s.total_iterations == r.total_iterations
+        assert_bits(gpu.download_positions(), ref.download_positions(), "positions after capture")
+
+
+def test_compacted_checkpoint_restores_cell(hip_device):
+    print("[case] compacted checkpoint", flush=True)
+    pk = bimodal_bounded()
+    cfg = fx.hard_gate_settings(7e-6, neighbor_capacity=256).to_config(7e-6)
+    with _native.World(pk, cfg) as a:
+        a.run_batch(cfg, 20)
+        lo, hi = a.cell_bounds()
+        new_hi = [hi[0], hi[1], hi[2] - 0.05 * (hi[2] - lo[2])]
+        a.compact_cell(lo, new_hi, "moving_walls")
+        ck = a.checkpoint()
+        assert ck.cell_upper == tuple(new_hi) or np.allclose(ck.cell_upper, np.float32(new_hi))
+        with _native.World.restore(pk, cfg, ck) as b:
+            assert b.cell_bounds() == a.cell_bounds()
+            assert_bits(b.download_positions(), a.download_positions(), "restored positions")
+            sa, sb = a.run_batch(cfg, 30), b.run_batch(cfg, 30)
+            assert sa.total_iterations == sb.total_iterations
+            assert_bits(b.download_positions(), a.download_positions(), "continued positions")
+            m = _native.CompactionEnergyModel()
+            assert_bits(metrics_array(a.compaction_metrics(cfg.correction_fraction, m)),
+                        metrics_array(b.compaction_metrics(cfg.correction_fraction, m)), "continued metrics")
+
+
+def test_fiber_maximum_curvatures_bitwise(hip_device):
+    print("[case] fiber curvature limits", flush=True)
+    pk = fx.bimodal_recipe(scale=0.05)._packed(False)
+    cfg = fx.hard_gate_settings(7e-6).to_config(7e-6)
+    F = len(pk.counts)
+    limits = np.where(np.arange(F) % 3 == 0, 0.0, 1.0 / 150e-6).astype(np.float32)
+    with _native.World(pk, cfg) as gpu, oracle.OracleWorld(pk, cfg) as ref:
+        gpu.set_fiber_maximum_curvatures(limits)
+        ref.set_fiber_maximum_curvatures(limits)
+        s, r = gpu.run_batch(cfg, 30), ref.run_batch(cfg, 30)
+        assert s.total_iterations == r.total_iterations
+        assert_bits(np.float32([s.max_curvature_ratio]), np.float32([r.max_curvature_ratio]), "curvature ratio")
+        assert_bits(gpu.download_positions(), ref.download_positions(), "positions")
+        with pytest.raises(ValueError):
+            gpu.set_fiber_maximum_curvatures(limits[:-1])
+        assert not math.isnan(s.max_curvature_ratio)
