@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--tta-warm", type=int, default=2560, help="default-settings iterations before the final stage")
     ap.add_argument("--skip-tta", action="store_true", help="skip the time-to-admissibility run")
     ap.add_argument("--skip-learned", action="store_true", help="skip the learned warm-start run")
-    ap.add_argument("--learned-warm", type=int, default=640, help="default-settings iterations of the learned plan")
+    ap.add_argument("--learned-warm", type=int, default=320, help="default-settings iterations of the learned plan")
     ap.add_argument("--learned-scale", type=float, default=8.0, help="proposal scale (intervals extrapolated)")
     ap.add_argument("--learned-interval", type=int, default=32, help="iterations between proposals")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")

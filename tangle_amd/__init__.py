@@ -731,6 +731,7 @@ class _RunState:
         self.config = config
         self.total = 0
         self.status = None
+        self.status_current = False  # the status describes the current positions
         self.converged = False
         self.events: list[str] = []
         self.warnings: list[str] = []
@@ -778,18 +779,23 @@ class _RunState:
         def residual(st):
             return max(st.max_penetration / pen_tol, (st.max_curvature_ratio - 1.0) / curv_tol)
 
-        before = residual(learned.measure(world, config))
+        # a batch's last pass only measures, so the status of the batch that
+        # brought the positions here describes them; it is reused rather
+        # than measured again
+        prior = self.status if self.status is not None and self.status_current else learned.measure(world, config)
+        before = residual(prior)
         learned.propose(world, ws.model, ws.scale)
         if ws.settle > 0:
             settle = _native.Config.from_buffer_copy(config)
             settle.force_full_iterations = 1
-            self._batch(world, settle, ws.settle)
-        after = residual(learned.measure(world, config))
+            after = residual(self._batch(world, settle, ws.settle))
+        else:
+            after = residual(learned.measure(world, config))
         if after > ws.tolerance * before:
             learned.revert(world)
             self.proposals_reverted += 1
-            # the last measurement describes the reverted positions again
-            self.status = learned.measure(world, config)
+            # the reverted positions are the ones `prior` describes
+            self.status = prior
             self.converged = False
         else:
             self.proposals_kept += 1
@@ -802,6 +808,7 @@ class _RunState:
         self.downloaded += 11 * 4 + 3 * 4
         self.total = int(status.total_iterations)
         self.status = status
+        self.status_current = True
         self.converged = bool(status.converged)
         if self.checkpoint_hook is not None and self.total > 0 and self.last_saved != self.total:
             n = self.checkpoint_interval
@@ -1282,6 +1289,8 @@ class Recipe:
         tangle_generate/src/recipe.rs:857-1652)."""
         from . import formation
         axis = self._stack_axis
+        if kind not in ("solve", "relax_for", "relax_until_converged", "settle_targets"):
+            run.status_current = False  # the operation may move or (de)activate material
         if kind == "activate":
             segments, vertices = world.activate_formation_step(arg)
             run.active_segments = segments
