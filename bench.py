@@ -269,6 +269,17 @@ def main():
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(packed, cfg, S, args.cpu_iterations)
+        tta = result.get("time_to_hard_admissibility") or {}
+        same = result["cpu_baseline"].get("gpu_vs_cpu_same_sample")
+        if tta.get("learned_speedup") and same:
+            # the CPU port follows the plain trajectory bit for bit, so it
+            # needs the same iterations; its time is estimated at the GPU/CPU
+            # ratio measured on the same first iterations
+            tta["learned_vs_cpu_estimate"] = {
+                "value": round(tta["learned_speedup"] * same, 1),
+                "basis": "learned HIP vs plain HIP time to the hard gates, times the plain HIP / CPU port speed "
+                         "ratio on the first iterations of the same specimen (the CPU port's trajectory is the "
+                         "plain path's, bit for bit)"}
 
     print(json.dumps(result), flush=True)
 
@@ -460,6 +471,16 @@ def cpu_baseline(packed, cfg, S, iterations):
         status = w.run_batch(cfg, iterations)
         t2 = time.perf_counter()
     done = int(status.batch_iterations)
+    # the plain HIP path on the same iterations from the same state (graph
+    # capture included), for a like-for-like ratio
+    from tangle_amd import _native
+    with _native.World(packed, cfg) as g:
+        g.run_batch(cfg, 1)  # library and stream warm-up on a throwaway world
+    with _native.World(packed, cfg) as g:
+        t3 = time.perf_counter()
+        gpu_status = g.run_batch(cfg, iterations)
+        t4 = time.perf_counter()
+    same = int(gpu_status.batch_iterations) == done
     return {
         "value": round(S * done / (t2 - t1), 1),
         "unit": "segment-iters/s",
@@ -467,6 +488,8 @@ def cpu_baseline(packed, cfg, S, iterations):
         "kind": "port",
         "sample": f"first {done} iterations of the same specimen from the generated state "
                   f"({t2 - t1:.1f} s; upload and sizing {t1 - t0:.2f} s excluded)",
+        "gpu_same_sample_seconds": round(t4 - t3, 4),
+        "gpu_vs_cpu_same_sample": round((t2 - t1) / (t4 - t3), 1) if same else None,
     }
 
 

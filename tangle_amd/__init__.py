@@ -59,6 +59,16 @@ __all__ = [
 _AXES = {"x": 0, "y": 1, "z": 2, 0: 0, 1: 1, 2: 2}
 
 
+def _rust_debug(text: str) -> str:
+    """A string as Rust's `{:?}` writes it (plain names: in double quotes)."""
+    return '"' + text.replace("\\", "\\\\").replace('"', '\\"') + '"'
+
+
+def _rust_display(value: float) -> str:
+    """An f64 as Rust's `{}` writes it: shortest round trip, never exponent form."""
+    return np.format_float_positional(float(value), unique=True, trim="-")
+
+
 def _axis_set(value) -> list[bool]:
     if isinstance(value, str):
         flags = [False, False, False]
@@ -680,6 +690,7 @@ class RunResult:
     last_checkpoint_bytes: int | None = None
     _fibers: list = field(default=None, repr=False)  # the recipe's fibers, in packed order
     _cell: object = field(default=None, repr=False)
+    _bend: list = field(default=None, repr=False)  # per-fiber bend radius at the end of the run
 
     def write_ovito(self, path, *, view_script_path=None, session_path=None, coloring: str = "fiber",
                     bond_radius_ratio: float | None = 1.0) -> None:
@@ -696,9 +707,9 @@ class RunResult:
             raise RuntimeError("this result does not carry its fibers")
         xyz = self.positions.reshape(-1, 3).astype(np.float64)
         rows, start = [], 0
+        bend = self._bend or [f.material.min_bend_radius for f in self._fibers]
         for index, (fiber, n) in enumerate(zip(self._fibers, self.vertex_counts)):
-            rows.append((index + 1, fiber.radius, xyz[start:start + n], fiber.intrinsic,
-                         fiber.material.min_bend_radius))
+            rows.append((index + 1, fiber.radius, xyz[start:start + n], fiber.intrinsic, bend[index]))
             start += n
         cell = self._cell
         lower = [float(o) for o in cell.origin]
@@ -992,6 +1003,35 @@ def _unit_fraction(value, name):
     return value
 
 
+def _curvature_limits(bend_radii) -> np.ndarray:
+    """Per-fiber maximum curvature, f32 of 1 / radius, 0 without a limit
+    (FiberBendLimit::maximum_curvature as the recipe uploads it)."""
+    return np.array([0.0 if r is None else np.float32(1.0 / r) for r in bend_radii], dtype=np.float32)
+
+
+def _active_capsule_bounds(fibers, positions, starts, active, padding):
+    """Bounds of the active segments' capsules, each endpoint padded by its
+    radius plus `padding`, in f32 (active_capsule_bounds,
+    tangle_generate/src/recipe/compaction_control.rs:446-474); None when no
+    segment is active."""
+    lower = np.full(3, np.inf, dtype=np.float32)
+    upper = np.full(3, -np.inf, dtype=np.float32)
+    found = False
+    pad = np.float32(padding)
+    for index, fiber in enumerate(fibers):
+        n = fiber.placed.shape[0]
+        if not active[index] or n < 2:
+            continue
+        found = True
+        radius = np.float32(np.float32(fiber.radius) + pad)
+        pts = positions[int(starts[index]):int(starts[index]) + n]
+        lower = np.minimum(lower, (pts - radius).min(axis=0))
+        upper = np.maximum(upper, (pts + radius).max(axis=0))
+    if not found:
+        return None, None
+    return [np.float32(v) for v in lower], [np.float32(v) for v in upper]
+
+
 class Recipe:
     """An ordered list of formation operations on one cell; ``insert`` adds
     fibers at once, everything else runs in order inside ``run``
@@ -1129,6 +1169,48 @@ class Recipe:
     def release_needles(self) -> None:
         self._push(("release_needles", None), "release needles")
 
+    def set_min_bend_radius(self, material, min_bend_radius: float) -> None:
+        """Sets the minimum bend radius of every fiber of `material` (a
+        Material or its name) for the rest of the run; the resident world
+        takes the new curvature limits at once (SetMaterialBendRadius,
+        tangle_python/src/recipe.rs:365-392; executed as in
+        tangle_generate/src/recipe.rs:957-1026)."""
+        if isinstance(material, Material):
+            name = material.name
+        elif isinstance(material, str):
+            name = material
+        else:
+            raise TypeError("material must be a Material or a material name")
+        if not name.strip():
+            raise ValueError("material name must not be empty")
+        radius = float(min_bend_radius)
+        if not (radius > 0.0 and math.isfinite(radius)):
+            raise ValueError("min_bend_radius must be positive and finite")
+        self._push(("bend_radius", (name, radius)),
+                   f"set {_rust_debug(name)} min bend radius to {_rust_display(radius)}")
+
+    def fit_cell_to_active_fibers(self, *, axes=None, padding: float = 0.0) -> None:
+        """Shrinks the cell along `axes` (default: the stack axis) to the
+        active fibers' capsules plus `padding`, moving those walls in
+        (FitCellToActiveFibers, tangle_python/src/recipe.rs:513-543; executed
+        as in tangle_generate/src/recipe.rs:1206-1252 with
+        recipe/compaction_control.rs:446-483)."""
+        if axes is None:
+            mask = [False, False, False]
+            mask[self._stack_axis] = True
+        else:
+            mask = _axis_set(axes)
+        if not any(mask):
+            raise ValueError("at least one cell-fit axis must be selected")
+        padding = float(padding)
+        if not (padding >= 0.0 and math.isfinite(padding)):
+            raise ValueError("padding must be nonnegative and finite")
+        for k in range(3):
+            if mask[k] and self._cell.periodic[k]:
+                raise ValueError(f"cannot fit periodic cell axis {k} to active fibers")
+        label = "".join("xyz"[k] for k in range(3) if mask[k])
+        self._push(("fit_cell", (tuple(mask), padding)), f"fit cell axes {label} to active fibers")
+
     def operations(self) -> list[str]:
         return list(self._descriptions)
 
@@ -1202,7 +1284,11 @@ class Recipe:
         starts = np.concatenate([[0], np.cumsum(packed.counts.astype(np.int64))[:-1]])
         fingerprint = self._fingerprint(packed)
         run = _RunState(settings, config, warm_start)
-        state = {"active_step": None, "targets": None if initial_targets is None else list(initial_targets)}
+        state = {"active_step": None, "targets": None if initial_targets is None else list(initial_targets),
+                 # the host cell (fit_cell_to_active_fibers moves it) and the
+                 # per-fiber bend radii (set_min_bend_radius changes them)
+                 "cell": (list(self._cell.origin), list(self._cell.lengths)),
+                 "bend": [f.material.min_bend_radius for f in self._fibers]}
         report = {"resumed": False, "resumed_iteration": None, "saves": 0, "last_iteration": None, "last_bytes": None}
         start_index, first_begin, restored = 0, None, None
         if checkpoint is not None and checkpoint.resume:
@@ -1226,18 +1312,26 @@ class Recipe:
                 run.needling = [tuple(n) for n in progress["needling"]]
                 run.post_recipe_iterations = int(progress.get("post_recipe_iterations", 0))
                 state["active_step"] = progress["active_step"]
+                if progress.get("cell") is not None:
+                    state["cell"] = (list(progress["cell"][0]), list(progress["cell"][1]))
+                if progress.get("bend") is not None:
+                    state["bend"] = list(progress["bend"])
                 if progress["layer_targets"] is not None:
                     state["targets"] = [np.float32(t) for t in progress["layer_targets"]]
         world = (_native.World(packed, config) if restored is None
                  else _native.World.restore(packed, config, restored))
         if restored is not None:
             run.active_segments = int(((packed.counts.astype(np.int64) - 1) * (restored.fiber_active != 0)).sum())
+            if state["bend"] != [f.material.min_bend_radius for f in self._fibers]:
+                # curvature limits are recipe state, not device checkpoint state
+                world.set_fiber_maximum_curvatures(_curvature_limits(state["bend"]))
 
         def save(failure=None):
             progress = {"fingerprint": fingerprint, "operation": run.op_index, "operation_begin": run.op_begin,
                         "events": run.events, "warnings": run.warnings, "needling": run.needling,
                         "active_step": state["active_step"], "post_recipe_iterations": run.post_recipe_iterations,
                         "layer_targets": None if state["targets"] is None else [float(t) for t in state["targets"]],
+                        "cell": [list(state["cell"][0]), list(state["cell"][1])], "bend": list(state["bend"]),
                         "failure": failure}
             nbytes = ckpt.save(checkpoint.path, checkpoint.case_id, world.checkpoint(), progress)
             report.update(saves=report["saves"] + 1, last_iteration=run.total, last_bytes=nbytes)
@@ -1281,7 +1375,10 @@ class Recipe:
             post_recipe_iterations=run.post_recipe_iterations, segment_iterations=run.segment_iterations,
             needling=list(run.needling), resumed=report["resumed"], resumed_iteration=report["resumed_iteration"],
             checkpoint_saves=report["saves"], last_checkpoint_iteration=report["last_iteration"],
-            last_checkpoint_bytes=report["last_bytes"], _fibers=list(self._fibers), _cell=self._cell,
+            last_checkpoint_bytes=report["last_bytes"], _fibers=list(self._fibers),
+            _cell=Cell(state["cell"][1], "".join("xyz"[k] for k in range(3) if self._cell.periodic[k]),
+                       state["cell"][0], stack_axis=self._cell.stack_axis),
+            _bend=list(state["bend"]),
         )
 
     def _run_operation(self, world, run, state, index, desc, kind, arg, begin, initial_targets, starts):
@@ -1306,7 +1403,8 @@ class Recipe:
             run.settle(world, index, desc, *arg, begin=begin)
         elif kind == "scale_layers":
             factor, stiffness, max_translation = arg
-            center = np.float32(self._cell.origin[axis] + 0.5 * self._cell.lengths[axis])
+            origin, lengths = state["cell"]
+            center = np.float32(origin[axis] + 0.5 * lengths[axis])
             state["targets"] = [np.float32(center + np.float32(np.float32(t - center) * np.float32(factor)))
                                 for t in initial_targets]
             world.set_layer_targets(axis, state["targets"], 0, len(state["targets"]) - 1, stiffness,
@@ -1335,8 +1433,9 @@ class Recipe:
                                   f"cannot needle layer {n.layer}: it has not been inserted yet")
             positions = world.download_positions().reshape(-1, 3)
             run.downloaded += positions.nbytes
-            lower = [np.float32(o) for o in self._cell.origin]
-            upper = [np.float32(o + l) for o, l in zip(self._cell.origin, self._cell.lengths)]
+            origin, lengths = state["cell"]
+            lower = [np.float32(o) for o in origin]
+            upper = [np.float32(o + l) for o, l in zip(origin, lengths)]
             vertices = formation.select_needle_vertices(self._fibers, positions, starts, active_step, axis,
                                                         self._cell.periodic, lower, upper, n)
             world.clear_vertex_targets()
@@ -1354,6 +1453,47 @@ class Recipe:
             world.clear_vertex_targets()
             run.converged = False
             run.events.append("release needle targets")
+        elif kind == "fit_cell":
+            mask, padding = arg
+            positions = world.download_positions().reshape(-1, 3)
+            run.downloaded += positions.nbytes
+            active_step = state["active_step"]
+            active = [active_step is None or f.step <= active_step for f in self._fibers]
+            fit_lower, fit_upper = _active_capsule_bounds(self._fibers, positions, starts, active, padding)
+            if fit_lower is None:
+                raise RecipeError(index, desc, run.total, "cannot fit a cell without active fiber segments")
+            old_lower, old_upper = world.cell_bounds()
+            new_lower = [np.float32(max(fit_lower[k], np.float32(old_lower[k]))) if mask[k]
+                         else np.float32(old_lower[k]) for k in range(3)]
+            new_upper = [np.float32(min(fit_upper[k], np.float32(old_upper[k]))) if mask[k]
+                         else np.float32(old_upper[k]) for k in range(3)]
+            try:
+                world.compact_cell(new_lower, new_upper, "moving_walls")
+            except (ValueError, _native.NativeError) as err:
+                raise RecipeError(index, desc, run.total, f"cannot fit the cell: {err}") from None
+            # update_host_cell (compaction_control.rs:476-483)
+            extent = [np.float32(new_upper[k] - new_lower[k]) for k in range(3)]
+            state["cell"] = ([float(v) for v in new_lower], [float(v) for v in extent])
+            run.converged = False
+            from .export import rust_exp
+            run.events.append(f"fit cell to active fibers with {rust_exp(np.float32(padding), 3)} padding: "
+                              f"[{', '.join(rust_exp(e, 3) for e in extent)}]")
+        elif kind == "bend_radius":
+            name, radius = arg
+            matching = [i for i, f in enumerate(self._fibers) if f.material.name == name]
+            if not matching:
+                names = list(dict.fromkeys(f.material.name for f in self._fibers))
+                reason = (f"material {_rust_debug(name)} has no fibers" if name in names else
+                          f"unknown material {_rust_debug(name)}; known materials: "
+                          f"{', '.join(_rust_debug(n) for n in names)}")
+                raise RecipeError(index, desc, run.total, reason)
+            for i in matching:
+                state["bend"][i] = radius
+            world.set_fiber_maximum_curvatures(_curvature_limits(state["bend"]))
+            run.converged = False
+            from .export import rust_exp
+            run.events.append(f"set {len(matching)} {_rust_debug(name)} fibers to minimum bend radius "
+                              f"{rust_exp(radius, 3)}")
         else:
             raise AssertionError(kind)
 

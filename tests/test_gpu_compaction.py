@@ -209,3 +209,87 @@ def test_fiber_maximum_curvatures_bitwise(hip_device):
         with pytest.raises(ValueError):
             gpu.set_fiber_maximum_curvatures(limits[:-1])
         assert not math.isnan(s.max_curvature_ratio)
+
+
+def test_fit_cell_to_capsules_bitwise(hip_device):
+    """The cell fit's path through the world: capsule bounds of the active
+    fibers (+ padding) on the bounded axis, moving walls, relaxation on the
+    fitted grid: bit-exact against the oracle."""
+    from tangle_amd import _active_capsule_bounds
+
+    print("[case] fit cell to capsules", flush=True)
+    pk = bimodal_bounded()
+    cfg = fx.hard_gate_settings(7e-6).to_config(7e-6)
+    recipe = fx.bimodal_recipe(scale=0.05)
+    starts = np.concatenate([[0], np.cumsum(pk.counts.astype(np.int64))[:-1]])
+    with _native.World(pk, cfg) as gpu, oracle.OracleWorld(pk, cfg) as ref:
+        gpu.run_batch(cfg, 40)
+        ref.run_batch(cfg, 40)
+        pos = gpu.download_positions()
+        assert_bits(pos, ref.download_positions(), "positions before the fit")
+        lo_fit, hi_fit = _active_capsule_bounds(recipe._fibers, pos.reshape(-1, 3), starts,
+                                                [True] * len(recipe._fibers), np.float32(1e-6))
+        lo, hi = gpu.cell_bounds()
+        new_lo = [lo[0], lo[1], max(lo_fit[2], np.float32(lo[2]))]
+        new_hi = [hi[0], hi[1], min(hi_fit[2], np.float32(hi[2]))]
+        gpu.compact_cell(new_lo, new_hi, "moving_walls")
+        ref.compact_cell(new_lo, new_hi, KIN["moving_walls"])
+        assert_bits(gpu.download_positions(), ref.download_positions(), "positions after the fit")
+        s, r = gpu.run_batch(cfg, 40), ref.run_batch(cfg, 40)
+        assert s.total_iterations == r.total_iterations
+        assert_bits(gpu.download_positions(), ref.download_positions(), "positions on the fitted cell")
+
+
+def bounded_bimodal_recipe(scale=0.05):
+    import tangle_amd as tg
+
+    side = 1e-3 * scale ** (1.0 / 3.0)
+    cell = tg.Cell([side] * 3, periodic="xy")
+    recipe = tg.Recipe(cell)
+    for spec, name in ((fx.BIMODAL_FINE, "fine"), (fx.BIMODAL_COARSE, "coarse")):
+        material = tg.Material(name, spec["diameter"], spec["bend"])
+        count = max(1, int(round(spec["count"] * scale)))
+        segments = max(2, int(round(spec["segments"] * side / 1e-3)))
+        pop = tg.FiberPopulation(material=material, count=count, length=side, segments_per_fiber=segments,
+                                 seed=spec["seed"], orientation=tg.SchladitzOrientation(beta=1.0 / 0.3, axis="z"))
+        recipe.insert(tg.generate_fiber_population(cell, pop))
+    return recipe
+
+
+def test_recipe_fit_cell_and_bend_radius(hip_device):
+    """Recipe.fit_cell_to_active_fibers and Recipe.set_min_bend_radius in a
+    run: the events read as the reference writes them, the result's cell is
+    the fitted one and every vertex stays inside its walls."""
+    import tangle_amd as tg
+
+    print("[case] recipe fit cell and bend radius", flush=True)
+    recipe = bounded_bimodal_recipe()
+    recipe.relax_for(64)
+    recipe.set_min_bend_radius("coarse", 120e-6)
+    recipe.fit_cell_to_active_fibers(padding=1e-6)
+    recipe.relax_for(64)
+    settings = fx.hard_gate_settings(7e-6, max_iterations=2000)
+    try:
+        res = recipe.run(settings)
+    except tg.RecipeError as err:  # the hard gates need not be met in the budget
+        pytest.fail(f"recipe failed: {err}")
+    bend = [e for e in res.events if e.startswith("set ")]
+    fit = [e for e in res.events if e.startswith("fit cell")]
+    n_coarse = sum(f.material.name == "coarse" for f in recipe._fibers)
+    assert bend == [f'set {n_coarse} "coarse" fibers to minimum bend radius 1.200e-4']
+    assert len(fit) == 1 and fit[0].startswith("fit cell to active fibers with 1.000e-6 padding: [")
+    cell = res._cell
+    side = recipe._cell.lengths[2]
+    # the host cell is rebuilt from the f32 bounds (update_host_cell), so
+    # the unfitted axes keep their length to f32 precision
+    assert abs(cell.lengths[0] - recipe._cell.lengths[0]) < 1e-6 * recipe._cell.lengths[0]
+    assert cell.lengths[2] < side
+    xyz = res.positions.reshape(-1, 3)
+    lo, hi = np.float32(cell.origin[2]), np.float32(cell.origin[2] + cell.lengths[2])
+    radius = np.repeat([np.float32(f.radius) for f in recipe._fibers], res.vertex_counts)
+    assert np.all(xyz[:, 2] >= lo + radius) and np.all(xyz[:, 2] <= hi + np.float32(1e-9) - radius)
+    assert res._bend[[f.material.name for f in recipe._fibers].index("coarse")] == 120e-6
+    bad = bounded_bimodal_recipe()
+    bad.set_min_bend_radius("wool", 1e-4)
+    with pytest.raises(tg.RecipeError, match='unknown material "wool"; known materials: "fine", "coarse"'):
+        bad.run(settings)
