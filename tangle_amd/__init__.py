@@ -24,6 +24,9 @@ import numpy as np
 
 from . import _native
 from .checkpoint import CheckpointError, CheckpointSettings
+from .compaction import (AxisWeightsPath, CellLengthsTarget, CellVolumeTarget, CompactionSettings,
+                         DirectionalPressureTarget, EqualPressurePath, MeanPressureTarget, MinimumWorkPath,
+                         PenaltyEnergyTarget, StressRatioPath, VolumeFractionTarget)
 from .formation import CircularFootprint, RandomFiberFraction
 from .units import um
 
@@ -53,6 +56,17 @@ __all__ = [
     "CheckpointSettings",
     "CheckpointError",
     "relax_batch",
+    "CompactionSettings",
+    "VolumeFractionTarget",
+    "CellVolumeTarget",
+    "CellLengthsTarget",
+    "MeanPressureTarget",
+    "DirectionalPressureTarget",
+    "PenaltyEnergyTarget",
+    "AxisWeightsPath",
+    "EqualPressurePath",
+    "StressRatioPath",
+    "MinimumWorkPath",
     "um",
 ]
 
@@ -683,6 +697,8 @@ class RunResult:
     post_recipe_iterations: int = 0
     segment_iterations: int = 0  # active segments x iterations over the run
     needling: list = field(default_factory=list)  # (layer, selected vertices, depth) per needle command
+    compactions: list = field(default_factory=list)  # one report per compaction operation
+    compaction_steps: list = field(default_factory=list)  # one report per accepted compaction step
     resumed: bool = False
     resumed_iteration: int | None = None
     checkpoint_saves: int = 0
@@ -743,6 +759,8 @@ class _RunState:
         self.total = 0
         self.status = None
         self.status_current = False  # the status describes the current positions
+        self.compactions = []        # CompactionReport per compaction operation
+        self.compaction_steps = []   # CompactionStepReport per accepted step
         self.converged = False
         self.events: list[str] = []
         self.warnings: list[str] = []
@@ -964,6 +982,220 @@ class _RunState:
             if status.cell_list_overflow:
                 raise RecipeError(index, description, self.total, "cell list overflow")
 
+    def _hold(self, world, config, index, description, remaining):
+        """One relaxation batch of at most `remaining` iterations, converging
+        early when the run's tolerances hold (hold_for_relaxation with
+        force_full_batch off, compaction_control.rs:428-432; the batch size
+        rule of plugin.rs:1050-1075)."""
+        run_remaining = self._budget(index, description)
+        batch = max(1, min(run_remaining, self.settings.iterations_per_batch, max(1, remaining)))
+        status = self._batch(world, config, batch)
+        if status.cell_list_overflow:
+            raise RecipeError(index, description, self.total, "cell list overflow")
+        return status
+
+    def compact(self, ctx, index, description, spec, overrides):
+        """Closed-loop compaction (advance_compaction and its helpers,
+        tangle_generate/src/recipe/compaction_control.rs:5-432): an admissible
+        baseline, then compact-relax-measure steps with adaptive strain,
+        rollback to the last accepted state when a window does not relax, and
+        a stop at the target, a guard or a jam."""
+        from . import compaction as cp
+        f32 = np.float32
+        world = ctx["world"]
+        config = _native.Config.from_buffer_copy(self.config)
+        if overrides is not None:
+            config = overrides.apply(config)
+        config.force_full_iterations = 0
+        model = _native.CompactionEnergyModel(spec.contact_energy_stiffness, spec.stretch_energy_stiffness,
+                                              spec.bending_energy_stiffness)
+        cf = float(config.correction_fraction)
+        state = ctx["state"]
+        active_step = state["active_step"]
+        fibers = ctx["fibers"]
+        formed = [f for f in fibers if active_step is None or f.step <= active_step]
+        path = spec.weights_for(ctx["stack_axis"])
+        R = int(spec.relax_iterations)
+        solid = cp.nominal_fiber_volume(fibers, active_step)
+
+        def lengths_now():
+            return list(state["cell"][1])
+
+        metrics = world.compaction_metrics(cf, model)
+        self.downloaded += 13 * 4
+        a = dict(next=f32(spec.initial_log_strain), started=self.total, windows=0, steps=0, last=f32(0.0),
+                 previous=lengths_now(), work=[f32(0.0)] * 3, metrics=metrics, baseline=True,
+                 accepted=None, accepted_status=None)
+
+        def finish(reason):
+            lengths = lengths_now()
+            fraction = solid / (lengths[0] * lengths[1] * lengths[2])
+            m = a["metrics"]
+            self.compactions.append({"operation": index, "reason": cp.reason_text(reason), "steps": a["steps"],
+                                     "cell_lengths": lengths, "nominal_volume_fraction": fraction,
+                                     "pressure": list(m.pressure), "total_energy": m.total_energy,
+                                     "cumulative_work": [float(w) for w in a["work"]]})
+            self.events.append(f"compact in {a['steps']} steps to nominal volume fraction {fraction:.4f} "
+                               f"({cp.reason_text(reason)})")
+
+        def accept(m):
+            lengths = lengths_now()
+            origin = state["cell"][0]
+            accepted = a["accepted"]
+            cur_lo = [f32(origin[k]) for k in range(3)]
+            cur_hi = [f32(origin[k] + lengths[k]) for k in range(3)]
+            motion, work = [[f32(0.0)] * 2 for _ in range(3)], [[f32(0.0)] * 2 for _ in range(3)]
+            fp = m.face_pressures
+            for k in range(3):
+                area = lengths[(k + 1) % 3] * lengths[(k + 2) % 3]
+                shortening = max(a["previous"][k] - lengths[k], 0.0)
+                a["work"][k] = f32(a["work"][k] + f32(f32(m.pressure[k]) * f32(area * shortening)))
+                lower_motion = f32(max(f32(cur_lo[k] - f32(accepted.cell_lower[k])), f32(0.0)))
+                upper_motion = f32(max(f32(f32(accepted.cell_upper[k]) - cur_hi[k]), f32(0.0)))
+                motion[k] = [lower_motion, upper_motion]
+                work[k] = [f32(f32(f32(fp[k][0]) * f32(area)) * lower_motion),
+                           f32(f32(f32(fp[k][1]) * f32(area)) * upper_motion)]
+            a["steps"] += 1
+            a["metrics"] = m
+            st = self.status
+            self.compaction_steps.append({
+                "operation": index, "step": a["steps"], "iteration": self.total, "cell_lengths": lengths,
+                "nominal_volume_fraction": solid / (lengths[0] * lengths[1] * lengths[2]),
+                "log_volume_strain": float(a["last"]), "max_penetration": float(st.max_penetration),
+                "max_bend_ratio": float(st.max_curvature_ratio), "pressure": list(m.pressure),
+                "contact_energy": m.contact_energy, "stretch_energy": m.stretch_energy,
+                "bending_energy": m.bending_energy, "total_energy": m.total_energy,
+                "cumulative_work": [float(w) for w in a["work"]]})
+            readings = []
+            for k, name in enumerate("xyz"):
+                if any(v > 0.0 for v in fp[k]) or any(v > 0.0 for v in work[k]):
+                    readings.append(
+                        f"{name}- p={cp.rust_exp3(fp[k][0])} dx={cp.rust_exp3(motion[k][0])} "
+                        f"dW={cp.rust_exp3(work[k][0])}, {name}+ p={cp.rust_exp3(fp[k][1])} "
+                        f"dx={cp.rust_exp3(motion[k][1])} dW={cp.rust_exp3(work[k][1])}")
+            if readings:
+                self.events.append(f"compaction step {a['steps']} wall pressure/motion/work: "
+                                   f"{'; '.join(readings)}")
+
+        def rollback():
+            ck = a["accepted"]
+            ck.total_iterations = self.total
+            new_world = ctx["restore"](ck)
+            state["cell"] = ([float(v) for v in ck.cell_lower],
+                             [float(f32(f32(ck.cell_upper[k]) - f32(ck.cell_lower[k]))) for k in range(3)])
+            self.status = a["accepted_status"]
+            self.status_current = True
+            return new_world
+
+        # an unconstrained admissible baseline first
+        self.converged = False
+        self._hold(world, config, index, description, R)
+        while True:
+            if a["started"] is not None:
+                elapsed = self.total - a["started"]
+                if not self.converged and elapsed < R:
+                    self._hold(world, config, index, description, R - elapsed)
+                    continue
+                if not self.converged:
+                    a["windows"] += 1
+                    if a["baseline"] and a["windows"] < spec.max_relax_windows:
+                        a["started"] = self.total
+                        self.converged = False
+                        self._hold(world, config, index, description, R)
+                        continue
+                    if a["baseline"]:
+                        a["metrics"] = world.compaction_metrics(cf, model)
+                        self.downloaded += 13 * 4
+                        finish(("GuardReached", "initial formation state did not relax (penalty energy "
+                                f"{cp.rust_exp3(a['metrics'].total_energy)})"))
+                        return
+                    world = rollback()
+                    attempted = a["next"]
+                    reduced = f32(max(f32(attempted * f32(spec.shrink_factor)), f32(spec.min_log_strain)))
+                    at_minimum = attempted <= f32(f32(spec.min_log_strain) * f32(1.0 + 16.0 * cp.F32_EPS))
+                    if at_minimum or a["windows"] >= spec.max_relax_windows:
+                        finish("Jammed")
+                        return
+                    a["next"] = reduced
+                    a["started"] = None
+                    continue
+                m = world.compaction_metrics(cf, model)
+                self.downloaded += 13 * 4
+                a["metrics"] = m
+                if a["baseline"]:
+                    a["baseline"] = False
+                    a["started"] = None
+                    a["windows"] = 0
+                    a["accepted"] = world.checkpoint()
+                    a["accepted_status"] = self.status
+                    continue
+                accept(m)
+                a["accepted"] = world.checkpoint()
+                a["accepted_status"] = self.status
+                a["started"] = None
+                a["windows"] = 0
+                if elapsed <= R // 2:
+                    a["next"] = f32(min(f32(a["next"] * f32(spec.growth_factor)), f32(spec.max_log_strain)))
+                else:
+                    a["next"] = f32(max(f32(a["next"] * f32(spec.shrink_factor)), f32(spec.min_log_strain)))
+            lengths = lengths_now()
+            fraction = solid / (lengths[0] * lengths[1] * lengths[2])
+            m = a["metrics"]
+            st = self.status
+            guard = cp.guard_reached(spec, st.max_penetration, st.max_curvature_ratio, m.pressure, m.total_energy,
+                                     a["steps"])
+            if guard is not None:
+                finish(("GuardReached", guard))
+                return
+            if cp.target_reached(spec.target, spec.target_tolerance, lengths, fraction, m.pressure, m.total_energy):
+                finish("TargetReached")
+                return
+            weights, remaining = cp.strain_direction_and_remaining(spec.target, path, lengths, fraction, m.pressure)
+            requested = a["next"] if remaining is None else f32(min(a["next"], remaining))
+            requested = f32(max(requested, f32(cp.F32_EPS)))
+            if not formed:
+                raise RecipeError(index, description, self.total, "compaction needs active fibers")
+            min_diameter = min(2.0 * f.radius for f in formed)
+            requested = cp.diameter_limited_log_strain(requested, weights, lengths, min_diameter,
+                                                       spec.max_shortening_over_min_diameter)
+            minimum_length = 2.001 * max(f.radius for f in formed)
+            new_lengths = [max(lengths[k] * math.exp(-float(f32(requested * weights[k]))), minimum_length)
+                           for k in range(3)]
+            applied = f32(sum(math.log(lengths[k] / new_lengths[k]) for k in range(3)))
+            if applied <= f32(cp.F32_EPS):
+                finish(("GuardReached", "cell reached the minimum fiber-diameter extent"))
+                return
+            old_lower, _ = world.cell_bounds()
+            anchor = [f32(v) for v in spec.cell_anchor]
+            if spec.balance_opposing_faces:
+                a["metrics"] = world.compaction_metrics(cf, model)
+                self.downloaded += 13 * 4
+                fp = a["metrics"].face_pressures
+                floor = f32(spec.face_pressure_floor)
+                strength = f32(spec.face_balance_strength)
+                for k in range(3):
+                    if weights[k] > 0.0 and not ctx["periodic"][k]:
+                        lower_ease = f32(f32(1.0) / f32(f32(fp[k][0]) + floor))
+                        upper_ease = f32(f32(1.0) / f32(f32(fp[k][1]) + floor))
+                        adaptive = f32(lower_ease / f32(lower_ease + upper_ease))
+                        base = f32(spec.cell_anchor[k])
+                        anchor[k] = f32(base + f32(strength * f32(adaptive - base)))
+            new_lower = [f32(f32(old_lower[k]) + f32(anchor[k] * f32(f32(lengths[k]) - f32(new_lengths[k]))))
+                         for k in range(3)]
+            new_upper = [f32(new_lower[k] + f32(new_lengths[k])) for k in range(3)]
+            try:
+                world.compact_cell(new_lower, new_upper, spec.kinematics)
+            except (ValueError, _native.NativeError) as err:
+                raise RecipeError(index, description, self.total, f"compaction step rejected: {err}") from None
+            state["cell"] = ([float(v) for v in new_lower],
+                             [float(f32(new_upper[k] - new_lower[k])) for k in range(3)])
+            self.status_current = False
+            a["previous"] = lengths
+            a["last"] = applied
+            a["started"] = self.total
+            self.converged = False
+            self._hold(world, config, index, description, R)
+
 
 class HeldTargets:
     """Returned by operations that hold fibers on targets
@@ -1169,6 +1401,18 @@ class Recipe:
     def release_needles(self) -> None:
         self._push(("release_needles", None), "release needles")
 
+    def compact(self, settings, overrides: "RelaxationOverrides | None" = None) -> None:
+        """Closed-loop compaction toward `settings.target`
+        (tangle_python/src/recipe.rs:544-563; the controller is
+        tangle_amd/compaction.py and _RunState.compact)."""
+        from . import compaction as cp
+        if not isinstance(settings, cp.CompactionSettings):
+            raise TypeError("settings must be CompactionSettings")
+        if overrides is not None and not isinstance(overrides, RelaxationOverrides):
+            raise TypeError("overrides must be RelaxationOverrides")
+        settings.validate()
+        self._push(("compact", (settings.copy(), overrides)), f"compact toward {settings.target!r}")
+
     def set_min_bend_radius(self, material, min_bend_radius: float) -> None:
         """Sets the minimum bend radius of every fiber of `material` (a
         Material or its name) for the rest of the run; the resident world
@@ -1320,6 +1564,15 @@ class Recipe:
                     state["targets"] = [np.float32(t) for t in progress["layer_targets"]]
         world = (_native.World(packed, config) if restored is None
                  else _native.World.restore(packed, config, restored))
+        holder = {"world": world}
+
+        def restore_world(ck):
+            # roll the resident world back to an accepted state (DeviceWorld::restore)
+            holder["world"].close()
+            holder["world"] = _native.World.restore(packed, config, ck)
+            if state["bend"] != [f.material.min_bend_radius for f in self._fibers]:
+                holder["world"].set_fiber_maximum_curvatures(_curvature_limits(state["bend"]))
+            return holder["world"]
         if restored is not None:
             run.active_segments = int(((packed.counts.astype(np.int64) - 1) * (restored.fiber_active != 0)).sum())
             if state["bend"] != [f.material.min_bend_radius for f in self._fibers]:
@@ -1333,32 +1586,39 @@ class Recipe:
                         "layer_targets": None if state["targets"] is None else [float(t) for t in state["targets"]],
                         "cell": [list(state["cell"][0]), list(state["cell"][1])], "bend": list(state["bend"]),
                         "failure": failure}
-            nbytes = ckpt.save(checkpoint.path, checkpoint.case_id, world.checkpoint(), progress)
+            nbytes = ckpt.save(checkpoint.path, checkpoint.case_id, holder["world"].checkpoint(), progress)
             report.update(saves=report["saves"] + 1, last_iteration=run.total, last_bytes=nbytes)
 
         if checkpoint is not None:
             run.checkpoint_interval = int(checkpoint.interval_iterations)
             run.checkpoint_hook = save
-        with world:
+        try:
             info = world.info()
             run.uploaded = int(info.device_bytes)
+            ctx = {"state": state, "fibers": self._fibers, "stack_axis": axis, "periodic": self._cell.periodic,
+                   "restore": restore_world, "holder": holder}
             try:
                 for index in range(start_index, len(operations)):
                     kind, arg = operations[index]
                     begin = first_begin if index == start_index and first_begin is not None else run.total
                     run.op_index, run.op_begin = index, begin
-                    self._run_operation(world, run, state, index, descriptions[index], kind, arg, begin,
-                                        initial_targets, starts)
+                    ctx["world"] = holder["world"]
+                    self._run_operation(holder["world"], run, state, index, descriptions[index], kind, arg, begin,
+                                        initial_targets, starts, ctx)
                 resumed_in_finish = start_index == len(operations) and first_begin is not None
                 run.op_index, run.op_begin = len(operations), first_begin if resumed_in_finish else run.total
-                run.finish(world, begin=run.op_begin)
+                run.finish(holder["world"], begin=run.op_begin)
             except RecipeError as err:
                 run.save_checkpoint(failure=err.reason)
                 raise
             run.save_checkpoint()
+            world = holder["world"]
+            info = world.info()
             positions = world.download_positions()
             run.downloaded += positions.nbytes
             rebuilds = int(world.info().neighbor_list_rebuilds)
+        finally:
+            holder["world"].close()
         status = run.status
         wall = time.perf_counter() - started
         return RunResult(
@@ -1373,7 +1633,8 @@ class Recipe:
             warnings=run.warnings, neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
             positions=positions, vertex_counts=packed.counts.copy(),
             post_recipe_iterations=run.post_recipe_iterations, segment_iterations=run.segment_iterations,
-            needling=list(run.needling), resumed=report["resumed"], resumed_iteration=report["resumed_iteration"],
+            needling=list(run.needling), compactions=list(run.compactions),
+            compaction_steps=list(run.compaction_steps), resumed=report["resumed"], resumed_iteration=report["resumed_iteration"],
             checkpoint_saves=report["saves"], last_checkpoint_iteration=report["last_iteration"],
             last_checkpoint_bytes=report["last_bytes"], _fibers=list(self._fibers),
             _cell=Cell(state["cell"][1], "".join("xyz"[k] for k in range(3) if self._cell.periodic[k]),
@@ -1381,7 +1642,7 @@ class Recipe:
             _bend=list(state["bend"]),
         )
 
-    def _run_operation(self, world, run, state, index, desc, kind, arg, begin, initial_targets, starts):
+    def _run_operation(self, world, run, state, index, desc, kind, arg, begin, initial_targets, starts, ctx=None):
         """One recorded operation (control_formation_recipe,
         tangle_generate/src/recipe.rs:857-1652)."""
         from . import formation
@@ -1478,6 +1739,9 @@ class Recipe:
             from .export import rust_exp
             run.events.append(f"fit cell to active fibers with {rust_exp(np.float32(padding), 3)} padding: "
                               f"[{', '.join(rust_exp(e, 3) for e in extent)}]")
+        elif kind == "compact":
+            spec, overrides = arg
+            run.compact(ctx, index, desc, spec, overrides)
         elif kind == "bend_radius":
             name, radius = arg
             matching = [i for i, f in enumerate(self._fibers) if f.material.name == name]

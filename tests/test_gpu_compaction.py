@@ -293,3 +293,74 @@ def test_recipe_fit_cell_and_bend_radius(hip_device):
     bad.set_min_bend_radius("wool", 1e-4)
     with pytest.raises(tg.RecipeError, match='unknown material "wool"; known materials: "fine", "coarse"'):
         bad.run(settings)
+
+
+def lenient_settings(**changes):
+    import tangle_amd as tg
+
+    return tg.RelaxationSettings(penetration_tolerance=0.05 * 7e-6, curvature_ratio_tolerance=0.01,
+                                 max_iterations=changes.pop("max_iterations", 6000), **changes)
+
+
+def test_recipe_compact_reaches_volume_fraction(hip_device):
+    """Recipe.compact: a bounded felt compacted along z to a nominal volume
+    fraction; the step reports grow the fraction, x and y stay, and the
+    operation ends TargetReached with the reference's event text."""
+    import tangle_amd as tg
+    from tangle_amd import compaction as cp
+
+    print("[case] recipe compact to volume fraction", flush=True)
+    recipe = bounded_bimodal_recipe()
+    recipe.relax_for(128)
+    solid = cp.nominal_fiber_volume(recipe._fibers, None)
+    side = recipe._cell.lengths
+    start_fraction = solid / (side[0] * side[1] * side[2])
+    target = round(start_fraction * 1.12, 4)
+    spec = tg.CompactionSettings.volume_fraction(target, kinematics="moving_walls", max_curvature_ratio=1.05)
+    recipe.compact(spec)
+    res = recipe.run(lenient_settings())
+    assert len(res.compactions) == 1
+    report = res.compactions[0]
+    assert report["reason"] == "TargetReached", report
+    assert report["nominal_volume_fraction"] >= target * (1 - 1e-3)
+    assert report["steps"] >= 2 and report["steps"] == len(res.compaction_steps)
+    fractions = [s["nominal_volume_fraction"] for s in res.compaction_steps]
+    assert all(b > a for a, b in zip(fractions, fractions[1:]))
+    lengths = report["cell_lengths"]
+    assert abs(lengths[0] - side[0]) < 1e-6 * side[0] and abs(lengths[1] - side[1]) < 1e-6 * side[1]
+    assert lengths[2] < side[2]
+    done = [e for e in res.events if e.startswith("compact in ")]
+    assert done == [f"compact in {report['steps']} steps to nominal volume fraction "
+                    f"{report['nominal_volume_fraction']:.4f} (TargetReached)"]
+    # walls moved in on z: every vertex inside the final cell
+    cell = res._cell
+    xyz = res.positions.reshape(-1, 3)
+    radius = np.repeat([np.float32(f.radius) for f in recipe._fibers], res.vertex_counts)
+    assert np.all(xyz[:, 2] >= np.float32(cell.origin[2]) + radius)
+    assert np.all(xyz[:, 2] <= np.float32(cell.origin[2] + cell.lengths[2]) + np.float32(1e-9) - radius)
+
+
+def test_recipe_compact_stop_reasons(hip_device):
+    """A baseline that cannot relax in its windows stops with the
+    reference's guard text; a step whose one-iteration window cannot relax
+    rolls the world back to the accepted state and ends Jammed."""
+    import tangle_amd as tg
+
+    print("[case] recipe compact stop reasons", flush=True)
+    strict = tg.RelaxationSettings(penetration_tolerance=1e-12, curvature_ratio_tolerance=1e-9, max_iterations=400)
+    recipe = bounded_bimodal_recipe()
+    recipe.compact(tg.CompactionSettings.volume_fraction(0.3, relax_iterations=8, max_relax_windows=2))
+    res = recipe.run(strict)
+    assert res.compactions[0]["reason"].startswith('GuardReached("initial formation state did not relax '
+                                                   '(penalty energy ')
+    assert res.compactions[0]["steps"] == 0
+    recipe = bounded_bimodal_recipe()
+    recipe.relax_for(400)
+    recipe.compact(tg.CompactionSettings.volume_fraction(0.3, relax_iterations=1, max_relax_windows=1,
+                                                         initial_log_strain=0.05, max_log_strain=0.05,
+                                                         kinematics="affine_vertices", max_curvature_ratio=1.05))
+    res = recipe.run(lenient_settings(max_iterations=1200))
+    report = res.compactions[0]
+    assert report["reason"] == "Jammed", report
+    # the cell of the accepted (baseline) state was restored
+    assert abs(report["cell_lengths"][2] - recipe._cell.lengths[2]) < 1e-6 * recipe._cell.lengths[2]
