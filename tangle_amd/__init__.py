@@ -28,6 +28,7 @@ from .compaction import (AxisWeightsPath, CellLengthsTarget, CellVolumeTarget, C
                          DirectionalPressureTarget, EqualPressurePath, MeanPressureTarget, MinimumWorkPath,
                          PenaltyEnergyTarget, StressRatioPath, VolumeFractionTarget)
 from .formation import CircularFootprint, RandomFiberFraction
+from .junctions import JunctionPolicy
 from .units import um
 
 __all__ = [
@@ -57,6 +58,7 @@ __all__ = [
     "CheckpointError",
     "relax_batch",
     "CompactionSettings",
+    "JunctionPolicy",
     "VolumeFractionTarget",
     "CellVolumeTarget",
     "CellLengthsTarget",
@@ -698,6 +700,8 @@ class RunResult:
     segment_iterations: int = 0  # active segments x iterations over the run
     needling: list = field(default_factory=list)  # (layer, selected vertices, depth) per needle command
     compactions: list = field(default_factory=list)  # one report per compaction operation
+    junctions: list = field(default_factory=list)  # tangle_amd.junctions.Junction records
+    junction_captures: list = field(default_factory=list)  # one report per junction capture
     compaction_steps: list = field(default_factory=list)  # one report per accepted compaction step
     resumed: bool = False
     resumed_iteration: int | None = None
@@ -713,9 +717,9 @@ class RunResult:
         """Writes the relaxed fibers as one OVITO frame, one spherocylinder per
         segment, and optionally the script that styles it and saves a session
         (RunResult.write_ovito, crates/tangle_python/src/recipe.rs:1005-1030;
-        tangle_export/src/ovito.rs). Junction bonds are not captured by this
-        backend, so `bond_radius_ratio` (None: no bonds) only adds their
-        styling to the script."""
+        tangle_export/src/ovito.rs). Junctions captured by the run are drawn
+        as binder bonds of `bond_radius_ratio` times the thinner fiber's
+        radius (None: no bonds; tangle_export/src/bridges.rs)."""
         from . import export
         if coloring not in export.COLORINGS:
             raise ValueError(f"coloring must be one of {', '.join(export.COLORINGS)}")
@@ -730,10 +734,17 @@ class RunResult:
         cell = self._cell
         lower = [float(o) for o in cell.origin]
         upper = [float(o) + float(l) for o, l in zip(cell.origin, cell.lengths)]
-        export.write_ovito_frame(path, rows, lower, upper, cell.periodic, self.iterations)
+        export.write_ovito_frame(path, rows, lower, upper, cell.periodic, self.iterations,
+                                 junctions=self.junctions, bond_radius_ratio=bond_radius_ratio)
         if view_script_path is not None:
             export.write_ovito_view_script(view_script_path, path, session_path=session_path, coloring=coloring,
                                            bond_radius_ratio=bond_radius_ratio)
+
+    @property
+    def junction_count(self) -> int:
+        """Persistent junctions captured by the run (junction_count,
+        tangle_python/src/recipe.rs:1001-1004)."""
+        return len(self.junctions)
 
     def centerlines(self):
         out, start = [], 0
@@ -760,6 +771,8 @@ class _RunState:
         self.status = None
         self.status_current = False  # the status describes the current positions
         self.compactions = []        # CompactionReport per compaction operation
+        self.junctions = []          # persistent junctions, in creation order
+        self.junction_captures = []  # JunctionCaptureReport per capture
         self.compaction_steps = []   # CompactionStepReport per accepted step
         self.converged = False
         self.events: list[str] = []
@@ -981,6 +994,20 @@ class _RunState:
             measured = True
             if status.cell_list_overflow:
                 raise RecipeError(index, description, self.total, "cell list overflow")
+
+    def capture_junctions(self, world, index, description, policy, ctx):
+        """One junction capture: device candidates, then the host policy
+        (capture_junctions, tangle_generate/src/recipe.rs:1660-1671)."""
+        from . import junctions as jn
+        capture = world.capture_contact_arrays(np.float32(policy.max_surface_gap), int(policy.candidate_capacity))
+        self.downloaded += 24 * len(capture[2]) + 8
+        try:
+            report = jn.form_junctions(self.junctions, ctx["fibers"], ctx["counts"], ctx["intrinsic"], capture,
+                                       policy)
+        except OverflowError as err:
+            raise RecipeError(index, description, self.total, str(err)) from None
+        self.junction_captures.append(report)
+        return report
 
     def _hold(self, world, config, index, description, remaining):
         """One relaxation batch of at most `remaining` iterations, converging
@@ -1413,6 +1440,28 @@ class Recipe:
         settings.validate()
         self._push(("compact", (settings.copy(), overrides)), f"compact toward {settings.target!r}")
 
+    def capture_junctions(self, policy) -> None:
+        """Promotes the current contacts to junctions under `policy`
+        (tangle_python/src/recipe.rs:564-571; tangle_generate/src/recipe.rs:1597-1605)."""
+        from .junctions import JunctionPolicy
+        if not isinstance(policy, JunctionPolicy):
+            raise TypeError("policy must be a JunctionPolicy")
+        policy.validate()
+        self._push(("capture_junctions", policy.replace()), f"capture junctions using policy {_rust_debug(policy.name)}")
+
+    def relax_and_capture(self, *, iterations: int, capture_every: int, policy) -> None:
+        """Relaxes for `iterations`, capturing junctions every `capture_every`
+        (tangle_python/src/recipe.rs:573-598; tangle_generate/src/recipe.rs:1606-1651)."""
+        from .junctions import JunctionPolicy
+        iterations, capture_every = int(iterations), int(capture_every)
+        if iterations <= 0 or capture_every <= 0 or capture_every > iterations:
+            raise ValueError("iterations must be positive and capture_every must lie within 1..=iterations")
+        if not isinstance(policy, JunctionPolicy):
+            raise TypeError("policy must be a JunctionPolicy")
+        policy.validate()
+        self._push(("relax_and_capture", (iterations, capture_every, policy.replace())),
+                   f"relax for {iterations} iterations and capture junctions every {capture_every}")
+
     def set_min_bend_radius(self, material, min_bend_radius: float) -> None:
         """Sets the minimum bend radius of every fiber of `material` (a
         Material or its name) for the rest of the run; the resident world
@@ -1560,6 +1609,11 @@ class Recipe:
                     state["cell"] = (list(progress["cell"][0]), list(progress["cell"][1]))
                 if progress.get("bend") is not None:
                     state["bend"] = list(progress["bend"])
+                if progress.get("junctions"):
+                    from .junctions import Junction
+                    run.junctions = [Junction(int(i), law, int(ps), tuple((int(f), float(a)) for f, a in anchors))
+                                     for i, law, ps, anchors in progress["junctions"]]
+                    run.junction_captures = list(progress.get("junction_captures", []))
                 if progress["layer_targets"] is not None:
                     state["targets"] = [np.float32(t) for t in progress["layer_targets"]]
         world = (_native.World(packed, config) if restored is None
@@ -1585,6 +1639,9 @@ class Recipe:
                         "active_step": state["active_step"], "post_recipe_iterations": run.post_recipe_iterations,
                         "layer_targets": None if state["targets"] is None else [float(t) for t in state["targets"]],
                         "cell": [list(state["cell"][0]), list(state["cell"][1])], "bend": list(state["bend"]),
+                        "junctions": [[j.id, j.law, j.parameter_set, [list(a) for a in j.anchors]]
+                                      for j in run.junctions],
+                        "junction_captures": list(run.junction_captures),
                         "failure": failure}
             nbytes = ckpt.save(checkpoint.path, checkpoint.case_id, holder["world"].checkpoint(), progress)
             report.update(saves=report["saves"] + 1, last_iteration=run.total, last_bytes=nbytes)
@@ -1596,7 +1653,8 @@ class Recipe:
             info = world.info()
             run.uploaded = int(info.device_bytes)
             ctx = {"state": state, "fibers": self._fibers, "stack_axis": axis, "periodic": self._cell.periodic,
-                   "restore": restore_world, "holder": holder}
+                   "restore": restore_world, "holder": holder, "counts": packed.counts,
+                   "intrinsic": np.asarray(packed.intrinsic, dtype=np.float32).reshape(-1, 3)}
             try:
                 for index in range(start_index, len(operations)):
                     kind, arg = operations[index]
@@ -1633,7 +1691,8 @@ class Recipe:
             warnings=run.warnings, neighbor_list_rebuilds=rebuilds, wall_seconds=wall,
             positions=positions, vertex_counts=packed.counts.copy(),
             post_recipe_iterations=run.post_recipe_iterations, segment_iterations=run.segment_iterations,
-            needling=list(run.needling), compactions=list(run.compactions),
+            needling=list(run.needling), compactions=list(run.compactions), junctions=list(run.junctions),
+            junction_captures=list(run.junction_captures),
             compaction_steps=list(run.compaction_steps), resumed=report["resumed"], resumed_iteration=report["resumed_iteration"],
             checkpoint_saves=report["saves"], last_checkpoint_iteration=report["last_iteration"],
             last_checkpoint_bytes=report["last_bytes"], _fibers=list(self._fibers),
@@ -1739,6 +1798,31 @@ class Recipe:
             from .export import rust_exp
             run.events.append(f"fit cell to active fibers with {rust_exp(np.float32(padding), 3)} padding: "
                               f"[{', '.join(rust_exp(e, 3) for e in extent)}]")
+        elif kind == "capture_junctions":
+            report = run.capture_junctions(world, index, desc, arg, ctx)
+            run.events.append(f"capture {report['created']} junctions with '{report['policy_name']}' from "
+                              f"{report['candidates']} candidates")
+        elif kind == "relax_and_capture":
+            iterations, every, policy = arg
+            config = _native.Config.from_buffer_copy(run.config)
+            config.force_full_iterations = 1
+            last_capture = None
+            while True:
+                completed = run.total - begin
+                if completed > 0 and completed % every == 0 and last_capture != run.total:
+                    report = run.capture_junctions(world, index, desc, policy, ctx)
+                    run.events.append(f"periodic capture created {report['created']} '{report['policy_name']}' "
+                                      f"junctions from {report['candidates']} candidates")
+                    last_capture = run.total
+                if completed >= iterations:
+                    run.events.append(f"relax for {iterations} iterations with junction capture every {every}")
+                    break
+                next_capture = (completed // every + 1) * every
+                remaining = run._budget(index, desc)
+                batch = max(1, min(remaining, run.settings.iterations_per_batch,
+                                   min(next_capture, iterations) - completed))
+                run.converged = False
+                run._batch(world, config, batch)
         elif kind == "compact":
             spec, overrides = arg
             run.compact(ctx, index, desc, spec, overrides)

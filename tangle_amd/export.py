@@ -113,14 +113,69 @@ def fiber_capsules(fibers, atom_type: int = 1):
     return out
 
 
+def _round_half_away(x: float) -> float:
+    return math.floor(x + 0.5) if x >= 0.0 else -math.floor(-x + 0.5)
+
+
+def anchor_point(placed, intrinsic, rest_arc_length: float):
+    """Placed centerline point of a material point given by its rest arc
+    length: walk the rest centerline to it, then take the same segment
+    coordinate on the placed one (anchor_point, bridges.rs:115-170)."""
+    traversed, last = 0.0, None
+    for j in range(len(intrinsic) - 1):
+        length = _norm(_sub(intrinsic[j + 1], intrinsic[j]))
+        if length <= 2.220446049250313e-16:
+            continue
+        last = j
+        if rest_arc_length <= traversed + length:
+            t = min(max((rest_arc_length - traversed) / length, 0.0), 1.0)
+            a, b = placed[j], placed[j + 1]
+            return tuple(a[k] + (b[k] - a[k]) * t for k in range(3))
+        traversed += length
+    if last is None:
+        raise ValueError("an anchor's fiber has no segment of positive length")
+    return tuple(placed[last + 1])
+
+
+def junction_bridges(fibers, junctions, lower, upper, periodic, radius_ratio: float):
+    """One binder bridge per junction, from its first anchor's placed point to
+    the other's nearest periodic image, radius `radius_ratio` times the
+    thinner fiber's (junction_bridges, bridges.rs:55-110). `fibers` as for
+    fiber_capsules; anchors name fibers by id."""
+    if not (math.isfinite(radius_ratio) and radius_ratio > 0.0):
+        raise ValueError("bond radius ratio must be positive and finite")
+    by_id = {f[0]: f for f in fibers}
+    lengths = [float(upper[k]) - float(lower[k]) for k in range(3)]
+    out = []
+    for junction in junctions:
+        (f0, s0), *others = junction.anchors
+        a = by_id[f0]
+        start = anchor_point([tuple(float(v) for v in p) for p in a[2]], [tuple(float(v) for v in p) for p in a[3]],
+                             s0)
+        for f1, s1 in others:
+            b = by_id[f1]
+            end = list(anchor_point([tuple(float(v) for v in p) for p in b[2]],
+                                    [tuple(float(v) for v in p) for p in b[3]], s1))
+            for k in range(3):
+                if periodic[k]:
+                    end[k] -= _round_half_away((end[k] - start[k]) / lengths[k]) * lengths[k]
+            out.append(dict(junction_id=junction.id, start=start, end=tuple(end),
+                            radius=radius_ratio * min(float(a[1]), float(b[1]))))
+    return out
+
+
 def write_ovito_frame(path, fibers, lower, upper, periodic, timestep: int, *, atom_type: int = 1,
-                      append: bool = False) -> int:
+                      append: bool = False, junctions=(), bond_radius_ratio=None) -> int:
     """Writes (or appends) one fiber-segments frame (write_fiber_segment_frame
-    and write_frame_header, ovito.rs:385-440, 747-770); returns the number
-    of glyph rows."""
+    and write_frame_header, ovito.rs:385-440, 747-770), with one binder-bond
+    glyph of type atom_type + 2 per junction bridge when `bond_radius_ratio`
+    is given (write_bond_rows, ovito.rs:456-490); returns the number of
+    glyph rows."""
     capsules = fiber_capsules(fibers, atom_type)
+    bridges = ([] if bond_radius_ratio is None or not junctions else
+               junction_bridges(fibers, junctions, lower, upper, periodic, float(bond_radius_ratio)))
     f = rust_exp
-    lines = ["ITEM: TIMESTEP", str(int(timestep)), "ITEM: NUMBER OF ATOMS", str(len(capsules)),
+    lines = ["ITEM: TIMESTEP", str(int(timestep)), "ITEM: NUMBER OF ATOMS", str(len(capsules) + len(bridges)),
              "ITEM: BOX BOUNDS " + " ".join("pp" if p else "ff" for p in periodic)]
     lines += [f"{f(lower[k])} {f(upper[k])}" for k in range(3)]
     lines.append(ATOM_HEADER)
@@ -131,11 +186,21 @@ def write_ovito_frame(path, fibers, lower, upper, periodic, timestep: int, *, at
             f(q[0]), f(q[1]), f(q[2]), f(q[3]), f(p[0]), f(p[1]), f(p[2]), str(c["local_segment"]),
             str(c["fiber_points"]), f(c["natural_curvature"]), f(c["current_curvature"]),
             f(c["curvature_ratio"]), f(c["curvature_excess"]), f(c["refinement_level"]), "0"]))
+    g = lambda v: rust_exp(v, 9)  # noqa: E731  (write_fiber_glyph's {:.9e})
+    for i, bridge in enumerate(bridges):
+        delta = _sub(bridge["end"], bridge["start"])
+        length = _norm(delta)
+        q = z_axis_orientation(delta, length)
+        mid = tuple((bridge["start"][k] + bridge["end"][k]) * 0.5 for k in range(3))
+        r = bridge["radius"]
+        lines.append(" ".join([str(len(capsules) + 1 + i), "0", str(atom_type + 2), g(r), g(r), g(length),
+                               g(q[0]), g(q[1]), g(q[2]), g(q[3]), g(mid[0]), g(mid[1]), g(mid[2]), "0", "0",
+                               g(0.0), g(0.0), g(0.0), g(0.0), g(0.0), str(bridge["junction_id"])]))
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
     with open(path, "a" if append else "w") as out:
         out.write("\n".join(lines) + "\n")
-    return len(capsules)
+    return len(capsules) + len(bridges)
 
 
 def _path_expression(path: Path, script_path: Path) -> str:

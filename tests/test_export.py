@@ -85,3 +85,45 @@ def test_dump_layout_and_script(tmp_path):
     assert "ParticlesVis.Shape.Spherocylinder" in text and "ovito.scene.save" in text
     with pytest.raises(ValueError):
         res.write_ovito(dump, coloring="rainbow")
+
+
+def _touching_cross(periodic):
+    # bridges.rs:186-228: two fibers of radius 0.1 crossing and touching, one
+    # junction (id 9) at rest arc length 0.3 on both
+    from tangle_amd.junctions import Junction
+
+    if periodic:
+        first, second = [[0.05, 0.2, 0.5], [0.05, 0.8, 0.5]], [[0.85, 0.5, 0.2], [0.85, 0.5, 0.8]]
+    else:
+        first, second = [[0.2, 0.5, 0.4], [0.8, 0.5, 0.4]], [[0.5, 0.2, 0.6], [0.5, 0.8, 0.6]]
+    fibers = [(1, 0.1, np.array(first), np.array(first), None), (2, 0.1, np.array(second), np.array(second), None)]
+    return fibers, [Junction(9, "bond", 0, ((1, 0.3), (2, 0.3)))], [periodic, False, False]
+
+
+def test_bridge_spans_the_crossing_between_centerlines():
+    fibers, junctions, periodic = _touching_cross(False)
+    bridges = export.junction_bridges(fibers, junctions, [0.0] * 3, [1.0] * 3, periodic, 0.5)
+    assert len(bridges) == 1 and bridges[0]["junction_id"] == 9
+    assert abs(bridges[0]["radius"] - 0.05) < 1e-12
+    assert np.allclose(bridges[0]["start"], [0.5, 0.5, 0.4], atol=1e-12)
+    assert np.allclose(bridges[0]["end"], [0.5, 0.5, 0.6], atol=1e-12)
+
+
+def test_bridge_across_a_periodic_wall_takes_the_nearest_image(tmp_path):
+    fibers, junctions, periodic = _touching_cross(True)
+    bridges = export.junction_bridges(fibers, junctions, [0.0] * 3, [1.0] * 3, periodic, 0.5)
+    b = bridges[0]
+    assert abs(np.linalg.norm(np.subtract(b["end"], b["start"])) - 0.2) < 1e-12
+    assert np.allclose(b["end"], [-0.15, 0.5, 0.5], atol=1e-12)
+    for bad in (0.0, float("nan")):
+        with pytest.raises(ValueError):
+            export.junction_bridges(fibers, junctions, [0.0] * 3, [1.0] * 3, periodic, bad)
+    # the frame carries one bond row of type atom_type + 2 after the capsules
+    path = tmp_path / "bonded.dump"
+    rows = export.write_ovito_frame(path, fibers, [0.0] * 3, [1.0] * 3, periodic, 0, junctions=junctions,
+                                    bond_radius_ratio=0.5)
+    text = path.read_text().splitlines()
+    assert rows == 3 and text[3] == "3"
+    bond = text[-1].split()
+    assert bond[:3] == ["3", "0", "3"] and bond[-1] == "9"
+    assert bond[3] == "5.000000000e-2" and bond[5] == "2.000000000e-1"

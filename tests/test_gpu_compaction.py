@@ -364,3 +364,50 @@ def test_recipe_compact_stop_reasons(hip_device):
     assert report["reason"] == "Jammed", report
     # the cell of the accepted (baseline) state was restored
     assert abs(report["cell_lengths"][2] - recipe._cell.lengths[2]) < 1e-6 * recipe._cell.lengths[2]
+
+
+def test_recipe_junction_capture_matches_oracle(hip_device):
+    """Recipe.capture_junctions and relax_and_capture: the first capture's
+    junctions equal those the host policy forms from the oracle's capture
+    of the same state, and the events read as the reference writes them."""
+    import tangle_amd as tg
+    from tangle_amd import junctions as jn
+
+    print("[case] recipe junction capture", flush=True)
+    recipe = bounded_bimodal_recipe()
+    recipe.relax_for(64)
+    policy = tg.JunctionPolicy("bonds", max_surface_gap=0.5e-6, max_per_fiber_pair=2)
+    recipe.capture_junctions(policy)
+    recipe.relax_and_capture(iterations=64, capture_every=32, policy=policy.replace(name="periodic"))
+    settings = lenient_settings(max_iterations=600)
+    res = recipe.run(settings)
+    first = res.junction_captures[0]
+    assert first["created"] > 0 and len(res.junction_captures) == 3
+    events = [e for e in res.events if "junction" in e]
+    assert events[0] == f"capture {first['created']} junctions with 'bonds' from {first['candidates']} candidates"
+    assert events[1].startswith("periodic capture created ") and events[-1] == (
+        "relax for 64 iterations with junction capture every 32")
+    assert res.junction_count == sum(c["created"] for c in res.junction_captures)
+    # the same state on the oracle: activation of both steps, 64 iterations
+    packed = recipe._packed(False)
+    cfg = settings.to_config(min(2.0 * f.radius for f in recipe._fibers))
+    cfg.force_full_iterations = 1
+    with oracle.OracleWorld(packed, cfg) as ref:
+        ref.activate_formation_step(1)
+        ref.run_batch(cfg, 64)
+        cap = ref.capture_contact_arrays(np.float32(policy.max_surface_gap), policy.candidate_capacity)
+    store = []
+    intrinsic = np.asarray(packed.intrinsic, dtype=np.float32).reshape(-1, 3)
+    report = jn.form_junctions(store, recipe._fibers, packed.counts, intrinsic, cap, policy)
+    assert report == first
+    assert store == res.junctions[:first["created"]]
+    # the OVITO frame carries one bond glyph per junction after the segments
+    import tempfile
+    from pathlib import Path
+    with tempfile.TemporaryDirectory() as tmp:
+        dump = Path(tmp) / "felt.dump"
+        res.write_ovito(dump, bond_radius_ratio=0.5)
+        lines = dump.read_text().splitlines()
+    segments = int((res.vertex_counts - 1).sum())
+    assert int(lines[3]) == segments + res.junction_count
+    assert all(row.split()[2] == "3" for row in lines[-res.junction_count:])
