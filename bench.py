@@ -61,6 +61,10 @@ def parse():
                     help="only copy a MIB-sized device buffer 5 times (reference kernel for the HBM counters)")
     ap.add_argument("--skip-batched", action="store_true", help="skip the batched-specimens throughput run")
     ap.add_argument("--skip-formation", action="store_true", help="skip the layered, needled felt workload")
+    ap.add_argument("--skip-adjacent", action="store_true",
+                    help="skip the compaction / contact-capture timings on the slab")
+    ap.add_argument("--adjacent-only", action="store_true",
+                    help="run only the compaction / contact-capture timings (for profiling)")
     ap.add_argument("--formation-plies", type=int, default=20, help="plies of the layered felt workload")
     ap.add_argument("--batched-specimens", type=int, default=256, help="specimens in the batched run")
     ap.add_argument("--batched-steps", type=int, default=2, help="timed 128-iteration batches of the batched run")
@@ -101,7 +105,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of this "
-                                   "build (see DESIGN.md for how the path would shard)", "n_gpus": args.gpus}))
+                                   "build (the sharding plan is written up with the design)", "n_gpus": args.gpus}))
         sys.exit(2)
 
     if args.copy_calibration:
@@ -111,6 +115,10 @@ def main():
     import fixtures as fx
     import tangle_amd as tg
     from tangle_amd import _native
+
+    if args.adjacent_only:
+        print(json.dumps({"compaction_capture": adjacent_workload(fx, _native)}), flush=True)
+        return
 
     d_min = 7e-6
     recipe = fx.bimodal_recipe(scale=args.scale)
@@ -267,6 +275,9 @@ def main():
     if not args.skip_formation:
         result["formation"] = formation_workload(args, fx, tg)
 
+    if not args.skip_adjacent:
+        result["compaction_capture"] = adjacent_workload(fx, _native)
+
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(packed, cfg, S, args.cpu_iterations)
         tta = result.get("time_to_hard_admissibility") or {}
@@ -414,6 +425,61 @@ def copy_calibration(mib: int) -> None:
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "dispatches": 5}), flush=True)
+
+
+def adjacent_workload(fx, _native):
+    """The compaction and contact-capture calls on the slab workload (20,000
+    bimodal fibers, 1.31 M segments, walled in z) after 256 iterations:
+    wall time per call (synchronised, so kernel time plus launch and copy
+    overhead) and the HBM rate its algorithmic bytes imply. Algorithmic
+    bytes per call: compaction metrics read 100 B per segment (record,
+    activity, both corrections, both endpoints, rest length) and 20 B per
+    vertex (ratio, wall reaction); moving walls and affine read 48 B and
+    write 32 B per vertex; rigid centres read each vertex twice (16 B each)
+    and write 32 B, plus 32 B per fiber."""
+    pk = fx.slab_recipe()._packed(False)
+    F, V = int(pk.counts.size), int(pk.vertex_count)
+    S = V - F
+    cfg = fx.hard_gate_settings(7e-6).to_config(7e-6)
+    cfg.force_full_iterations = 1
+    out = {"workload": "slab: 20,000 bimodal fibers, 3 x 3 x 1 mm, periodic x y, walled z, after 256 iterations",
+           "segments": S, "vertices": V}
+
+    def timed(fn, repeats):
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            r = fn()
+            times.append(time.perf_counter() - t0)
+        return min(times), r
+
+    def rate(nbytes, seconds):
+        return {"ms": round(1e3 * seconds, 3), "algorithmic_MB": round(nbytes / 1e6, 2),
+                "achieved_GBps": round(nbytes / seconds / 1e9, 1), "peak_GBps": PEAK_HBM_GBS,
+                "frac": round(nbytes / seconds / 1e9 / PEAK_HBM_GBS, 4)}
+
+    with _native.World(pk, cfg) as world:
+        world.run_batch(cfg, 256)
+        model = _native.CompactionEnergyModel(1.0, 1.0, 1.0)
+        sec, m = timed(lambda: world.compaction_metrics(cfg.correction_fraction, model), 3)
+        out["compaction_metrics"] = dict(rate(100 * S + 20 * V, sec),
+                                         bound="latency: each sum is one lane adding its terms in index "
+                                               "order, as the reference's single thread does",
+                                         total_energy=m.total_energy)
+        sec, cap = timed(lambda: world.capture_contact_arrays(1.0e-6, 1 << 22), 3)
+        out["capture_contacts"] = {"ms": round(1e3 * sec, 3), "candidates": int(len(cap[2])),
+                                   "overflow": bool(cap[4]),
+                                   "note": "whole-segment cell list on the capture grid, 27-cell scan, "
+                                           "device radix sort of the candidates, download"}
+        lo, hi = world.cell_bounds()
+        for name, nbytes in (("moving_walls", 80 * V), ("affine_vertices", 80 * V),
+                             ("rigid_fiber_centers", 64 * V + 32 * F)):
+            lo, hi = world.cell_bounds()
+            dz = 0.002 * (hi[2] - lo[2])
+            t0 = time.perf_counter()
+            world.compact_cell([lo[0], lo[1], lo[2] + dz], [hi[0], hi[1], hi[2] - dz], name)
+            out[f"compact_{name}"] = rate(nbytes, time.perf_counter() - t0)
+    return out
 
 
 def formation_workload(args, fx, tg):

@@ -159,10 +159,16 @@ hipError_t launch_activate(const WorldDev &w, uint32_t maximum_step, hipStream_t
 hipError_t launch_apply_activity(const WorldDev &w, hipStream_t stream);
 // compaction: kinematics 0 rigid fiber centres, 1 moving walls, 2 affine
 hipError_t launch_compact(const WorldDev &w, const CellChange &box, uint32_t kinematics, hipStream_t stream);
-// the 13 compaction metrics (measure_compaction_metrics order) into out[13]
+// the 13 compaction metrics (measure_compaction_metrics order) into out[13];
+// `terms` is scratch of 5 S + 7 V floats
 hipError_t launch_compaction_metrics(const WorldDev &w, const GridParams *grid, float correction_fraction,
                                      float contact_stiffness, float stretch_stiffness, float bending_stiffness,
-                                     float *out, hipStream_t stream);
+                                     float *terms, float *out, hipStream_t stream);
+// the first n captured pairs sorted by (first, second) segment on the device
+// (radix sort of 64-bit keys), gathered as 24-byte candidate records (the
+// tgh_contact_candidate layout) into `records`; synchronises
+hipError_t launch_capture_sort(uint32_t n, const uint2 *pairs, const float2 *coordinates, const float *gaps,
+                               const float *angles, void *records, hipStream_t stream);
 hipError_t launch_capture(const WorldDev &w, const GridParams *capture_grid, uint32_t capture_cells, float max_gap,
                           uint32_t capacity, uint32_t *counter, uint2 *pairs, float2 *coordinates, float *gaps,
                           float *angles, hipStream_t stream);

@@ -39,6 +39,7 @@
 // so its projections reproduce a host evaluation bit for bit as well.
 
 #include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
 #include <stdint.h>
 
 #include "acosf_f32.h"
@@ -1756,95 +1757,125 @@ __global__ void k_compact_walls(float4 *pos, const float4 *vA, const float4 *vB,
     if (wall) wall[v] = make_float4(r[0], r[1], r[2], 0.f);
 }
 
-// measure_compaction_metrics (kernels.rs:811-923). The reference reduces on
-// one thread in index order; so does lane 0 here, so every f32 sum rounds
-// alike. The other lanes only fetch: each chunk of 64 segments (then
-// vertices) has its per-item terms computed by all lanes into LDS, then
-// lane 0 adds them in order. out[13] = {pressure x, y, z, contact, stretch,
-// bending and total energy, lower x, upper x, lower y, upper y, lower z,
-// upper z face pressures}.
-__global__ __launch_bounds__(64) void k_compaction_metrics(const GridParams *grid, const float4 *pos,
-                                                           const uint4 *sstat, const uint32_t *factive,
-                                                           const float4 *vA, const float4 *corr0,
-                                                           const float4 *corr1, const float *ratios,
-                                                           const float4 *wall, uint32_t S, uint32_t V,
-                                                           float correction_fraction, float contact_stiffness,
-                                                           float stretch_stiffness, float bending_stiffness,
-                                                           float *out) {
-    __shared__ float terms[7][64];
-    const uint32_t lane = threadIdx.x;
-    float load_x = 0.f, load_y = 0.f, load_z = 0.f, contact = 0.f, stretch = 0.f;
-    for (uint32_t base = 0; base < S; base += 64) {
-        const uint32_t s = base + lane;
+// measure_compaction_metrics (kernels.rs:811-923), in two launches. The
+// reference reduces on one thread in index order. Here k_compaction_terms
+// computes every item's terms in parallel (zero where the reference skips
+// the item, which leaves its nonnegative sums unchanged) into rows [5][S]
+// (segments) and [7][V] (vertices); k_compaction_sums then keeps each sum's
+// index order on a lane of its own, streaming the rows through LDS tiles
+// that the whole workgroup loads one tile ahead, so every f32 sum rounds as
+// the reference's does while the memory latency hides under the additions.
+// out[13] = {pressure x, y, z, contact, stretch, bending and total energy,
+// lower x, upper x, lower y, upper y, lower z, upper z face pressures}.
+__global__ void k_compaction_terms(const float4 *pos, const uint4 *sstat, const uint32_t *factive,
+                                   const float4 *vA, const float4 *corr0, const float4 *corr1,
+                                   const float *ratios, const float4 *wall, uint32_t S, uint32_t V,
+                                   float contact_stiffness, float stretch_stiffness, float bending_stiffness,
+                                   float *terms) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S) {
         float t[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-        if (s < S) {
-            const uint4 me = sstat[s];
-            if (factive[me.y] != 0u) {
-                const float4 c0 = corr0[s], c1 = corr1[s];
-                t[0] = fabsf(c0.x) + fabsf(c1.x);
-                t[1] = fabsf(c0.y) + fabsf(c1.y);
-                t[2] = fabsf(c0.z) + fabsf(c1.z);
-                const float penetration = c0.w;
-                t[3] = 0.5f * contact_stiffness * penetration * penetration;
-                const float4 a = pos[me.x], b = pos[me.x + 1];
-                const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
-                const float length = sqrtf(dx * dx + dy * dy + dz * dz);
-                const float rest = vA[me.x].z;
-                if (rest > 1.0e-7f) {
-                    const float extension = length - rest;
-                    t[4] = 0.5f * stretch_stiffness * extension * extension / rest;
-                }
+        const uint4 me = sstat[i];
+        if (factive[me.y] != 0u) {
+            const float4 c0 = corr0[i], c1 = corr1[i];
+            t[0] = fabsf(c0.x) + fabsf(c1.x);
+            t[1] = fabsf(c0.y) + fabsf(c1.y);
+            t[2] = fabsf(c0.z) + fabsf(c1.z);
+            const float penetration = c0.w;
+            t[3] = 0.5f * contact_stiffness * penetration * penetration;
+            const float4 a = pos[me.x], b = pos[me.x + 1];
+            const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+            const float length = sqrtf(dx * dx + dy * dy + dz * dz);
+            const float rest = vA[me.x].z;
+            if (rest > 1.0e-7f) {
+                const float extension = length - rest;
+                t[4] = 0.5f * stretch_stiffness * extension * extension / rest;
             }
         }
-        // skipped items add +0, which leaves the nonnegative sums unchanged
-        for (int k = 0; k < 5; ++k) terms[k][lane] = t[k];
-        __syncthreads();
-        if (lane == 0) {
-            const uint32_t n = min(64u, S - base);
-            for (uint32_t i = 0; i < n; ++i) {
-                load_x += terms[0][i];
-                load_y += terms[1][i];
-                load_z += terms[2][i];
-                contact += terms[3][i];
-                stretch += terms[4][i];
-            }
-        }
-        __syncthreads();
+        for (int k = 0; k < 5; ++k) terms[(size_t)k * S + i] = t[k];
     }
-    float bending = 0.f, lower_x = 0.f, upper_x = 0.f, lower_y = 0.f, upper_y = 0.f, lower_z = 0.f, upper_z = 0.f;
-    for (uint32_t base = 0; base < V; base += 64) {
-        const uint32_t v = base + lane;
+    if (i < V) {
         float t[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (v < V) {
-            const float excess = fmaxf(ratios[v] - 1.0f, 0.f);
-            t[0] = 0.5f * bending_stiffness * excess * excess;
-            if (wall) {
-                const float4 r = wall[v];
-                t[1] = fmaxf(-r.x, 0.f);
-                t[2] = fmaxf(r.x, 0.f);
-                t[3] = fmaxf(-r.y, 0.f);
-                t[4] = fmaxf(r.y, 0.f);
-                t[5] = fmaxf(-r.z, 0.f);
-                t[6] = fmaxf(r.z, 0.f);
+        const float excess = fmaxf(ratios[i] - 1.0f, 0.f);
+        t[0] = 0.5f * bending_stiffness * excess * excess;
+        if (wall) {
+            const float4 r = wall[i];
+            t[1] = fmaxf(-r.x, 0.f);
+            t[2] = fmaxf(r.x, 0.f);
+            t[3] = fmaxf(-r.y, 0.f);
+            t[4] = fmaxf(r.y, 0.f);
+            t[5] = fmaxf(-r.z, 0.f);
+            t[6] = fmaxf(r.z, 0.f);
+        }
+        float *vt = terms + (size_t)5 * S;
+        for (int k = 0; k < 7; ++k) vt[(size_t)k * V + i] = t[k];
+    }
+}
+
+constexpr uint32_t kSumThreads = 256, kSumPerThread = 4, kSumTile = kSumThreads * kSumPerThread;
+
+// Adds rows[k][0..n) in index order on lane k (k < nrows), one LDS tile of
+// kSumTile items at a time; the next tile's global loads are in flight while
+// the current one is added.
+__device__ void ordered_row_sums(const float *rows, uint32_t nrows, uint32_t n, float (*tile)[7][kSumTile + 4],
+                                 float *sums) {
+    const uint32_t tid = threadIdx.x;
+    float acc = 0.f;
+    float regs[7][kSumPerThread];
+    auto load = [&](uint32_t base) {
+        for (uint32_t k = 0; k < nrows; ++k)
+            for (uint32_t j = 0; j < kSumPerThread; ++j) {
+                const uint32_t idx = base + tid + j * kSumThreads;
+                regs[k][j] = idx < n ? rows[(size_t)k * n + idx] : 0.f;
+            }
+    };
+    auto store = [&](uint32_t buf) {
+        for (uint32_t k = 0; k < nrows; ++k)
+            for (uint32_t j = 0; j < kSumPerThread; ++j) tile[buf][k][tid + j * kSumThreads] = regs[k][j];
+    };
+    if (n > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    uint32_t buf = 0;
+    for (uint32_t base = 0; base < n; base += kSumTile, buf ^= 1u) {
+        const bool more = base + kSumTile < n;
+        if (more) load(base + kSumTile);
+        if (tid < nrows) {
+            const float *row = tile[buf][tid];
+            const uint32_t m = min(kSumTile, n - base);
+            if (m == kSumTile) {
+                for (uint32_t i = 0; i < kSumTile; i += 4u) {
+                    const float4 q = *reinterpret_cast<const float4 *>(row + i);
+                    acc += q.x;
+                    acc += q.y;
+                    acc += q.z;
+                    acc += q.w;
+                }
+            } else {
+                for (uint32_t i = 0; i < m; ++i) acc += row[i];
             }
         }
-        for (int k = 0; k < 7; ++k) terms[k][lane] = t[k];
-        __syncthreads();
-        if (lane == 0) {
-            const uint32_t n = min(64u, V - base);
-            for (uint32_t i = 0; i < n; ++i) {
-                bending += terms[0][i];
-                lower_x += terms[1][i];
-                upper_x += terms[2][i];
-                lower_y += terms[3][i];
-                upper_y += terms[4][i];
-                lower_z += terms[5][i];
-                upper_z += terms[6][i];
-            }
-        }
+        if (more) store(buf ^ 1u);
         __syncthreads();
     }
-    if (lane != 0) return;
+    if (tid < nrows) sums[tid] = acc;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kSumThreads) void k_compaction_sums(const GridParams *grid, const float *terms,
+                                                                 uint32_t S, uint32_t V,
+                                                                 float correction_fraction,
+                                                                 float contact_stiffness, float *out) {
+    __shared__ __align__(16) float tile[2][7][kSumTile + 4];
+    __shared__ float sums[12];
+    ordered_row_sums(terms, 5u, S, tile, sums);
+    ordered_row_sums(terms + (size_t)5 * S, 7u, V, tile, sums + 5);
+    if (threadIdx.x != 0) return;
+    const float load_x = sums[0], load_y = sums[1], load_z = sums[2], contact = sums[3], stretch = sums[4];
+    const float bending = sums[5], lower_x = sums[6], upper_x = sums[7], lower_y = sums[8], upper_y = sums[9],
+                lower_z = sums[10], upper_z = sums[11];
     const GridParams g = *grid;
     const float length_x = g.upper[0] - g.lower[0];
     const float length_y = g.upper[1] - g.lower[1];
@@ -1980,6 +2011,37 @@ __global__ void k_capture(const GridParams *grid, const float4 *pos, const uint4
             }
         }
     }
+}
+
+// Sort keys of the captured pairs: (first segment, second segment) in one
+// 64-bit key, with the append slot as the value.
+__global__ void k_capture_keys(const uint2 *pairs, uint32_t n, uint64_t *keys, uint32_t *slots) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = ((uint64_t)pairs[i].x << 32) | pairs[i].y;
+    slots[i] = i;
+}
+
+// One captured candidate as the C ABI returns it (tgh_contact_candidate).
+struct CandidateRecord {
+    uint2 pair;
+    float2 coordinates;
+    float gap, angle;
+};
+static_assert(sizeof(CandidateRecord) == 24, "candidate record layout");
+
+// The captured arrays in sorted order, as candidate records.
+__global__ void k_capture_gather(const uint32_t *slots, uint32_t n, const uint2 *pairs, const float2 *coordinates,
+                                 const float *gaps, const float *angles, CandidateRecord *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = slots[i];
+    CandidateRecord r;
+    r.pair = pairs[k];
+    r.coordinates = coordinates[k];
+    r.gap = gaps[k];
+    r.angle = angles[k];
+    out[i] = r;
 }
 
 }  // namespace tgh
@@ -2147,10 +2209,14 @@ hipError_t launch_compact(const WorldDev &w, const CellChange &box, uint32_t kin
 
 hipError_t launch_compaction_metrics(const WorldDev &w, const GridParams *grid, float correction_fraction,
                                      float contact_stiffness, float stretch_stiffness, float bending_stiffness,
-                                     float *out, hipStream_t stream) {
-    hipLaunchKernelGGL(k_compaction_metrics, dim3(1), dim3(64), 0, stream, grid, w.pos, w.sstat, w.factive, w.vA,
-                       w.corr0, w.corr1, w.ratio, w.track_walls ? w.wall : nullptr, w.S, w.V, correction_fraction,
-                       contact_stiffness, stretch_stiffness, bending_stiffness, out);
+                                     float *terms, float *out, hipStream_t stream) {
+    const uint32_t n = w.S > w.V ? w.S : w.V;
+    if (n > 0)
+        hipLaunchKernelGGL(k_compaction_terms, dim3(blocks_for(n, 256)), dim3(256), 0, stream, w.pos, w.sstat,
+                           w.factive, w.vA, w.corr0, w.corr1, w.ratio, w.track_walls ? w.wall : nullptr, w.S, w.V,
+                           contact_stiffness, stretch_stiffness, bending_stiffness, terms);
+    hipLaunchKernelGGL(k_compaction_sums, dim3(1), dim3(kSumThreads), 0, stream, grid, terms, w.S, w.V,
+                       correction_fraction, contact_stiffness, out);
     return hipGetLastError();
 }
 
@@ -2179,6 +2245,40 @@ hipError_t launch_capture(const WorldDev &w, const GridParams *capture_grid, uin
     // the rebuild request stays pending: the next batch rebuilds the lists
     // on the relaxation grid
     return hipGetLastError();
+}
+
+hipError_t launch_capture_sort(uint32_t n, const uint2 *pairs, const float2 *coordinates, const float *gaps,
+                               const float *angles, void *records, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    uint64_t *keys = nullptr, *keys_sorted = nullptr;
+    uint32_t *slots = nullptr, *slots_sorted = nullptr;
+    void *temp = nullptr;
+    size_t temp_bytes = 0;
+    hipError_t e = hipMalloc((void **)&keys, 2 * (size_t)n * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&slots, 2 * (size_t)n * sizeof(uint32_t));
+    if (e == hipSuccess) {
+        keys_sorted = keys + n;
+        slots_sorted = slots + n;
+        hipLaunchKernelGGL(k_capture_keys, dim3(blocks_for(n, 256)), dim3(256), 0, stream, pairs, n, keys, slots);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys_sorted, slots, slots_sorted, (int)n, 0,
+                                               64, stream);
+    if (e == hipSuccess) e = hipMalloc(&temp, temp_bytes > 0 ? temp_bytes : 1);
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_sorted, slots, slots_sorted, (int)n, 0,
+                                               64, stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_capture_gather, dim3(blocks_for(n, 256)), dim3(256), 0, stream, slots_sorted, n, pairs,
+                           coordinates, gaps, angles, static_cast<CandidateRecord *>(records));
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(temp);
+    (void)hipFree(slots);
+    (void)hipFree(keys);
+    return e;
 }
 
 hipError_t launch_apply_activity(const WorldDev &w, hipStream_t stream) {

@@ -1157,14 +1157,18 @@ int tgh_world_compaction_metrics(tgh_world *world, float correction_fraction, fl
     if (!(correction_fraction > 0.0f)) return set_error(TGH_ERR_INVALID, "correction_fraction must be positive");
     if (!(contact_stiffness >= 0.0f) || !(stretch_stiffness >= 0.0f) || !(bending_stiffness >= 0.0f))
         return set_error(TGH_ERR_INVALID, "stiffnesses must be nonnegative");
-    float *dev = nullptr;
-    HIP_TRY(hipMalloc((void **)&dev, 13 * sizeof(float)));
+    if (!world->metrics_scratch) {
+        // allocated on first use and kept: every call on this world needs the same size
+        const size_t scratch = 13 + 5 * (size_t)world->d.S + 7 * (size_t)world->d.V;
+        HIP_TRY(world->alloc(&world->metrics_scratch, scratch));
+    }
+    float *dev = world->metrics_scratch;
     float values[13];
     hipError_t e = tgh::launch_compaction_metrics(world->d, world->d.grids, correction_fraction, contact_stiffness,
-                                                  stretch_stiffness, bending_stiffness, dev, world->stream);
+                                                  stretch_stiffness, bending_stiffness, dev + 13, dev,
+                                                  world->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(values, dev, sizeof(values), hipMemcpyDeviceToHost, world->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(world->stream);
-    (void)hipFree(dev);
     if (e != hipSuccess) return set_error(TGH_ERR_DEVICE, std::string("compaction metrics: ") + hipGetErrorString(e));
     for (int k = 0; k < 3; ++k) {
         out->pressure[k] = values[k];
@@ -1243,27 +1247,31 @@ int tgh_world_capture_contacts(tgh_world *world, float maximum_surface_gap, size
     HIP_TRY(hipMemcpyAsync(counter, b.counter, sizeof(counter), hipMemcpyDeviceToHost, world->stream));
     HIP_TRY(hipStreamSynchronize(world->stream));
     const size_t n = std::min<size_t>(counter[0], capacity);
-    std::vector<uint2> pairs(n);
-    std::vector<float2> coordinates(n);
-    std::vector<float> gaps(n), angles(n);
     if (n) {
-        HIP_TRY(hipMemcpy(pairs.data(), b.pairs, n * sizeof(uint2), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(coordinates.data(), b.coordinates, n * sizeof(float2), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(gaps.data(), b.gaps, n * sizeof(float), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(angles.data(), b.angles, n * sizeof(float), hipMemcpyDeviceToHost));
-    }
-    std::vector<size_t> order(n);
-    for (size_t i = 0; i < n; ++i) order[i] = i;
-    std::sort(order.begin(), order.end(), [&](size_t l, size_t r) {
-        if (pairs[l].x != pairs[r].x) return pairs[l].x < pairs[r].x;
-        if (pairs[l].y != pairs[r].y) return pairs[l].y < pairs[r].y;
-        const int32_t a = total_order_key(coordinates[l].x), b2 = total_order_key(coordinates[r].x);
-        if (a != b2) return a < b2;
-        return total_order_key(coordinates[l].y) < total_order_key(coordinates[r].y);
-    });
-    for (size_t i = 0; i < n; ++i) {
-        const size_t k = order[i];
-        out[i] = tgh_contact_candidate{pairs[k].x, pairs[k].y, coordinates[k].x, coordinates[k].y, gaps[k], angles[k]};
+        // sorted by (first, second) on the device, then copied out as records
+        void *records = nullptr;
+        HIP_TRY(hipMalloc(&records, n * sizeof(tgh_contact_candidate)));
+        hipError_t e = tgh::launch_capture_sort((uint32_t)n, b.pairs, b.coordinates, b.gaps, b.angles, records,
+                                                world->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, records, n * sizeof(tgh_contact_candidate), hipMemcpyDeviceToHost);
+        (void)hipFree(records);
+        if (e != hipSuccess) return set_error(TGH_ERR_DEVICE, std::string("capture sort: ") + hipGetErrorString(e));
+        // the reference breaks ties of (first, second) by the coordinates'
+        // total order; a pair is found once per capture, but order any run of
+        // equal pairs that way all the same
+        auto less = [](const tgh_contact_candidate &l, const tgh_contact_candidate &r) {
+            const int32_t a = total_order_key(l.first_coordinate), b2 = total_order_key(r.first_coordinate);
+            if (a != b2) return a < b2;
+            return total_order_key(l.second_coordinate) < total_order_key(r.second_coordinate);
+        };
+        for (size_t i = 0; i < n;) {
+            size_t j = i + 1;
+            while (j < n && out[j].first_segment == out[i].first_segment &&
+                   out[j].second_segment == out[i].second_segment)
+                ++j;
+            if (j - i > 1) std::sort(out + i, out + j, less);
+            i = j;
+        }
     }
     *count = n;
     *overflow = counter[1] != 0u ? 1u : 0u;

@@ -70,6 +70,7 @@ struct tgh_world {
     float4 *proposal_y = nullptr;
     float4 *proposal_backup = nullptr;
     float *proposal_scale = nullptr;  // per segment: history length (m) x diameter
+    float *metrics_scratch = nullptr;  // compaction metrics: 13 outputs, then 5 S + 7 V terms
     float4 *history = nullptr;        // per segment: midpoint at the last history mark
     bool history_valid = false;
     bool proposal_pending = false;
