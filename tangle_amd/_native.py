@@ -700,7 +700,7 @@ class World:
     def capture_contact_arrays(self, maximum_surface_gap: float, capacity: int):
         """capture_contacts as arrays: (pairs (n, 2) u32, coordinates (n, 2),
         gaps (n,), angles (n,), overflow)."""
-        out = np.zeros((max(1, int(capacity)), 6), dtype=np.float32)
+        out = np.empty((max(1, int(capacity)), 6), dtype=np.float32)
         count, overflow = C.c_size_t(), C.c_uint32()
         check(library().tgh_world_capture_contacts(self._handle, float(maximum_surface_gap), int(capacity),
                                                    out.ctypes.data_as(C.POINTER(ContactCandidateStruct)),

@@ -1906,11 +1906,16 @@ __global__ __launch_bounds__(kSumThreads) void k_compaction_sums(const GridParam
 // appends each later segment of another fiber whose surface gap is at most
 // `max_gap`, with the closest-point coordinates and the crossing angle. The
 // append order is unordered; the host sorts the pairs.
+// kCaptureLanes lanes share one segment and split each cell's entries; the
+// arithmetic of every pair is the one-lane arithmetic, only the append order
+// differs, and the host sorts the pairs.
+constexpr uint32_t kCaptureLanes = 8;
 __global__ void k_capture(const GridParams *grid, const float4 *pos, const uint4 *sstat, const uint32_t *factive,
                           const uint32_t *cell_counts, const uint32_t *cell_offsets, const uint32_t *cell_segs,
                           uint32_t S, float max_gap, uint32_t capacity, uint32_t *counter, uint2 *pairs,
                           float2 *coordinates, float *gaps, float *angles) {
-    const uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t seg = gid / kCaptureLanes, sub = gid % kCaptureLanes;
     if (seg >= S) return;
     const uint4 me = sstat[seg];
     if (factive[me.y] == 0u) return;
@@ -1936,7 +1941,7 @@ __global__ void k_capture(const GridParams *grid, const float4 *pos, const uint4
             else if (raw >= (int32_t)g.cells[k]) shift[k] = g.upper[k] - g.lower[k];
         }
         const uint32_t start = cell_offsets[cell], count = cell_counts[cell];
-        for (uint32_t j = 0; j < count; ++j) {
+        for (uint32_t j = sub; j < count; j += kCaptureLanes) {
             const uint32_t other = cell_segs[start + j];
             if (other <= seg) continue;
             const uint4 ot = sstat[other];
@@ -2239,7 +2244,8 @@ hipError_t launch_capture(const WorldDev &w, const GridParams *capture_grid, uin
                        c.cell_offsets, c.cell_cursor, c.scattered, c.S);
     hipLaunchKernelGGL(k_rank_cells, dim3(blocks_for(c.C, 4)), dim3(256), 0, stream, c.st, c.cell_counts,
                        c.cell_offsets, c.scattered, c.cell_segs, c.pos, c.sstat, c.slotA, c.slotB, c.slotC, c.C);
-    hipLaunchKernelGGL(k_capture, dim3(blocks_for(c.S, 64)), dim3(64), 0, stream, capture_grid, c.pos, c.sstat,
+    hipLaunchKernelGGL(k_capture, dim3(blocks_for((size_t)c.S * kCaptureLanes, 256)), dim3(256), 0, stream,
+                       capture_grid, c.pos, c.sstat,
                        c.factive, c.cell_counts, c.cell_offsets, c.cell_segs, c.S, max_gap, capacity, counter, pairs,
                        coordinates, gaps, angles);
     // the rebuild request stays pending: the next batch rebuilds the lists
