@@ -117,7 +117,8 @@ def main():
     from tangle_amd import _native
 
     if args.adjacent_only:
-        print(json.dumps({"compaction_capture": adjacent_workload(fx, _native)}), flush=True)
+        print(json.dumps({"compaction_capture": adjacent_workload(fx, _native),
+                          "checkpoint_export": checkpoint_export_workload(fx, tg, _native)}), flush=True)
         return
 
     d_min = 7e-6
@@ -277,6 +278,7 @@ def main():
 
     if not args.skip_adjacent:
         result["compaction_capture"] = adjacent_workload(fx, _native)
+        result["checkpoint_export"] = checkpoint_export_workload(fx, tg, _native)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(packed, cfg, S, args.cpu_iterations)
@@ -479,6 +481,48 @@ def adjacent_workload(fx, _native):
             t0 = time.perf_counter()
             world.compact_cell([lo[0], lo[1], lo[2] + dz], [hi[0], hi[1], hi[2] - dz], name)
             out[f"compact_{name}"] = rate(nbytes, time.perf_counter() - t0)
+    return out
+
+
+def checkpoint_export_workload(fx, tg, _native):
+    """Checkpoint and OVITO export of the benchmark felt after 256
+    iterations: device-state download, atomic file write and read, restore
+    into a new world, and one fiber-segment dump frame."""
+    import tempfile
+    from tangle_amd import checkpoint as ckpt
+
+    recipe = fx.bimodal_recipe()
+    pk = recipe._packed(False)
+    cfg = fx.hard_gate_settings(7e-6).to_config(7e-6)
+    cfg.force_full_iterations = 1
+    out = {"workload": "bimodal felt (2,000 fibers, 133,335 vertices) after 256 iterations"}
+    with _native.World(pk, cfg) as world, tempfile.TemporaryDirectory() as tmp:
+        world.run_batch(cfg, 256)
+        t0 = time.perf_counter()
+        state = world.checkpoint()
+        t1 = time.perf_counter()
+        nbytes = ckpt.save(f"{tmp}/felt.ckpt", "bench", state, {"operation": 0})
+        t2 = time.perf_counter()
+        loaded, _ = ckpt.load(f"{tmp}/felt.ckpt", "bench")
+        t3 = time.perf_counter()
+        with _native.World.restore(pk, cfg, loaded) as again:
+            t4 = time.perf_counter()
+            same = bool(np.array_equal(again.download_positions().view(np.uint32),
+                                       world.download_positions().view(np.uint32)))
+        positions = world.download_positions()
+        result = tg.RunResult(fiber_count=int(pk.counts.size), iterations=256, converged=False, max_penetration=0.0,
+                              max_curvature_ratio=0.0, max_displacement=0.0, active_segments=0, active_vertices=0,
+                              cell_count=0, uploaded_bytes=0, downloaded_bytes=0, events=[], warnings=[],
+                              neighbor_list_rebuilds=0, wall_seconds=0.0, positions=positions,
+                              vertex_counts=pk.counts.copy(), _fibers=list(recipe._fibers), _cell=recipe._cell)
+        t5 = time.perf_counter()
+        result.write_ovito(f"{tmp}/felt.dump")
+        t6 = time.perf_counter()
+        dump_bytes = os.path.getsize(f"{tmp}/felt.dump")
+    out.update({"checkpoint_download_ms": round(1e3 * (t1 - t0), 2), "file_bytes": int(nbytes),
+                "file_write_ms": round(1e3 * (t2 - t1), 2), "file_read_ms": round(1e3 * (t3 - t2), 2),
+                "restore_ms": round(1e3 * (t4 - t3), 2), "restored_positions_bitwise_equal": same,
+                "ovito_frame_ms": round(1e3 * (t6 - t5), 1), "ovito_frame_bytes": int(dump_bytes)})
     return out
 
 
