@@ -79,3 +79,31 @@ def test_presets_follow_the_reference():
     assert p.curvature_cleanup_sweeps == 16
     with pytest.raises(ValueError):
         tg.RelaxationOverrides.preset("no_such_preset")
+
+
+@pytest.mark.parametrize("case,message", [
+    ("empty", "no fibers"), ("single_vertex", "fewer than two vertices"), ("nan", "finite"),
+    ("zero_radius", "radius must be positive"), ("zero_cell", "cell lengths"),
+])
+def test_world_creation_rejects_invalid_assemblies(case, message):
+    """Invalid assemblies are rejected by the C ABI before any device call
+    (so this runs without a GPU)."""
+    import numpy as np
+
+    import fixtures as fx
+    from tangle_amd import _native
+
+    two = [[0.5, 0.5, 0.5], [0.6, 0.5, 0.5]]
+    if case == "empty":
+        pk = _native.PackedAssembly([], np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0), None, [0, 0, 0], [1, 1, 1],
+                                    [False] * 3)
+    elif case == "single_vertex":
+        pk = fx.packed([[[0.5, 0.5, 0.5]]], 0.1, [1.0] * 3, [False] * 3)
+    elif case == "nan":
+        pk = fx.packed([[[0.5, 0.5, float("nan")], [0.6, 0.5, 0.5]]], 0.1, [1.0] * 3, [False] * 3)
+    elif case == "zero_radius":
+        pk = fx.packed([two], 0.0, [1.0] * 3, [False] * 3)
+    else:
+        pk = fx.packed([two], 0.1, [1.0, 0.0, 1.0], [False] * 3)
+    with pytest.raises(ValueError, match=message):
+        _native.World(pk, fx.config())
