@@ -134,3 +134,49 @@ def test_recipe_resume_matches_the_uninterrupted_run(hip_device, tmp_path, monke
     other.insert(tg.FiberCollection.from_centerlines([[[0.1, 0.5, 0.1], [0.9, 0.5, 0.1]]], tg.Material("f", 0.037)))
     with pytest.raises(tg.CheckpointError):
         other.run(settings, checkpoint=settings_b)
+
+
+def test_resume_keeps_cell_bend_radii_and_junctions(hip_device, tmp_path, monkeypatch):
+    """A recipe that fits the cell, changes a bend radius and captures
+    junctions, resumed from a checkpoint taken after those operations, ends
+    as the uninterrupted run does: same iterations, events, junctions, cell
+    and positions bit for bit."""
+    from test_gpu_compaction import bounded_bimodal_recipe
+
+    def recipe():
+        r = bounded_bimodal_recipe()
+        r.relax_for(128)
+        r.set_min_bend_radius("coarse", 120e-6)
+        r.fit_cell_to_active_fibers(padding=1e-6)
+        r.capture_junctions(tg.JunctionPolicy("bonds", max_surface_gap=0.5e-6))
+        r.relax_for(256)
+        r.capture_junctions(tg.JunctionPolicy("late", max_surface_gap=0.2e-6))
+        return r
+
+    settings = tg.RelaxationSettings(penetration_tolerance=0.05 * 7e-6, curvature_ratio_tolerance=0.01,
+                                     max_iterations=700, neighbor_capacity=256)
+    mid = tmp_path / "mid.ckpt"
+    original_save = ck.save
+
+    def save_and_keep(path, case_id, device, progress):
+        size = original_save(path, case_id, device, progress)
+        if device.total_iterations >= 200 and not mid.exists():
+            shutil.copyfile(path, mid)
+        return size
+
+    monkeypatch.setattr(ck, "save", save_and_keep)
+    full = recipe().run(settings, checkpoint=tg.CheckpointSettings("cells", tmp_path / "a.ckpt",
+                                                                   interval_iterations=100))
+    monkeypatch.setattr(ck, "save", original_save)
+    assert mid.exists() and full.junction_count > 0
+    resumed = recipe().run(settings, checkpoint=tg.CheckpointSettings("cells", tmp_path / "b.ckpt",
+                                                                      interval_iterations=100, resume=True,
+                                                                      resume_path=mid))
+    print(f"[resume] at {resumed.resumed_iteration}: {resumed.iterations} vs {full.iterations} iterations",
+          flush=True)
+    assert resumed.resumed and 200 <= resumed.resumed_iteration < full.iterations
+    assert resumed.iterations == full.iterations
+    assert resumed.events == full.events
+    assert resumed.junctions == full.junctions
+    assert resumed._cell.origin == full._cell.origin and resumed._cell.lengths == full._cell.lengths
+    assert np.array_equal(bits(resumed.positions), bits(full.positions))
