@@ -272,6 +272,7 @@ def main():
 
     if not args.skip_batched:
         result["batched"] = batched_throughput(args, fx, tg, _native)
+        result["batched_time_to_gate"] = batched_warm_start(fx, tg)
 
     if not args.skip_formation:
         result["formation"] = formation_workload(args, fx, tg)
@@ -427,6 +428,37 @@ def copy_calibration(mib: int) -> None:
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "dispatches": 5}), flush=True)
+
+
+def batched_warm_start(fx, tg):
+    """The learned warm start on a batched world: 64 of the batched
+    specimens (fiber counts log-uniform in [50, 5,000], same seeds), relaxed
+    together until every one has penetration <= 1e-3 d (d = 10 um), plain
+    and with proposals from a model trained on two other specimens of the
+    same kind (training time reported apart)."""
+    from tangle_amd import learned
+
+    d = 10e-6
+    t0 = time.perf_counter()
+    train = [r._packed(False) for r in fx.batched_specimen_recipes(count=2, n_min=1500, n_max=2000, seed=777)]
+    model, _ = learned.train_warm_start(train, d, interval=32, default_iterations=1024, cleanup_iterations=0,
+                                        epochs=12, batch_rows=16384)
+    out = {"workload": "64 batched specimens (fiber counts log-uniform in [50, 5000], 10 um x 0.5 mm, 10 % fill) "
+                       "to penetration <= 1e-3 d", "training_seconds": round(time.perf_counter() - t0, 3)}
+    for label in ("plain", "learned"):
+        recipes = fx.batched_specimen_recipes(count=64)
+        settings = tg.RelaxationSettings(penetration_tolerance=1e-3 * d, curvature_ratio_tolerance=1e-5)
+        ws = (learned.WarmStart(model, interval=32, scale=8.0, settle=4, active_iterations=10 ** 9)
+              if label == "learned" else None)
+        t1 = time.perf_counter()
+        res = tg.relax_batch(recipes, settings, max_iterations=30000, warm_start=ws)
+        out[label] = {"seconds": round(time.perf_counter() - t1, 3),
+                      "slowest_specimen_iterations": max(r.iterations for r in res),
+                      "converged": sum(r.converged for r in res)}
+    model.close()
+    if out["plain"]["converged"] == out["learned"]["converged"] == 64:
+        out["learned_speedup"] = round(out["plain"]["seconds"] / out["learned"]["seconds"], 3)
+    return out
 
 
 def adjacent_workload(fx, _native):

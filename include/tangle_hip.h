@@ -348,6 +348,9 @@ int tgh_world_features(tgh_world *world, uint16_t *out, size_t count);
  * neighbour-list rebuild. Segments without recent motion do not move. */
 int tgh_world_propose(tgh_world *world, tgh_mlp *mlp, float scale);
 int tgh_world_revert_proposal(tgh_world *world);
+/* Reverts the last proposal for the specimens with revert[s] != 0 only
+ * (revert: [specimen_count]); the others keep it. Requests a rebuild. */
+int tgh_world_revert_proposal_specimens(tgh_world *world, const uint32_t *revert, size_t count);
 /* Average device time of one forward pass and of one gradient pass (forward,
  * backward and the fixed-order reduction) over `rows` feature rows, each
  * timed over `repeats` launches with HIP events. */

@@ -1846,8 +1846,56 @@ class Recipe:
             raise AssertionError(kind)
 
 
+class _BatchedProposals:
+    """Learned proposals on a batched world: the single-world rule of
+    _RunState._propose applied per specimen (no counterpart in the
+    reference)."""
+
+    def __init__(self, warm_start, config):
+        self.ws = warm_start
+        self.config = config
+        self.marked = False
+        self.last = 0
+        self.kept = self.reverted = 0
+
+    def _residual(self, st):
+        pen_tol = max(self.config.penetration_tolerance, 1e-30)
+        curv_tol = max(self.config.curvature_ratio_tolerance, 1e-30)
+        return max(st.max_penetration / pen_tol, (st.max_curvature_ratio - 1.0) / curv_tol)
+
+    def step(self, world, done, batch):
+        """Proposes when one is due; returns the next batch size, cut so the
+        following proposal falls on a batch boundary."""
+        from . import learned
+        ws = self.ws
+        if done >= ws.active_iterations:
+            return batch
+        if not self.marked:
+            learned.mark_history(world)
+            self.marked, self.last = True, done
+        elif done - self.last >= ws.interval:
+            before = world.specimen_status()
+            learned.propose(world, ws.model, ws.scale)
+            # specimens already converged keep their positions
+            learned.revert_specimens(world, [bool(st.converged) for st in before])
+            if ws.settle > 0:
+                world.run_batch(self.config, ws.settle)
+                after = world.specimen_status()
+            else:
+                after = before
+            worse = [not b.converged and self._residual(a) > ws.tolerance * self._residual(b)
+                     for b, a in zip(before, after)]
+            if any(worse):
+                learned.revert_specimens(world, worse)
+            self.reverted += sum(worse)
+            self.kept += sum(1 for b, w in zip(before, worse) if not b.converged and not w)
+            learned.mark_history(world)
+            self.last = int(max(st.total_iterations for st in world.specimen_status()))
+        return max(1, min(batch, self.last + ws.interval - done))
+
+
 def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iterations: int | None = None,
-                iterations: int | None = None, autotune: bool = False) -> list[RunResult]:
+                iterations: int | None = None, autotune: bool = False, warm_start=None) -> list[RunResult]:
     """Relaxes independent specimens together (no counterpart in the
     reference, which runs one world at a time): every recipe's fibers and
     cell become one specimen of a batched device world, all specimens step
@@ -1858,9 +1906,13 @@ def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iter
     `max_iterations` (default: settings.max_iterations). With `autotune`,
     the cell size, skin and launch shapes are first picked by
     tangle_amd.autotune on a scratch copy (same positions bit for bit,
-    fewer seconds). Returns one RunResult per recipe, in order; a specimen
-    that does not converge in the budget is returned with converged=False
-    rather than raising."""
+    fewer seconds). With `warm_start` (a tangle_amd.learned.WarmStart) every
+    specimen still relaxing takes the network's proposal every `interval`
+    iterations, runs `settle` iterations and keeps it only when its own
+    normalised residual did not rise; converged specimens are left as they
+    are. Returns one RunResult per recipe, in order; a specimen that does not
+    converge in the budget is returned with converged=False rather than
+    raising."""
     recipes = list(recipes)
     if not recipes:
         raise ValueError("relax_batch needs at least one recipe")
@@ -1890,8 +1942,12 @@ def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iter
             tuning.apply(world)
         info = world.info()
         done = 0
+        proposals = _BatchedProposals(warm_start, config) if warm_start is not None and iterations is None else None
         while done < budget:
-            status = world.run_batch(config, min(settings.iterations_per_batch, budget - done))
+            batch = min(settings.iterations_per_batch, budget - done)
+            if proposals is not None:
+                batch = proposals.step(world, done, batch)
+            status = world.run_batch(config, batch)
             done = int(status.total_iterations)
             if status.converged or status.batch_iterations == 0:
                 break

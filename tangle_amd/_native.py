@@ -374,6 +374,7 @@ SIGNATURES = {
     "tgh_mlp_time_kernels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "tgh_world_propose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float]),
     "tgh_world_revert_proposal": (C.c_int, [C.c_void_p]),
+    "tgh_world_revert_proposal_specimens": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_size_t]),
     "tgh_mfma_probe": (C.c_int, [C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), _f32p]),
 }
 

@@ -618,6 +618,15 @@ __global__ void k_restore(float4 *pos, const float4 *backup, uint32_t V) {
     if (v < V) pos[v] = backup[v];
 }
 
+// Restores the vertices of the specimens flagged in `revert` ([nspec]).
+__global__ void k_restore_specimens(float4 *pos, const float4 *backup, const float4 *vB, const uint32_t *revert,
+                                    uint32_t nspec, uint32_t V) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const uint32_t spec = __float_as_uint(vB[v].z) >> kSpecShift;
+    if (spec < nspec && revert[spec] != 0u) pos[v] = backup[v];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 
@@ -688,6 +697,13 @@ hipError_t launch_apply_proposal(const WorldDev &w, const float4 *Y, const float
 
 hipError_t launch_restore(const WorldDev &w, const float4 *backup, hipStream_t s) {
     hipLaunchKernelGGL(k_restore, dim3(nblocks(w.V, 256)), dim3(256), 0, s, w.pos, backup, w.V);
+    return hipGetLastError();
+}
+
+hipError_t launch_restore_specimens(const WorldDev &w, const float4 *backup, const uint32_t *revert,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL(k_restore_specimens, dim3(nblocks(w.V, 256)), dim3(256), 0, s, w.pos, backup, w.vB, revert,
+                       w.nspec, w.V);
     return hipGetLastError();
 }
 

@@ -55,5 +55,8 @@ hipError_t launch_mlp_publish(const MlpDevice &m, hipStream_t s);
 hipError_t launch_apply_proposal(const WorldDev &w, const float4 *Y, const float *seg_scale, float4 *backup,
                                  float scale, hipStream_t s);
 hipError_t launch_restore(const WorldDev &w, const float4 *backup, hipStream_t s);
+// restores the vertices of the specimens with revert[specimen] != 0
+hipError_t launch_restore_specimens(const WorldDev &w, const float4 *backup, const uint32_t *revert,
+                                    hipStream_t s);
 
 }  // namespace tgh

@@ -408,6 +408,23 @@ int tgh_world_revert_proposal(tgh_world *world) {
     return TGH_OK;
 }
 
+int tgh_world_revert_proposal_specimens(tgh_world *world, const uint32_t *revert, size_t count) {
+    if (!world || !revert) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (!world->proposal_pending) return tgh::fail(TGH_ERR_INVALID, "no proposal to revert");
+    if (count != world->d.nspec) return tgh::fail(TGH_ERR_SIZE, "one flag per specimen");
+    uint32_t *flags = nullptr;
+    HIP_CHECK(hipMalloc((void **)&flags, count * sizeof(uint32_t)));
+    hipError_t e = hipMemcpyAsync(flags, revert, count * sizeof(uint32_t), hipMemcpyHostToDevice, world->stream);
+    if (e == hipSuccess) e = tgh::launch_restore_specimens(world->d, world->proposal_backup, flags, world->stream);
+    if (e == hipSuccess) e = tgh::launch_request_rebuild(world->d, world->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(world->stream);
+    (void)hipFree(flags);
+    if (e != hipSuccess) return tgh::fail(TGH_ERR_DEVICE, std::string("revert: ") + hipGetErrorString(e));
+    // the backup still holds every specimen's pre-proposal positions, so a
+    // later call may revert more specimens until the next proposal
+    return TGH_OK;
+}
+
 int tgh_mlp_time_kernels(tgh_mlp *mlp, uint32_t rows, uint32_t repeats, float *forward_ms, float *gradient_ms) {
     if (!mlp || !forward_ms || !gradient_ms) return tgh::fail(TGH_ERR_INVALID, "null argument");
     if (rows == 0 || repeats == 0) return tgh::fail(TGH_ERR_INVALID, "rows and repeats must be positive");

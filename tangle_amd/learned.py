@@ -133,6 +133,13 @@ def revert(world: _native.World) -> None:
     _native.check(_native.library().tgh_world_revert_proposal(world._handle))
 
 
+def revert_specimens(world: _native.World, flags) -> None:
+    """Reverts the last proposal for the specimens whose flag is set."""
+    arr = np.ascontiguousarray([1 if f else 0 for f in flags], dtype=np.uint32)
+    _native.check(_native.library().tgh_world_revert_proposal_specimens(world._handle, _native.ptr(arr, C.c_uint32),
+                                                                       arr.size))
+
+
 def measure(world: _native.World, config: _native.Config) -> _native.BatchStatus:
     """Residuals of the current positions, without moving anything: a batch
     whose tolerances every state meets stops at its first (measuring) pass."""

@@ -90,3 +90,31 @@ def test_relax_batch_facade(hip_device):
         assert r.converged, r
         assert r.positions.size == 3 * sum(len(f.placed) for f in recipe._fibers)
         assert r.max_penetration <= 0.01 * 10e-6
+
+
+def test_batched_warm_start_reaches_the_gate_in_fewer_iterations(hip_device):
+    """The learned warm start on a batched world: each specimen still
+    relaxing takes the proposals and keeps them only when its own residual
+    does not rise. 32 specimens reach penetration <= 1e-3 d with every
+    specimen converged, the slowest in well under half the plain
+    iterations (measured on 64 specimens: 8,788 -> 1,436)."""
+    import tangle_amd as tg
+    from tangle_amd import learned
+
+    d = 10e-6
+    train = [r._packed(False) for r in fx.batched_specimen_recipes(count=2, n_min=1500, n_max=2000, seed=777)]
+    model, losses = learned.train_warm_start(train, d, interval=32, default_iterations=1024, cleanup_iterations=0,
+                                             epochs=12, batch_rows=16384)
+    assert losses[-1] < losses[0]
+    out = {}
+    for label in ("plain", "learned"):
+        recipes = fx.batched_specimen_recipes(count=32, n_max=2000)
+        settings = tg.RelaxationSettings(penetration_tolerance=1e-3 * d, curvature_ratio_tolerance=1e-5)
+        ws = (learned.WarmStart(model, interval=32, scale=8.0, settle=4, active_iterations=10 ** 9)
+              if label == "learned" else None)
+        res = tg.relax_batch(recipes, settings, max_iterations=20000, warm_start=ws)
+        assert all(r.converged and r.max_penetration <= 1e-3 * d for r in res), label
+        out[label] = max(r.iterations for r in res)
+        print(f"[batched warm start] {label}: slowest specimen {out[label]} iterations", flush=True)
+    model.close()
+    assert out["learned"] <= 0.5 * out["plain"], out
