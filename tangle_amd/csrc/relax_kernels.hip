@@ -618,8 +618,12 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // stored stencil-order row; a list longer than that row scans the 27 cells
 // sequentially in lane 0 of its group. Thread 0 of the grid first closes a
 // pending rebuild (finish_neighbor_list_rebuild, neighbor.rs:424-432).
+// The kernel waits on dependent gathers (list entry, segment record, two
+// positions), so it is held to 80 VGPRs for six waves per SIMD instead of
+// five: a few spills on the cold overflow paths, 7-11 % less time per
+// launch (7 waves needs 14 spills and is slower again).
 template <int G>
-__global__ __launch_bounds__(256) void k_contact(DevState *st, const SpecState *ss_all, const GridParams *grids,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_contact(DevState *st, const SpecState *ss_all, const GridParams *grids,
                                                  uint32_t red_slots, const float4 *pos, const uint4 *sstat,
                                                  const uint4 *slotC,
                                                  const uint32_t *cell_segs, const uint32_t *seg_cell,
