@@ -137,6 +137,7 @@ def main():
     if args.curvature_lanes >= 0:
         world.set_tuning(curvature_lanes=args.curvature_lanes)
     lanes = int(world.tuning().curvature_lanes)
+    contact_lanes = int(world.tuning().contact_lanes)
     info = world.info()
     # warmup: builds and caches the batch graph
     for _ in range(args.warmup):
@@ -183,7 +184,7 @@ def main():
         "curvature_gs": (f"tgh::k_curvature_gs_group<{lanes}>" if lanes in (4, 8, 16, 32)
                          else ("tgh::k_curvature_gs_wave64" if lanes == 64 else "tgh::k_curvature_gs_lane"),
                          curvature_bytes_per_launch(V)),
-        "contact": ("tgh::k_contact<16>", contact_bytes_per_launch(S, V, mean_list)),
+        "contact": (f"tgh::k_contact<{contact_lanes}>", contact_bytes_per_launch(S, V, mean_list)),
     }
 
     def kernel_roofline(group):
@@ -407,7 +408,7 @@ def batched_throughput(args, fx, tg, _native):
                      "trial_ms": {k: {str(kk): round(vv, 3) for kk, vv in v.items()}
                                   for k, v in tuning.trials.items()}},
         "group_ms_per_iteration": per_iter,
-        "contact_roofline": {"bound": "hbm", "kernel": "tgh::k_contact<16>", "bytes_per_launch": int(contact_bytes),
+        "contact_roofline": {"bound": "hbm", "kernel": f"tgh::k_contact<{tuning.contact_lanes}>", "bytes_per_launch": int(contact_bytes),
                              "avg_launch_us": round(1000.0 * contact_avg, 2),
                              "achieved": round(contact_bytes / (contact_avg * 1e-3) / 1e9, 1),
                              "peak": PEAK_HBM_GBS, "unit": "GB/s",
