@@ -15,7 +15,6 @@ constexpr uint32_t kPinned = 16u;
 constexpr uint32_t kInactive = 32u;  // the vertex's fiber is not active (formation)
 
 constexpr uint32_t kBuildThreads = 128;
-constexpr uint32_t kLdsListCapacity = 64;
 // Entries kept per overflowed list (candidates in the reference's scan order).
 constexpr uint32_t kOverflowListCapacity = 256;
 // Longest fiber the per-wave curvature kernel stages in LDS (9 floats each).

@@ -281,13 +281,15 @@ __global__ __launch_bounds__(256) void k_rank_cells(const DevState *st, const ui
 // One lane per cell-list slot builds that slot's segment list from the 27
 // stencil cells (build_segment_neighbor_lists, neighbor.rs:155-364). Lists are
 // rows of `nbr` (row `slot`, `block` entries) kept in ascending segment order,
-// which replaces the separate sort (neighbor.rs:366-408). A segment with more
-// than `block` neighbours keeps its candidates in stencil-then-cell order in
-// a row of `ovf` (`ovf_cap` entries): exactly the order in which the
-// reference's overflow scan visits them (contact.rs:348-428), restricted to
-// the pairs within the skin, the only ones that can touch before the next
-// rebuild. The first kLdsListCapacity entries are staged in LDS.
-template <bool Lds>
+// which replaces the separate sort (neighbor.rs:366-408). Candidates first
+// land in the slot's row of `ovf` (`ovf_cap` entries) in stencil-then-cell
+// order: exactly the order in which the reference's overflow scan visits
+// them (contact.rs:348-428), restricted to the pairs within the skin, the
+// only ones that can touch before the next rebuild. A listed segment's row is
+// then copied to `nbr` and insertion-sorted there; an overflowed segment
+// keeps its `ovf` row. The rows stay in global memory (L2): staging them in
+// LDS held the kernel to about 2.5 waves per SIMD and was 26 % slower on
+// 7.2 M batched segments.
 __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *st, const GridParams *grids,
                                                                const float4 *slotA, const float4 *slotB,
                                                                const uint4 *slotC, const uint32_t *seg_cell,
@@ -296,10 +298,8 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
                                                                uint32_t *nbr, uint32_t *ovf,
                                                                uint32_t block, uint32_t ovf_cap, uint32_t S) {
     if (st->nstate[0] == 0) return;
-    __shared__ uint32_t lds[Lds ? kLdsListCapacity * kBuildThreads : 1];
     uint32_t slot = blockIdx.x * kBuildThreads + threadIdx.x;
     if (slot >= S || slot >= st->slots) return;
-    const uint32_t tid = threadIdx.x;
     float4 A = slotA[slot], B = slotB[slot];
     uint4 M = slotC[slot];
     const uint32_t seg = M.x, owner = M.y;
@@ -351,44 +351,24 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
             float reach = half_length + 0.5f * sqrtf(d2x * d2x + d2y * d2y + d2z * d2z) + inter;
             if (!(mdx * mdx + mdy * mdy + mdz * mdz <= reach * reach)) continue;
             if (!(axis_distance(p1x, p1y, p1z, d1x, d1y, d1z, p2x, p2y, p2z, d2x, d2y, d2z) <= inter)) continue;
-            if (Lds && count < kLdsListCapacity) {
-                lds[count * kBuildThreads + tid] = other;
-            } else if (count < ovf_cap) {
-                row[count] = other;
-            }
+            if (count < ovf_cap) row[count] = other;
             ++count;
         }
     }
     ncount[slot] = count;
-    const uint32_t staged = Lds ? (count < kLdsListCapacity ? count : kLdsListCapacity) : 0u;
     if (count <= block) {
-        // listed: insertion sort into ascending order
+        // listed: copy the row, then insertion sort into ascending order
         uint32_t *out = nbr + (size_t)slot * block;
-        if (Lds) {
-            for (uint32_t i = 1; i < count; ++i) {
-                uint32_t key = lds[i * kBuildThreads + tid];
-                uint32_t at = i;
-                while (at > 0 && lds[(at - 1) * kBuildThreads + tid] > key) {
-                    lds[at * kBuildThreads + tid] = lds[(at - 1) * kBuildThreads + tid];
-                    --at;
-                }
-                lds[at * kBuildThreads + tid] = key;
+        for (uint32_t i = 0; i < count; ++i) out[i] = row[i];
+        for (uint32_t i = 1; i < count; ++i) {
+            uint32_t key = out[i];
+            uint32_t at = i;
+            while (at > 0 && out[at - 1] > key) {
+                out[at] = out[at - 1];
+                --at;
             }
-            for (uint32_t i = 0; i < count; ++i) out[i] = lds[i * kBuildThreads + tid];
-        } else {
-            for (uint32_t i = 0; i < count; ++i) out[i] = row[i];
-            for (uint32_t i = 1; i < count; ++i) {
-                uint32_t key = out[i];
-                uint32_t at = i;
-                while (at > 0 && out[at - 1] > key) {
-                    out[at] = out[at - 1];
-                    --at;
-                }
-                out[at] = key;
-            }
+            out[at] = key;
         }
-    } else if (count <= ovf_cap) {
-        for (uint32_t i = 0; i < staged; ++i) row[i] = lds[i * kBuildThreads + tid];
     }
 }
 
@@ -2085,15 +2065,9 @@ hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
                        w.cell_offsets, w.cell_cursor, w.scattered, w.S);
     hipLaunchKernelGGL(k_rank_cells, dim3(blocks_for(w.C, 4)), dim3(256), 0, stream, w.st, w.cell_counts,
                        w.cell_offsets, w.scattered, w.cell_segs, w.pos, w.sstat, w.slotA, w.slotB, w.slotC, w.C);
-    if (w.block <= kLdsListCapacity) {
-        hipLaunchKernelGGL(k_build_lists<true>, dim3(blocks_for(w.S, kBuildThreads)), dim3(kBuildThreads), 0, stream,
-                           w.st, w.grids, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets,
-                           w.ncount, w.nbr, w.ovf, w.block, w.ovf_cap, w.S);
-    } else {
-        hipLaunchKernelGGL(k_build_lists<false>, dim3(blocks_for(w.S, kBuildThreads)), dim3(kBuildThreads), 0,
-                           stream, w.st, w.grids, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts,
-                           w.cell_offsets, w.ncount, w.nbr, w.ovf, w.block, w.ovf_cap, w.S);
-    }
+    hipLaunchKernelGGL(k_build_lists, dim3(blocks_for(w.S, kBuildThreads)), dim3(kBuildThreads), 0, stream, w.st,
+                       w.grids, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets, w.ncount,
+                       w.nbr, w.ovf, w.block, w.ovf_cap, w.S);
     return hipGetLastError();
 }
 
