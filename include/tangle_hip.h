@@ -124,6 +124,46 @@ typedef struct tgh_world tgh_world;
 void tgh_config_default(tgh_config *config);
 
 int tgh_world_create(const tgh_assembly *assembly, const tgh_config *config, tgh_world **out);
+
+/* PackedAssembly (crates/tangle_relax/src/device/packed.rs:47-141), the f32
+ * structure of arrays DeviceWorld::new takes (plugin.rs:422-424), for the
+ * uniform round layout this backend relaxes. Field meanings are the
+ * reference's: fibers follow one another (fiber_vertex_spans: first vertex
+ * and count, each span starting where the previous ended); segment first - f
+ * + l of fiber f joins vertices first + l and first + l + 1 (the layout
+ * from_assembly_with_options writes, packed.rs:150-320); one radius per
+ * fiber. Arrays the uniform layout does not need (segment_children,
+ * refinement levels and epochs, vertex_active, directors, lane offsets,
+ * junction tethers) are not taken; `adaptive` and `has_ovals` must be 0 and
+ * segment_active, when given, all 1. */
+typedef struct tgh_packed {
+    uint32_t fiber_count;
+    uint32_t vertex_count;
+    uint32_t segment_count;                /* vertex_count - fiber_count */
+    const float *positions;                /* [3 * vertex_count] */
+    const float *intrinsic_positions;      /* [3 * vertex_count] */
+    const uint32_t *segment_vertices;      /* [2 * segment_count] */
+    const uint32_t *segment_fibers;        /* [segment_count] */
+    const float *segment_radii;            /* [segment_count] */
+    const float *segment_rest_lengths;     /* [segment_count] */
+    const uint32_t *segment_active;        /* [segment_count] or NULL (all active) */
+    const uint32_t *fiber_vertex_spans;    /* [2 * fiber_count] */
+    const uint32_t *fiber_formation_layers; /* [fiber_count] or NULL; 0xffffffff: none */
+    const uint32_t *fiber_formation_steps;  /* [fiber_count] or NULL: all 0 */
+    const float *vertex_max_curvature;     /* [vertex_count]; 0: unrestricted */
+    const uint32_t *vertex_pinned;         /* [vertex_count] */
+    float cell_lower[3];
+    float cell_upper[3];
+    uint32_t cell_periodic[3];
+    uint32_t adaptive;                     /* must be 0 */
+    uint32_t has_ovals;                    /* must be 0 */
+} tgh_packed;
+
+/* DeviceWorld::new(&RelaxationConfig, PackedAssembly) (plugin.rs:422-424,
+ * world.rs:308-690): creates a world from a packed assembly. A layout that is
+ * not the uniform round one above is rejected with TGH_ERR_INVALID before any
+ * device call. */
+int tgh_world_create_packed(const tgh_packed *packed, const tgh_config *config, tgh_world **out);
 /* Batched specimens (no counterpart in the reference, which relaxes one
  * world per DeviceWorld): `count` independent assemblies, each with its own
  * cell, grid, skin and convergence, relaxed together by the same launches.

@@ -795,25 +795,30 @@ class _RunState:
         self.checkpoint_hook = None
         self.last_saved = None
 
-    def _propose(self, world, config):
+    def _propose(self, world, config, room):
         """Learned warm start: every `interval` iterations of the first
         `active_iterations`, move the vertices by the network's prediction,
         run `settle` plain iterations, and revert the move (keeping the
         iteration count) if the normalised residual max(penetration /
         tolerance, curvature excess / tolerance) rose above `tolerance` times
-        its value before the move. Plain PBD still runs every later pass."""
+        its value before the move. Plain PBD still runs every later pass.
+        A due proposal whose settle iterations would not leave one iteration
+        of the `room` the stage and the run have left is skipped; returns
+        False then, True otherwise."""
         ws = self.warm_start
         if ws is None or self.total >= ws.active_iterations:
-            return
+            return True
         from . import learned
         if not self.history_marked:
             # the first interval only provides the motion history
             learned.mark_history(world)
             self.history_marked = True
             self.last_proposal = self.total
-            return
+            return True
         if self.total - self.last_proposal < ws.interval:
-            return
+            return True
+        if ws.settle >= room:
+            return False
         self.last_proposal = self.total
         pen_tol = max(config.penetration_tolerance, 1e-30)
         curv_tol = max(config.curvature_ratio_tolerance, 1e-30)
@@ -830,7 +835,9 @@ class _RunState:
         if ws.settle > 0:
             settle = _native.Config.from_buffer_copy(config)
             settle.force_full_iterations = 1
-            after = residual(self._batch(world, settle, ws.settle))
+            # no checkpoint inside the settle: its positions may be reverted;
+            # a save that falls due is taken after the keep/revert decision
+            after = residual(self._batch(world, settle, ws.settle, checkpoint=False))
         else:
             after = residual(learned.measure(world, config))
         if after > ws.tolerance * before:
@@ -843,8 +850,10 @@ class _RunState:
             self.proposals_kept += 1
         learned.mark_history(world)
         self.last_proposal = self.total
+        self._checkpoint_if_due()
+        return True
 
-    def _batch(self, world, config, iterations):
+    def _batch(self, world, config, iterations, checkpoint=True):
         status = world.run_batch(config, iterations)
         self.segment_iterations += self.active_segments * int(status.batch_iterations)
         self.downloaded += 11 * 4 + 3 * 4
@@ -852,12 +861,18 @@ class _RunState:
         self.status = status
         self.status_current = True
         self.converged = bool(status.converged)
+        if checkpoint:
+            self._checkpoint_if_due()
+        return status
+
+    def _checkpoint_if_due(self):
+        """Saves when the iteration count has crossed an interval boundary
+        since the last save."""
         if self.checkpoint_hook is not None and self.total > 0 and self.last_saved != self.total:
             n = self.checkpoint_interval
             crossed = self.total >= n if self.last_saved is None else self.total // n > self.last_saved // n
             if crossed:
                 self.save_checkpoint()
-        return status
 
     def save_checkpoint(self, failure: str | None = None):
         """Saves the world and the recipe's progress (save_checkpoint,
@@ -868,13 +883,16 @@ class _RunState:
         self.checkpoint_hook(failure)
         self.last_saved = self.total
 
-    def _proposal_window(self, world, config, batch):
-        """Proposes when one is due and shortens the batch so that the next
-        proposal falls on a batch boundary."""
+    def _proposal_window(self, world, config, batch, room):
+        """Proposes when one is due and fits in `room` iterations, and
+        shortens the batch so that the next proposal falls on a batch
+        boundary; the caller recounts its budget afterwards, since the
+        proposal's settle iterations count."""
         ws = self.warm_start
         if ws is None or self.total >= ws.active_iterations:
             return batch
-        self._propose(world, config)
+        if not self._propose(world, config, room):
+            return batch
         return max(1, min(batch, self.last_proposal + ws.interval - self.total))
 
     def _budget(self, index, description):
@@ -906,9 +924,13 @@ class _RunState:
                 raise RecipeError(index, description, self.total,
                                   f"failed to converge in {amount} iterations: {self._residuals()}")
             remaining = self._budget(index, description)
-            batch = min(remaining, self.settings.iterations_per_batch, amount - completed)
-            batch = self._proposal_window(world, config, batch)
-            status = self._batch(world, config, batch)
+            room = min(remaining, amount - completed)
+            batch = min(room, self.settings.iterations_per_batch)
+            batch = self._proposal_window(world, config, batch, room)
+            room = min(self.settings.max_iterations - self.total, amount - (self.total - begin))
+            if room <= 0:
+                continue
+            status = self._batch(world, config, min(batch, room))
             if status.cell_list_overflow:
                 raise RecipeError(index, description, self.total, "cell list overflow")
 
@@ -951,9 +973,13 @@ class _RunState:
         config.force_full_iterations = 1 if self.settings.force_full_iterations else 0
         self.converged = False
         while self.total < self.settings.max_iterations:
-            batch = min(self.settings.max_iterations - self.total, self.settings.iterations_per_batch)
-            batch = self._proposal_window(world, config, batch)
-            status = self._batch(world, config, batch)
+            room = self.settings.max_iterations - self.total
+            batch = min(room, self.settings.iterations_per_batch)
+            batch = self._proposal_window(world, config, batch, room)
+            room = self.settings.max_iterations - self.total
+            if room <= 0:
+                break
+            status = self._batch(world, config, min(batch, room))
             if status.cell_list_overflow or self.converged:
                 break
         self.post_recipe_iterations = self.total - begin
@@ -988,9 +1014,14 @@ class _RunState:
                 raise RecipeError(index, description, self.total,
                                   f"{policy.name} failed after {completed} iterations: {self._residuals()}")
             run_remaining = self._budget(index, description)
-            batch = max(1, min(run_remaining, self.settings.iterations_per_batch, remaining))
-            batch = self._proposal_window(world, config, batch)
-            status = self._batch(world, config, batch)
+            room = min(run_remaining, remaining)
+            batch = max(1, min(room, self.settings.iterations_per_batch))
+            before = self.total
+            batch = self._proposal_window(world, config, batch, room)
+            room -= self.total - before
+            if room <= 0:
+                continue
+            status = self._batch(world, config, min(batch, room))
             measured = True
             if status.cell_list_overflow:
                 raise RecipeError(index, description, self.total, "cell list overflow")
@@ -1594,6 +1625,11 @@ class Recipe:
             report.update(resumed=True, resumed_iteration=run.total, last_iteration=run.total)
             if checkpoint.fresh_formation_on_resume:
                 restored.layer_target = restored.vertex_target = None
+                # the world restores onto the checkpoint's (possibly compacted) cell
+                if restored.cell_lower is not None and all(np.isfinite(restored.cell_lower)):
+                    state["cell"] = ([float(v) for v in restored.cell_lower],
+                                     [float(np.float32(np.float32(restored.cell_upper[k])
+                                                       - np.float32(restored.cell_lower[k]))) for k in range(3)])
             else:
                 if progress.get("failure"):
                     raise RecipeError(progress["operation"], descriptions[progress["operation"]]
@@ -1863,17 +1899,19 @@ class _BatchedProposals:
         curv_tol = max(self.config.curvature_ratio_tolerance, 1e-30)
         return max(st.max_penetration / pen_tol, (st.max_curvature_ratio - 1.0) / curv_tol)
 
-    def step(self, world, done, batch):
-        """Proposes when one is due; returns the next batch size, cut so the
-        following proposal falls on a batch boundary."""
+    def step(self, world, done, batch, room):
+        """Proposes when one is due and its settle iterations leave one of
+        the `room` the budget has left; returns the iteration count after
+        the proposal and the next batch size, cut so that the following
+        proposal falls on a batch boundary and the budget holds."""
         from . import learned
         ws = self.ws
         if done >= ws.active_iterations:
-            return batch
+            return done, batch
         if not self.marked:
             learned.mark_history(world)
             self.marked, self.last = True, done
-        elif done - self.last >= ws.interval:
+        elif done - self.last >= ws.interval and ws.settle < room:
             before = world.specimen_status()
             learned.propose(world, ws.model, ws.scale)
             # specimens already converged keep their positions
@@ -1891,7 +1929,11 @@ class _BatchedProposals:
             self.kept += sum(1 for b, w in zip(before, worse) if not b.converged and not w)
             learned.mark_history(world)
             self.last = int(max(st.total_iterations for st in world.specimen_status()))
-        return max(1, min(batch, self.last + ws.interval - done))
+            room -= self.last - done
+            done = self.last
+        elif done - self.last >= ws.interval:
+            return done, batch
+        return done, max(1, min(batch, room, self.last + ws.interval - done))
 
 
 def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iterations: int | None = None,
@@ -1946,7 +1988,7 @@ def relax_batch(recipes, settings: RelaxationSettings | None = None, *, max_iter
         while done < budget:
             batch = min(settings.iterations_per_batch, budget - done)
             if proposals is not None:
-                batch = proposals.step(world, done, batch)
+                done, batch = proposals.step(world, done, batch, budget - done)
             status = world.run_batch(config, batch)
             done = int(status.total_iterations)
             if status.converged or status.batch_iterations == 0:

@@ -55,6 +55,32 @@ class Assembly(C.Structure):
     ]
 
 
+class PackedStruct(C.Structure):
+    """tgh_packed: PackedAssembly's f32 arrays (packed.rs:47-141)."""
+    _fields_ = [
+        ("fiber_count", C.c_uint32),
+        ("vertex_count", C.c_uint32),
+        ("segment_count", C.c_uint32),
+        ("positions", _f32p),
+        ("intrinsic_positions", _f32p),
+        ("segment_vertices", _u32p),
+        ("segment_fibers", _u32p),
+        ("segment_radii", _f32p),
+        ("segment_rest_lengths", _f32p),
+        ("segment_active", _u32p),
+        ("fiber_vertex_spans", _u32p),
+        ("fiber_formation_layers", _u32p),
+        ("fiber_formation_steps", _u32p),
+        ("vertex_max_curvature", _f32p),
+        ("vertex_pinned", _u32p),
+        ("cell_lower", C.c_float * 3),
+        ("cell_upper", C.c_float * 3),
+        ("cell_periodic", C.c_uint32 * 3),
+        ("adaptive", C.c_uint32),
+        ("has_ovals", C.c_uint32),
+    ]
+
+
 class Config(C.Structure):
     _fields_ = [
         ("penetration_tolerance", C.c_float),
@@ -319,6 +345,7 @@ class PopulationSpec(C.Structure):
 SIGNATURES = {
     "tgh_config_default": (None, [C.POINTER(Config)]),
     "tgh_world_create": (C.c_int, [C.POINTER(Assembly), C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    "tgh_world_create_packed": (C.c_int, [C.POINTER(PackedStruct), C.POINTER(Config), C.POINTER(C.c_void_p)]),
     "tgh_world_create_batch": (C.c_int, [C.POINTER(Assembly), C.c_uint32, C.POINTER(Config), C.POINTER(C.c_void_p)]),
     "tgh_world_specimen_status": (C.c_int, [C.c_void_p, C.POINTER(BatchStatus), C.c_size_t]),
     "tgh_world_run_batch": (C.c_int, [C.c_void_p, C.POINTER(Config), C.c_uint32, C.POINTER(BatchStatus)]),
@@ -479,6 +506,109 @@ class PackedAssembly:
         return int(self.struct.vertex_count)
 
 
+class PackedLayout:
+    """PackedAssembly's f32 structure of arrays for the uniform round layout
+    (crates/tangle_relax/src/device/packed.rs:47-141), the input of
+    DeviceWorld::new; holds the arrays alive for a tgh_packed view.
+    `from_assembly` packs a PackedAssembly (f64 fiber arrays) the way
+    from_assembly_with_options does (packed.rs:150-320): f32 positions, rest
+    lengths as f32 of the f64 intrinsic distance (packed.rs:1058-1063), the
+    bend limit 1/r_min on internal vertices, pins."""
+
+    def __init__(self, positions, intrinsic_positions, segment_vertices, segment_fibers, segment_radii,
+                 segment_rest_lengths, fiber_vertex_spans, vertex_max_curvature, vertex_pinned, cell_lower,
+                 cell_upper, cell_periodic, *, segment_active=None, fiber_formation_layers=None,
+                 fiber_formation_steps=None, adaptive=False, has_ovals=False):
+        f32, u32 = np.float32, np.uint32
+        self.positions = np.ascontiguousarray(positions, dtype=f32).reshape(-1)
+        self.intrinsic_positions = np.ascontiguousarray(intrinsic_positions, dtype=f32).reshape(-1)
+        self.segment_vertices = np.ascontiguousarray(segment_vertices, dtype=u32).reshape(-1)
+        self.segment_fibers = np.ascontiguousarray(segment_fibers, dtype=u32)
+        self.segment_radii = np.ascontiguousarray(segment_radii, dtype=f32)
+        self.segment_rest_lengths = np.ascontiguousarray(segment_rest_lengths, dtype=f32)
+        self.fiber_vertex_spans = np.ascontiguousarray(fiber_vertex_spans, dtype=u32).reshape(-1)
+        self.vertex_max_curvature = np.ascontiguousarray(vertex_max_curvature, dtype=f32)
+        self.vertex_pinned = np.ascontiguousarray(vertex_pinned, dtype=u32)
+        self.segment_active = None if segment_active is None else np.ascontiguousarray(segment_active, dtype=u32)
+        self.fiber_formation_layers = (None if fiber_formation_layers is None
+                                       else np.ascontiguousarray(fiber_formation_layers, dtype=u32))
+        self.fiber_formation_steps = (None if fiber_formation_steps is None
+                                      else np.ascontiguousarray(fiber_formation_steps, dtype=u32))
+        V, S, F = self.positions.size // 3, self.segment_fibers.size, self.fiber_vertex_spans.size // 2
+        sizes = {"intrinsic_positions": (self.intrinsic_positions, 3 * V),
+                 "segment_vertices": (self.segment_vertices, 2 * S), "segment_radii": (self.segment_radii, S),
+                 "segment_rest_lengths": (self.segment_rest_lengths, S),
+                 "vertex_max_curvature": (self.vertex_max_curvature, V), "vertex_pinned": (self.vertex_pinned, V),
+                 "segment_active": (self.segment_active, S),
+                 "fiber_formation_layers": (self.fiber_formation_layers, F),
+                 "fiber_formation_steps": (self.fiber_formation_steps, F)}
+        for name, (arr, n) in sizes.items():
+            if arr is not None and arr.size != n:
+                raise ValueError(f"{name} must hold {n} values")
+        st = PackedStruct()
+        st.fiber_count, st.vertex_count, st.segment_count = F, V, S
+        st.positions = ptr(self.positions, C.c_float)
+        st.intrinsic_positions = ptr(self.intrinsic_positions, C.c_float)
+        st.segment_vertices = ptr(self.segment_vertices, C.c_uint32)
+        st.segment_fibers = ptr(self.segment_fibers, C.c_uint32)
+        st.segment_radii = ptr(self.segment_radii, C.c_float)
+        st.segment_rest_lengths = ptr(self.segment_rest_lengths, C.c_float)
+        st.segment_active = None if self.segment_active is None else ptr(self.segment_active, C.c_uint32)
+        st.fiber_vertex_spans = ptr(self.fiber_vertex_spans, C.c_uint32)
+        st.fiber_formation_layers = (None if self.fiber_formation_layers is None
+                                     else ptr(self.fiber_formation_layers, C.c_uint32))
+        st.fiber_formation_steps = (None if self.fiber_formation_steps is None
+                                    else ptr(self.fiber_formation_steps, C.c_uint32))
+        st.vertex_max_curvature = ptr(self.vertex_max_curvature, C.c_float)
+        st.vertex_pinned = ptr(self.vertex_pinned, C.c_uint32)
+        for k in range(3):
+            st.cell_lower[k] = float(cell_lower[k])
+            st.cell_upper[k] = float(cell_upper[k])
+            st.cell_periodic[k] = 1 if cell_periodic[k] else 0
+        st.adaptive = 1 if adaptive else 0
+        st.has_ovals = 1 if has_ovals else 0
+        self.struct = st
+
+    @property
+    def vertex_count(self) -> int:
+        return int(self.struct.vertex_count)
+
+    @classmethod
+    def from_assembly(cls, packed: PackedAssembly) -> "PackedLayout":
+        counts = packed.counts.astype(np.int64)
+        F, V = counts.size, int(counts.sum())
+        starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        intr64 = packed.intrinsic.reshape(-1, 3)
+        seg_vtx, seg_fib, rest = [], [], []
+        for f in range(F):
+            first = np.arange(starts[f], starts[f] + counts[f] - 1)
+            seg_vtx.append(np.stack([first, first + 1], axis=1))
+            seg_fib.append(np.full(first.size, f))
+            d = intr64[first + 1] - intr64[first]
+            rest.append(np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]))
+        seg_vtx = np.concatenate(seg_vtx)
+        seg_fib = np.concatenate(seg_fib)
+        kmax = np.zeros(V, dtype=np.float32)
+        if packed.min_bend_radius is not None:
+            for f in range(F):
+                r = packed.min_bend_radius[f]
+                if r > 0.0 and np.isfinite(r):
+                    kmax[starts[f] + 1:starts[f] + counts[f] - 1] = np.float32(1.0 / r)
+        pinned = np.zeros(V, dtype=np.uint32)
+        if packed.vertex_pinned is not None:
+            pinned |= (packed.vertex_pinned != 0).astype(np.uint32)
+        if packed.struct.pin_fiber_ends:
+            pinned[starts] = 1
+            pinned[starts + counts - 1] = 1
+        origin = [packed.struct.cell_origin[k] for k in range(3)]
+        upper = [packed.struct.cell_origin[k] + packed.struct.cell_lengths[k] for k in range(3)]
+        return cls(packed.placed.astype(np.float32), packed.intrinsic.astype(np.float32), seg_vtx, seg_fib,
+                   packed.radius.astype(np.float32)[seg_fib], np.concatenate(rest).astype(np.float32),
+                   np.stack([starts, counts], axis=1), kmax, pinned, np.float32(origin), np.float32(upper),
+                   [packed.struct.cell_periodic[k] for k in range(3)],
+                   fiber_formation_layers=packed.formation_layers, fiber_formation_steps=packed.formation_steps)
+
+
 class World:
     """One resident device world (DeviceFiberWorld on the HIP backend), or
     several specimens relaxed together (World.batch)."""
@@ -491,6 +621,20 @@ class World:
         handle = C.c_void_p()
         check(lib.tgh_world_create(C.byref(packed.struct), C.byref(config), C.byref(handle)))
         self._handle = handle
+
+    @classmethod
+    def from_packed(cls, layout: "PackedLayout", config: Config) -> "World":
+        """DeviceWorld::new from PackedAssembly's f32 arrays (tgh_world_create_packed)."""
+        lib = library()
+        self = cls.__new__(cls)
+        self._packed = layout
+        self._specimens = [layout]
+        self._vertex_count = layout.vertex_count
+        self._handle = None
+        handle = C.c_void_p()
+        check(lib.tgh_world_create_packed(C.byref(layout.struct), C.byref(config), C.byref(handle)))
+        self._handle = handle
+        return self
 
     @classmethod
     def batch(cls, specimens, config: Config) -> "World":

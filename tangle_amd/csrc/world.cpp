@@ -157,6 +157,105 @@ int pack_assembly(const tgh_assembly *a, Packed &p) {
     return TGH_OK;
 }
 
+// PackedAssembly (packed.rs:47-141) of a uniform, round layout: the f32
+// arrays are taken as they are; the layout must be the one
+// from_assembly_with_options writes for such an assembly (packed.rs:150-320):
+// fibers contiguous, segment f-th fiber's l-th = vertices (first + l, first +
+// l + 1) numbered first - f + l, one radius per fiber, every segment active.
+int pack_packed(const tgh_packed *a, Packed &p) {
+    if (!a) return set_error(TGH_ERR_INVALID, "packed assembly is null");
+    if (a->fiber_count == 0 || a->vertex_count == 0 || a->segment_count == 0)
+        return set_error(TGH_ERR_INVALID, "packed assembly holds no fibers");
+    if (!a->positions || !a->intrinsic_positions || !a->segment_vertices || !a->segment_fibers ||
+        !a->segment_radii || !a->segment_rest_lengths || !a->fiber_vertex_spans || !a->vertex_max_curvature ||
+        !a->vertex_pinned)
+        return set_error(TGH_ERR_INVALID, "packed assembly is missing an array");
+    if (a->adaptive) return set_error(TGH_ERR_INVALID, "adaptive (dyadic tree) layouts are not supported");
+    if (a->has_ovals) return set_error(TGH_ERR_INVALID, "oval sections are not supported");
+    p.F = a->fiber_count;
+    p.V = a->vertex_count;
+    if ((uint64_t)p.V < (uint64_t)p.F * 2 || a->segment_count != p.V - p.F)
+        return set_error(TGH_ERR_INVALID, "segment_count must equal vertex_count - fiber_count");
+    p.S = a->segment_count;
+    p.fstart.resize(p.F);
+    p.fcount.resize(p.F);
+    p.radius.resize(p.F);
+    uint64_t next = 0;
+    for (uint32_t f = 0; f < p.F; ++f) {
+        uint32_t st = a->fiber_vertex_spans[2 * (size_t)f], n = a->fiber_vertex_spans[2 * (size_t)f + 1];
+        if (st != next || n < 2)
+            return set_error(TGH_ERR_INVALID, "fiber " + std::to_string(f) +
+                                                  " vertex span must follow the previous one and hold two vertices");
+        p.fstart[f] = st;
+        p.fcount[f] = n;
+        next += n;
+    }
+    if (next != p.V) return set_error(TGH_ERR_INVALID, "fiber vertex spans do not cover vertex_count");
+    for (uint32_t f = 0; f < p.F; ++f) {
+        const uint32_t st = p.fstart[f], n = p.fcount[f];
+        for (uint32_t l = 0; l + 1 < n; ++l) {
+            const size_t s = (size_t)st - f + l;
+            if (a->segment_vertices[2 * s] != st + l || a->segment_vertices[2 * s + 1] != st + l + 1 ||
+                a->segment_fibers[s] != f)
+                return set_error(TGH_ERR_INVALID, "segment " + std::to_string(s) +
+                                                      " is not the uniform layout's segment of fiber " +
+                                                      std::to_string(f));
+            if (a->segment_active && a->segment_active[s] == 0)
+                return set_error(TGH_ERR_INVALID, "inactive segments (adaptive layouts) are not supported");
+            float r = a->segment_radii[s];
+            if (!(r > 0.0f) || !std::isfinite(r))
+                return set_error(TGH_ERR_INVALID, "segment " + std::to_string(s) + " radius must be positive");
+            if (l == 0)
+                p.radius[f] = r;
+            else if (r != p.radius[f])
+                return set_error(TGH_ERR_INVALID, "fiber " + std::to_string(f) + " segments differ in radius");
+        }
+    }
+    p.pos.assign(a->positions, a->positions + 3 * (size_t)p.V);
+    p.intr.assign(a->intrinsic_positions, a->intrinsic_positions + 3 * (size_t)p.V);
+    for (size_t i = 0; i < 3 * (size_t)p.V; ++i)
+        if (!std::isfinite(p.pos[i]) || !std::isfinite(p.intr[i]))
+            return set_error(TGH_ERR_INVALID, "packed positions must be finite");
+    p.rest.assign(a->segment_rest_lengths, a->segment_rest_lengths + p.S);
+    for (float r : p.rest)
+        if (!(r >= 0.0f) || !std::isfinite(r)) return set_error(TGH_ERR_INVALID, "rest lengths must be finite");
+    p.kmax.assign(a->vertex_max_curvature, a->vertex_max_curvature + p.V);
+    for (float k : p.kmax)
+        if (!(k >= 0.0f) || !std::isfinite(k))
+            return set_error(TGH_ERR_INVALID, "vertex_max_curvature must be finite and non-negative");
+    p.pinned.resize(p.V);
+    for (uint32_t v = 0; v < p.V; ++v) p.pinned[v] = a->vertex_pinned[v] ? 1 : 0;
+    p.fstep.assign(p.F, 0u);
+    p.flayer.assign(p.F, 0xffffffffu);
+    for (uint32_t f = 0; f < p.F; ++f) {
+        if (a->fiber_formation_steps) p.fstep[f] = a->fiber_formation_steps[f];
+        if (a->fiber_formation_layers) p.flayer[f] = a->fiber_formation_layers[f];
+    }
+    // vertex_arc_lengths (packed.rs:561-580): f64 sums over the f32 intrinsic chain
+    p.arc.assign(p.V, 0.0f);
+    for (uint32_t f = 0; f < p.F; ++f) {
+        double total = 0.0;
+        for (uint32_t v = p.fstart[f] + 1; v < p.fstart[f] + p.fcount[f]; ++v) {
+            double sq = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                double d = (double)p.intr[3 * (size_t)v + k] - (double)p.intr[3 * (size_t)(v - 1) + k];
+                sq += d * d;
+            }
+            total += std::sqrt(sq);
+            p.arc[v] = (float)total;
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (!std::isfinite(a->cell_lower[k]) || !std::isfinite(a->cell_upper[k]) ||
+            !(a->cell_upper[k] > a->cell_lower[k]))
+            return set_error(TGH_ERR_INVALID, "cell bounds must be finite with upper > lower");
+        p.lower[k] = a->cell_lower[k];
+        p.upper[k] = a->cell_upper[k];
+        p.periodic[k] = a->cell_periodic[k] ? 1u : 0u;
+    }
+    return TGH_OK;
+}
+
 int validate_config(const tgh_config *c) {
     if (!c) return set_error(TGH_ERR_INVALID, "config is null");
     if (!(c->cell_size_scale >= 1.0f) || !std::isfinite(c->cell_size_scale))
@@ -343,17 +442,20 @@ int upload_static(tgh_world *w) {
     return TGH_OK;
 }
 
-int create_world(const tgh_assembly *a, uint32_t count, const tgh_config *c, tgh_world **out) {
+// Creates a world from specimens already in the f32 layout (pack_assembly or
+// pack_packed); every input check has run before the first HIP call.
+int create_world_packed(std::vector<Packed> &&packed, const tgh_config *c, tgh_world **out) {
     int rc = validate_config(c);
     if (rc) return rc;
-    if (!a || count == 0) return set_error(TGH_ERR_INVALID, "no assembly");
-    if (count > (1u << (32 - tgh::kSpecShift))) return set_error(TGH_ERR_INVALID, "too many specimens");
+    const uint32_t count = (uint32_t)packed.size();
+    if (count == 0) return set_error(TGH_ERR_INVALID, "no assembly");
+    if (packed.size() > (size_t(1) << (32 - tgh::kSpecShift))) return set_error(TGH_ERR_INVALID, "too many specimens");
     tgh_world *w = new tgh_world();
     auto fail = [&](int code) {
         delete w;
         return code;
     };
-    w->specimens.resize(count);
+    w->specimens = std::move(packed);
     w->sizes.resize(count);
     w->vbase.assign(count + 1, 0);
     w->fbase.assign(count + 1, 0);
@@ -361,8 +463,7 @@ int create_world(const tgh_assembly *a, uint32_t count, const tgh_config *c, tgh
     w->cbase.assign(count + 1, 0);
     uint64_t V = 0, F = 0, S = 0, C = 0;
     for (uint32_t sp = 0; sp < count; ++sp) {
-        rc = pack_assembly(&a[sp], w->specimens[sp]);
-        if (!rc) rc = size_grid(w->specimens[sp], c, w->sizes[sp]);
+        rc = size_grid(w->specimens[sp], c, w->sizes[sp]);
         if (rc) {
             if (count > 1) set_error(rc, "specimen " + std::to_string(sp) + ": " + tgh_last_error());
             return fail(rc);
@@ -470,6 +571,22 @@ int create_world(const tgh_assembly *a, uint32_t count, const tgh_config *c, tgh
     if (rc) return fail(rc);
     *out = w;
     return TGH_OK;
+}
+
+int create_world(const tgh_assembly *a, uint32_t count, const tgh_config *c, tgh_world **out) {
+    int rc = validate_config(c);
+    if (rc) return rc;
+    if (!a || count == 0) return set_error(TGH_ERR_INVALID, "no assembly");
+    if (count > (1u << (32 - tgh::kSpecShift))) return set_error(TGH_ERR_INVALID, "too many specimens");
+    std::vector<Packed> packed(count);
+    for (uint32_t sp = 0; sp < count; ++sp) {
+        rc = pack_assembly(&a[sp], packed[sp]);
+        if (rc) {
+            if (count > 1) set_error(rc, "specimen " + std::to_string(sp) + ": " + tgh_last_error());
+            return rc;
+        }
+    }
+    return create_world_packed(std::move(packed), c, out);
 }
 
 std::string graph_key(const IterParams &q, bool timing) {
@@ -646,6 +763,15 @@ int tgh_world_create(const tgh_assembly *assembly, const tgh_config *config, tgh
     if (!out) return set_error(TGH_ERR_INVALID, "output pointer is null");
     *out = nullptr;
     return create_world(assembly, 1, config, out);
+}
+
+int tgh_world_create_packed(const tgh_packed *packed, const tgh_config *config, tgh_world **out) {
+    if (!out) return set_error(TGH_ERR_INVALID, "output pointer is null");
+    *out = nullptr;
+    std::vector<Packed> specimens(1);
+    int rc = pack_packed(packed, specimens[0]);
+    if (rc) return rc;
+    return create_world_packed(std::move(specimens), config, out);
 }
 
 int tgh_world_create_batch(const tgh_assembly *specimens, uint32_t count, const tgh_config *config, tgh_world **out) {

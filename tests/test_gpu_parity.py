@@ -278,3 +278,18 @@ def test_bimodal_reaches_hard_admissibility(hip_device, scale):
         assert check.total_iterations == 0 and check.converged, check.as_dict()
         assert check.max_penetration <= 1e-3 * 7e-6
         assert check.max_curvature_ratio <= 1.0 + 1e-5
+
+
+def test_create_packed_equals_create(hip_device):
+    """DeviceWorld::new from PackedAssembly's f32 arrays
+    (tgh_world_create_packed) gives the world tgh_world_create builds from
+    the f64 assembly: same sizing, lists and trajectory bit for bit."""
+    packed = fx.bimodal_recipe(scale=0.02)._packed(False)
+    layout = _native.PackedLayout.from_assembly(packed)
+    cfg = fx.hard_gate_settings(7e-6, force_full_iterations=True).to_config(7e-6)
+    with _native.World(packed, cfg) as a, _native.World.from_packed(layout, cfg) as b:
+        ia, ib = a.info(), b.info()
+        assert list(ia.cells) == list(ib.cells) and ia.cell_size == ib.cell_size
+        sa, sb = a.run_batch(cfg, 30), b.run_batch(cfg, 30)
+        assert sa.as_dict() == sb.as_dict()
+        assert np.array_equal(a.download_positions().view(np.uint32), b.download_positions().view(np.uint32))
