@@ -362,6 +362,17 @@ int tgh_mlp_get_parameters(const tgh_mlp *mlp, float *out, size_t count);
 int tgh_mlp_set_parameters(tgh_mlp *mlp, const float *in, size_t count);
 /* features: rows x 64 bf16 bit patterns; out: rows x 3 */
 int tgh_mlp_forward(tgh_mlp *mlp, const uint16_t *features, size_t rows, float *out);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): tgh_mlp_forward with the features and predictions staged in a
+ * static device array of the library instead of per-call allocations. It
+ * calls no hipMalloc or hipFree. At most TGH_MLP_FORWARD_STATIC_MAX_ROWS rows
+ * (TGH_ERR_SIZE beyond); the array is 9.75 MiB on each GPU the library runs
+ * on, held until the process ends. Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (from several threads or models)
+ * race: serialise them. Results equal tgh_mlp_forward's bit for bit. */
+#define TGH_MLP_FORWARD_STATIC_MAX_ROWS 65536u
+int tgh_mlp_forward_static(tgh_mlp *mlp, const uint16_t *features, size_t rows, float *out);
 /* loss = mean over rows of the squared error summed over the 3 outputs;
  * grad: tgh_mlp_parameter_count() floats */
 int tgh_mlp_gradients(tgh_mlp *mlp, const uint16_t *features, const float *targets, size_t rows, float *grad,

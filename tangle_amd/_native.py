@@ -20,6 +20,7 @@ TGH_OK = 0
 TGH_ERR_INVALID = -1
 TGH_ERR_DEVICE = -2
 TGH_ERR_SIZE = -3
+TGH_MLP_FORWARD_STATIC_MAX_ROWS = 65536  # include/tangle_hip.h
 
 ORIENT_ISOTROPIC = 0
 ORIENT_PLANAR = 1
@@ -390,6 +391,7 @@ SIGNATURES = {
     "tgh_mlp_get_parameters": (C.c_int, [C.c_void_p, _f32p, C.c_size_t]),
     "tgh_mlp_set_parameters": (C.c_int, [C.c_void_p, _f32p, C.c_size_t]),
     "tgh_mlp_forward": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint16), C.c_size_t, _f32p]),
+    "tgh_mlp_forward_static": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint16), C.c_size_t, _f32p]),
     "tgh_mlp_gradients": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint16), _f32p, C.c_size_t, _f32p, _f32p]),
     "tgh_mlp_record_begin": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tgh_mlp_record_end": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <string>
@@ -28,6 +29,7 @@ using tgh::MlpDevice;
     } while (0)
 
 struct tgh_mlp {
+    int device = 0;  // the device current at tgh_mlp_create
     hipStream_t stream = nullptr;
     MlpDevice m{};
     float *grad = nullptr;     // kMlpPartialStride
@@ -149,7 +151,8 @@ int tgh_mlp_create(uint64_t seed, tgh_mlp **out) {
         delete m;
         return code;
     };
-    hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    hipError_t e = hipGetDevice(&m->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e)));
     float *master = nullptr, *am = nullptr, *av = nullptr;
     uint16_t *w1, *w2, *w3, *w2n, *w3n;
@@ -229,6 +232,88 @@ int tgh_mlp_forward(tgh_mlp *mlp, const uint16_t *features, size_t rows, float *
         out[3 * r + 1] = host[r].y;
         out[3 * r + 2] = host[r].z;
     }
+    return TGH_OK;
+}
+
+// ---- forward pass without device allocation -------------------------------
+// tgh_mlp_forward_static stages the features, the network's float4 outputs
+// and the packed three-float predictions in `scratch`, a static device array
+// of this translation unit: every GPU the library runs on holds its own copy
+// until the process ends. Host code never passes the array's name to a
+// kernel or a copy: it uses the address hipGetSymbolAddress returns for the
+// device, taken once per device with that device current.
+namespace {
+constexpr size_t kStaticRows = TGH_MLP_FORWARD_STATIC_MAX_ROWS;
+constexpr size_t kStaticX = kStaticRows * tgh::kFeatureWidth * sizeof(uint16_t);  // 8 MiB
+constexpr size_t kStaticY = kStaticRows * sizeof(float4);                          // 1 MiB
+constexpr size_t kStaticP = kStaticRows * 3 * sizeof(float);                       // 0.75 MiB
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kStaticX + kStaticY + kStaticP];
+
+namespace {
+
+__global__ void k_pack_predictions(const float4 *__restrict__ Y, float *__restrict__ out, uint32_t rows) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows) {
+        const float4 y = Y[r];
+        out[3 * (size_t)r] = y.x;
+        out[3 * (size_t)r + 1] = y.y;
+        out[3 * (size_t)r + 2] = y.z;
+    }
+}
+
+// The device address of `scratch` on `device` (which must be current).
+int scratch_address(int device, unsigned char **out) {
+    if (device < 0 || device >= kMaxDevices) return tgh::fail(TGH_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return tgh::fail(TGH_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[device] = (unsigned char *)ptr;
+    }
+    *out = g_scratch[device];
+    return TGH_OK;
+}
+
+}  // namespace
+
+int tgh_mlp_forward_static(tgh_mlp *mlp, const uint16_t *features, size_t rows, float *out) {
+    if (!mlp || !features || !out) return tgh::fail(TGH_ERR_INVALID, "null argument");
+    if (rows == 0) return TGH_OK;
+    if (rows > kStaticRows)
+        return tgh::fail(TGH_ERR_SIZE, "tgh_mlp_forward_static takes at most " + std::to_string(kStaticRows) +
+                                           " rows (" + std::to_string(rows) + " given)");
+    int current = 0;
+    HIP_CHECK(hipGetDevice(&current));
+    if (current != mlp->device) HIP_CHECK(hipSetDevice(mlp->device));
+    unsigned char *base = nullptr;
+    int rc = scratch_address(mlp->device, &base);
+    hipError_t e = hipSuccess;
+    if (rc == TGH_OK) {
+        uint16_t *X = (uint16_t *)base;
+        float4 *Y = (float4 *)(base + kStaticX);
+        float *P = (float *)(base + kStaticX + kStaticY);
+        e = hipMemcpyAsync(X, features, rows * tgh::kFeatureWidth * sizeof(uint16_t), hipMemcpyHostToDevice,
+                           mlp->stream);
+        if (e == hipSuccess) e = tgh::launch_mlp_forward(mlp->m, X, Y, (uint32_t)rows, mlp->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_pack_predictions, dim3((uint32_t)((rows + 255) / 256)), dim3(256), 0, mlp->stream, Y,
+                               P, (uint32_t)rows);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, P, rows * 3 * sizeof(float), hipMemcpyDeviceToHost, mlp->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(mlp->stream);
+    }
+    if (current != mlp->device) (void)hipSetDevice(current);
+    if (rc) return rc;
+    if (e != hipSuccess) return tgh::fail(TGH_ERR_DEVICE, hipGetErrorString(e));
     return TGH_OK;
 }
 
