@@ -37,7 +37,10 @@ TGH_HOST_DEVICE inline uint32_t f32_bits(float f) {
     return u;
 }
 
-TGH_HOST_DEVICE inline float acosf_f32(float x) {
+// `Sqrt` is the correctly rounded square root to use for s = sqrt(z); z is
+// 0 or at least 2^-25 whenever s is used (|x| >= 1/2).
+template <typename Sqrt>
+TGH_HOST_DEVICE inline float acosf_f32_with(float x, Sqrt sqrt_rn) {
     const float pi = f32_from_bits(0x40490fdau);
     const float pio2_hi = f32_from_bits(0x3fc90fdau);
     const float pio2_lo = f32_from_bits(0x33a22168u);
@@ -52,7 +55,7 @@ TGH_HOST_DEVICE inline float acosf_f32(float x) {
     const float p = z * (p0 + z * (p1 + z * (p2 + z * (p3 + z * (p4 + z * p5)))));
     const float q = 1.0f + z * (q1 + z * (q2 + z * (q3 + z * q4)));
     const float r = p / q;
-    const float s = sqrtf(z);
+    const float s = sqrt_rn(z);
     const float near_zero = pio2_hi - (x - (pio2_lo - x * r));
     const float below = pi - 2.0f * (s + (r * s - pio2_lo));
     const float s_hi = f32_from_bits(f32_bits(s) & 0xfffff000u);
@@ -64,5 +67,11 @@ TGH_HOST_DEVICE inline float acosf_f32(float x) {
     if (ix > 0x3f800000u) result = (x - x) / (x - x);  // |x| > 1, NaN: NaN
     return result;
 }
+
+struct SqrtRn {
+    TGH_HOST_DEVICE float operator()(float z) const { return sqrtf(z); }
+};
+
+TGH_HOST_DEVICE inline float acosf_f32(float x) { return acosf_f32_with(x, SqrtRn{}); }
 
 }  // namespace tgh

@@ -989,8 +989,8 @@ __device__ __forceinline__ bool triplet_projection(float3 pp, float3 pc, float3 
     return true;
 }
 
-// Curvature sweep with L lanes per fiber (64 / L fibers per wave), each
-// fiber staged in LDS as float4 (x, y, z, wall extent) plus (kmax, weight).
+// Curvature sweep with 4 lanes per fiber (16 fibers per wave), each fiber
+// staged in LDS as float4 (x, y, z, wall extent) plus (kmax, weight).
 //
 // A group alternates between two modes. Screening: the next L triplets of
 // the sweep are tested from the current positions with a cheap bound; a
@@ -1019,20 +1019,11 @@ __device__ __forceinline__ float quad_bcast(float x, int k) {
 __device__ __forceinline__ float axis_sum(float x) {  // (x + y) + z over lanes 0, 1, 2 of the quad
     return (quad_bcast(x, 0) + quad_bcast(x, 1)) + quad_bcast(x, 2);
 }
-// Exchange between the two quads of an 8-lane group (lanes 0..3 <-> 4..7),
-// with DPP row shifts by 4 inside a 16-lane row: row_shl:4 gives lane i the
-// value of lane i + 4, row_shr:4 that of lane i - 4.
-__device__ __forceinline__ float swap_quads(float x, bool second) {
-    const int from_next = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x104, 0xF, 0xF, false);
-    const int from_prev = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, false);
-    return __int_as_float(second ? from_prev : from_next);
-}
-
 __device__ __forceinline__ void lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 __host__ __device__ inline size_t curvature_group_stride_floats(uint32_t max_vertices, uint32_t track_walls) {
     const size_t n = (max_vertices + 1u) & ~1u;  // keeps every region 16-byte aligned
-    return n * (4u + 2u + (track_walls ? 4u : 0u));
+    return n * (4u + 2u + (track_walls ? 4u : 0u)) + 4u;  // + a sink slot
 }
 
 template <int L>
@@ -1042,7 +1033,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, const S
                                                            float4 *wall, uint32_t F, uint32_t sweeps, float stiffness,
                                                            float margin, uint32_t track_walls,
                                                            uint32_t max_vertices) {
-    static_assert(L == 4 || L == 8 || L == 16 || L == 32, "group width");
+    static_assert(L == 4, "group width (8, 16 and 32 lanes: k_curvature_gs_lean)");
     constexpr uint32_t G = 64 / L;
     extern __shared__ float4 lds4[];
     const uint32_t lane = threadIdx.x, group = lane / L, gl = lane % L, gbase = group * L;
@@ -1118,79 +1109,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, const S
         const uint32_t j = mine ? (uint32_t)__builtin_ctzll(mine) : 0u;
         const uint32_t t = dir == 0 ? cursor + j : trip - 1 - (cursor + j);
         bool hit = false;
-        if constexpr (L >= 8) {
-            // Two quads per group. Lanes 0..2 hold the axes of the first
-            // segment (previous -> center), lanes 4..6 those of the second
-            // (center -> next): one square root, two divisions and one arc
-            // cosine serve both segments, and the values the formula pairs
-            // are swapped between the quads. Every operation and its operand
-            // order are the sequential projection's.
-            const bool second = (gl & 4u) != 0u;
-            const uint32_t ax = gl & 3u;
-            if (mine != 0 && ax < 3u && gl < 8u) {
-                const uint32_t o = second ? 1u : 0u;
-                const float a0 = Pf[4 * (t + o) + ax], a1 = Pf[4 * (t + o + 1) + ax];
-                const float e0 = P[t + o].w, e1 = P[t + o + 1].w;
-                const float2 kwp = KW[t], kwc = KW[t + 1], kwn = KW[t + 2];
-                const float kmax = kwc.x * (1.0f - margin);
-                const float d = a1 - a0;
-                const float len = sqrtf(axis_sum(d * d));
-                const float dn = d / len;
-                const float dn_o = swap_quads(dn, second), len_o = swap_quads(len, second);
-                const float l1 = second ? len_o : len, l2 = second ? len : len_o;
-                const float un = second ? dn_o : dn, vn = second ? dn : dn_o;
-                const float cosine = rclamp(axis_sum(un * vn), -1.f, 1.f);
-                const float sine_half = 0.25f * kmax * (l1 + l2);
-                const float min_cos = 1.0f - 2.0f * sine_half * sine_half;
-                float violation = min_cos - cosine;
-                bool ok = kmax > 0.f && l1 > 1.0e-7f && l2 > 1.0e-7f && sine_half < 1.0f && violation > 0.f;
-                const float sine = sqrtf(fmaxf(1.0f - cosine * cosine, 0.f));
-                // the first quad takes the turning angle, the second the limit
-                const float angle = acosf_f32(second ? min_cos : cosine);
-                const float angle_o = swap_quads(angle, second);
-                const float turn = second ? angle_o : angle, limit_angle = second ? angle : angle_o;
-                if (sine > 1.0e-3f) violation = (turn - limit_angle) * sine;
-                ok = ok && violation > 0.f;
-                // (vn - cos un) / l1 in the first quad, (un - cos vn) / l2 in the second
-                const float g = (dn_o - cosine * dn) / len;
-                const float g_o = swap_quads(g, second);
-                const float gu = second ? g_o : g, gv = second ? g : g_o;
-                const float gc = gu - gv;
-                const float fwp = kwp.y, fwc = kwc.y, fwn = kwn.y;
-                const float sg = axis_sum(g * g), sg_o = swap_quads(sg, second);
-                const float suu = second ? sg_o : sg, svv = second ? sg : sg_o;
-                const float den = fwp * suu + fwc * axis_sum(gc * gc) + fwn * svv;
-                ok = ok && den > 1.0e-12f;
-                if (ok) {
-                    const float scale = stiffness * violation / den;
-                    float *W = (float *)WR;
-                    if (!second) {
-                        const float m_p = a0 - fwp * scale * gu;
-                        const float m_c = a1 + fwc * scale * gc;
-                        float v_p = m_p, v_c = m_c;
-                        if (walled) {
-                            v_p = rclamp(m_p, lo + e0, hi - e0);
-                            v_c = rclamp(m_c, lo + e1, hi - e1);
-                            if (track_walls) {
-                                if (fwp > 0.f) W[4 * t + ax] += m_p - v_p;
-                                if (fwc > 0.f) W[4 * (t + 1) + ax] += m_c - v_c;
-                            }
-                        }
-                        if (fwp > 0.f) Pf[4 * t + ax] = v_p;
-                        if (fwc > 0.f) Pf[4 * (t + 1) + ax] = v_c;
-                    } else {
-                        const float m_n = a1 + fwn * scale * gv;
-                        float v_n = m_n;
-                        if (walled) {
-                            v_n = rclamp(m_n, lo + e1, hi - e1);
-                            if (track_walls && fwn > 0.f) W[4 * (t + 2) + ax] += m_n - v_n;
-                        }
-                        if (fwn > 0.f) Pf[4 * (t + 2) + ax] = v_n;
-                    }
-                }
-                hit = ok;
-            }
-        } else if (mine != 0 && gl < 3) {
+        if (mine != 0 && gl < 3) {
             float pp = Pf[4 * t + k], pc = Pf[4 * (t + 1) + k], pn = Pf[4 * (t + 2) + k];
             const float ep = P[t].w, ec = P[t + 1].w, en = P[t + 2].w;
             const float2 kwp = KW[t], kwc = KW[t + 1], kwn = KW[t + 2];
@@ -1244,6 +1163,247 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, const S
             cursor += j + 1;
             if ((hits >> gbase) & 1ull) any = true;
         }
+        // one wave: its LDS accesses complete in order; keep the compiler's
+        lds_order();
+    }
+    __syncthreads();
+    const float limit = half_skin * half_skin;
+    bool flag = false;
+    for (uint32_t i = gl; i < n; i += L) {
+        const uint32_t v = start + i;
+        const float4 old = in[v], q = P[i];
+        pos[v] = make_float4(q.x, q.y, q.z, old.w);
+        if (track_walls) wall[v] = make_float4(WR[i].x, WR[i].y, WR[i].z, 0.f);
+        const float4 r = refpos[v];
+        const float dx = q.x - r.x, dy = q.y - r.y, dz = q.z - r.z;
+        if (dx * dx + dy * dy + dz * dz > limit) flag = true;
+    }
+    if (__ballot(flag) != 0 && lane == 0) st->nstate[0] = 1u;
+}
+
+// Curvature sweep, straight-line form (L = 8, 16 or 32 lanes per fiber).
+//
+// The same screening and exact two-quad projection as k_curvature_gs_group,
+// in the same order, so the result is again the sequential sweep's, bit for
+// bit; what changes is the control flow. The step is issue-bound with about
+// one wave per SIMD, and divergent branches (each an exec-mask save and
+// restore plus an instruction-buffer refill) cost as much as the arithmetic,
+// so the loop is written without them: the sweep bookkeeping is a set of
+// selects, screening reads a clamped position instead of branching on its
+// validity, and every lane of a wave runs the exact evaluation whenever any
+// group has a candidate; groups without one evaluate a straight dummy
+// triplet with no bend limit, which never projects. Stores that must not
+// land are sent to a per-group sink slot in LDS.
+//
+// Cross-lane sums use DPP with bound_ctrl set: every lane of these patterns
+// reads an active lane of its own row, so the flag never supplies a value,
+// and it lets the compiler fold the lane move into the add that uses it.
+__device__ __forceinline__ float quad_lane(float x, int k) {
+    switch (k) {
+        case 0: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x00, 0xF, 0xF, true));
+        case 1: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x55, 0xF, 0xF, true));
+        default: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xAA, 0xF, 0xF, true));
+    }
+}
+__device__ __forceinline__ float quad_axis_sum(float x) {  // (x + y) + z over lanes 0, 1, 2 of the quad
+    return (quad_lane(x, 0) + quad_lane(x, 1)) + quad_lane(x, 2);
+}
+// lanes 0..3 of an 8-lane group read lanes 4..7 and lanes 4..7 read 0..3
+__device__ __forceinline__ float other_quad(float x, bool second) {
+    const int from_next = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x104, 0xF, 0xF, true);
+    const int from_prev = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, true);
+    return __int_as_float(second ? from_prev : from_next);
+}
+
+// Correctly rounded square root for x = 0, x = +inf or x >= 2^-96: the IEEE
+// expansion (the hardware root s, then s - 1 ulp or s + 1 ulp where an exact
+// fused residual shows the root is past it) without the scaling that only
+// inputs below 2^-96 take, so for these inputs it is sqrtf bit for bit. The
+// curvature sweep's inputs meet it: 1 - c^2 for f32 c in [-1, 1] is 0 or at
+// least 2^-24, acos's z is 0 or at least 2^-25 where its root is used, and a
+// squared segment length below 2^-96 belongs to a segment far shorter than
+// the 1e-7 below which the projection is not taken, whatever the root.
+__device__ __forceinline__ float sqrt_rn_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float down = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float r = __builtin_fmaf(-down, s, x) <= 0.f ? down : s;
+    const float q = __builtin_fmaf(-up, s, x) > 0.f ? up : r;
+    return __builtin_amdgcn_class(x, 0x260) ? x : q;  // +-0 and +inf are their own roots
+}
+struct SqrtRnNormal {
+    __device__ float operator()(float z) const { return sqrt_rn_normal(z); }
+};
+// f32::clamp for non-NaN x: when lo <= hi the median of (x, lo, hi); when
+// lo > hi clamp's second test takes hi whatever x, which the median of
+// (x, hi, hi) also gives
+__device__ __forceinline__ float clamp_med3(float x, float lo, float hi) {
+    return __builtin_amdgcn_fmed3f(x, fminf(lo, hi), hi);
+}
+
+template <int L>
+__global__ __launch_bounds__(64) void k_curvature_gs_lean(DevState *st, const SpecState *ss, const GridParams *grids,
+                                                          const float4 *in, float4 *pos, const float4 *refpos,
+                                                          const float4 *vA, const float4 *vB, const uint4 *fspan,
+                                                          float4 *wall, uint32_t F, uint32_t sweeps, float stiffness,
+                                                          float margin, uint32_t track_walls,
+                                                          uint32_t max_vertices) {
+    static_assert(L == 8 || L == 16 || L == 32, "group width");
+    constexpr uint32_t G = 64 / L;
+    extern __shared__ float4 lds4[];
+    const uint32_t lane = threadIdx.x, group = lane / L, gl = lane % L, gbase = group * L;
+    const uint32_t nv = (max_vertices + 1u) & ~1u;
+    float4 *P = (float4 *)((float *)lds4 + group * curvature_group_stride_floats(max_vertices, track_walls));
+    float4 *WR = P + nv;                                     // wall reactions (when tracked)
+    float2 *KW = (float2 *)(track_walls ? WR + nv : WR);     // (kmax, 1 - pinned)
+    float *Pf = (float *)P;
+    float *W = (float *)WR;
+    float *sink = (float *)(KW + nv);                        // absorbs stores that must not land
+    const uint32_t fiber = blockIdx.x * G + group;
+    uint4 span = make_uint4(0u, 0u, 0u, 0u);
+    if (fiber < F) span = fspan[fiber];
+    // a specimen that stopped, or an inactive fiber, keeps its positions
+    if (!running(ss, span.z) || (span.y && (__float_as_uint(vB[span.x].z) & kInactive))) span.y = 0u;
+    const uint32_t start = span.x, n = span.y;
+    const GridParams &g = grids[span.z];
+    const float half_skin = 0.5f * g.skin;
+    for (uint32_t i = gl; i < n; i += L) {
+        const float4 p = in[start + i];
+        const float4 a = vA[start + i];
+        P[i] = make_float4(p.x, p.y, p.z, a.w);
+        KW[i] = make_float2(a.x, (__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f);
+        if (track_walls) WR[i] = wall[start + i];
+    }
+    // Lane roles, repeated in every 8 lanes of the group: lanes 0..2 hold the
+    // axes of the triplet's first segment (previous -> center), lanes 4..6 those
+    // of its second (center -> next); lanes 3 and 7 (and the lanes past 8 of a
+    // wider group) compute the same values as another lane and store nothing.
+    const bool second = (gl & 4u) != 0u;
+    const uint32_t k = (gl & 3u) < 3u ? (gl & 3u) : 0u;
+    const uint32_t o = second ? 1u : 0u;
+    const bool writer = gl < 8u && (gl & 3u) < 3u;
+    const bool walled = !g.periodic[k];
+    const float lo = g.lower[k], hi = g.upper[k];
+    __syncthreads();
+    const uint32_t trip = n >= 3 ? n - 2 : 0;  // triplet t has center t + 1
+    const uint64_t gmask = (1ull << (L % 64)) - 1ull;
+    bool done = trip == 0 || sweeps == 0;
+    bool any = false;
+    uint32_t sweep = 0, dir = 0, cursor = 0;
+    while (true) {
+        // the end of a sweep direction: finish, or turn around
+        const bool end = !done && cursor >= trip;
+        const bool finish = end && (!any || (dir == 1 && sweep + 1 >= sweeps));
+        const bool turn = end && !finish;
+        done = done || finish;
+        sweep = (turn && dir == 1u) ? sweep + 1u : sweep;
+        dir = turn ? 1u - dir : dir;
+        cursor = turn ? 0u : cursor;
+        any = any && !turn;
+        if (__ballot(!done) == 0) break;
+        // the operands of the block's first position, read with the screen:
+        // after a projection the next candidate is most often that one
+        const uint32_t tf = done ? 0u : (dir == 0 ? cursor : trip - 1 - cursor);
+        float a0 = Pf[4 * (tf + o) + k], a1 = Pf[4 * (tf + o + 1) + k];
+        float e0 = P[tf + o].w, e1 = P[tf + o + 1].w;
+        float2 kwp = KW[tf], kwc = KW[tf + 1], kwn = KW[tf + 2];
+        // screen the next L positions of every group still sweeping
+        const uint32_t order = cursor + gl;
+        const bool valid = !done && order < trip;
+        const uint32_t ts = valid ? (dir == 0 ? order : trip - 1 - order) : 0u;
+        bool maybe;
+        {
+            const float4 a = P[ts], b = P[ts + 1], c = P[ts + 2];
+            const float kmax = KW[ts + 1].x * (1.0f - margin);
+            const float ux = b.x - a.x, uy = b.y - a.y, uz = b.z - a.z;
+            const float vx = c.x - b.x, vy = c.y - b.y, vz = c.z - b.z;
+            const float uu = ux * ux + uy * uy + uz * uz, vv = vx * vx + vy * vy + vz * vz;
+            const float uv = ux * vx + uy * vy + uz * vz;
+            const float ru = __builtin_amdgcn_rsqf(uu), rv = __builtin_amdgcn_rsqf(vv);
+            const float cos_est = uv * ru * rv;
+            const float half_est = 0.25f * kmax * (uu * ru + vv * rv);
+            const float min_cos_est = 1.0f - 2.0f * half_est * half_est;
+            maybe = valid & (kmax > 0.f) & !(cos_est - min_cos_est > 1.0e-5f);
+        }
+        const uint64_t mine = ((uint64_t)__ballot(maybe) >> gbase) & gmask;
+        const bool found = mine != 0;
+        const uint32_t j = found ? (uint32_t)__builtin_ctzll(mine) : 0u;
+        cursor = (!done && !found) ? cursor + L : cursor;
+        if (__ballot(found) == 0) continue;
+        // exact evaluation of each group's first candidate
+        const uint32_t ord = cursor + j;
+        const uint32_t t = found ? (dir == 0 ? ord : trip - 1 - ord) : 0u;
+        if (__ballot(found && j != 0u) != 0) {
+            if (found && j != 0u) {
+                a0 = Pf[4 * (t + o) + k];
+                a1 = Pf[4 * (t + o + 1) + k];
+                e0 = P[t + o].w;
+                e1 = P[t + o + 1].w;
+                kwp = KW[t];
+                kwc = KW[t + 1];
+                kwn = KW[t + 2];
+            }
+        }
+        if (!found) {  // a straight unit segment pair along x with no limit: never projects
+            a0 = 0.f;
+            a1 = k == 0u ? 1.f : 0.f;
+            kwc.x = 0.f;
+        }
+        // the sequential projection's operations, in its order
+        // The sequential projection's operations, in its order. Values the
+        // formula pairs (l1 and l2, un and vn, ...) sit on the two quads; where
+        // the formula combines them symmetrically (l1 + l2, un . vn) each quad
+        // uses its own value first, which commutes exactly.
+        const float kmax = kwc.x * (1.0f - margin);
+        const float d = a1 - a0;
+        const float len = sqrt_rn_normal(quad_axis_sum(d * d));
+        const float dn = d / len;
+        const float dn_o = other_quad(dn, second), len_o = other_quad(len, second);
+        const float cosine = __builtin_amdgcn_fmed3f(quad_axis_sum(dn * dn_o), -1.f, 1.f);  // finite when used
+        const float sine_half = 0.25f * kmax * (len + len_o);
+        const float min_cos = 1.0f - 2.0f * sine_half * sine_half;
+        float violation = min_cos - cosine;
+        bool ok = kmax > 0.f && len > 1.0e-7f && len_o > 1.0e-7f && sine_half < 1.0f && violation > 0.f;
+        if (__ballot(ok) != 0) {
+            const float sine = sqrt_rn_normal(fmaxf(1.0f - cosine * cosine, 0.f));
+            // the first quad takes the turning angle, the second the limit;
+            // turn - limit is the negated difference on the second quad
+            const float angle = acosf_f32_with(second ? min_cos : cosine, SqrtRnNormal{});
+            const float diff = angle - other_quad(angle, second);
+            if (sine > 1.0e-3f) violation = (second ? -diff : diff) * sine;
+            ok = ok && violation > 0.f;
+            // (vn - cos un) / l1 in the first quad (gu), (un - cos vn) / l2 in
+            // the second (gv); gu - gv is exact on the first quad, which alone
+            // moves the center, and only its square is used on the second
+            const float gq = (dn_o - cosine * dn) / len;
+            const float gq_o = other_quad(gq, second);
+            const float gc = gq - gq_o;
+            const float fwp = kwp.y, fwc = kwc.y, fwn = kwn.y;
+            const float sg = quad_axis_sum(gq * gq), sg_o = other_quad(sg, second);
+            const float suu = second ? sg_o : sg, svv = second ? sg : sg_o;
+            const float den = fwp * suu + fwc * quad_axis_sum(gc * gc) + fwn * svv;
+            ok = ok && den > 1.0e-12f;
+            const float scale = stiffness * violation / den;
+            // first quad: previous (- gu) and center (+ gc); second quad: next (+ gv)
+            const float f0 = fwp, f1 = second ? fwn : fwc;
+            const float m0 = a0 - fwp * scale * gq;
+            const float m1 = a1 + f1 * scale * (second ? gq : gc);
+            const float v0 = walled ? clamp_med3(m0, lo + e0, hi - e0) : m0;
+            const float v1 = walled ? clamp_med3(m1, lo + e1, hi - e1) : m1;
+            const bool s0 = writer && ok && !second && f0 > 0.f, s1 = writer && ok && f1 > 0.f;
+            const uint32_t i0 = t + o, i1 = t + o + 1;
+            if (track_walls) {
+                float *w0 = (s0 && walled) ? &W[4 * i0 + k] : &sink[k];
+                float *w1 = (s1 && walled) ? &W[4 * i1 + k] : &sink[k];
+                *w0 = *w0 + (m0 - v0);
+                *w1 = *w1 + (m1 - v1);
+            }
+            *(s0 ? &Pf[4 * i0 + k] : &sink[k]) = v0;
+            *(s1 ? &Pf[4 * i1 + k] : &sink[k]) = v1;
+        }
+        const bool hit = ((uint64_t)__ballot(ok && gl == 0) >> gbase) & 1ull;
+        cursor = found ? ord + 1u : cursor;
+        any = any || hit;
         // one wave: its LDS accesses complete in order; keep the compiler's
         lds_order();
     }
@@ -2146,9 +2306,9 @@ hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t 
         p.curvature_margin, w.track_walls, w.max_fiber_vertices
         switch (w.gs_lanes) {
             case 4: hipLaunchKernelGGL(k_curvature_gs_group<4>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
-            case 8: hipLaunchKernelGGL(k_curvature_gs_group<8>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
-            case 16: hipLaunchKernelGGL(k_curvature_gs_group<16>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
-            case 32: hipLaunchKernelGGL(k_curvature_gs_group<32>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
+            case 8: hipLaunchKernelGGL(k_curvature_gs_lean<8>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
+            case 16: hipLaunchKernelGGL(k_curvature_gs_lean<16>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
+            case 32: hipLaunchKernelGGL(k_curvature_gs_lean<32>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
             case 64: hipLaunchKernelGGL(k_curvature_gs_wave64, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
             default: return hipErrorInvalidValue;
         }
