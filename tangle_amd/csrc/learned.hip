@@ -129,18 +129,21 @@ __device__ __forceinline__ void closest_offset(const GridParams &g, float4 P1, f
     dist = sqrtf(ox * ox + oy * oy + oz * oz);
 }
 
-__global__ __launch_bounds__(256) void k_features(const DevState *st, const GridParams *grids, const float4 *pos,
-                                                  const uint4 *sstat,
-                                                  const uint32_t *cell_segs, const uint32_t *ncount,
+// Lists hold cell-list slots; every segment's geometry is read from the
+// compact slot records (first vertex and radius, second vertex and segment)
+// and the specimen from slotC, which the refresh before this kernel writes
+// from `pos`.
+__global__ __launch_bounds__(256) void k_features(const DevState *st, const GridParams *grids, const float4 *cP,
+                                                  const float4 *cQ, const uint4 *slotC, const uint32_t *ncount,
                                                   const uint32_t *nbr, const uint32_t *ovf, const float4 *history,
                                                   uint16_t *X, float *scale, uint32_t block, uint32_t ovf_cap,
                                                   uint32_t S) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= S || slot >= st->slots) return;  // inactive segments keep zero rows
-    const uint32_t seg = cell_segs[slot];
-    const uint4 me = sstat[seg];
+    const uint4 me = slotC[slot];
+    const uint32_t seg = me.x;
     const GridParams &g = grids[me.w];
-    const float4 P1 = pos[me.x], Q1 = pos[me.x + 1];
+    const float4 P1 = cP[slot], Q1 = cQ[slot];
     const float r1 = __uint_as_float(me.z);
     const uint32_t count = ncount[slot];
     const uint32_t *list = count <= block ? nbr + (size_t)slot * block
@@ -155,10 +158,10 @@ __global__ __launch_bounds__(256) void k_features(const DevState *st, const Grid
     }
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t other = list[k];
-        const uint4 ot = sstat[other];
         float ox, oy, oz, dist;
-        closest_offset(g, P1, Q1, pos[ot.x], pos[ot.x + 1], ox, oy, oz, dist);
-        float gk = dist - (r1 + __uint_as_float(ot.z));
+        const float4 P2 = cP[other];
+        closest_offset(g, P1, Q1, P2, cQ[other], ox, oy, oz, dist);
+        float gk = dist - (r1 + P2.w);
         uint32_t wk = other;
         // keep the 16 smallest gaps in ascending order (unrolled insertion)
 #pragma unroll
@@ -177,10 +180,10 @@ __global__ __launch_bounds__(256) void k_features(const DevState *st, const Grid
     for (uint32_t k = 0; k < kFeatureNeighbors; ++k) {
         float fx = 0.f, fy = 0.f, fz = 0.f;
         if (who[k] != 0xffffffffu) {
-            const uint4 ot = sstat[who[k]];
             float ox, oy, oz, dist;
-            closest_offset(g, P1, Q1, pos[ot.x], pos[ot.x + 1], ox, oy, oz, dist);
-            const float contact = r1 + __uint_as_float(ot.z);
+            const float4 P2 = cP[who[k]];
+            closest_offset(g, P1, Q1, P2, cQ[who[k]], ox, oy, oz, dist);
+            const float contact = r1 + P2.w;
             const float d = fmaxf(dist, 1.0e-3f * contact);
             const float w = fminf(contact / d, 8.0f) / d;
             fx = ox * w;
@@ -644,7 +647,9 @@ hipError_t launch_features(const WorldDev &w, const float4 *history, uint16_t *X
     if (e != hipSuccess) return e;
     if (scale) e = hipMemsetAsync(scale, 0, (size_t)w.S * sizeof(float), s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_features, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.st, w.grids, w.pos, w.sstat, w.cell_segs,
+    e = launch_refresh_slots(w, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_features, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.st, w.grids, w.cP, w.cQ, w.slotC,
                        w.ncount, w.nbr, w.ovf, history, X, scale, w.block, w.ovf_cap, w.S);
     return hipGetLastError();
 }

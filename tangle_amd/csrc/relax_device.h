@@ -110,8 +110,16 @@ struct WorldDev {
     uint32_t *flayer;   // per fiber: formation layer (0xffffffff: none)
     uint32_t *factive;  // per fiber: 1 active, 0 not
     uint32_t *cell_counts, *cell_offsets, *cell_cursor, *cell_segs, *seg_cell, *scattered;
+    // slot records in cell-list order, refreshed from `pos` every iteration:
+    // slotA/slotB = the segment's two vertex records (x, y, z, rest arc
+    // length), slotC = (segment, fiber, radius bits, specimen)
     float4 *slotA, *slotB;
     uint4 *slotC;
+    uint32_t *slot_of;  // per segment: its cell-list slot
+    // the contact pass's compact slot records, refreshed every iteration:
+    // cP = (first vertex x, y, z, radius), cQ = (second vertex x, y, z,
+    // segment bits)
+    float4 *cP, *cQ;
     uint32_t *ncount, *nbr, *ovf;
     float4 *corr0, *corr1;
     float *ratio;  // per vertex: curvature ratio at the last measure (0 until measured)
@@ -135,6 +143,9 @@ struct IterParams {
 };
 
 // Dynamic LDS of one curvature-sweep workgroup (64 / lanes fibers).
+// The slot records' positions from `pos` (before any reader of the records
+// that does not follow a rebuild).
+hipError_t launch_refresh_slots(const WorldDev &w, hipStream_t stream);
 size_t curvature_lds_bytes(uint32_t max_vertices, uint32_t lanes, uint32_t track_walls);
 
 // Enqueue work on `stream`; every launcher checks its grid against the

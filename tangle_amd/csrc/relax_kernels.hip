@@ -253,12 +253,15 @@ __global__ void k_scatter_segments(const DevState *st, const uint32_t *seg_cell,
 
 // One wave per cell: every lane ranks its scattered entries against the whole
 // cell, which puts the cell in ascending segment order (the order
-// rank_cell_segments produces, cell_list.rs:344-378), and copies the slot
-// geometry the list build reads (gather_cell_slot_geometry, neighbor.rs:99-153).
+// rank_cell_segments produces, cell_list.rs:344-378), and writes the slot
+// records the list build and the contact pass read (gather_cell_slot_geometry,
+// neighbor.rs:99-153): the two vertex records, then (segment, fiber, radius,
+// specimen); and each segment's slot.
 __global__ __launch_bounds__(256) void k_rank_cells(const DevState *st, const uint32_t *cell_counts,
                                                     const uint32_t *cell_offsets, const uint32_t *scattered,
                                                     uint32_t *cell_segs, const float4 *pos, const uint4 *sstat,
-                                                    float4 *slotA, float4 *slotB, uint4 *slotC, uint32_t C) {
+                                                    float4 *slotA, float4 *slotB, uint4 *slotC, uint32_t *slot_of,
+                                                    float4 *cP, float4 *cQ, uint32_t C) {
     if (st->nstate[0] == 0) return;
     uint32_t cell = blockIdx.x * 4 + (threadIdx.x >> 6);
     uint32_t lane = threadIdx.x & 63;
@@ -270,29 +273,49 @@ __global__ __launch_bounds__(256) void k_rank_cells(const DevState *st, const ui
         for (uint32_t k = 0; k < n; ++k) rank += scattered[off + k] < mine ? 1u : 0u;
         uint32_t slot = off + rank;
         cell_segs[slot] = mine;
+        slot_of[mine] = slot;
         uint4 ss = sstat[mine];
-        float4 p = pos[ss.x], q = pos[ss.x + 1];
-        slotA[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(ss.z));
-        slotB[slot] = make_float4(q.x, q.y, q.z, p.w);
-        slotC[slot] = make_uint4(mine, ss.y, __float_as_uint(q.w), ss.w);
+        const float4 p = pos[ss.x], q = pos[ss.x + 1];
+        slotA[slot] = p;
+        slotB[slot] = q;
+        slotC[slot] = make_uint4(mine, ss.y, ss.z, ss.w);
+        cP[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(ss.z));
+        cQ[slot] = make_float4(q.x, q.y, q.z, __uint_as_float(mine));
     }
+}
+
+// The contact pass's compact slot records from the current vertex records
+// (the first vertex of segment s of fiber f is s + f: fibers are
+// contiguous), every iteration before the contact pass, so that a candidate
+// is one 32-byte gather of records stored in cell order.
+__global__ void k_refresh_slots(const DevState *st, const float4 *pos, const uint4 *slotC, float4 *cP, float4 *cQ,
+                                uint32_t S) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= S || slot >= st->slots) return;
+    const uint4 sc = slotC[slot];
+    const uint32_t v = sc.x + sc.y;
+    const float4 p = pos[v], q = pos[v + 1];
+    cP[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(sc.z));
+    cQ[slot] = make_float4(q.x, q.y, q.z, __uint_as_float(sc.x));
 }
 
 // One lane per cell-list slot builds that slot's segment list from the 27
 // stencil cells (build_segment_neighbor_lists, neighbor.rs:155-364). Lists are
-// rows of `nbr` (row `slot`, `block` entries) kept in ascending segment order,
-// which replaces the separate sort (neighbor.rs:366-408). Candidates first
-// land in the slot's row of `ovf` (`ovf_cap` entries) in stencil-then-cell
-// order: exactly the order in which the reference's overflow scan visits
-// them (contact.rs:348-428), restricted to the pairs within the skin, the
-// only ones that can touch before the next rebuild. A listed segment's row is
-// then copied to `nbr` and insertion-sorted there; an overflowed segment
-// keeps its `ovf` row. The rows stay in global memory (L2): staging them in
-// LDS held the kernel to about 2.5 waves per SIMD and was 26 % slower on
-// 7.2 M batched segments.
+// rows of `nbr` (row `slot`, `block` entries) holding the neighbours' slots in
+// ascending segment order, which replaces the separate sort (neighbor.rs:
+// 366-408) and keeps the reference's summation order. Candidates first land
+// in the slot's row of `ovf` (`ovf_cap` slots) in stencil-then-cell order:
+// exactly the order in which the reference's overflow scan visits them
+// (contact.rs:348-428), restricted to the pairs within the skin, the only ones
+// that can touch before the next rebuild. A listed segment's row is then
+// copied to `nbr` as segments, insertion-sorted there and mapped back to
+// slots; an overflowed segment keeps its `ovf` row. The rows stay in global
+// memory (L2): staging them in LDS held the kernel to about 2.5 waves per
+// SIMD and was 26 % slower on 7.2 M batched segments.
 __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *st, const GridParams *grids,
                                                                const float4 *slotA, const float4 *slotB,
-                                                               const uint4 *slotC, const uint32_t *seg_cell,
+                                                               const uint4 *slotC, const uint32_t *slot_of,
+                                                               const uint32_t *seg_cell,
                                                                const uint32_t *cell_counts,
                                                                const uint32_t *cell_offsets, uint32_t *ncount,
                                                                uint32_t *nbr, uint32_t *ovf,
@@ -306,8 +329,8 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
     const GridParams g = grids[M.w];
     const float skin = g.skin;
     const Image im = image_of(g);
-    const float arc_p = B.w, arc_q = __uint_as_float(M.z);
-    const float p1x = A.x, p1y = A.y, p1z = A.z, radius = A.w;
+    const float arc_p = A.w, arc_q = B.w;
+    const float p1x = A.x, p1y = A.y, p1z = A.z, radius = __uint_as_float(M.z);
     const float q1x = B.x, q1y = B.y, q1z = B.z;
     const float d1x = q1x - p1x, d1y = q1y - p1y, d1z = q1z - p1z;
     const float half_length = 0.5f * sqrtf(d1x * d1x + d1y * d1y + d1z * d1z);
@@ -324,8 +347,9 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
             uint4 OM = slotC[o];
             uint32_t other = OM.x;
             float4 OA = slotA[o], OB = slotB[o];
-            float gap = fmaxf(OB.w - arc_q, arc_p - __uint_as_float(OM.z));
-            bool near = OM.y == owner && gap < radius + OA.w;
+            const float other_radius = __uint_as_float(OM.z);
+            float gap = fmaxf(OA.w - arc_q, arc_p - OB.w);
+            bool near = OM.y == owner && gap < radius + other_radius;
             if (other == seg || near) continue;
             float p2x = OA.x, p2y = OA.y, p2z = OA.z, q2x = OB.x, q2y = OB.y, q2z = OB.z;
             if (im.periodic[0]) {
@@ -344,22 +368,23 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
                 q2z -= sh;
             }
             float d2x = q2x - p2x, d2y = q2y - p2y, d2z = q2z - p2z;
-            float inter = (radius + OA.w + skin) * (1.0f + 1.0e-4f);
+            float inter = (radius + other_radius + skin) * (1.0f + 1.0e-4f);
             float mdx = 0.5f * (p2x + q2x - p1x - q1x);
             float mdy = 0.5f * (p2y + q2y - p1y - q1y);
             float mdz = 0.5f * (p2z + q2z - p1z - q1z);
             float reach = half_length + 0.5f * sqrtf(d2x * d2x + d2y * d2y + d2z * d2z) + inter;
             if (!(mdx * mdx + mdy * mdy + mdz * mdz <= reach * reach)) continue;
             if (!(axis_distance(p1x, p1y, p1z, d1x, d1y, d1z, p2x, p2y, p2z, d2x, d2y, d2z) <= inter)) continue;
-            if (count < ovf_cap) row[count] = other;
+            if (count < ovf_cap) row[count] = o;
             ++count;
         }
     }
     ncount[slot] = count;
     if (count <= block) {
-        // listed: copy the row, then insertion sort into ascending order
+        // listed: copy the row as segments, insertion sort them into
+        // ascending order, then store their slots
         uint32_t *out = nbr + (size_t)slot * block;
-        for (uint32_t i = 0; i < count; ++i) out[i] = row[i];
+        for (uint32_t i = 0; i < count; ++i) out[i] = slotC[row[i]].x;
         for (uint32_t i = 1; i < count; ++i) {
             uint32_t key = out[i];
             uint32_t at = i;
@@ -369,6 +394,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *s
             }
             out[at] = key;
         }
+        for (uint32_t i = 0; i < count; ++i) out[i] = slot_of[out[i]];
     }
 }
 
@@ -390,16 +416,13 @@ struct Term {
     float aw;
 };
 
-__device__ __forceinline__ bool contact_term(const Image &g, const SelfSeg &me, uint32_t other,
-                                             const float4 *pos, const uint4 *sstat, float cf, uint32_t aggregation,
+// The term of one candidate, from its endpoints (.xyz of P2 and Q2), radius
+// and segment: the capsule closest points, penetration and correction of
+// find_segment_corrections (contact.rs:453-891), in its operation order.
+__device__ __forceinline__ bool contact_term(const Image &g, const SelfSeg &me, const float4 &P2, const float4 &Q2,
+                                             float other_radius, uint32_t other, float cf, uint32_t aggregation,
                                              Term &t) {
     const float eps_len = 1.0e-20f, eps_par = 1.0e-6f;
-    uint4 ot = sstat[other];
-    float4 P2 = pos[ot.x], Q2 = pos[ot.x + 1];
-    float other_radius = __uint_as_float(ot.z);
-    float gap = fmaxf(P2.w - me.q.w, me.p.w - Q2.w);
-    bool near = ot.y == me.owner && gap < me.radius + other_radius;
-    if (other == me.seg || near) return false;
     const float p1x = me.p.x, p1y = me.p.y, p1z = me.p.z, q1x = me.q.x, q1y = me.q.y, q1z = me.q.z;
     const float d1x = me.d1x, d1y = me.d1y, d1z = me.d1z;
     float p2x = P2.x, p2y = P2.y, p2z = P2.z, q2x = Q2.x, q2y = Q2.y, q2z = Q2.z;
@@ -598,15 +621,21 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // stored stencil-order row; a list longer than that row scans the 27 cells
 // sequentially in lane 0 of its group. Thread 0 of the grid first closes a
 // pending rebuild (finish_neighbor_list_rebuild, neighbor.rs:424-432).
-// The kernel waits on dependent gathers (list entry, segment record, two
-// positions), so it is held to 80 VGPRs for six waves per SIMD instead of
-// five: a few spills on the cold overflow paths, 7-11 % less time per
-// launch (7 waves needs 14 spills and is slower again).
+// Lists hold slots, and a listed candidate's endpoints, radius and segment
+// are one 32-byte compact record in cell order, so a candidate costs one
+// gather (list entry, then its record) and neighbouring groups share cache
+// lines. The list build already left out the segment itself and its
+// along-fiber neighbours (with the same static arc lengths), so listed
+// candidates skip that test; the 27-cell scan applies it. Each lane issues
+// the list entries and records of up to kContactRounds rounds before it
+// evaluates any, to keep more gathers in flight; the terms are still added
+// in list order.
+constexpr uint32_t kContactRounds = 3;
 template <int G>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_contact(DevState *st, const SpecState *ss_all, const GridParams *grids,
-                                                 uint32_t red_slots, const float4 *pos, const uint4 *sstat,
-                                                 const uint4 *slotC,
-                                                 const uint32_t *cell_segs, const uint32_t *seg_cell,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_contact(DevState *st, const SpecState *ss_all, const GridParams *grids,
+                                                 uint32_t red_slots, const float4 *cP, const float4 *cQ,
+                                                 const float4 *slotA, const float4 *slotB,
+                                                 const uint4 *slotC, const uint32_t *seg_cell,
                                                  const uint32_t *cell_counts, const uint32_t *cell_offsets,
                                                  const uint32_t *ncount, const uint32_t *nbr, const uint32_t *ovf,
                                                  float4 *corr0, float4 *corr1, uint32_t *red, float cf,
@@ -620,77 +649,106 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     const uint32_t lane = threadIdx.x % G, group = threadIdx.x / G;
     const uint32_t wave_lane = threadIdx.x & 63u, group_base = wave_lane - lane;
     const uint32_t slot = xcd_block(blockIdx.x, gridDim.x) * groups + group;
-    bool active = slot < S && slot < st->slots;
-    SelfSeg me;
-    uint32_t count = 0, spec = 0;
-    const uint32_t *list = nullptr;
-    bool slow = false;
-    uint4 ss = make_uint4(0u, 0u, 0u, 0u);
-    if (active) {
-        // the slot record carries the segment and its specimen, so the
-        // segment record and the specimen's state load side by side
-        const uint4 sc = slotC[slot];
-        me.seg = sc.x;
-        spec = sc.w;
-        ss = sstat[me.seg];
-        active = running(ss_all, spec);
+    // The kernel is bound by chains of dependent gathers, so every load that
+    // depends only on the slot is issued at once: the slot's records, its
+    // list count and the first entries of its listed row (entries past the
+    // count are never used as addresses).
+    const uint32_t sl = slot < S ? slot : 0u;
+    const uint32_t slots = st->slots;
+    const uint4 sc = slotC[sl];
+    const float4 P1 = cP[sl], Q1 = cQ[sl];
+    const uint32_t listed_count = ncount[sl];
+    const uint32_t *row = nbr + (size_t)sl * block;
+    uint32_t first[kContactRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kContactRounds; ++r) {
+        const uint32_t k = r * G + lane;
+        first[r] = k < block ? row[k] : 0u;
     }
+    const bool in_list = slot < S && slot < slots;
+    const uint32_t spec = in_list ? sc.w : 0u;
+    const bool active = in_list && running(ss_all, spec);
     const GridParams &g = grids[spec];
     const Image im = image_of(g);
-    if (active) {
-        me.p = pos[ss.x];
-        me.q = pos[ss.x + 1];
-        me.owner = ss.y;
-        me.radius = __uint_as_float(ss.z);
-        me.d1x = me.q.x - me.p.x;
-        me.d1y = me.q.y - me.p.y;
-        me.d1z = me.q.z - me.p.z;
-        me.half_length = 0.5f * sqrtf(me.d1x * me.d1x + me.d1y * me.d1y + me.d1z * me.d1z);
-        count = ncount[slot];
-        if (count <= block) {
-            list = nbr + (size_t)slot * block;
-        } else if (count <= ovf_cap) {
-            list = ovf + (size_t)slot * ovf_cap;
-        } else {
-            slow = true;
-        }
-    }
+    SelfSeg me;
+    me.seg = sc.x;
+    me.owner = sc.y;
+    me.radius = __uint_as_float(sc.z);
+    me.p = P1;  // .w: radius, unused here
+    me.q = Q1;
+    me.d1x = me.q.x - me.p.x;
+    me.d1y = me.q.y - me.p.y;
+    me.d1z = me.q.z - me.p.z;
+    me.half_length = 0.5f * sqrtf(me.d1x * me.d1x + me.d1y * me.d1y + me.d1z * me.d1z);
+    const uint32_t count = active ? listed_count : 0u;
+    const bool listed = count <= block;
+    const bool slow = count > ovf_cap;
+    const uint32_t *list = listed ? row : ovf + (size_t)sl * ovf_cap;
     Accum acc;
     if (active && !slow) {
-        for (uint32_t base = 0; base < count; base += G) {
-            const uint32_t k = base + lane;
-            Term t;
-            bool hit = k < count && contact_term(im, me, list[k], pos, sstat, cf, aggregation, t);
-            uint64_t mask = (uint64_t)__ballot(hit);
-            uint32_t mine = (uint32_t)((mask >> group_base) & (G == 64 ? ~0ull : ((1ull << G) - 1ull)));
-            while (mine) {
-                const int j = __builtin_ctz(mine);
-                mine &= mine - 1u;
-                const int src = (int)group_base + j;
-                Term u;
-                u.pen = __shfl(t.pen, src, 64);
-                u.t0x = __shfl(t.t0x, src, 64);
-                u.t0y = __shfl(t.t0y, src, 64);
-                u.t0z = __shfl(t.t0z, src, 64);
-                u.t1x = __shfl(t.t1x, src, 64);
-                u.t1y = __shfl(t.t1y, src, 64);
-                u.t1z = __shfl(t.t1z, src, 64);
-                u.aw = __shfl(t.aw, src, 64);
-                acc.add(u, aggregation);
+        for (uint32_t base = 0; base < count; base += G * kContactRounds) {
+            uint32_t o[kContactRounds];
+            float4 P2[kContactRounds], Q2[kContactRounds];
+            if (base == 0 && listed) {
+#pragma unroll
+                for (uint32_t r = 0; r < kContactRounds; ++r) o[r] = r * G + lane < count ? first[r] : sl;
+            } else {
+#pragma unroll
+                for (uint32_t r = 0; r < kContactRounds; ++r) {
+                    const uint32_t k = base + r * G + lane;
+                    o[r] = k < count ? list[k] : sl;
+                }
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kContactRounds; ++r) {
+                P2[r] = cP[o[r]];
+                Q2[r] = cQ[o[r]];
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kContactRounds; ++r) {
+                const uint32_t k = base + r * G + lane;
+                Term t;
+                bool hit = k < count && contact_term(im, me, P2[r], Q2[r], P2[r].w, __float_as_uint(Q2[r].w), cf,
+                                                     aggregation, t);
+                uint64_t mask = (uint64_t)__ballot(hit);
+                uint32_t mine = (uint32_t)((mask >> group_base) & (G == 64 ? ~0ull : ((1ull << G) - 1ull)));
+                while (mine) {
+                    const int j = __builtin_ctz(mine);
+                    mine &= mine - 1u;
+                    const int src = (int)group_base + j;
+                    Term u;
+                    u.pen = __shfl(t.pen, src, 64);
+                    u.t0x = __shfl(t.t0x, src, 64);
+                    u.t0y = __shfl(t.t0y, src, 64);
+                    u.t0z = __shfl(t.t0z, src, 64);
+                    u.t1x = __shfl(t.t1x, src, 64);
+                    u.t1y = __shfl(t.t1y, src, 64);
+                    u.t1z = __shfl(t.t1z, src, 64);
+                    u.aw = __shfl(t.aw, src, 64);
+                    acc.add(u, aggregation);
+                }
             }
         }
     } else if (active && slow && lane == 0) {
         // the reference's overflow scan: 27 stencil cells of the home cell at
-        // the list-build positions, each cell in ascending segment order
+        // the list-build positions, each cell in ascending segment order,
+        // leaving out the segment itself and its along-fiber neighbours
         const uint32_t home = seg_cell[me.seg] - g.cell_base;
+        const float arc_p = slotA[slot].w, arc_q = slotB[slot].w;
         for (uint32_t n = 0; n < 27; ++n) {
             uint32_t cell;
             if (!stencil_cell(g, home, n, cell)) continue;
             cell += g.cell_base;
             const uint32_t cs = cell_offsets[cell], cn = cell_counts[cell];
             for (uint32_t k = 0; k < cn; ++k) {
+                const uint32_t o = cs + k;
+                const uint4 ot = slotC[o];
+                const float other_radius = __uint_as_float(ot.z);
+                const float gap = fmaxf(slotA[o].w - arc_q, arc_p - slotB[o].w);
+                const bool near = ot.y == me.owner && gap < me.radius + other_radius;
+                if (ot.x == me.seg || near) continue;
                 Term t;
-                if (contact_term(im, me, cell_segs[cs + k], pos, sstat, cf, aggregation, t)) acc.add(t, aggregation);
+                if (contact_term(im, me, cP[o], cQ[o], other_radius, ot.x, cf, aggregation, t)) acc.add(t, aggregation);
             }
         }
     }
@@ -2224,10 +2282,11 @@ hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
     hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.seg_cell,
                        w.cell_offsets, w.cell_cursor, w.scattered, w.S);
     hipLaunchKernelGGL(k_rank_cells, dim3(blocks_for(w.C, 4)), dim3(256), 0, stream, w.st, w.cell_counts,
-                       w.cell_offsets, w.scattered, w.cell_segs, w.pos, w.sstat, w.slotA, w.slotB, w.slotC, w.C);
+                       w.cell_offsets, w.scattered, w.cell_segs, w.pos, w.sstat, w.slotA, w.slotB, w.slotC,
+                       w.slot_of, w.cP, w.cQ, w.C);
     hipLaunchKernelGGL(k_build_lists, dim3(blocks_for(w.S, kBuildThreads)), dim3(kBuildThreads), 0, stream, w.st,
-                       w.grids, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets, w.ncount,
-                       w.nbr, w.ovf, w.block, w.ovf_cap, w.S);
+                       w.grids, w.slotA, w.slotB, w.slotC, w.slot_of, w.seg_cell, w.cell_counts, w.cell_offsets,
+                       w.ncount, w.nbr, w.ovf, w.block, w.ovf_cap, w.S);
     return hipGetLastError();
 }
 
@@ -2246,15 +2305,23 @@ hipError_t launch_refresh_lists(const WorldDev &w, hipStream_t stream) {
     return hipGetLastError();
 }
 
+hipError_t launch_refresh_slots(const WorldDev &w, hipStream_t stream) {
+    hipLaunchKernelGGL(k_refresh_slots, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.pos, w.slotC, w.cP,
+                       w.cQ, w.S);
+    return hipGetLastError();
+}
+
 hipError_t launch_request_rebuild(const WorldDev &w, hipStream_t stream) {
     hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(64), 0, stream, w.st);
     return hipGetLastError();
 }
 
 hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t stream) {
-#define TGH_CONTACT_ARGS                                                                                           \
-    w.st, w.ss, w.grids, w.red_slots, w.pos, w.sstat, w.slotC, w.cell_segs, w.seg_cell, w.cell_counts, w.cell_offsets, w.ncount, \
-        w.nbr, w.ovf, w.corr0, w.corr1, w.red, p.correction_fraction, p.aggregation, w.block, w.ovf_cap, w.S
+    hipError_t e = launch_refresh_slots(w, stream);
+    if (e != hipSuccess) return e;
+#define TGH_CONTACT_ARGS                                                                                     \
+    w.st, w.ss, w.grids, w.red_slots, w.cP, w.cQ, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets, \
+        w.ncount, w.nbr, w.ovf, w.corr0, w.corr1, w.red, p.correction_fraction, p.aggregation, w.block, w.ovf_cap, w.S
     const dim3 grid(blocks_for(w.S, 256u / w.contact_lanes));
     switch (w.contact_lanes) {
         case 4: hipLaunchKernelGGL(k_contact<4>, grid, dim3(256), 0, stream, TGH_CONTACT_ARGS); break;
@@ -2381,7 +2448,8 @@ hipError_t launch_capture(const WorldDev &w, const GridParams *capture_grid, uin
     hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.seg_cell,
                        c.cell_offsets, c.cell_cursor, c.scattered, c.S);
     hipLaunchKernelGGL(k_rank_cells, dim3(blocks_for(c.C, 4)), dim3(256), 0, stream, c.st, c.cell_counts,
-                       c.cell_offsets, c.scattered, c.cell_segs, c.pos, c.sstat, c.slotA, c.slotB, c.slotC, c.C);
+                       c.cell_offsets, c.scattered, c.cell_segs, c.pos, c.sstat, c.slotA, c.slotB, c.slotC,
+                       c.slot_of, c.cP, c.cQ, c.C);
     hipLaunchKernelGGL(k_capture, dim3(blocks_for((size_t)c.S * kCaptureLanes, 256)), dim3(256), 0, stream,
                        capture_grid, c.pos, c.sstat,
                        c.factive, c.cell_counts, c.cell_offsets, c.cell_segs, c.S, max_gap, capacity, counter, pairs,

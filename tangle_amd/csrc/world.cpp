@@ -560,6 +560,9 @@ int create_world_packed(std::vector<Packed> &&packed, const tgh_config *c, tgh_w
     if (e == hipSuccess) e = w->alloc(&d.slotA, S);
     if (e == hipSuccess) e = w->alloc(&d.slotB, S);
     if (e == hipSuccess) e = w->alloc(&d.slotC, S);
+    if (e == hipSuccess) e = w->alloc(&d.slot_of, S);
+    if (e == hipSuccess) e = w->alloc(&d.cP, S);
+    if (e == hipSuccess) e = w->alloc(&d.cQ, S);
     if (e == hipSuccess) e = w->alloc(&d.ncount, S);
     if (e == hipSuccess) e = w->alloc(&d.nbr, std::max<size_t>(1, (size_t)d.block * S));
     if (e == hipSuccess) e = w->alloc(&d.corr0, S);
@@ -879,8 +882,14 @@ int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t 
     for (size_t slot = 0; slot < listed_slots; ++slot) {
         const uint32_t seg = slot_seg[slot];
         counts[seg] = slot_count[slot];
-        const size_t keep = std::min<size_t>(slot_count[slot], B);
-        for (size_t k = 0; k < keep; ++k) lists[(size_t)seg * B + k] = cols[slot * B + k];
+        // rows hold cell-list slots; the boundary returns segments. An
+        // overflowed list's row is never written (unspecified), so only
+        // listed rows are mapped.
+        if (slot_count[slot] > B) continue;
+        for (size_t k = 0; k < slot_count[slot]; ++k) {
+            const uint32_t other = cols[slot * B + k];
+            lists[(size_t)seg * B + k] = other < S ? slot_seg[other] : 0xffffffffu;
+        }
     }
     return TGH_OK;
 }
