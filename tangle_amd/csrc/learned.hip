@@ -133,14 +133,14 @@ __device__ __forceinline__ void closest_offset(const GridParams &g, float4 P1, f
 // compact slot records (first vertex and radius, second vertex and segment)
 // and the specimen from slotC, which the refresh before this kernel writes
 // from `pos`.
-__global__ __launch_bounds__(256) void k_features(const DevState *st, const GridParams *grids, const float4 *cP,
-                                                  const float4 *cQ, const uint4 *slotC, const uint32_t *ncount,
-                                                  const uint32_t *nbr, const uint32_t *ovf, const float4 *history,
-                                                  uint16_t *X, float *scale, uint32_t block, uint32_t ovf_cap,
-                                                  uint32_t S) {
+__global__ __launch_bounds__(256) void k_features(const GridParams *grids, const float4 *cP, const float4 *cQ,
+                                                  const uint4 *slotC, const uint32_t *ncount, const uint32_t *nbr,
+                                                  const uint32_t *ovf, const float4 *history, uint16_t *X,
+                                                  float *scale, uint32_t block, uint32_t ovf_cap, uint32_t S) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= S || slot >= st->slots) return;  // inactive segments keep zero rows
+    if (slot >= S) return;
     const uint4 me = slotC[slot];
+    if (me.x == kNoSegment) return;  // inactive segments keep zero rows
     const uint32_t seg = me.x;
     const GridParams &g = grids[me.w];
     const float4 P1 = cP[slot], Q1 = cQ[slot];
@@ -649,7 +649,7 @@ hipError_t launch_features(const WorldDev &w, const float4 *history, uint16_t *X
     if (e != hipSuccess) return e;
     e = launch_refresh_slots(w, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_features, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.st, w.grids, w.cP, w.cQ, w.slotC,
+    hipLaunchKernelGGL(k_features, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.grids, w.cP, w.cQ, w.slotC,
                        w.ncount, w.nbr, w.ovf, history, X, scale, w.block, w.ovf_cap, w.S);
     return hipGetLastError();
 }

@@ -501,7 +501,7 @@ int tgh_world_revert_proposal_specimens(tgh_world *world, const uint32_t *revert
     HIP_CHECK(hipMalloc((void **)&flags, count * sizeof(uint32_t)));
     hipError_t e = hipMemcpyAsync(flags, revert, count * sizeof(uint32_t), hipMemcpyHostToDevice, world->stream);
     if (e == hipSuccess) e = tgh::launch_restore_specimens(world->d, world->proposal_backup, flags, world->stream);
-    if (e == hipSuccess) e = tgh::launch_request_rebuild(world->d, world->stream);
+    if (e == hipSuccess) e = tgh::launch_request_rebuild_specimens(world->d, flags, world->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(world->stream);
     (void)hipFree(flags);
     if (e != hipSuccess) return tgh::fail(TGH_ERR_DEVICE, std::string("revert: ") + hipGetErrorString(e));

@@ -33,7 +33,9 @@ constexpr uint32_t kFlagMask = (1u << kSpecShift) - 1u;
 // One specimen's uniform grid: widths are (upper - lower) / cells computed
 // once in f32, exactly as the reference's proxy_cell computes them per call.
 // The specimen's cells occupy [cell_base, cell_base + cells product) of the
-// world's cell list; `skin` is its Verlet skin.
+// world's cell list and its segments [seg_base, seg_base + seg_count) of the
+// world's segments, which are also the cell-list slots its active segments
+// fill from seg_base on; `skin` is its Verlet skin.
 // A change of the cell: old and new bounds and the periodic axes.
 struct CellChange {
     float old_lower[3], old_upper[3], new_lower[3], new_upper[3];
@@ -49,26 +51,29 @@ struct GridParams {
     uint32_t periodic[3];
     uint32_t cell_base;
     float skin;
-    uint32_t pad[2];
+    uint32_t seg_base, seg_count;
 };
 
 // Per-specimen control block. `control` follows the reference's buffer
 // (world.rs:466-467): {running, lifetime iterations, batch iterations,
 // terminal failure}; `metrics` = {max penetration, max vertex step, max
-// curvature ratio} measured at the specimen's last assessment.
+// curvature ratio} measured at the specimen's last assessment; `rebuild` =
+// the specimen's lists are stale (its own neighbor_state[0], neighbor.rs:1-9:
+// each specimen rebuilds when its own vertices have moved, as it would alone).
 struct SpecState {
     uint32_t control[4];
     float metrics[3];
-    uint32_t pad;
+    uint32_t rebuild;
 };
 
-// World-wide state: nstate = {rebuild pending, rebuild count} (neighbor.rs:1-9);
-// slots = the segments in the cell list (the active ones) after a rebuild.
+// World-wide state: nstate = {some specimen's rebuild is pending, passes that
+// rebuilt lists} (neighbor.rs:1-9). A cell-list slot that holds no segment
+// (past a specimen's active segments) has slotC.x = cell_segs = kNoSegment.
 struct DevState {
     uint32_t nstate[2];
-    uint32_t slots;
-    uint32_t pad[5];
+    uint32_t pad[6];
 };
+constexpr uint32_t kNoSegment = 0xffffffffu;
 
 // Installed formation targets: layer targets (per layer coordinate along
 // `layer_axis`, fibers of layers first..last held) and vertex targets
@@ -101,6 +106,7 @@ struct WorldDev {
     uint32_t max_fiber_vertices;
     uint32_t gs_lanes;       // lanes per fiber in the curvature sweep (0, 4, 8, 16, 32 or 64)
     uint32_t contact_lanes;  // lanes per segment in the contact pass (4, 8, 16 or 32)
+    uint32_t build_lanes;    // lanes per slot in the neighbour-list build (4, 8, 16 or 32)
     DevState *st;
     uint32_t *red;  // [nspec][3][red_slots] u32 bit patterns of nonnegative floats
     float4 *pos, *alt, *refpos, *wall, *vA, *vB;
@@ -110,12 +116,12 @@ struct WorldDev {
     uint32_t *flayer;   // per fiber: formation layer (0xffffffff: none)
     uint32_t *factive;  // per fiber: 1 active, 0 not
     uint32_t *cell_counts, *cell_offsets, *cell_cursor, *cell_segs, *seg_cell, *scattered;
+    uint32_t *cell_spec;  // per cell: its specimen
     // slot records in cell-list order, refreshed from `pos` every iteration:
     // slotA/slotB = the segment's two vertex records (x, y, z, rest arc
     // length), slotC = (segment, fiber, radius bits, specimen)
     float4 *slotA, *slotB;
     uint4 *slotC;
-    uint32_t *slot_of;  // per segment: its cell-list slot
     // the contact pass's compact slot records, refreshed every iteration:
     // cP = (first vertex x, y, z, radius), cQ = (second vertex x, y, z,
     // segment bits)
@@ -156,8 +162,10 @@ hipError_t launch_rebuild_now(const WorldDev &w, hipStream_t stream);
 // Finishes a pending rebuild (if any) now, from the current positions: the
 // same lists, snapshot and rebuild count the next pass would have produced.
 hipError_t launch_refresh_lists(const WorldDev &w, hipStream_t stream);
-// Marks the lists stale so the next pass rebuilds them.
+// Marks the lists stale so the next pass rebuilds them: every specimen's,
+// or those of the specimens whose entry of `flags` ([nspec], device) is set.
 hipError_t launch_request_rebuild(const WorldDev &w, hipStream_t stream);
+hipError_t launch_request_rebuild_specimens(const WorldDev &w, const uint32_t *flags, hipStream_t stream);
 // One relaxation iteration is these five groups in order (world.rs:1038-1501).
 hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t stream);
 hipError_t launch_measure(const WorldDev &w, const IterParams &p, hipStream_t stream);

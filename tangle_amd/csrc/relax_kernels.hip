@@ -178,28 +178,35 @@ __global__ void k_begin_batch(SpecState *ss, uint32_t nspec) {
 __device__ __forceinline__ bool running(const SpecState *ss, uint32_t spec) { return ss[spec].control[0] != 0u; }
 
 // ---------------------------------------------------------------------------
-// neighbour-list rebuild, every kernel gated on the device-side request flag
-// (world/neighbor_list.rs:19-62)
+// neighbour-list rebuild (world/neighbor_list.rs:19-62). Every kernel returns
+// at once unless some specimen's rebuild is pending, and then works only on
+// the cells, segments, slots and vertices of the specimens whose own request
+// flag is set: each specimen rebuilds exactly when it would alone.
 
-__global__ void k_rebuild_clear(const DevState *st, uint32_t *cell_counts, uint32_t *cell_cursor, uint32_t C,
-                                const float4 *pos, float4 *refpos, uint32_t V) {
+__device__ __forceinline__ bool rebuilding(const SpecState *ss, uint32_t spec) { return ss[spec].rebuild != 0u; }
+
+__global__ void k_rebuild_clear(const DevState *st, const SpecState *ss, const uint32_t *cell_spec,
+                                uint32_t *cell_counts, uint32_t *cell_cursor, uint32_t C, const float4 *pos,
+                                float4 *refpos, const float4 *vB, uint32_t V) {
     if (st->nstate[0] == 0) return;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < C) {
+    if (i < C && rebuilding(ss, cell_spec[i])) {
         cell_counts[i] = 0;
         cell_cursor[i] = 0;
     }
     // The list snapshot (neighbor.rs:410-422) is taken from the positions the
     // lists are built from; nothing moves them before the build finishes.
-    if (i < V) refpos[i] = pos[i];
+    if (i < V && rebuilding(ss, __float_as_uint(vB[i].z) >> kSpecShift)) refpos[i] = pos[i];
 }
 
-__global__ void k_bin_segments(const DevState *st, const GridParams *grids, const float4 *pos, const uint4 *sstat,
-                               const uint32_t *factive, uint32_t *seg_cell, uint32_t *cell_counts, uint32_t S) {
+__global__ void k_bin_segments(const DevState *st, const SpecState *ss, const GridParams *grids, const float4 *pos,
+                               const uint4 *sstat, const uint32_t *factive, uint32_t *seg_cell, uint32_t *cell_counts,
+                               uint32_t S) {
     if (st->nstate[0] == 0) return;
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     const uint4 me = sstat[s];
+    if (!rebuilding(ss, me.w)) return;
     if (factive[me.y] == 0u) {  // inactive fibers are not in the cell list
         seg_cell[s] = 0xffffffffu;
         return;
@@ -211,16 +218,26 @@ __global__ void k_bin_segments(const DevState *st, const GridParams *grids, cons
     atomicAdd(&cell_counts[cell], 1u);
 }
 
-// Exclusive scan of the cell counts in one 1024-thread workgroup: each thread
-// sums a contiguous chunk, the 1024 chunk sums are scanned in LDS, then each
-// thread writes its chunk's offsets.
-__global__ __launch_bounds__(1024) void k_scan_cells(DevState *st, const uint32_t *counts,
-                                                     uint32_t *offsets, uint32_t C) {
+// Exclusive scan of one specimen's cell counts per 1024-thread workgroup
+// (one workgroup per specimen): each thread sums a contiguous chunk, the 1024
+// chunk sums are scanned in LDS, then each thread writes its chunk's offsets,
+// which start at the specimen's first segment. The slots past its active
+// segments are marked empty.
+__global__ __launch_bounds__(1024) void k_scan_cells(const DevState *st, const SpecState *ss,
+                                                     const GridParams *grids, const uint32_t *counts,
+                                                     uint32_t *offsets, uint32_t *cell_segs, uint32_t *scattered,
+                                                     uint4 *slotC) {
     if (st->nstate[0] == 0) return;
+    const uint32_t sp = blockIdx.x;
+    if (!rebuilding(ss, sp)) return;
+    const GridParams &g = grids[sp];
+    const uint32_t C = g.cells[0] * g.cells[1] * g.cells[2];
+    counts += g.cell_base;
+    offsets += g.cell_base;
     __shared__ uint32_t sums[1024];
     uint32_t t = threadIdx.x;
     uint32_t chunk = (C + 1023) / 1024;
-    uint32_t begin = t * chunk;
+    uint32_t begin = min(t * chunk, C);
     uint32_t end = min(begin + chunk, C);
     uint32_t local = 0;
     for (uint32_t i = begin; i < end; ++i) local += counts[i];
@@ -232,169 +249,266 @@ __global__ __launch_bounds__(1024) void k_scan_cells(DevState *st, const uint32_
         sums[t] += add;
         __syncthreads();
     }
-    uint32_t run = sums[t] - local;
+    uint32_t run = g.seg_base + sums[t] - local;
     for (uint32_t i = begin; i < end; ++i) {
         offsets[i] = run;
         run += counts[i];
     }
-    if (t == 1023) st->slots = sums[1023];  // segments in the cell list
+    const uint32_t filled = sums[1023];  // the specimen's active segments
+    for (uint32_t i = g.seg_base + filled + t; i < g.seg_base + g.seg_count; i += 1024) {
+        cell_segs[i] = kNoSegment;
+        scattered[i] = kNoSegment;
+        slotC[i] = make_uint4(kNoSegment, kNoSegment, 0u, sp);
+    }
 }
 
-__global__ void k_scatter_segments(const DevState *st, const uint32_t *seg_cell, const uint32_t *cell_offsets,
-                                   uint32_t *cell_cursor, uint32_t *scattered, uint32_t S) {
+__global__ void k_scatter_segments(const DevState *st, const SpecState *ss, const uint32_t *cell_spec,
+                                   const uint32_t *seg_cell, const uint32_t *cell_offsets, uint32_t *cell_cursor,
+                                   uint32_t *scattered, uint32_t S) {
     if (st->nstate[0] == 0) return;
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     uint32_t cell = seg_cell[s];
     if (cell == 0xffffffffu) return;  // inactive
+    if (!rebuilding(ss, cell_spec[cell])) return;
     uint32_t slot = cell_offsets[cell] + atomicAdd(&cell_cursor[cell], 1u);
     scattered[slot] = s;
 }
 
-// One wave per cell: every lane ranks its scattered entries against the whole
-// cell, which puts the cell in ascending segment order (the order
-// rank_cell_segments produces, cell_list.rs:344-378), and writes the slot
-// records the list build and the contact pass read (gather_cell_slot_geometry,
-// neighbor.rs:99-153): the two vertex records, then (segment, fiber, radius,
-// specimen); and each segment's slot.
-__global__ __launch_bounds__(256) void k_rank_cells(const DevState *st, const uint32_t *cell_counts,
+// Whether any of specimens lo..hi (those of a block's first and last
+// element) is rebuilding: blocks of other specimens return at once.
+__device__ __forceinline__ bool any_rebuilding(const SpecState *ss, uint32_t lo, uint32_t hi) {
+    for (uint32_t s = lo; s <= hi; ++s)
+        if (ss[s].rebuild != 0u) return true;
+    return false;
+}
+
+// One thread per cell-list slot: the segment scattered into the slot is
+// ranked against the other entries of its cell, which puts every cell in
+// ascending segment order (the order rank_cell_segments produces,
+// cell_list.rs:344-378), and the slot records the list build and the contact
+// pass read are written (gather_cell_slot_geometry, neighbor.rs:99-153): the
+// two vertex records, (segment, fiber, radius, specimen) and the compact
+// records. Slot i belongs to the specimen of
+// segment i (a specimen's slots are its segment range).
+__global__ __launch_bounds__(256) void k_rank_slots(const DevState *st, const SpecState *ss, const uint4 *sstat,
+                                                    const uint32_t *seg_cell, const uint32_t *cell_counts,
                                                     const uint32_t *cell_offsets, const uint32_t *scattered,
-                                                    uint32_t *cell_segs, const float4 *pos, const uint4 *sstat,
-                                                    float4 *slotA, float4 *slotB, uint4 *slotC, uint32_t *slot_of,
-                                                    float4 *cP, float4 *cQ, uint32_t C) {
+                                                    uint32_t *cell_segs, const float4 *pos, float4 *slotA,
+                                                    float4 *slotB, uint4 *slotC, float4 *cP,
+                                                    float4 *cQ, uint32_t S) {
     if (st->nstate[0] == 0) return;
-    uint32_t cell = blockIdx.x * 4 + (threadIdx.x >> 6);
-    uint32_t lane = threadIdx.x & 63;
-    if (cell >= C) return;
-    uint32_t n = cell_counts[cell], off = cell_offsets[cell];
-    for (uint32_t i = lane; i < n; i += 64) {
-        uint32_t mine = scattered[off + i];
-        uint32_t rank = 0;
-        for (uint32_t k = 0; k < n; ++k) rank += scattered[off + k] < mine ? 1u : 0u;
-        uint32_t slot = off + rank;
-        cell_segs[slot] = mine;
-        slot_of[mine] = slot;
-        uint4 ss = sstat[mine];
-        const float4 p = pos[ss.x], q = pos[ss.x + 1];
-        slotA[slot] = p;
-        slotB[slot] = q;
-        slotC[slot] = make_uint4(mine, ss.y, ss.z, ss.w);
-        cP[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(ss.z));
-        cQ[slot] = make_float4(q.x, q.y, q.z, __uint_as_float(mine));
-    }
+    const uint32_t first = blockIdx.x * 256u;
+    if (first >= S || !any_rebuilding(ss, sstat[first].w, sstat[min(first + 255u, S - 1u)].w)) return;
+    const uint32_t i = first + threadIdx.x;
+    if (i >= S || !rebuilding(ss, sstat[i].w)) return;
+    const uint32_t mine = scattered[i];
+    if (mine == kNoSegment) return;  // past the specimen's active segments
+    const uint32_t cell = seg_cell[mine];
+    const uint32_t n = cell_counts[cell], off = cell_offsets[cell];
+    uint32_t rank = 0;
+    for (uint32_t k = 0; k < n; ++k) rank += scattered[off + k] < mine ? 1u : 0u;
+    const uint32_t slot = off + rank;
+    cell_segs[slot] = mine;
+    const uint4 rec = sstat[mine];
+    const float4 p = pos[rec.x], q = pos[rec.x + 1];
+    slotA[slot] = p;
+    slotB[slot] = q;
+    slotC[slot] = make_uint4(mine, rec.y, rec.z, rec.w);
+    cP[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(rec.z));
+    cQ[slot] = make_float4(q.x, q.y, q.z, __uint_as_float(mine));
 }
 
 // The contact pass's compact slot records from the current vertex records
 // (the first vertex of segment s of fiber f is s + f: fibers are
 // contiguous), every iteration before the contact pass, so that a candidate
 // is one 32-byte gather of records stored in cell order.
-__global__ void k_refresh_slots(const DevState *st, const float4 *pos, const uint4 *slotC, float4 *cP, float4 *cQ,
-                                uint32_t S) {
+__global__ void k_refresh_slots(const float4 *pos, const uint4 *slotC, float4 *cP, float4 *cQ, uint32_t S) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= S || slot >= st->slots) return;
+    if (slot >= S) return;
     const uint4 sc = slotC[slot];
+    if (sc.x == kNoSegment) return;
     const uint32_t v = sc.x + sc.y;
     const float4 p = pos[v], q = pos[v + 1];
     cP[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(sc.z));
     cQ[slot] = make_float4(q.x, q.y, q.z, __uint_as_float(sc.x));
 }
 
-// One lane per cell-list slot builds that slot's segment list from the 27
-// stencil cells (build_segment_neighbor_lists, neighbor.rs:155-364). Lists are
-// rows of `nbr` (row `slot`, `block` entries) holding the neighbours' slots in
-// ascending segment order, which replaces the separate sort (neighbor.rs:
-// 366-408) and keeps the reference's summation order. Candidates first land
-// in the slot's row of `ovf` (`ovf_cap` slots) in stencil-then-cell order:
-// exactly the order in which the reference's overflow scan visits them
-// (contact.rs:348-428), restricted to the pairs within the skin, the only ones
-// that can touch before the next rebuild. A listed segment's row is then
-// copied to `nbr` as segments, insertion-sorted there and mapped back to
-// slots; an overflowed segment keeps its `ovf` row. The rows stay in global
-// memory (L2): staging them in LDS held the kernel to about 2.5 waves per
-// SIMD and was 26 % slower on 7.2 M batched segments.
-__global__ __launch_bounds__(kBuildThreads) void k_build_lists(const DevState *st, const GridParams *grids,
-                                                               const float4 *slotA, const float4 *slotB,
-                                                               const uint4 *slotC, const uint32_t *slot_of,
-                                                               const uint32_t *seg_cell,
-                                                               const uint32_t *cell_counts,
-                                                               const uint32_t *cell_offsets, uint32_t *ncount,
-                                                               uint32_t *nbr, uint32_t *ovf,
-                                                               uint32_t block, uint32_t ovf_cap, uint32_t S) {
+// Orders LDS and global accesses of the lanes of one wave: what a lane wrote
+// before is seen by every lane of the wave after.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Entries per list whose segments a build group keeps in LDS for the sort.
+constexpr uint32_t kBuildSortEntries = 64;
+
+// G lanes per cell-list slot build that slot's segment list from the 27
+// stencil cells (build_segment_neighbor_lists, neighbor.rs:155-364). The
+// stencil's candidates form one sequence, cell by cell in stencil order and
+// in slot (ascending segment) order within a cell: exactly the order in
+// which the reference's overflow scan visits them (contact.rs:348-428). Each
+// round the group's lanes test G consecutive candidates, whose compact slot
+// records are adjacent in memory, and the accepted ones (the pairs within the
+// skin, the only ones that can touch before the next rebuild) are appended to
+// the slot's row of `ovf` (`ovf_cap` slots) in sequence order through a
+// ballot. A listed segment's row (at most `block` entries) is then written
+// to `nbr` as slots in ascending segment order, which replaces the separate
+// sort (neighbor.rs:366-408) and keeps the reference's summation order; an
+// overflowed segment keeps its `ovf` row.
+template <int G>
+__global__ __launch_bounds__(256) void k_build_lists(const DevState *st, const SpecState *ss, const GridParams *grids,
+                                                     const uint4 *sstat, const uint4 *fspan, const float4 *slotA,
+                                                     const float4 *slotB, const uint4 *slotC, const float4 *cP,
+                                                     const float4 *cQ, const uint32_t *seg_cell,
+                                                     const uint32_t *cell_counts, const uint32_t *cell_offsets,
+                                                     uint32_t *ncount, uint32_t *nbr, uint32_t *ovf, uint32_t block,
+                                                     uint32_t ovf_cap, uint32_t S) {
+    static_assert(G == 4 || G == 8 || G == 16 || G == 32, "lanes per slot");
+    constexpr uint32_t groups = 256 / G;
+    constexpr uint32_t per_lane = (27 + G - 1) / G;  // stencil cells per lane
+    __shared__ uint32_t cell_start[groups][28];      // candidate sequence start of each stencil cell
+    __shared__ uint32_t cell_slot[groups][27];       // first slot of each stencil cell
+    __shared__ uint32_t keys[groups][kBuildSortEntries];
     if (st->nstate[0] == 0) return;
-    uint32_t slot = blockIdx.x * kBuildThreads + threadIdx.x;
-    if (slot >= S || slot >= st->slots) return;
-    float4 A = slotA[slot], B = slotB[slot];
-    uint4 M = slotC[slot];
+    const uint32_t first = blockIdx.x * groups;
+    if (first >= S || !any_rebuilding(ss, sstat[first].w, sstat[min(first + groups - 1u, S - 1u)].w)) return;
+    const uint32_t lane = threadIdx.x % G, group = threadIdx.x / G;
+    const uint32_t group_base = (threadIdx.x & 63u) - lane;
+    const uint32_t slot = first + group;
+    const uint4 M = slotC[slot < S ? slot : first];
+    const bool live = slot < S && M.x != kNoSegment && rebuilding(ss, M.w);
     const uint32_t seg = M.x, owner = M.y;
-    const GridParams g = grids[M.w];
+    const GridParams &g = grids[live ? M.w : 0u];
     const float skin = g.skin;
     const Image im = image_of(g);
-    const float arc_p = A.w, arc_q = B.w;
-    const float p1x = A.x, p1y = A.y, p1z = A.z, radius = __uint_as_float(M.z);
-    const float q1x = B.x, q1y = B.y, q1z = B.z;
-    const float d1x = q1x - p1x, d1y = q1y - p1y, d1z = q1z - p1z;
-    const float half_length = 0.5f * sqrtf(d1x * d1x + d1y * d1y + d1z * d1z);
-    const uint32_t home = seg_cell[seg] - g.cell_base;
-    uint32_t *row = ovf + (size_t)slot * ovf_cap;  // encounter order
-    uint32_t count = 0;
-    for (uint32_t n = 0; n < 27; ++n) {
-        uint32_t cell;
-        if (!stencil_cell(g, home, n, cell)) continue;
-        cell += g.cell_base;
-        uint32_t cs = cell_offsets[cell], cn = cell_counts[cell];
-        for (uint32_t k = 0; k < cn; ++k) {
-            uint32_t o = cs + k;
-            uint4 OM = slotC[o];
-            uint32_t other = OM.x;
-            float4 OA = slotA[o], OB = slotB[o];
-            const float other_radius = __uint_as_float(OM.z);
-            float gap = fmaxf(OA.w - arc_q, arc_p - OB.w);
-            bool near = OM.y == owner && gap < radius + other_radius;
-            if (other == seg || near) continue;
-            float p2x = OA.x, p2y = OA.y, p2z = OA.z, q2x = OB.x, q2y = OB.y, q2z = OB.z;
-            if (im.periodic[0]) {
-                float sh = roundf(0.5f * (p2x + q2x - p1x - q1x) / im.length[0]) * im.length[0];
-                p2x -= sh;
-                q2x -= sh;
+    uint32_t total = 0, count = 0;
+    float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
+    uint32_t first_seg = 0, nseg = 0;
+    if (live) {
+        A = slotA[slot];
+        B = slotB[slot];
+        const uint4 span = fspan[owner];
+        first_seg = span.x - owner;  // segment s of fiber f starts at vertex s + f
+        nseg = span.y - 1u;
+        // the lane's stencil cells, then their starts in the candidate sequence
+        const uint32_t home = seg_cell[seg] - g.cell_base;
+        uint32_t local = 0;
+        for (uint32_t j = 0; j < per_lane; ++j) {
+            const uint32_t n = lane * per_lane + j;
+            if (n >= 27) break;
+            uint32_t cell, cn = 0, cs = 0;
+            if (stencil_cell(g, home, n, cell)) {
+                cell += g.cell_base;
+                cn = cell_counts[cell];
+                cs = cell_offsets[cell];
             }
-            if (im.periodic[1]) {
-                float sh = roundf(0.5f * (p2y + q2y - p1y - q1y) / im.length[1]) * im.length[1];
-                p2y -= sh;
-                q2y -= sh;
-            }
-            if (im.periodic[2]) {
-                float sh = roundf(0.5f * (p2z + q2z - p1z - q1z) / im.length[2]) * im.length[2];
-                p2z -= sh;
-                q2z -= sh;
-            }
-            float d2x = q2x - p2x, d2y = q2y - p2y, d2z = q2z - p2z;
-            float inter = (radius + other_radius + skin) * (1.0f + 1.0e-4f);
-            float mdx = 0.5f * (p2x + q2x - p1x - q1x);
-            float mdy = 0.5f * (p2y + q2y - p1y - q1y);
-            float mdz = 0.5f * (p2z + q2z - p1z - q1z);
-            float reach = half_length + 0.5f * sqrtf(d2x * d2x + d2y * d2y + d2z * d2z) + inter;
-            if (!(mdx * mdx + mdy * mdy + mdz * mdz <= reach * reach)) continue;
-            if (!(axis_distance(p1x, p1y, p1z, d1x, d1y, d1z, p2x, p2y, p2z, d2x, d2y, d2z) <= inter)) continue;
-            if (count < ovf_cap) row[count] = o;
-            ++count;
+            cell_start[group][n] = local;  // chunk-relative for now
+            cell_slot[group][n] = cs;
+            local += cn;
         }
+        uint32_t incl = local;
+#pragma unroll
+        for (uint32_t d = 1; d < G; d <<= 1) {
+            const uint32_t up = (uint32_t)__shfl_up((int)incl, d, G);
+            if (lane >= d) incl += up;
+        }
+        const uint32_t excl = incl - local;
+        total = (uint32_t)__shfl((int)incl, G - 1, G);
+        for (uint32_t j = 0; j < per_lane; ++j) {
+            const uint32_t n = lane * per_lane + j;
+            if (n >= 27) break;
+            cell_start[group][n] += excl;
+        }
+        if (lane == 0) cell_start[group][27] = total;
     }
-    ncount[slot] = count;
-    if (count <= block) {
-        // listed: copy the row as segments, insertion sort them into
-        // ascending order, then store their slots
-        uint32_t *out = nbr + (size_t)slot * block;
-        for (uint32_t i = 0; i < count; ++i) out[i] = slotC[row[i]].x;
-        for (uint32_t i = 1; i < count; ++i) {
-            uint32_t key = out[i];
+    wave_sync();
+    uint32_t *row = ovf + (size_t)(live ? slot : 0u) * ovf_cap;  // encounter order
+    if (live) {
+        const float arc_p = A.w, arc_q = B.w;
+        const float p1x = A.x, p1y = A.y, p1z = A.z, radius = __uint_as_float(M.z);
+        const float q1x = B.x, q1y = B.y, q1z = B.z;
+        const float d1x = q1x - p1x, d1y = q1y - p1y, d1z = q1z - p1z;
+        const float half_length = 0.5f * sqrtf(d1x * d1x + d1y * d1y + d1z * d1z);
+        uint32_t n = 0;  // this lane's stencil cell
+        for (uint32_t base = 0; base < total; base += G) {
+            const uint32_t t = base + lane;
+            bool accept = false;
+            uint32_t o = 0, other = 0;
+            if (t < total) {
+                while (cell_start[group][n + 1] <= t) ++n;
+                o = cell_slot[group][n] + (t - cell_start[group][n]);
+                const float4 P2 = cP[o], Q2 = cQ[o];
+                other = __float_as_uint(Q2.w);
+                const float other_radius = P2.w;
+                bool skip = other == seg;
+                if (!skip && other - first_seg < nseg) {  // same fiber: along-fiber neighbours are skipped
+                    const float gap = fmaxf(slotA[o].w - arc_q, arc_p - slotB[o].w);
+                    skip = gap < radius + other_radius;
+                }
+                if (!skip) {
+                    float p2x = P2.x, p2y = P2.y, p2z = P2.z, q2x = Q2.x, q2y = Q2.y, q2z = Q2.z;
+                    if (im.periodic[0]) {
+                        float sh = roundf(0.5f * (p2x + q2x - p1x - q1x) / im.length[0]) * im.length[0];
+                        p2x -= sh;
+                        q2x -= sh;
+                    }
+                    if (im.periodic[1]) {
+                        float sh = roundf(0.5f * (p2y + q2y - p1y - q1y) / im.length[1]) * im.length[1];
+                        p2y -= sh;
+                        q2y -= sh;
+                    }
+                    if (im.periodic[2]) {
+                        float sh = roundf(0.5f * (p2z + q2z - p1z - q1z) / im.length[2]) * im.length[2];
+                        p2z -= sh;
+                        q2z -= sh;
+                    }
+                    float d2x = q2x - p2x, d2y = q2y - p2y, d2z = q2z - p2z;
+                    float inter = (radius + other_radius + skin) * (1.0f + 1.0e-4f);
+                    float mdx = 0.5f * (p2x + q2x - p1x - q1x);
+                    float mdy = 0.5f * (p2y + q2y - p1y - q1y);
+                    float mdz = 0.5f * (p2z + q2z - p1z - q1z);
+                    float reach = half_length + 0.5f * sqrtf(d2x * d2x + d2y * d2y + d2z * d2z) + inter;
+                    accept = mdx * mdx + mdy * mdy + mdz * mdz <= reach * reach &&
+                             axis_distance(p1x, p1y, p1z, d1x, d1y, d1z, p2x, p2y, p2z, d2x, d2y, d2z) <= inter;
+                }
+            }
+            const uint32_t mask = (uint32_t)(__ballot(accept) >> group_base) & (uint32_t)((1ull << G) - 1ull);
+            if (accept) {
+                const uint32_t at = count + __popc(mask & ((1u << lane) - 1u));
+                if (at < ovf_cap) row[at] = o;
+                if (at < kBuildSortEntries) keys[group][at] = other;
+            }
+            count += __popc(mask);
+        }
+        if (lane == 0) ncount[slot] = count;
+    }
+    // the row and the keys the group wrote, seen by all of its lanes
+    __syncthreads();
+    if (!live || count > block) return;
+    uint32_t *out = nbr + (size_t)slot * block;
+    if (count <= kBuildSortEntries) {
+        // rank sort: segments are distinct, so every entry has its own rank
+        for (uint32_t i = lane; i < count; i += G) {
+            const uint32_t key = keys[group][i];
+            uint32_t rank = 0;
+            for (uint32_t k = 0; k < count; ++k) rank += keys[group][k] < key ? 1u : 0u;
+            out[rank] = row[i];
+        }
+    } else if (lane == 0) {
+        // long lists (blocks wider than the LDS keys): insertion sort by segment
+        for (uint32_t i = 0; i < count; ++i) {
+            const uint32_t o = row[i];
+            const uint32_t key = __float_as_uint(cQ[o].w);
             uint32_t at = i;
-            while (at > 0 && out[at - 1] > key) {
+            while (at > 0 && __float_as_uint(cQ[out[at - 1]].w) > key) {
                 out[at] = out[at - 1];
                 --at;
             }
-            out[at] = key;
+            out[at] = o;
         }
-        for (uint32_t i = 0; i < count; ++i) out[i] = slot_of[out[i]];
     }
 }
 
@@ -632,7 +746,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // in list order.
 constexpr uint32_t kContactRounds = 3;
 template <int G>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_contact(DevState *st, const SpecState *ss_all, const GridParams *grids,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_contact(DevState *st, SpecState *ss_all, uint32_t nspec, const GridParams *grids,
                                                  uint32_t red_slots, const float4 *cP, const float4 *cQ,
                                                  const float4 *slotA, const float4 *slotB,
                                                  const uint4 *slotC, const uint32_t *seg_cell,
@@ -641,9 +755,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                                                  float4 *corr0, float4 *corr1, uint32_t *red, float cf,
                                                  uint32_t aggregation, uint32_t block, uint32_t ovf_cap,
                                                  uint32_t S) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && st->nstate[0] != 0) {
-        st->nstate[0] = 0;
-        st->nstate[1] += 1;
+    if (blockIdx.x == 0) {  // the rebuilds (if any) ran before this pass
+        const bool pending = st->nstate[0] != 0u;
+        __syncthreads();
+        if (pending) {
+            for (uint32_t s = threadIdx.x; s < nspec; s += 256) ss_all[s].rebuild = 0u;
+            if (threadIdx.x == 0) {
+                st->nstate[0] = 0;
+                st->nstate[1] += 1;
+            }
+        }
     }
     constexpr uint32_t groups = 256 / G;
     const uint32_t lane = threadIdx.x % G, group = threadIdx.x / G;
@@ -654,7 +775,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     // list count and the first entries of its listed row (entries past the
     // count are never used as addresses).
     const uint32_t sl = slot < S ? slot : 0u;
-    const uint32_t slots = st->slots;
     const uint4 sc = slotC[sl];
     const float4 P1 = cP[sl], Q1 = cQ[sl];
     const uint32_t listed_count = ncount[sl];
@@ -665,7 +785,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         const uint32_t k = r * G + lane;
         first[r] = k < block ? row[k] : 0u;
     }
-    const bool in_list = slot < S && slot < slots;
+    const bool in_list = slot < S && sc.x != kNoSegment;
     const uint32_t spec = in_list ? sc.w : 0u;
     const bool active = in_list && running(ss_all, spec);
     const GridParams &g = grids[spec];
@@ -1085,7 +1205,7 @@ __host__ __device__ inline size_t curvature_group_stride_floats(uint32_t max_ver
 }
 
 template <int L>
-__global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, const SpecState *ss, const GridParams *grids,
+__global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, SpecState *ss, const GridParams *grids,
                                                            const float4 *in, float4 *pos, const float4 *refpos,
                                                            const float4 *vA, const float4 *vB, const uint4 *fspan,
                                                            float4 *wall, uint32_t F, uint32_t sweeps, float stiffness,
@@ -1236,6 +1356,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, const S
         const float dx = q.x - r.x, dy = q.y - r.y, dz = q.z - r.z;
         if (dx * dx + dy * dy + dz * dz > limit) flag = true;
     }
+    if (flag) ss[span.z].rebuild = 1u;
     if (__ballot(flag) != 0 && lane == 0) st->nstate[0] = 1u;
 }
 
@@ -1300,7 +1421,7 @@ __device__ __forceinline__ float clamp_med3(float x, float lo, float hi) {
 }
 
 template <int L>
-__global__ __launch_bounds__(64) void k_curvature_gs_lean(DevState *st, const SpecState *ss, const GridParams *grids,
+__global__ __launch_bounds__(64) void k_curvature_gs_lean(DevState *st, SpecState *ss, const GridParams *grids,
                                                           const float4 *in, float4 *pos, const float4 *refpos,
                                                           const float4 *vA, const float4 *vB, const uint4 *fspan,
                                                           float4 *wall, uint32_t F, uint32_t sweeps, float stiffness,
@@ -1477,6 +1598,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_lean(DevState *st, const Sp
         const float dx = q.x - r.x, dy = q.y - r.y, dz = q.z - r.z;
         if (dx * dx + dy * dy + dz * dz > limit) flag = true;
     }
+    if (flag) ss[span.z].rebuild = 1u;
     if (__ballot(flag) != 0 && lane == 0) st->nstate[0] = 1u;
 }
 
@@ -1488,7 +1610,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_lean(DevState *st, const Sp
 // next position is neither changed by the last projection nor a screened
 // candidate. LDS accesses of one wave complete in order, so the screen that
 // follows reads the chain's writes without a barrier.
-__global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, const SpecState *ss,
+__global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, SpecState *ss,
                                                             const GridParams *grids, const float4 *in, float4 *pos,
                                                             const float4 *refpos, const float4 *vA, const float4 *vB,
                                                             const uint4 *fspan, float4 *wall, uint32_t F,
@@ -1670,12 +1792,13 @@ __global__ __launch_bounds__(64) void k_curvature_gs_wave64(DevState *st, const 
         const float dx = q.x - r.x, dy = q.y - r.y, dz = q.z - r.z;
         if (dx * dx + dy * dy + dz * dz > limit) flag = true;
     }
+    if (flag) ss[span.z].rebuild = 1u;
     if (__ballot(flag) != 0 && lane == 0) st->nstate[0] = 1u;
 }
 
 // Long fibers that do not fit in LDS: one lane per fiber walks the chain in
 // global memory with the same evaluate-then-apply projection.
-__global__ __launch_bounds__(64) void k_curvature_gs_lane(DevState *st, const SpecState *ss, const GridParams *grids,
+__global__ __launch_bounds__(64) void k_curvature_gs_lane(DevState *st, SpecState *ss, const GridParams *grids,
                                                           const float4 *in, float4 *pos, const float4 *refpos,
                                                           const float4 *vA, const float4 *vB, const uint4 *fspan,
                                                           float4 *wall, uint32_t F, uint32_t sweeps, float stiffness,
@@ -1725,19 +1848,35 @@ __global__ __launch_bounds__(64) void k_curvature_gs_lane(DevState *st, const Sp
         float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
         if (dx * dx + dy * dy + dz * dz > limit) flag = true;
     }
-    if (flag) st->nstate[0] = 1u;
+    if (flag) {
+        ss[span.z].rebuild = 1u;
+        st->nstate[0] = 1u;
+    }
 }
 
 // Rebuild on request: request_neighbor_list_rebuild (neighbor.rs:434-441),
 // the rebuild, and finish_neighbor_list_rebuild.
-__global__ void k_request_rebuild(DevState *st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) st->nstate[0] = 1u;
+// One workgroup: every specimen's lists are marked stale (or, with `only`,
+// those of the flagged specimens).
+__global__ void k_request_rebuild(DevState *st, SpecState *ss, const uint32_t *only, uint32_t nspec) {
+    bool any = false;
+    for (uint32_t s = threadIdx.x; s < nspec; s += blockDim.x) {
+        if (only && only[s] == 0u) continue;
+        ss[s].rebuild = 1u;
+        any = true;
+    }
+    if (__syncthreads_or(any) && threadIdx.x == 0) st->nstate[0] = 1u;
 }
 
-__global__ void k_finish_rebuild(DevState *st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || st->nstate[0] == 0) return;
-    st->nstate[0] = 0u;
-    st->nstate[1] += 1u;
+__global__ void k_finish_rebuild(DevState *st, SpecState *ss, uint32_t nspec) {
+    const bool pending = st->nstate[0] != 0u;
+    __syncthreads();
+    if (!pending) return;
+    for (uint32_t s = threadIdx.x; s < nspec; s += blockDim.x) ss[s].rebuild = 0u;
+    if (threadIdx.x == 0) {
+        st->nstate[0] = 0u;
+        st->nstate[1] += 1u;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2274,45 +2413,60 @@ hipError_t launch_begin_batch(const WorldDev &w, hipStream_t stream) {
 hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
     if (w.S == 0 || w.C == 0 || w.V == 0) return hipErrorInvalidValue;
     const uint32_t clear_n = w.C > w.V ? w.C : w.V;
-    hipLaunchKernelGGL(k_rebuild_clear, dim3(blocks_for(clear_n, 256)), dim3(256), 0, stream, w.st, w.cell_counts,
-                       w.cell_cursor, w.C, w.pos, w.refpos, w.V);
-    hipLaunchKernelGGL(k_bin_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.grids, w.pos,
+    hipLaunchKernelGGL(k_rebuild_clear, dim3(blocks_for(clear_n, 256)), dim3(256), 0, stream, w.st, w.ss, w.cell_spec,
+                       w.cell_counts, w.cell_cursor, w.C, w.pos, w.refpos, w.vB, w.V);
+    hipLaunchKernelGGL(k_bin_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.grids, w.pos,
                        w.sstat, w.factive, w.seg_cell, w.cell_counts, w.S);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, w.st, w.cell_counts, w.cell_offsets, w.C);
-    hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.seg_cell,
-                       w.cell_offsets, w.cell_cursor, w.scattered, w.S);
-    hipLaunchKernelGGL(k_rank_cells, dim3(blocks_for(w.C, 4)), dim3(256), 0, stream, w.st, w.cell_counts,
-                       w.cell_offsets, w.scattered, w.cell_segs, w.pos, w.sstat, w.slotA, w.slotB, w.slotC,
-                       w.slot_of, w.cP, w.cQ, w.C);
-    hipLaunchKernelGGL(k_build_lists, dim3(blocks_for(w.S, kBuildThreads)), dim3(kBuildThreads), 0, stream, w.st,
-                       w.grids, w.slotA, w.slotB, w.slotC, w.slot_of, w.seg_cell, w.cell_counts, w.cell_offsets,
-                       w.ncount, w.nbr, w.ovf, w.block, w.ovf_cap, w.S);
+    hipLaunchKernelGGL(k_scan_cells, dim3(w.nspec), dim3(1024), 0, stream, w.st, w.ss, w.grids, w.cell_counts,
+                       w.cell_offsets, w.cell_segs, w.scattered, w.slotC);
+    hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.cell_spec,
+                       w.seg_cell, w.cell_offsets, w.cell_cursor, w.scattered, w.S);
+    hipLaunchKernelGGL(k_rank_slots, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.sstat,
+                       w.seg_cell, w.cell_counts, w.cell_offsets, w.scattered, w.cell_segs, w.pos, w.slotA, w.slotB,
+                       w.slotC, w.cP, w.cQ, w.S);
+#define TGH_BUILD_ARGS                                                                                            \
+    w.st, w.ss, w.grids, w.sstat, w.fspan, w.slotA, w.slotB, w.slotC, w.cP, w.cQ, w.seg_cell, w.cell_counts,      \
+        w.cell_offsets, w.ncount, w.nbr, w.ovf, w.block, w.ovf_cap, w.S
+    switch (w.build_lanes) {
+        case 4: hipLaunchKernelGGL(k_build_lists<4>, dim3(blocks_for(w.S, 64)), dim3(256), 0, stream, TGH_BUILD_ARGS); break;
+        case 8: hipLaunchKernelGGL(k_build_lists<8>, dim3(blocks_for(w.S, 32)), dim3(256), 0, stream, TGH_BUILD_ARGS); break;
+        case 16: hipLaunchKernelGGL(k_build_lists<16>, dim3(blocks_for(w.S, 16)), dim3(256), 0, stream, TGH_BUILD_ARGS); break;
+        case 32: hipLaunchKernelGGL(k_build_lists<32>, dim3(blocks_for(w.S, 8)), dim3(256), 0, stream, TGH_BUILD_ARGS); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef TGH_BUILD_ARGS
     return hipGetLastError();
 }
 
 hipError_t launch_rebuild_now(const WorldDev &w, hipStream_t stream) {
-    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(64), 0, stream, w.st);
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(256), 0, stream, w.st, w.ss, nullptr, w.nspec);
     hipError_t e = launch_rebuild(w, stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_finish_rebuild, dim3(1), dim3(64), 0, stream, w.st);
+    hipLaunchKernelGGL(k_finish_rebuild, dim3(1), dim3(256), 0, stream, w.st, w.ss, w.nspec);
     return hipGetLastError();
 }
 
 hipError_t launch_refresh_lists(const WorldDev &w, hipStream_t stream) {
     hipError_t e = launch_rebuild(w, stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_finish_rebuild, dim3(1), dim3(64), 0, stream, w.st);
+    hipLaunchKernelGGL(k_finish_rebuild, dim3(1), dim3(256), 0, stream, w.st, w.ss, w.nspec);
     return hipGetLastError();
 }
 
 hipError_t launch_refresh_slots(const WorldDev &w, hipStream_t stream) {
-    hipLaunchKernelGGL(k_refresh_slots, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.pos, w.slotC, w.cP,
-                       w.cQ, w.S);
+    hipLaunchKernelGGL(k_refresh_slots, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.pos, w.slotC, w.cP, w.cQ,
+                       w.S);
     return hipGetLastError();
 }
 
 hipError_t launch_request_rebuild(const WorldDev &w, hipStream_t stream) {
-    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(64), 0, stream, w.st);
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(256), 0, stream, w.st, w.ss, nullptr, w.nspec);
+    return hipGetLastError();
+}
+
+hipError_t launch_request_rebuild_specimens(const WorldDev &w, const uint32_t *flags, hipStream_t stream) {
+    if (!flags) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(256), 0, stream, w.st, w.ss, flags, w.nspec);
     return hipGetLastError();
 }
 
@@ -2320,7 +2474,7 @@ hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t st
     hipError_t e = launch_refresh_slots(w, stream);
     if (e != hipSuccess) return e;
 #define TGH_CONTACT_ARGS                                                                                     \
-    w.st, w.ss, w.grids, w.red_slots, w.cP, w.cQ, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets, \
+    w.st, w.ss, w.nspec, w.grids, w.red_slots, w.cP, w.cQ, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets, \
         w.ncount, w.nbr, w.ovf, w.corr0, w.corr1, w.red, p.correction_fraction, p.aggregation, w.block, w.ovf_cap, w.S
     const dim3 grid(blocks_for(w.S, 256u / w.contact_lanes));
     switch (w.contact_lanes) {
@@ -2433,23 +2587,24 @@ hipError_t launch_compaction_metrics(const WorldDev &w, const GridParams *grid, 
 hipError_t launch_capture(const WorldDev &w, const GridParams *capture_grid, uint32_t capture_cells, float max_gap,
                           uint32_t capacity, uint32_t *counter, uint2 *pairs, float2 *coordinates, float *gaps,
                           float *angles, hipStream_t stream) {
-    if (w.S == 0 || capture_cells == 0) return hipErrorInvalidValue;
+    if (w.S == 0 || capture_cells == 0 || w.nspec != 1) return hipErrorInvalidValue;
     // the whole-segment cell list on the capture grid, in the shared buffers
     WorldDev c = w;
     c.grids = const_cast<GridParams *>(capture_grid);
     c.C = capture_cells;
-    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(64), 0, stream, c.st);
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(256), 0, stream, c.st, c.ss, nullptr, c.nspec);
     const uint32_t clear_n = c.C > c.V ? c.C : c.V;
-    hipLaunchKernelGGL(k_rebuild_clear, dim3(blocks_for(clear_n, 256)), dim3(256), 0, stream, c.st, c.cell_counts,
-                       c.cell_cursor, c.C, c.pos, c.refpos, c.V);
-    hipLaunchKernelGGL(k_bin_segments, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.grids, c.pos,
+    hipLaunchKernelGGL(k_rebuild_clear, dim3(blocks_for(clear_n, 256)), dim3(256), 0, stream, c.st, c.ss, c.cell_spec,
+                       c.cell_counts, c.cell_cursor, c.C, c.pos, c.refpos, c.vB, c.V);
+    hipLaunchKernelGGL(k_bin_segments, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.grids, c.pos,
                        c.sstat, c.factive, c.seg_cell, c.cell_counts, c.S);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, c.st, c.cell_counts, c.cell_offsets, c.C);
-    hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.seg_cell,
-                       c.cell_offsets, c.cell_cursor, c.scattered, c.S);
-    hipLaunchKernelGGL(k_rank_cells, dim3(blocks_for(c.C, 4)), dim3(256), 0, stream, c.st, c.cell_counts,
-                       c.cell_offsets, c.scattered, c.cell_segs, c.pos, c.sstat, c.slotA, c.slotB, c.slotC,
-                       c.slot_of, c.cP, c.cQ, c.C);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, c.st, c.ss, c.grids, c.cell_counts,
+                       c.cell_offsets, c.cell_segs, c.scattered, c.slotC);
+    hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.cell_spec,
+                       c.seg_cell, c.cell_offsets, c.cell_cursor, c.scattered, c.S);
+    hipLaunchKernelGGL(k_rank_slots, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.sstat,
+                       c.seg_cell, c.cell_counts, c.cell_offsets, c.scattered, c.cell_segs, c.pos, c.slotA, c.slotB,
+                       c.slotC, c.cP, c.cQ, c.S);
     hipLaunchKernelGGL(k_capture, dim3(blocks_for((size_t)c.S * kCaptureLanes, 256)), dim3(256), 0, stream,
                        capture_grid, c.pos, c.sstat,
                        c.factive, c.cell_counts, c.cell_offsets, c.cell_segs, c.S, max_gap, capacity, counter, pairs,

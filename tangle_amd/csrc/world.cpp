@@ -428,11 +428,21 @@ int upload_static(tgh_world *w) {
     HIP_TRY(hipMemset(d.red, 0, (size_t)d.nspec * 3 * d.red_slots * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_counts, 0, d.C * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_offsets, 0, d.C * sizeof(uint32_t)));
+    {
+        std::vector<uint32_t> cell_spec(d.C);
+        for (uint32_t sp = 0; sp < d.nspec; ++sp)
+            std::fill(cell_spec.begin() + w->cbase[sp], cell_spec.begin() + w->cbase[sp + 1], sp);
+        HIP_TRY(hipMemcpy(d.cell_spec, cell_spec.data(), d.C * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    // no slot holds a segment until the first rebuild
+    HIP_TRY(hipMemset(d.cell_segs, 0xff, S * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(d.slotC, 0xff, S * sizeof(uint4)));
     std::vector<SpecState> spec(d.nspec);
     for (SpecState &sp : spec) {
         std::memset(&sp, 0, sizeof(sp));
         sp.metrics[0] = std::numeric_limits<float>::infinity();
         sp.metrics[2] = std::numeric_limits<float>::infinity();
+        sp.rebuild = 1;  // lists start stale so the first contact pass builds them (world.rs:510-511)
     }
     HIP_TRY(hipMemcpy(d.ss, spec.data(), d.nspec * sizeof(SpecState), hipMemcpyHostToDevice));
     DevState init;
@@ -517,6 +527,8 @@ int create_world_packed(std::vector<Packed> &&packed, const tgh_config *c, tgh_w
         }
         g.cell_base = w->cbase[sp];
         g.skin = w->sizes[sp].skin;
+        g.seg_base = w->sbase[sp];
+        g.seg_count = p.S;
     }
     // lanes per fiber in the curvature sweep: 8 (eight fibers per wave, the
     // fastest measured on the bimodal felt) unless a workgroup's LDS would
@@ -532,6 +544,12 @@ int create_world_packed(std::vector<Packed> &&packed, const tgh_config *c, tgh_w
     // ~20-candidate lists of the bimodal felt; the autotuner picks narrower
     // groups for shorter lists
     d.contact_lanes = 16u;
+    // lanes per slot in the list build, one stencil candidate each: about a
+    // dozen rounds over the mean 27-cell candidate count
+    {
+        const double candidates = 27.0 * (double)S / (double)std::max<uint64_t>(1, C);
+        d.build_lanes = candidates <= 48.0 ? 4u : candidates <= 160.0 ? 8u : candidates <= 480.0 ? 16u : 32u;
+    }
     e = hipSuccess;
     GridParams *grids = nullptr;
     if (e == hipSuccess) e = w->alloc(&d.st, 1);
@@ -554,13 +572,13 @@ int create_world_packed(std::vector<Packed> &&packed, const tgh_config *c, tgh_w
     if (e == hipSuccess) e = w->alloc(&d.cell_counts, C);
     if (e == hipSuccess) e = w->alloc(&d.cell_offsets, C);
     if (e == hipSuccess) e = w->alloc(&d.cell_cursor, C);
+    if (e == hipSuccess) e = w->alloc(&d.cell_spec, std::max<uint64_t>(1, C));
     if (e == hipSuccess) e = w->alloc(&d.cell_segs, S);
     if (e == hipSuccess) e = w->alloc(&d.seg_cell, S);
     if (e == hipSuccess) e = w->alloc(&d.scattered, S);
     if (e == hipSuccess) e = w->alloc(&d.slotA, S);
     if (e == hipSuccess) e = w->alloc(&d.slotB, S);
     if (e == hipSuccess) e = w->alloc(&d.slotC, S);
-    if (e == hipSuccess) e = w->alloc(&d.slot_of, S);
     if (e == hipSuccess) e = w->alloc(&d.cP, S);
     if (e == hipSuccess) e = w->alloc(&d.cQ, S);
     if (e == hipSuccess) e = w->alloc(&d.ncount, S);
@@ -826,10 +844,8 @@ int tgh_world_upload_positions(tgh_world *world, const float *positions, size_t 
     HIP_TRY(hipMemcpy(world->d.pos, tmp.data(), V * sizeof(float4), hipMemcpyHostToDevice));
     world->history_valid = false;  // the motion history described the replaced positions
     world->proposal_pending = false;
-    DevState s;
-    HIP_TRY(hipMemcpy(&s, world->d.st, sizeof(s), hipMemcpyDeviceToHost));
-    s.nstate[0] = 1;
-    HIP_TRY(hipMemcpy(world->d.st, &s, sizeof(s), hipMemcpyHostToDevice));
+    HIP_TRY(tgh::launch_request_rebuild(world->d, world->stream));
+    HIP_TRY(hipStreamSynchronize(world->stream));
     return TGH_OK;
 }
 
@@ -870,17 +886,15 @@ int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t 
     const size_t S = world->d.S, B = world->d.block;
     if (list_capacity < S * B) return set_error(TGH_ERR_SIZE, "list buffer smaller than segment_count*neighbor_block");
     std::vector<uint32_t> slot_seg(S), slot_count(S), cols(std::max<size_t>(1, S * B));  // rows by slot
-    DevState st;
     HIP_TRY(hipStreamSynchronize(world->stream));
-    HIP_TRY(hipMemcpy(&st, world->d.st, sizeof(st), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(slot_seg.data(), world->d.cell_segs, S * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(slot_count.data(), world->d.ncount, S * 4, hipMemcpyDeviceToHost));
     if (S * B) HIP_TRY(hipMemcpy(cols.data(), world->d.nbr, S * B * 4, hipMemcpyDeviceToHost));
     std::memset(lists, 0, S * B * 4);
     std::memset(counts, 0, S * 4);  // segments outside the cell list (inactive) have no list
-    const size_t listed_slots = std::min<size_t>(S, st.slots);
-    for (size_t slot = 0; slot < listed_slots; ++slot) {
+    for (size_t slot = 0; slot < S; ++slot) {
         const uint32_t seg = slot_seg[slot];
+        if (seg >= S) continue;  // no segment in this slot
         counts[seg] = slot_count[slot];
         // rows hold cell-list slots; the boundary returns segments. An
         // overflowed list's row is never written (unspecified), so only

@@ -93,8 +93,6 @@ def test_batched_256_sample_follows_solo_and_oracle(hip_device):
         with _native.World(packed[i], cfg) as solo, oracle.OracleWorld(packed[i], cfg) as ref:
             ss = solo.run_batch(cfg, 40)
             so = ref.run_batch(cfg, 40)
-            ncount, _ = solo.neighbor_lists()
-            assert ncount.max() <= solo.info().neighbor_block, "an overflowed list sums in stencil order"
             alone = solo.download_positions()
             assert np.array_equal(alone.view(np.uint32), ref.download_positions().view(np.uint32)), i
             assert ss.max_penetration == so.max_penetration
