@@ -106,7 +106,6 @@ struct WorldDev {
     uint32_t max_fiber_vertices;
     uint32_t gs_lanes;       // lanes per fiber in the curvature sweep (0, 4, 8, 16, 32 or 64)
     uint32_t contact_lanes;  // lanes per segment in the contact pass (4, 8, 16 or 32)
-    uint32_t build_lanes;    // lanes per slot in the neighbour-list build (4, 8, 16 or 32)
     DevState *st;
     uint32_t *red;  // [nspec][3][red_slots] u32 bit patterns of nonnegative floats
     float4 *pos, *alt, *refpos, *wall, *vA, *vB;

@@ -43,6 +43,7 @@
 #include <stdint.h>
 
 #include "acosf_f32.h"
+#include "periodic_image.h"
 #include "relax_device.h"
 
 namespace tgh {
@@ -72,7 +73,7 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // The minimum-image data of one specimen, held in registers for the
 // per-candidate loops.
 struct Image {
-    float length[3];
+    float length[3], inv_length[3];
     uint32_t periodic[3];
 };
 __device__ __forceinline__ Image image_of(const GridParams &g) {
@@ -80,6 +81,7 @@ __device__ __forceinline__ Image image_of(const GridParams &g) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         m.length[k] = g.length[k];
+        m.inv_length[k] = 1.0f / g.length[k];
         m.periodic[k] = g.periodic[k];
     }
     return m;
@@ -126,6 +128,27 @@ __device__ __forceinline__ bool stencil_cell(const GridParams &g, uint32_t home,
     uint32_t z = (uint32_t)(((rz % nz) + nz) % nz);
     cell = (z * cy + y) * cx + x;
     return true;
+}
+
+// stencil_cell from the home cell's coordinates (hx, hy, hz), without
+// divisions: the same cells, for the same n.
+__device__ __forceinline__ bool stencil_cell_at(const GridParams &g, uint32_t hx, uint32_t hy, uint32_t hz,
+                                                uint32_t n, uint32_t &cell) {
+    const uint32_t cx = g.cells[0], cy = g.cells[1], cz = g.cells[2];
+    const int32_t ox = (int32_t)(n % 3u) - 1, oy = (int32_t)((n / 3u) % 3u) - 1, oz = (int32_t)(n / 9u) - 1;
+    const int32_t rx = (int32_t)hx + ox, ry = (int32_t)hy + oy, rz = (int32_t)hz + oz;
+    const int32_t nx = (int32_t)cx, ny = (int32_t)cy, nz = (int32_t)cz;
+    bool ok = (g.periodic[0] || (rx >= 0 && rx < nx)) && (g.periodic[1] || (ry >= 0 && ry < ny)) &&
+              (g.periodic[2] || (rz >= 0 && rz < nz));
+    ok = ok && (!g.periodic[0] || !((cx == 1 && ox != 0) || (cx == 2 && ((hx == 0 && ox > 0) || (hx == 1 && ox < 0)))));
+    ok = ok && (!g.periodic[1] || !((cy == 1 && oy != 0) || (cy == 2 && ((hy == 0 && oy > 0) || (hy == 1 && oy < 0)))));
+    ok = ok && (!g.periodic[2] || !((cz == 1 && oz != 0) || (cz == 2 && ((hz == 0 && oz > 0) || (hz == 1 && oz < 0)))));
+    // |offset| <= 1, so one wrap suffices
+    const uint32_t x = (uint32_t)(rx < 0 ? rx + nx : (rx >= nx ? rx - nx : rx));
+    const uint32_t y = (uint32_t)(ry < 0 ? ry + ny : (ry >= ny ? ry - ny : ry));
+    const uint32_t z = (uint32_t)(rz < 0 ? rz + nz : (rz >= nz ? rz - nz : rz));
+    cell = (z * cy + y) * cx + x;
+    return ok;
 }
 
 // Distance between two segment axes (proxy_pair_axis_distance,
@@ -185,37 +208,66 @@ __device__ __forceinline__ bool running(const SpecState *ss, uint32_t spec) { re
 
 __device__ __forceinline__ bool rebuilding(const SpecState *ss, uint32_t spec) { return ss[spec].rebuild != 0u; }
 
-__global__ void k_rebuild_clear(const DevState *st, const SpecState *ss, const uint32_t *cell_spec,
-                                uint32_t *cell_counts, uint32_t *cell_cursor, uint32_t C, const float4 *pos,
-                                float4 *refpos, const float4 *vB, uint32_t V) {
+// The rebuild kernels run a capped grid over chunks of 256 elements (the
+// grid-stride form), so that a pass without a pending rebuild costs one short
+// launch each; a chunk none of whose specimens rebuilds is skipped whole.
+constexpr uint32_t kRebuildBlocks = 2048;
+
+// Whether any of specimens lo..hi (those of a chunk's first and last
+// element) is rebuilding.
+__device__ __forceinline__ bool any_rebuilding(const SpecState *ss, uint32_t lo, uint32_t hi) {
+    for (uint32_t s = lo; s <= hi; ++s)
+        if (ss[s].rebuild != 0u) return true;
+    return false;
+}
+
+__device__ __forceinline__ uint32_t vertex_spec(const float4 *vB, uint32_t v) {
+    return __float_as_uint(vB[v].z) >> kSpecShift;
+}
+
+__global__ __launch_bounds__(256) void k_rebuild_clear(const DevState *st, const SpecState *ss,
+                                                       const uint32_t *cell_spec, uint32_t *cell_counts,
+                                                       uint32_t *cell_cursor, uint32_t C, const float4 *pos,
+                                                       float4 *refpos, const float4 *vB, uint32_t V) {
     if (st->nstate[0] == 0) return;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < C && rebuilding(ss, cell_spec[i])) {
-        cell_counts[i] = 0;
-        cell_cursor[i] = 0;
+    for (uint32_t first = blockIdx.x * 256u; first < C; first += gridDim.x * 256u) {
+        if (!any_rebuilding(ss, cell_spec[first], cell_spec[min(first + 255u, C - 1u)])) continue;
+        const uint32_t i = first + threadIdx.x;
+        if (i < C && rebuilding(ss, cell_spec[i])) {
+            cell_counts[i] = 0;
+            cell_cursor[i] = 0;
+        }
     }
     // The list snapshot (neighbor.rs:410-422) is taken from the positions the
     // lists are built from; nothing moves them before the build finishes.
-    if (i < V && rebuilding(ss, __float_as_uint(vB[i].z) >> kSpecShift)) refpos[i] = pos[i];
+    for (uint32_t first = blockIdx.x * 256u; first < V; first += gridDim.x * 256u) {
+        if (!any_rebuilding(ss, vertex_spec(vB, first), vertex_spec(vB, min(first + 255u, V - 1u)))) continue;
+        const uint32_t i = first + threadIdx.x;
+        if (i < V && rebuilding(ss, vertex_spec(vB, i))) refpos[i] = pos[i];
+    }
 }
 
-__global__ void k_bin_segments(const DevState *st, const SpecState *ss, const GridParams *grids, const float4 *pos,
-                               const uint4 *sstat, const uint32_t *factive, uint32_t *seg_cell, uint32_t *cell_counts,
-                               uint32_t S) {
+__global__ __launch_bounds__(256) void k_bin_segments(const DevState *st, const SpecState *ss,
+                                                      const GridParams *grids, const float4 *pos, const uint4 *sstat,
+                                                      const uint32_t *factive, uint32_t *seg_cell,
+                                                      uint32_t *cell_counts, uint32_t S) {
     if (st->nstate[0] == 0) return;
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
-    const uint4 me = sstat[s];
-    if (!rebuilding(ss, me.w)) return;
-    if (factive[me.y] == 0u) {  // inactive fibers are not in the cell list
-        seg_cell[s] = 0xffffffffu;
-        return;
+    for (uint32_t first = blockIdx.x * 256u; first < S; first += gridDim.x * 256u) {
+        if (!any_rebuilding(ss, sstat[first].w, sstat[min(first + 255u, S - 1u)].w)) continue;
+        const uint32_t s = first + threadIdx.x;
+        if (s >= S) continue;
+        const uint4 me = sstat[s];
+        if (!rebuilding(ss, me.w)) continue;
+        if (factive[me.y] == 0u) {  // inactive fibers are not in the cell list
+            seg_cell[s] = 0xffffffffu;
+            continue;
+        }
+        const GridParams &g = grids[me.w];
+        const uint32_t v0 = me.x;
+        const uint32_t cell = g.cell_base + cell_index(g, pos[v0], pos[v0 + 1]);
+        seg_cell[s] = cell;
+        atomicAdd(&cell_counts[cell], 1u);
     }
-    const GridParams &g = grids[me.w];
-    uint32_t v0 = me.x;
-    uint32_t cell = g.cell_base + cell_index(g, pos[v0], pos[v0 + 1]);
-    seg_cell[s] = cell;
-    atomicAdd(&cell_counts[cell], 1u);
 }
 
 // Exclusive scan of one specimen's cell counts per 1024-thread workgroup
@@ -262,25 +314,21 @@ __global__ __launch_bounds__(1024) void k_scan_cells(const DevState *st, const S
     }
 }
 
-__global__ void k_scatter_segments(const DevState *st, const SpecState *ss, const uint32_t *cell_spec,
-                                   const uint32_t *seg_cell, const uint32_t *cell_offsets, uint32_t *cell_cursor,
-                                   uint32_t *scattered, uint32_t S) {
+__global__ __launch_bounds__(256) void k_scatter_segments(const DevState *st, const SpecState *ss,
+                                                          const uint4 *sstat, const uint32_t *cell_spec,
+                                                          const uint32_t *seg_cell, const uint32_t *cell_offsets,
+                                                          uint32_t *cell_cursor, uint32_t *scattered, uint32_t S) {
     if (st->nstate[0] == 0) return;
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
-    uint32_t cell = seg_cell[s];
-    if (cell == 0xffffffffu) return;  // inactive
-    if (!rebuilding(ss, cell_spec[cell])) return;
-    uint32_t slot = cell_offsets[cell] + atomicAdd(&cell_cursor[cell], 1u);
-    scattered[slot] = s;
-}
-
-// Whether any of specimens lo..hi (those of a block's first and last
-// element) is rebuilding: blocks of other specimens return at once.
-__device__ __forceinline__ bool any_rebuilding(const SpecState *ss, uint32_t lo, uint32_t hi) {
-    for (uint32_t s = lo; s <= hi; ++s)
-        if (ss[s].rebuild != 0u) return true;
-    return false;
+    for (uint32_t first = blockIdx.x * 256u; first < S; first += gridDim.x * 256u) {
+        if (!any_rebuilding(ss, sstat[first].w, sstat[min(first + 255u, S - 1u)].w)) continue;
+        const uint32_t s = first + threadIdx.x;
+        if (s >= S) continue;
+        const uint32_t cell = seg_cell[s];
+        if (cell == 0xffffffffu) continue;  // inactive
+        if (!rebuilding(ss, cell_spec[cell])) continue;
+        const uint32_t slot = cell_offsets[cell] + atomicAdd(&cell_cursor[cell], 1u);
+        scattered[slot] = s;
+    }
 }
 
 // One thread per cell-list slot: the segment scattered into the slot is
@@ -289,8 +337,8 @@ __device__ __forceinline__ bool any_rebuilding(const SpecState *ss, uint32_t lo,
 // cell_list.rs:344-378), and the slot records the list build and the contact
 // pass read are written (gather_cell_slot_geometry, neighbor.rs:99-153): the
 // two vertex records, (segment, fiber, radius, specimen) and the compact
-// records. Slot i belongs to the specimen of
-// segment i (a specimen's slots are its segment range).
+// records. Slot i belongs to the specimen of segment i (a specimen's slots
+// are its segment range).
 __global__ __launch_bounds__(256) void k_rank_slots(const DevState *st, const SpecState *ss, const uint4 *sstat,
                                                     const uint32_t *seg_cell, const uint32_t *cell_counts,
                                                     const uint32_t *cell_offsets, const uint32_t *scattered,
@@ -298,25 +346,26 @@ __global__ __launch_bounds__(256) void k_rank_slots(const DevState *st, const Sp
                                                     float4 *slotB, uint4 *slotC, float4 *cP,
                                                     float4 *cQ, uint32_t S) {
     if (st->nstate[0] == 0) return;
-    const uint32_t first = blockIdx.x * 256u;
-    if (first >= S || !any_rebuilding(ss, sstat[first].w, sstat[min(first + 255u, S - 1u)].w)) return;
-    const uint32_t i = first + threadIdx.x;
-    if (i >= S || !rebuilding(ss, sstat[i].w)) return;
-    const uint32_t mine = scattered[i];
-    if (mine == kNoSegment) return;  // past the specimen's active segments
-    const uint32_t cell = seg_cell[mine];
-    const uint32_t n = cell_counts[cell], off = cell_offsets[cell];
-    uint32_t rank = 0;
-    for (uint32_t k = 0; k < n; ++k) rank += scattered[off + k] < mine ? 1u : 0u;
-    const uint32_t slot = off + rank;
-    cell_segs[slot] = mine;
-    const uint4 rec = sstat[mine];
-    const float4 p = pos[rec.x], q = pos[rec.x + 1];
-    slotA[slot] = p;
-    slotB[slot] = q;
-    slotC[slot] = make_uint4(mine, rec.y, rec.z, rec.w);
-    cP[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(rec.z));
-    cQ[slot] = make_float4(q.x, q.y, q.z, __uint_as_float(mine));
+    for (uint32_t first = blockIdx.x * 256u; first < S; first += gridDim.x * 256u) {
+        if (!any_rebuilding(ss, sstat[first].w, sstat[min(first + 255u, S - 1u)].w)) continue;
+        const uint32_t i = first + threadIdx.x;
+        if (i >= S || !rebuilding(ss, sstat[i].w)) continue;
+        const uint32_t mine = scattered[i];
+        if (mine == kNoSegment) continue;  // past the specimen's active segments
+        const uint32_t cell = seg_cell[mine];
+        const uint32_t n = cell_counts[cell], off = cell_offsets[cell];
+        uint32_t rank = 0;
+        for (uint32_t k = 0; k < n; ++k) rank += scattered[off + k] < mine ? 1u : 0u;
+        const uint32_t slot = off + rank;
+        cell_segs[slot] = mine;
+        const uint4 rec = sstat[mine];
+        const float4 p = pos[rec.x], q = pos[rec.x + 1];
+        slotA[slot] = p;
+        slotB[slot] = q;
+        slotC[slot] = make_uint4(mine, rec.y, rec.z, rec.w);
+        cP[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(rec.z));
+        cQ[slot] = make_float4(q.x, q.y, q.z, __uint_as_float(mine));
+    }
 }
 
 // The contact pass's compact slot records from the current vertex records
@@ -342,173 +391,276 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Entries per list whose segments a build group keeps in LDS for the sort.
-constexpr uint32_t kBuildSortEntries = 64;
+// The list build: kBuildLanes lanes per cell-list slot, kBuildGroups slots
+// per wave, one wave per workgroup.
+constexpr uint32_t kBuildLanes = 8;
+constexpr uint32_t kBuildGroups = 64 / kBuildLanes;
+constexpr uint32_t kBuildKeys = 48;      // list entries per slot kept in LDS for the sort
+constexpr uint32_t kBuildQueue = 128;    // candidates past the reach test awaiting the axis test
+constexpr uint32_t kBuildBlocks = 16384; // grid cap of the one-wave build workgroups
 
-// G lanes per cell-list slot build that slot's segment list from the 27
-// stencil cells (build_segment_neighbor_lists, neighbor.rs:155-364). The
-// stencil's candidates form one sequence, cell by cell in stencil order and
-// in slot (ascending segment) order within a cell: exactly the order in
-// which the reference's overflow scan visits them (contact.rs:348-428). Each
-// round the group's lanes test G consecutive candidates, whose compact slot
-// records are adjacent in memory, and the accepted ones (the pairs within the
-// skin, the only ones that can touch before the next rebuild) are appended to
-// the slot's row of `ovf` (`ovf_cap` slots) in sequence order through a
-// ballot. A listed segment's row (at most `block` entries) is then written
-// to `nbr` as slots in ascending segment order, which replaces the separate
-// sort (neighbor.rs:366-408) and keeps the reference's summation order; an
-// overflowed segment keeps its `ovf` row.
-template <int G>
-__global__ __launch_bounds__(256) void k_build_lists(const DevState *st, const SpecState *ss, const GridParams *grids,
-                                                     const uint4 *sstat, const uint4 *fspan, const float4 *slotA,
-                                                     const float4 *slotB, const uint4 *slotC, const float4 *cP,
-                                                     const float4 *cQ, const uint32_t *seg_cell,
-                                                     const uint32_t *cell_counts, const uint32_t *cell_offsets,
-                                                     uint32_t *ncount, uint32_t *nbr, uint32_t *ovf, uint32_t block,
-                                                     uint32_t ovf_cap, uint32_t S) {
-    static_assert(G == 4 || G == 8 || G == 16 || G == 32, "lanes per slot");
-    constexpr uint32_t groups = 256 / G;
-    constexpr uint32_t per_lane = (27 + G - 1) / G;  // stencil cells per lane
-    __shared__ uint32_t cell_start[groups][28];      // candidate sequence start of each stencil cell
-    __shared__ uint32_t cell_slot[groups][27];       // first slot of each stencil cell
-    __shared__ uint32_t keys[groups][kBuildSortEntries];
-    if (st->nstate[0] == 0) return;
-    const uint32_t first = blockIdx.x * groups;
-    if (first >= S || !any_rebuilding(ss, sstat[first].w, sstat[min(first + groups - 1u, S - 1u)].w)) return;
-    const uint32_t lane = threadIdx.x % G, group = threadIdx.x / G;
-    const uint32_t group_base = (threadIdx.x & 63u) - lane;
-    const uint32_t slot = first + group;
-    const uint4 M = slotC[slot < S ? slot : first];
-    const bool live = slot < S && M.x != kNoSegment && rebuilding(ss, M.w);
-    const uint32_t seg = M.x, owner = M.y;
-    const GridParams &g = grids[live ? M.w : 0u];
-    const float skin = g.skin;
-    const Image im = image_of(g);
-    uint32_t total = 0, count = 0;
-    float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
-    uint32_t first_seg = 0, nseg = 0;
-    if (live) {
-        A = slotA[slot];
-        B = slotB[slot];
-        const uint4 span = fspan[owner];
-        first_seg = span.x - owner;  // segment s of fiber f starts at vertex s + f
-        nseg = span.y - 1u;
-        // the lane's stencil cells, then their starts in the candidate sequence
-        const uint32_t home = seg_cell[seg] - g.cell_base;
-        uint32_t local = 0;
-        for (uint32_t j = 0; j < per_lane; ++j) {
-            const uint32_t n = lane * per_lane + j;
-            if (n >= 27) break;
-            uint32_t cell, cn = 0, cs = 0;
-            if (stencil_cell(g, home, n, cell)) {
-                cell += g.cell_base;
-                cn = cell_counts[cell];
-                cs = cell_offsets[cell];
-            }
-            cell_start[group][n] = local;  // chunk-relative for now
-            cell_slot[group][n] = cs;
-            local += cn;
-        }
-        uint32_t incl = local;
-#pragma unroll
-        for (uint32_t d = 1; d < G; d <<= 1) {
-            const uint32_t up = (uint32_t)__shfl_up((int)incl, d, G);
-            if (lane >= d) incl += up;
-        }
-        const uint32_t excl = incl - local;
-        total = (uint32_t)__shfl((int)incl, G - 1, G);
-        for (uint32_t j = 0; j < per_lane; ++j) {
-            const uint32_t n = lane * per_lane + j;
-            if (n >= 27) break;
-            cell_start[group][n] += excl;
-        }
-        if (lane == 0) cell_start[group][27] = total;
+// One slot's data the axis test reads from LDS.
+struct BuildSlot {
+    float p[3], q[3];
+    float radius, skin;
+    float length[3], inv_length[3];
+    uint32_t periodic;
+};
+
+// The reference's list test of one candidate (neighbor.rs:155-364) is
+// evaluated in two parts: the nearest-image shift and the reach bound on
+// the segment midpoints, then the axis distance. Both parts compute their
+// quantities with the same operations as the single test.
+__device__ __forceinline__ void list_image(const float *length, const float *inv_length, uint32_t periodic,
+                                           float p1x, float p1y, float p1z, float q1x, float q1y, float q1z,
+                                           float &p2x, float &p2y, float &p2z, float &q2x, float &q2y, float &q2z) {
+    if (periodic & 1u) {
+        float sh = image_round(0.5f * (p2x + q2x - p1x - q1x), length[0], inv_length[0]) * length[0];
+        p2x -= sh;
+        q2x -= sh;
     }
-    wave_sync();
-    uint32_t *row = ovf + (size_t)(live ? slot : 0u) * ovf_cap;  // encounter order
-    if (live) {
+    if (periodic & 2u) {
+        float sh = image_round(0.5f * (p2y + q2y - p1y - q1y), length[1], inv_length[1]) * length[1];
+        p2y -= sh;
+        q2y -= sh;
+    }
+    if (periodic & 4u) {
+        float sh = image_round(0.5f * (p2z + q2z - p1z - q1z), length[2], inv_length[2]) * length[2];
+        p2z -= sh;
+        q2z -= sh;
+    }
+}
+
+// The list build (build_segment_neighbor_lists, neighbor.rs:155-364), one
+// wave per kBuildGroups consecutive cell-list slots, kBuildLanes lanes each.
+// A slot's candidates are its 27 stencil cells' segments, cell by cell in
+// stencil order and in slot (ascending segment) order within a cell: exactly
+// the order in which the reference's overflow scan visits them
+// (contact.rs:348-428). Each round every group tests its next kBuildLanes
+// candidates (adjacent compact records) against the reach bound; the few
+// that pass are queued in order in LDS and given the axis-distance test 64 at
+// a time, so that the costly part keeps every lane busy. The accepted ones
+// (the pairs within the skin, the only ones that can touch before the next
+// rebuild) are appended to each slot's row of `ovf` (`ovf_cap` slots) in
+// sequence order. A listed segment's row (at most `block` entries) is then
+// written to `nbr` as slots in ascending segment order, which replaces the
+// separate sort (neighbor.rs:366-408) and keeps the reference's summation
+// order; an overflowed segment keeps its `ovf` row.
+__global__ __launch_bounds__(64) void k_build_lists(const DevState *st, const SpecState *ss, const GridParams *grids,
+                                                    const uint4 *sstat, const uint4 *fspan, const float4 *slotA,
+                                                    const float4 *slotB, const uint4 *slotC, const float4 *cP,
+                                                    const float4 *cQ, const uint32_t *seg_cell,
+                                                    const uint32_t *cell_counts, const uint32_t *cell_offsets,
+                                                    uint32_t *ncount, uint32_t *nbr, uint32_t *ovf, uint32_t block,
+                                                    uint32_t ovf_cap, uint32_t S) {
+    constexpr uint32_t G = kBuildLanes, W = kBuildGroups;
+    constexpr uint32_t per_lane = (27 + G - 1) / G;  // stencil cells per lane
+    __shared__ BuildSlot slots[W];
+    __shared__ uint32_t cell_start[W][28];  // a slot's candidate start of each stencil cell
+    __shared__ uint32_t cell_slot[W][27];   // first slot of each stencil cell
+    __shared__ uint32_t counts[W];
+    __shared__ uint2 queue[kBuildQueue];    // (group, candidate slot)
+    __shared__ uint32_t keys[W][kBuildKeys];
+    __shared__ uint32_t listed[W][kBuildKeys];
+    if (st->nstate[0] == 0) return;
+    const uint32_t lane = threadIdx.x % G, group = threadIdx.x / G;
+    const uint64_t below = (1ull << threadIdx.x) - 1ull;
+    // Workgroups are dealt to the 8 XCDs in turn; each XCD walks its own
+    // eighth of the chunks, so that neighbouring slots share an L2.
+    const uint32_t chunks = (S + W - 1u) / W;
+    uint32_t c0 = blockIdx.x, c1 = chunks, step = gridDim.x;
+    if (gridDim.x >= 8u) {
+        const uint32_t xcd = blockIdx.x % 8u;
+        c0 = (uint32_t)((uint64_t)chunks * xcd / 8u) + blockIdx.x / 8u;
+        c1 = (uint32_t)((uint64_t)chunks * (xcd + 1u) / 8u);
+        step = (gridDim.x - xcd + 7u) / 8u;  // workgroups on this XCD
+    }
+    for (uint32_t chunk = c0; chunk < c1; chunk += step) {
+        const uint32_t first = chunk * W;
+        if (!any_rebuilding(ss, sstat[first].w, sstat[min(first + W - 1u, S - 1u)].w)) continue;
+        const uint32_t slot = first + group;
+        const uint4 M = slotC[slot < S ? slot : first];
+        const bool live = slot < S && M.x != kNoSegment && rebuilding(ss, M.w);
+        const uint32_t seg = M.x, owner = M.y;
+        const GridParams &g = grids[live ? M.w : 0u];
+        uint32_t periodic = 0;
+        for (int k = 0; k < 3; ++k)
+            if (g.periodic[k]) periodic |= 1u << k;
+        float length[3] = {g.length[0], g.length[1], g.length[2]};
+        float inv_length[3] = {1.0f / g.length[0], 1.0f / g.length[1], 1.0f / g.length[2]};
+        const float skin = g.skin;
+        uint32_t total = 0;
+        float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
+        uint32_t first_seg = 0, nseg = 0;
+        if (live) {
+            A = slotA[slot];
+            B = slotB[slot];
+            const uint4 span = fspan[owner];
+            first_seg = span.x - owner;  // segment s of fiber f starts at vertex s + f
+            nseg = span.y - 1u;
+            const uint32_t home = seg_cell[seg] - g.cell_base;
+            const uint32_t hx = home % g.cells[0], hy = (home / g.cells[0]) % g.cells[1],
+                           hz = home / (g.cells[0] * g.cells[1]);
+            // the lane's stencil cells, then their starts in the candidate sequence
+            uint32_t local = 0;
+            for (uint32_t j = 0; j < per_lane; ++j) {
+                const uint32_t n = lane * per_lane + j;
+                if (n >= 27) break;
+                uint32_t cell, cn = 0, cs = 0;
+                if (stencil_cell_at(g, hx, hy, hz, n, cell)) {
+                    cell += g.cell_base;
+                    cn = cell_counts[cell];
+                    cs = cell_offsets[cell];
+                }
+                cell_start[group][n] = local;  // lane-relative for now
+                cell_slot[group][n] = cs;
+                local += cn;
+            }
+            uint32_t incl = local;
+#pragma unroll
+            for (uint32_t d = 1; d < G; d <<= 1) {
+                const uint32_t up = (uint32_t)__shfl_up((int)incl, d, G);
+                if (lane >= d) incl += up;
+            }
+            const uint32_t excl = incl - local;
+            total = (uint32_t)__shfl((int)incl, G - 1, G);
+            for (uint32_t j = 0; j < per_lane; ++j) {
+                const uint32_t n = lane * per_lane + j;
+                if (n >= 27) break;
+                cell_start[group][n] += excl;
+            }
+            if (lane == 0) {
+                cell_start[group][27] = total;
+                BuildSlot me;
+                me.p[0] = A.x, me.p[1] = A.y, me.p[2] = A.z;
+                me.q[0] = B.x, me.q[1] = B.y, me.q[2] = B.z;
+                me.radius = __uint_as_float(M.z);
+                me.skin = skin;
+                me.length[0] = length[0], me.length[1] = length[1], me.length[2] = length[2];
+                me.inv_length[0] = inv_length[0], me.inv_length[1] = inv_length[1], me.inv_length[2] = inv_length[2];
+                me.periodic = periodic;
+                slots[group] = me;
+            }
+        }
+        if (lane == 0) counts[group] = 0u;
+        wave_sync();
         const float arc_p = A.w, arc_q = B.w;
         const float p1x = A.x, p1y = A.y, p1z = A.z, radius = __uint_as_float(M.z);
         const float q1x = B.x, q1y = B.y, q1z = B.z;
         const float d1x = q1x - p1x, d1y = q1y - p1y, d1z = q1z - p1z;
         const float half_length = 0.5f * sqrtf(d1x * d1x + d1y * d1y + d1z * d1z);
-        uint32_t n = 0;  // this lane's stencil cell
-        for (uint32_t base = 0; base < total; base += G) {
-            const uint32_t t = base + lane;
+        uint32_t n = 0, base = 0;               // this lane's stencil cell; the group's next candidate
+        uint32_t head = 0, tail = 0;            // queue read and write counts
+        bool screening = __ballot(base < total) != 0ull;
+        while (screening || head != tail) {
+            if (screening && tail - head < 64u) {
+                // every group tests its next G candidates against the reach bound
+                const uint32_t t = base + lane;
+                bool pass = false;
+                uint32_t o = 0;
+                if (t < total) {
+                    while (cell_start[group][n + 1] <= t) ++n;
+                    o = cell_slot[group][n] + (t - cell_start[group][n]);
+                    const float4 P2 = cP[o], Q2 = cQ[o];
+                    const uint32_t other = __float_as_uint(Q2.w);
+                    const float other_radius = P2.w;
+                    bool skip = other == seg;
+                    if (!skip && other - first_seg < nseg) {  // same fiber: along-fiber neighbours are skipped
+                        const float gap = fmaxf(slotA[o].w - arc_q, arc_p - slotB[o].w);
+                        skip = gap < radius + other_radius;
+                    }
+                    if (!skip) {
+                        float p2x = P2.x, p2y = P2.y, p2z = P2.z, q2x = Q2.x, q2y = Q2.y, q2z = Q2.z;
+                        list_image(length, inv_length, periodic, p1x, p1y, p1z, q1x, q1y, q1z, p2x, p2y, p2z, q2x, q2y,
+                                   q2z);
+                        float d2x = q2x - p2x, d2y = q2y - p2y, d2z = q2z - p2z;
+                        float inter = (radius + other_radius + skin) * (1.0f + 1.0e-4f);
+                        float mdx = 0.5f * (p2x + q2x - p1x - q1x);
+                        float mdy = 0.5f * (p2y + q2y - p1y - q1y);
+                        float mdz = 0.5f * (p2z + q2z - p1z - q1z);
+                        float reach = half_length + 0.5f * sqrtf(d2x * d2x + d2y * d2y + d2z * d2z) + inter;
+                        pass = mdx * mdx + mdy * mdy + mdz * mdz <= reach * reach;
+                    }
+                }
+                const uint64_t mask = __ballot(pass);
+                if (pass) queue[(tail + (uint32_t)__popcll(mask & below)) % kBuildQueue] = make_uint2(group, o);
+                tail += (uint32_t)__popcll(mask);
+                base += G;
+                screening = __ballot(base < total) != 0ull;
+                wave_sync();
+            }
+            const uint32_t waiting = tail - head;
+            if (waiting < 64u && screening) continue;
+            // the axis test of up to 64 queued candidates, in order
+            const uint32_t take = waiting < 64u ? waiting : 64u;
             bool accept = false;
-            uint32_t o = 0, other = 0;
-            if (t < total) {
-                while (cell_start[group][n + 1] <= t) ++n;
-                o = cell_slot[group][n] + (t - cell_start[group][n]);
+            uint32_t k = W, o = 0, other = 0;
+            if (threadIdx.x < take) {
+                const uint2 e = queue[(head + threadIdx.x) % kBuildQueue];
+                k = e.x;
+                o = e.y;
+                const BuildSlot &me = slots[k];
                 const float4 P2 = cP[o], Q2 = cQ[o];
                 other = __float_as_uint(Q2.w);
-                const float other_radius = P2.w;
-                bool skip = other == seg;
-                if (!skip && other - first_seg < nseg) {  // same fiber: along-fiber neighbours are skipped
-                    const float gap = fmaxf(slotA[o].w - arc_q, arc_p - slotB[o].w);
-                    skip = gap < radius + other_radius;
+                const float ax = me.p[0], ay = me.p[1], az = me.p[2];
+                const float bx = me.q[0], by = me.q[1], bz = me.q[2];
+                float p2x = P2.x, p2y = P2.y, p2z = P2.z, q2x = Q2.x, q2y = Q2.y, q2z = Q2.z;
+                list_image(me.length, me.inv_length, me.periodic, ax, ay, az, bx, by, bz, p2x, p2y, p2z, q2x, q2y, q2z);
+                const float inter = (me.radius + P2.w + me.skin) * (1.0f + 1.0e-4f);
+                accept = axis_distance(ax, ay, az, bx - ax, by - ay, bz - az, p2x, p2y, p2z, q2x - p2x, q2y - p2y,
+                                       q2z - p2z) <= inter;
+            }
+            head += take;
+            // an accepted entry's place: its slot's count plus the accepted
+            // entries of the same slot before it in the queue
+#pragma unroll
+            for (uint32_t w = 0; w < W; ++w) {
+                const uint64_t mine = __ballot(accept && k == w);
+                if (accept && k == w) {
+                    const uint32_t at = counts[w] + (uint32_t)__popcll(mine & below);
+                    if (at < ovf_cap) ovf[(size_t)(first + w) * ovf_cap + at] = o;
+                    if (at < kBuildKeys) {
+                        keys[w][at] = other;
+                        listed[w][at] = o;
+                    }
                 }
-                if (!skip) {
-                    float p2x = P2.x, p2y = P2.y, p2z = P2.z, q2x = Q2.x, q2y = Q2.y, q2z = Q2.z;
-                    if (im.periodic[0]) {
-                        float sh = roundf(0.5f * (p2x + q2x - p1x - q1x) / im.length[0]) * im.length[0];
-                        p2x -= sh;
-                        q2x -= sh;
+                wave_sync();
+                if (threadIdx.x == 0) counts[w] += (uint32_t)__popcll(mine);
+            }
+            wave_sync();
+        }
+        // counts, then the listed rows in ascending segment order
+        const uint32_t c = counts[group];
+        if (live && lane == 0) ncount[slot] = c;
+        uint32_t *out = nbr + (size_t)slot * block;
+        if (live && c <= block && c <= kBuildKeys) {
+            // rank sort: segments are distinct, so every entry has its own rank
+            for (uint32_t i = lane; i < c; i += G) {
+                const uint32_t key = keys[group][i];
+                uint32_t rank = 0;
+                for (uint32_t j = 0; j < c; ++j) rank += keys[group][j] < key ? 1u : 0u;
+                out[rank] = listed[group][i];
+            }
+        }
+        if (live && c <= block && c > kBuildKeys) {
+            // long lists (blocks wider than the LDS keys): insertion sort of the
+            // row this wave wrote
+            __threadfence();
+            if (lane == 0) {
+                const uint32_t *row = ovf + (size_t)slot * ovf_cap;
+                for (uint32_t i = 0; i < c; ++i) {
+                    const uint32_t o = row[i];
+                    const uint32_t key = __float_as_uint(cQ[o].w);
+                    uint32_t at = i;
+                    while (at > 0 && __float_as_uint(cQ[out[at - 1]].w) > key) {
+                        out[at] = out[at - 1];
+                        --at;
                     }
-                    if (im.periodic[1]) {
-                        float sh = roundf(0.5f * (p2y + q2y - p1y - q1y) / im.length[1]) * im.length[1];
-                        p2y -= sh;
-                        q2y -= sh;
-                    }
-                    if (im.periodic[2]) {
-                        float sh = roundf(0.5f * (p2z + q2z - p1z - q1z) / im.length[2]) * im.length[2];
-                        p2z -= sh;
-                        q2z -= sh;
-                    }
-                    float d2x = q2x - p2x, d2y = q2y - p2y, d2z = q2z - p2z;
-                    float inter = (radius + other_radius + skin) * (1.0f + 1.0e-4f);
-                    float mdx = 0.5f * (p2x + q2x - p1x - q1x);
-                    float mdy = 0.5f * (p2y + q2y - p1y - q1y);
-                    float mdz = 0.5f * (p2z + q2z - p1z - q1z);
-                    float reach = half_length + 0.5f * sqrtf(d2x * d2x + d2y * d2y + d2z * d2z) + inter;
-                    accept = mdx * mdx + mdy * mdy + mdz * mdz <= reach * reach &&
-                             axis_distance(p1x, p1y, p1z, d1x, d1y, d1z, p2x, p2y, p2z, d2x, d2y, d2z) <= inter;
+                    out[at] = o;
                 }
             }
-            const uint32_t mask = (uint32_t)(__ballot(accept) >> group_base) & (uint32_t)((1ull << G) - 1ull);
-            if (accept) {
-                const uint32_t at = count + __popc(mask & ((1u << lane) - 1u));
-                if (at < ovf_cap) row[at] = o;
-                if (at < kBuildSortEntries) keys[group][at] = other;
-            }
-            count += __popc(mask);
         }
-        if (lane == 0) ncount[slot] = count;
-    }
-    // the row and the keys the group wrote, seen by all of its lanes
-    __syncthreads();
-    if (!live || count > block) return;
-    uint32_t *out = nbr + (size_t)slot * block;
-    if (count <= kBuildSortEntries) {
-        // rank sort: segments are distinct, so every entry has its own rank
-        for (uint32_t i = lane; i < count; i += G) {
-            const uint32_t key = keys[group][i];
-            uint32_t rank = 0;
-            for (uint32_t k = 0; k < count; ++k) rank += keys[group][k] < key ? 1u : 0u;
-            out[rank] = row[i];
-        }
-    } else if (lane == 0) {
-        // long lists (blocks wider than the LDS keys): insertion sort by segment
-        for (uint32_t i = 0; i < count; ++i) {
-            const uint32_t o = row[i];
-            const uint32_t key = __float_as_uint(cQ[o].w);
-            uint32_t at = i;
-            while (at > 0 && __float_as_uint(cQ[out[at - 1]].w) > key) {
-                out[at] = out[at - 1];
-                --at;
-            }
-            out[at] = o;
-        }
+        wave_sync();  // LDS is reused by the next slots
     }
 }
 
@@ -544,17 +696,17 @@ __device__ __forceinline__ bool contact_term(const Image &g, const SelfSeg &me, 
     float idy = 0.5f * (p2y + q2y - p1y - q1y);
     float idz = 0.5f * (p2z + q2z - p1z - q1z);
     if (g.periodic[0]) {
-        float sh = roundf(idx / g.length[0]) * g.length[0];
+        float sh = image_round(idx, g.length[0], g.inv_length[0]) * g.length[0];
         p2x -= sh;
         q2x -= sh;
     }
     if (g.periodic[1]) {
-        float sh = roundf(idy / g.length[1]) * g.length[1];
+        float sh = image_round(idy, g.length[1], g.inv_length[1]) * g.length[1];
         p2y -= sh;
         q2y -= sh;
     }
     if (g.periodic[2]) {
-        float sh = roundf(idz / g.length[2]) * g.length[2];
+        float sh = image_round(idz, g.length[2], g.inv_length[2]) * g.length[2];
         p2z -= sh;
         q2z -= sh;
     }
@@ -2399,6 +2551,11 @@ namespace tgh {
 
 namespace {
 inline uint32_t blocks_for(uint64_t n, uint32_t threads) { return (uint32_t)((n + threads - 1) / threads); }
+// grid of a rebuild kernel over n elements, `per_block` a chunk
+inline uint32_t rebuild_blocks(uint64_t n, uint32_t per_block) {
+    const uint32_t b = blocks_for(n, per_block);
+    return b < kRebuildBlocks ? b : kRebuildBlocks;
+}
 }  // namespace
 
 size_t curvature_lds_bytes(uint32_t max_vertices, uint32_t lanes, uint32_t track_walls) {
@@ -2413,28 +2570,23 @@ hipError_t launch_begin_batch(const WorldDev &w, hipStream_t stream) {
 hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
     if (w.S == 0 || w.C == 0 || w.V == 0) return hipErrorInvalidValue;
     const uint32_t clear_n = w.C > w.V ? w.C : w.V;
-    hipLaunchKernelGGL(k_rebuild_clear, dim3(blocks_for(clear_n, 256)), dim3(256), 0, stream, w.st, w.ss, w.cell_spec,
+    hipLaunchKernelGGL(k_rebuild_clear, dim3(rebuild_blocks(clear_n, 256)), dim3(256), 0, stream, w.st, w.ss, w.cell_spec,
                        w.cell_counts, w.cell_cursor, w.C, w.pos, w.refpos, w.vB, w.V);
-    hipLaunchKernelGGL(k_bin_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.grids, w.pos,
+    hipLaunchKernelGGL(k_bin_segments, dim3(rebuild_blocks(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.grids, w.pos,
                        w.sstat, w.factive, w.seg_cell, w.cell_counts, w.S);
     hipLaunchKernelGGL(k_scan_cells, dim3(w.nspec), dim3(1024), 0, stream, w.st, w.ss, w.grids, w.cell_counts,
                        w.cell_offsets, w.cell_segs, w.scattered, w.slotC);
-    hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.cell_spec,
-                       w.seg_cell, w.cell_offsets, w.cell_cursor, w.scattered, w.S);
-    hipLaunchKernelGGL(k_rank_slots, dim3(blocks_for(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.sstat,
+    hipLaunchKernelGGL(k_scatter_segments, dim3(rebuild_blocks(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.sstat,
+                       w.cell_spec, w.seg_cell, w.cell_offsets, w.cell_cursor, w.scattered, w.S);
+    hipLaunchKernelGGL(k_rank_slots, dim3(rebuild_blocks(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.sstat,
                        w.seg_cell, w.cell_counts, w.cell_offsets, w.scattered, w.cell_segs, w.pos, w.slotA, w.slotB,
                        w.slotC, w.cP, w.cQ, w.S);
-#define TGH_BUILD_ARGS                                                                                            \
-    w.st, w.ss, w.grids, w.sstat, w.fspan, w.slotA, w.slotB, w.slotC, w.cP, w.cQ, w.seg_cell, w.cell_counts,      \
-        w.cell_offsets, w.ncount, w.nbr, w.ovf, w.block, w.ovf_cap, w.S
-    switch (w.build_lanes) {
-        case 4: hipLaunchKernelGGL(k_build_lists<4>, dim3(blocks_for(w.S, 64)), dim3(256), 0, stream, TGH_BUILD_ARGS); break;
-        case 8: hipLaunchKernelGGL(k_build_lists<8>, dim3(blocks_for(w.S, 32)), dim3(256), 0, stream, TGH_BUILD_ARGS); break;
-        case 16: hipLaunchKernelGGL(k_build_lists<16>, dim3(blocks_for(w.S, 16)), dim3(256), 0, stream, TGH_BUILD_ARGS); break;
-        case 32: hipLaunchKernelGGL(k_build_lists<32>, dim3(blocks_for(w.S, 8)), dim3(256), 0, stream, TGH_BUILD_ARGS); break;
-        default: return hipErrorInvalidValue;
+    {
+        const uint32_t blocks = blocks_for(w.S, kBuildGroups);
+        hipLaunchKernelGGL(k_build_lists, dim3(blocks < kBuildBlocks ? blocks : kBuildBlocks), dim3(64), 0, stream,
+                           w.st, w.ss, w.grids, w.sstat, w.fspan, w.slotA, w.slotB, w.slotC, w.cP, w.cQ, w.seg_cell,
+                           w.cell_counts, w.cell_offsets, w.ncount, w.nbr, w.ovf, w.block, w.ovf_cap, w.S);
     }
-#undef TGH_BUILD_ARGS
     return hipGetLastError();
 }
 
@@ -2594,15 +2746,15 @@ hipError_t launch_capture(const WorldDev &w, const GridParams *capture_grid, uin
     c.C = capture_cells;
     hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(256), 0, stream, c.st, c.ss, nullptr, c.nspec);
     const uint32_t clear_n = c.C > c.V ? c.C : c.V;
-    hipLaunchKernelGGL(k_rebuild_clear, dim3(blocks_for(clear_n, 256)), dim3(256), 0, stream, c.st, c.ss, c.cell_spec,
+    hipLaunchKernelGGL(k_rebuild_clear, dim3(rebuild_blocks(clear_n, 256)), dim3(256), 0, stream, c.st, c.ss, c.cell_spec,
                        c.cell_counts, c.cell_cursor, c.C, c.pos, c.refpos, c.vB, c.V);
-    hipLaunchKernelGGL(k_bin_segments, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.grids, c.pos,
+    hipLaunchKernelGGL(k_bin_segments, dim3(rebuild_blocks(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.grids, c.pos,
                        c.sstat, c.factive, c.seg_cell, c.cell_counts, c.S);
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, c.st, c.ss, c.grids, c.cell_counts,
                        c.cell_offsets, c.cell_segs, c.scattered, c.slotC);
-    hipLaunchKernelGGL(k_scatter_segments, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.cell_spec,
-                       c.seg_cell, c.cell_offsets, c.cell_cursor, c.scattered, c.S);
-    hipLaunchKernelGGL(k_rank_slots, dim3(blocks_for(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.sstat,
+    hipLaunchKernelGGL(k_scatter_segments, dim3(rebuild_blocks(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.sstat,
+                       c.cell_spec, c.seg_cell, c.cell_offsets, c.cell_cursor, c.scattered, c.S);
+    hipLaunchKernelGGL(k_rank_slots, dim3(rebuild_blocks(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.sstat,
                        c.seg_cell, c.cell_counts, c.cell_offsets, c.scattered, c.cell_segs, c.pos, c.slotA, c.slotB,
                        c.slotC, c.cP, c.cQ, c.S);
     hipLaunchKernelGGL(k_capture, dim3(blocks_for((size_t)c.S * kCaptureLanes, 256)), dim3(256), 0, stream,

@@ -544,12 +544,6 @@ int create_world_packed(std::vector<Packed> &&packed, const tgh_config *c, tgh_w
     // ~20-candidate lists of the bimodal felt; the autotuner picks narrower
     // groups for shorter lists
     d.contact_lanes = 16u;
-    // lanes per slot in the list build, one stencil candidate each: about a
-    // dozen rounds over the mean 27-cell candidate count
-    {
-        const double candidates = 27.0 * (double)S / (double)std::max<uint64_t>(1, C);
-        d.build_lanes = candidates <= 48.0 ? 4u : candidates <= 160.0 ? 8u : candidates <= 480.0 ? 16u : 32u;
-    }
     e = hipSuccess;
     GridParams *grids = nullptr;
     if (e == hipSuccess) e = w->alloc(&d.st, 1);
