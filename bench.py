@@ -14,9 +14,13 @@ outside the timing. value = segment-iterations per second over the timed
 region (segments x iterations / seconds).
 
 Also reported: the wall time for Recipe.run to reach the hard admissibility
-gates (max penetration <= 1e-3 d_min, max curvature ratio <= 1 + 1e-5), the
-HBM roofline of the dominant kernel from HIP events captured in the timed
-batches, and the CPU baseline (the C++ restatement of the reference loop,
+gates (max penetration <= 1e-3 d_min, max curvature ratio <= 1 + 1e-5),
+plain and with the learned warm start on five specimens (training time
+reported and counted separately), the oracle replaying the plain plan to the
+same gates on the host cores, the HBM roofline of the dominant kernel from
+HIP events captured in the timed batches and of the whole iteration's
+algorithmic bytes, the same throughput legs on the slab and on 256 batched
+specimens, and the CPU baseline (the C++ restatement of the reference loop,
 OpenMP on the host cores) on a bounded sample of the same workload.
 """
 
@@ -70,6 +74,13 @@ def parse():
     ap.add_argument("--batched-steps", type=int, default=2, help="timed 128-iteration batches of the batched run")
     ap.add_argument("--curvature-lanes", type=int, default=-1,
                     help="lanes per fiber in the curvature sweep (0, 16, 32, 64; default: the world's choice)")
+    ap.add_argument("--learned-specimens", type=int, default=5,
+                    help="specimens of the learned-vs-plain comparison (the benchmark felt first)")
+    ap.add_argument("--skip-cpu-tta", action="store_true",
+                    help="skip the oracle's replay of the plain plan to the hard gates")
+    ap.add_argument("--skip-slab", action="store_true", help="skip the slab throughput run")
+    ap.add_argument("--workload", choices=["felt", "slab", "batched"], default="",
+                    help="run only this workload's plain throughput leg (for profiling)")
     return ap.parse_args()
 
 
@@ -94,11 +105,55 @@ def profiled_traffic(kernel: str):
 
 
 def contact_bytes_per_launch(S: int, V: int, mean_list: float) -> float:
-    """Algorithmic bytes of one contact launch: each vertex position read once
-    (16 B float4 record), each segment's static record (16 B: first vertex,
-    fiber, radius), list count (4 B) and list entries (4 B each), and the two
-    corrections written (32 B)."""
-    return 16.0 * V + S * (16.0 + 4.0 + 4.0 * mean_list + 32.0)
+    """Algorithmic bytes of one contact launch: each cell-list slot's compact
+    record (both endpoints and radius, 32 B) and static record (16 B:
+    segment, fiber, radius, specimen) read once, its list count (4 B) and
+    list entries (4 B each), and the two corrections written (32 B)."""
+    return S * (32.0 + 16.0 + 4.0 + 4.0 * mean_list + 32.0)
+
+
+def iteration_bytes(S: int, V: int, mean_list: float) -> float:
+    """Algorithmic HBM bytes of one relaxation iteration (the whole loop, not
+    one kernel): every vertex's position read and written and its intrinsic
+    position, curvature limit, arc length and pin flag read once (48 B), and
+    every segment's radius, fiber, rest length and list count read once
+    (16 B) plus its list entries (4 B each); the list rebuild is excluded
+    (amortised over the iterations between rebuilds, reported apart)."""
+    return 48.0 * V + (16.0 + 4.0 * mean_list) * S
+
+
+def whole_iteration_roofline(S, V, mean_list, iterations_per_second):
+    nbytes = iteration_bytes(S, V, mean_list)
+    achieved = nbytes * iterations_per_second / 1e9
+    return {"bound": "hbm", "bytes_per_iteration": int(nbytes), "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5)}
+
+
+def gs_kernel_name(lanes: int) -> str:
+    return {4: "tgh::k_curvature_gs_group<4>", 8: "tgh::k_curvature_gs_lean<8>", 16: "tgh::k_curvature_gs_lean<16>",
+            32: "tgh::k_curvature_gs_lean<32>", 64: "tgh::k_curvature_gs_wave64"}.get(lanes,
+                                                                                      "tgh::k_curvature_gs_lane")
+
+
+def host_cpu():
+    """The host CPU as this process sees it: model, logical CPUs it may use
+    and the physical cores under them (distinct package and core ids)."""
+    allowed = sorted(os.sched_getaffinity(0))
+    model, cores, cur = "unknown", set(), {}
+    try:
+        for line in open("/proc/cpuinfo").read().splitlines() + [""]:
+            if not line.strip():
+                if cur.get("processor") is not None and int(cur["processor"]) in allowed:
+                    cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                cur = {}
+                continue
+            key, _, value = line.partition(":")
+            cur[key.strip()] = value.strip()
+            if key.strip() == "model name":
+                model = value.strip()
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": len(allowed), "physical_cores": len(cores) or len(allowed)}
 
 
 def main():
@@ -126,64 +181,31 @@ def main():
     packed = recipe._packed(False)
     settings = fx.hard_gate_settings(d_min, force_full_iterations=True)
     cfg = settings.to_config(d_min)
-    S = int(packed.vertex_count - packed.counts.size)
-    V = int(packed.vertex_count)
-    initial = packed.placed.astype(np.float32)
 
-    lib = _native.library()
-    import ctypes as C
+    if args.workload:
+        # one plain throughput leg only, for the profiler
+        if args.workload == "felt":
+            out = plain_leg(_native, packed, cfg, args.steps, args.warmup, args.batch, args.curvature_lanes)
+        elif args.workload == "slab":
+            out = slab_throughput(args, fx, _native)
+        else:
+            out = batched_throughput(args, fx, tg, _native)
+        print(json.dumps({"workload": args.workload, "leg": out}, default=float), flush=True)
+        return
 
-    world = _native.World(packed, cfg)
-    if args.curvature_lanes >= 0:
-        world.set_tuning(curvature_lanes=args.curvature_lanes)
-    lanes = int(world.tuning().curvature_lanes)
-    contact_lanes = int(world.tuning().contact_lanes)
-    info = world.info()
-    # warmup: builds and caches the batch graph
-    for _ in range(args.warmup):
-        world.run_batch(cfg, args.batch)
-    world.upload_positions(initial)
-    rebuilds_before = world.info().neighbor_list_rebuilds
-
-    t0 = time.perf_counter()
-    total_iterations = 0
-    for _ in range(args.steps):
-        status = world.run_batch(cfg, args.batch)
-        total_iterations += int(status.batch_iterations)
-    t1 = time.perf_counter()
-    seconds = t1 - t0
-    rebuilds = world.info().neighbor_list_rebuilds - rebuilds_before
-    final_timed = world.download_positions()
-    counts, _ = world.neighbor_lists()
-    block = info.neighbor_block
-    mean_list = float(np.mean(np.minimum(counts, block)))
-
-    # Kernel timing: the same K steps from the same start, replayed with every
-    # kernel group bracketed by HIP events on the world's stream (captured
-    # graphs do not expose event times, so these batches launch directly).
-    world.upload_positions(initial)
-    _native.check(lib.tgh_world_set_timing(world._handle, 1))
-    ms = (C.c_double * 5)()
-    launches = (C.c_uint64 * 5)()
-    _native.check(lib.tgh_world_timing(world._handle, ms, launches, 5, 1))
-    for _ in range(args.steps):
-        world.run_batch(cfg, args.batch)
-    _native.check(lib.tgh_world_timing(world._handle, ms, launches, 5, 1))
-    replay_matches = bool(np.array_equal(world.download_positions().view(np.uint32), final_timed.view(np.uint32)))
-    group_ms = {GROUPS[g]: ms[g] for g in range(5)}
-    group_launches = {GROUPS[g]: launches[g] for g in range(5)}
-    world.close()
-
-    seg_iters = S * total_iterations
-    value = seg_iters / seconds
-    ms_per_step = 1000.0 * seconds / args.steps
+    progress("felt throughput")
+    leg = plain_leg(_native, packed, cfg, args.steps, args.warmup, args.batch, args.curvature_lanes)
+    S, V = leg["segments"], leg["vertices"]
+    lanes, contact_lanes, mean_list = leg["curvature_lanes"], leg["contact_lanes"], leg["mean_list_length"]
+    total_iterations = leg["iterations"]
+    group_ms, group_launches = leg["group_ms"], leg["group_launches"]
+    value = leg["value"]
+    ms_per_step = leg["ms_per_step"]
 
     # dominant kernel group by event time
     dominant = max(GROUPS, key=lambda g: group_ms[g])
     kernels = {
-        "curvature_gs": (f"tgh::k_curvature_gs_group<{lanes}>" if lanes in (4, 8, 16, 32)
-                         else ("tgh::k_curvature_gs_wave64" if lanes == 64 else "tgh::k_curvature_gs_lane"),
-                         curvature_bytes_per_launch(V)),
+        "curvature_gs": (gs_kernel_name(lanes), curvature_bytes_per_launch(V)),
         "contact": (f"tgh::k_contact<{contact_lanes}>", contact_bytes_per_launch(S, V, mean_list)),
     }
 
@@ -207,10 +229,10 @@ def main():
     roofline["limiter"] = ("the sequential Gauss-Seidel projection chain of the slowest fiber "
                            "(one exact projection per step), not memory")
     roofline["contact"] = kernel_roofline("contact")
+    roofline["whole_iteration"] = whole_iteration_roofline(S, V, mean_list, value / S)
     roofline["dominant_group"] = dominant
-    roofline["group_ms_per_iteration"] = {g: round(group_ms[g] / max(1, total_iterations + args.steps), 4)
-                                          for g in GROUPS}
-    roofline["timing_replay_bitwise_equal"] = replay_matches
+    roofline["group_ms_per_iteration"] = leg["group_ms_per_iteration"]
+    roofline["timing_replay_bitwise_equal"] = leg["timing_replay_bitwise_equal"]
 
     result = {
         "metric": METRIC,
@@ -232,80 +254,235 @@ def main():
             "vertices": V,
             "iterations_per_step": args.batch,
             "scale": args.scale,
-            "cells": list(info.cells),
-            "neighbor_block": block,
+            "cells": leg["cells"],
+            "neighbor_block": leg["neighbor_block"],
             "mean_list_length": round(mean_list, 2),
-            "rebuilds_in_timed_region": int(rebuilds),
+            "rebuilds_in_timed_region": leg["rebuilds"],
             "curvature_lanes": lanes,
         },
         "roofline": roofline,
     }
 
     if not args.skip_tta:
-        # plain HIP PBD plan to the hard gates: default relaxation, then a
-        # final SolvePolicy at the hard limits under the reference's
-        # contact_cleanup preset (the best of the schedules tried)
-        r = fx.staged_hard_gate_recipe(fx.bimodal_recipe(scale=args.scale), warm_iterations=args.tta_warm,
-                                       cleanup_iterations=args.tta_max_iterations)
-        tta_settings = fx.hard_gate_settings(d_min, max_iterations=args.tta_warm + args.tta_max_iterations + 1)
         tta = {"plain": None, "learned": None, "gates": {"max_penetration_m": fx.HARD_PENETRATION,
                                                            "max_curvature_ratio": fx.HARD_CURVATURE_RATIO}}
-        t0 = time.perf_counter()
-        try:
-            res = r.run(tta_settings)
-            wall = time.perf_counter() - t0
-            tta["plain"] = {"seconds": round(wall, 3), "iterations": res.iterations, "converged": res.converged,
-                            "max_penetration": res.max_penetration, "max_curvature_ratio": res.max_curvature_ratio,
-                            "plan": r.operations()[2:]}
-            positions = res.positions
-        except tg.RecipeError as err:
-            wall = time.perf_counter() - t0
-            tta["plain"] = {"seconds": None, "iterations": err.iteration, "converged": False,
-                            "error": str(err), "elapsed_s": round(wall, 3)}
-            positions = None
+        progress("plain time to the hard gates")
+        plain, recorded, positions = plain_tta(args, fx, tg, _native, d_min, 0, record=True)
+        tta["plain"] = plain
         if not args.skip_learned:
-            tta["learned"] = learned_tta(args, fx, tg, d_min)
-            if tta["plain"] and tta["plain"].get("seconds") and tta["learned"].get("seconds"):
-                tta["learned_speedup"] = round(tta["plain"]["seconds"] / tta["learned"]["seconds"], 3)
+            progress("learned time to the hard gates")
+            tta["learned"] = learned_tta(args, fx, tg, _native, d_min, plain)
+            if plain.get("seconds") and tta["learned"].get("seconds"):
+                tta["learned_speedup"] = round(plain["seconds"] / tta["learned"]["seconds"], 3)
+        if not args.no_cpu and not args.skip_cpu_tta and plain.get("converged"):
+            progress("oracle replay of the plain plan")
+            tta["cpu_port"] = cpu_plan_replay(recorded, plain)
+            if tta["cpu_port"].get("seconds") and tta.get("learned") and tta["learned"].get("seconds"):
+                tta["learned_vs_cpu_port"] = round(tta["cpu_port"]["seconds"] / tta["learned"]["seconds"], 1)
+                tta["plain_vs_cpu_port"] = round(tta["cpu_port"]["seconds"] / plain["seconds"], 1)
         result["time_to_hard_admissibility"] = tta
         if args.dump and positions is not None:
             np.save(args.dump, positions)
 
+    if not args.skip_slab:
+        progress("slab throughput")
+        result["slab"] = slab_throughput(args, fx, _native)
+
     if not args.skip_batched:
+        progress("batched throughput")
         result["batched"] = batched_throughput(args, fx, tg, _native)
+        progress("batched time to the gate")
         result["batched_time_to_gate"] = batched_warm_start(fx, tg)
 
     if not args.skip_formation:
+        progress("layered felt formation")
         result["formation"] = formation_workload(args, fx, tg)
 
     if not args.skip_adjacent:
+        progress("compaction, capture, checkpoint")
         result["compaction_capture"] = adjacent_workload(fx, _native)
         result["checkpoint_export"] = checkpoint_export_workload(fx, tg, _native)
 
     if not args.no_cpu:
+        progress("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(packed, cfg, S, args.cpu_iterations)
-        tta = result.get("time_to_hard_admissibility") or {}
-        same = result["cpu_baseline"].get("gpu_vs_cpu_same_sample")
-        if tta.get("learned_speedup") and same:
-            # the CPU port follows the plain trajectory bit for bit, so it
-            # needs the same iterations; its time is estimated at the GPU/CPU
-            # ratio measured on the same first iterations
-            tta["learned_vs_cpu_estimate"] = {
-                "value": round(tta["learned_speedup"] * same, 1),
-                "basis": "learned HIP vs plain HIP time to the hard gates, times the plain HIP / CPU port speed "
-                         "ratio on the first iterations of the same specimen (the CPU port's trajectory is the "
-                         "plain path's, bit for bit)"}
 
     print(json.dumps(result), flush=True)
 
 
-def learned_tta(args, fx, tg, d_min):
+def progress(stage: str) -> None:
+    """A progress line on stderr (the result line goes to stdout)."""
+    print(f"[bench] {stage}", file=sys.stderr, flush=True)
+
+
+def plain_leg(_native, packed, cfg, steps, warmup, batch, curvature_lanes=-1):
+    """K timed batches of `batch` iterations (force_full_iterations) from the
+    generated positions, launched as captured graphs, then the same K
+    batches replayed from the same start with every kernel group bracketed
+    by HIP events on the world's stream (captured graphs do not expose
+    event times, so those batches launch directly)."""
+    import ctypes as C
+
+    lib = _native.library()
+    S = int(packed.vertex_count - packed.counts.size)
+    V = int(packed.vertex_count)
+    initial = packed.placed.astype(np.float32)
+    world = _native.World(packed, cfg)
+    if curvature_lanes >= 0:
+        world.set_tuning(curvature_lanes=curvature_lanes)
+    lanes = int(world.tuning().curvature_lanes)
+    contact_lanes = int(world.tuning().contact_lanes)
+    info = world.info()
+    for _ in range(warmup):  # builds and caches the batch graph
+        world.run_batch(cfg, batch)
+    world.upload_positions(initial)
+    rebuilds_before = world.info().neighbor_list_rebuilds
+    t0 = time.perf_counter()
+    total_iterations = 0
+    for _ in range(steps):
+        status = world.run_batch(cfg, batch)
+        total_iterations += int(status.batch_iterations)
+    seconds = time.perf_counter() - t0
+    rebuilds = world.info().neighbor_list_rebuilds - rebuilds_before
+    final_timed = world.download_positions()
+    counts, _ = world.neighbor_lists()
+    block = info.neighbor_block
+    mean_list = float(np.mean(np.minimum(counts, block)))
+    world.upload_positions(initial)
+    _native.check(lib.tgh_world_set_timing(world._handle, 1))
+    ms = (C.c_double * 5)()
+    launches = (C.c_uint64 * 5)()
+    _native.check(lib.tgh_world_timing(world._handle, ms, launches, 5, 1))
+    for _ in range(steps):
+        world.run_batch(cfg, batch)
+    _native.check(lib.tgh_world_timing(world._handle, ms, launches, 5, 1))
+    replay_matches = bool(np.array_equal(world.download_positions().view(np.uint32), final_timed.view(np.uint32)))
+    world.close()
+    group_ms = {GROUPS[g]: ms[g] for g in range(5)}
+    group_launches = {GROUPS[g]: int(launches[g]) for g in range(5)}
+    passes = total_iterations + steps  # each batch ends with a measuring pass
+    return {
+        "segments": S, "vertices": V, "value": round(S * total_iterations / seconds, 1), "unit": "segment-iters/s",
+        "seconds": round(seconds, 4), "ms_per_step": round(1000.0 * seconds / steps, 3),
+        "ms_per_iteration": round(1000.0 * seconds / total_iterations, 4), "iterations": total_iterations,
+        "rebuilds": int(rebuilds), "mean_list_length": round(mean_list, 3), "neighbor_block": int(block),
+        "cells": list(info.cells), "curvature_lanes": lanes, "contact_lanes": contact_lanes,
+        "group_ms": group_ms, "group_launches": group_launches,
+        "group_ms_per_iteration": {g: round(group_ms[g] / max(1, passes), 4) for g in GROUPS},
+        "timing_replay_bitwise_equal": replay_matches,
+    }
+
+
+def slab_throughput(args, fx, _native):
+    """The slab: 20,000 bimodal fibers 1 mm long in a 3 x 3 x 1 mm cell,
+    periodic in x and y, walled in z (1.31 M segments), two timed 128-
+    iteration batches from the generated state at the hard-gate settings,
+    with the contact kernel's roofline and the whole iteration's."""
+    packed = fx.slab_recipe()._packed(False)
+    cfg = fx.hard_gate_settings(7e-6, force_full_iterations=True).to_config(7e-6)
+    leg = plain_leg(_native, packed, cfg, 2, 1, args.batch)
+    S, V, k = leg["segments"], leg["vertices"], leg["mean_list_length"]
+    nbytes = contact_bytes_per_launch(S, V, k)
+    avg_ms = leg["group_ms"]["contact"] / max(1, leg["group_launches"]["contact"])
+    leg["workload"] = "slab: 20,000 bimodal fibers (17,630 x 7 um + 2,370 x 19 um), 3 x 3 x 1 mm, periodic x y, walled z"
+    leg["contact_roofline"] = {"bound": "hbm", "kernel": f"tgh::k_contact<{leg['contact_lanes']}>",
+                               "bytes_per_launch": int(nbytes), "avg_launch_us": round(1000.0 * avg_ms, 2),
+                               "achieved": round(nbytes / (avg_ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS,
+                               "unit": "GB/s", "frac": round(nbytes / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    leg["whole_iteration"] = whole_iteration_roofline(S, V, k, leg["value"] / S)
+    del leg["group_ms"], leg["group_launches"]
+    return leg
+
+
+class _BatchRecorder:
+    """Records the world a Recipe.run creates and every batch it runs (a copy
+    of the config and the iteration count), so that the oracle can replay
+    the same plan."""
+
+    def __init__(self, _native):
+        self._native, self.created, self.calls = _native, [], []
+
+    def __enter__(self):
+        World = self._native.World
+        self._init, self._run = World.__init__, World.run_batch
+        rec = self
+
+        def init(world, packed, config):
+            rec.created.append((packed, type(config).from_buffer_copy(config)))
+            rec._init(world, packed, config)
+
+        def run(world, config, iterations):
+            rec.calls.append((type(config).from_buffer_copy(config), int(iterations)))
+            return rec._run(world, config, iterations)
+
+        World.__init__, World.run_batch = init, run
+        return self
+
+    def __exit__(self, *exc):
+        self._native.World.__init__, self._native.World.run_batch = self._init, self._run
+
+
+def plain_tta(args, fx, tg, _native, d_min, seed_offset, record=False):
+    """Plain HIP PBD plan to the hard gates: default relaxation, then a final
+    SolvePolicy at the hard limits under the reference's contact_cleanup
+    preset (the best of the schedules tried), through Recipe.run."""
+    r = fx.staged_hard_gate_recipe(fx.bimodal_recipe(scale=args.scale, seed_offset=seed_offset),
+                                   warm_iterations=args.tta_warm, cleanup_iterations=args.tta_max_iterations)
+    settings = fx.hard_gate_settings(d_min, max_iterations=args.tta_warm + args.tta_max_iterations + 1)
+    recorder = _BatchRecorder(_native)
+    t0 = time.perf_counter()
+    try:
+        if record:
+            with recorder:
+                res = r.run(settings)
+        else:
+            res = r.run(settings)
+        wall = time.perf_counter() - t0
+        out = {"seconds": round(wall, 3), "iterations": res.iterations, "converged": res.converged,
+               "max_penetration": res.max_penetration, "max_curvature_ratio": res.max_curvature_ratio,
+               "plan": r.operations()[2:]}
+        positions = res.positions
+    except tg.RecipeError as err:
+        out = {"seconds": None, "iterations": err.iteration, "converged": False, "error": str(err),
+               "elapsed_s": round(time.perf_counter() - t0, 3)}
+        positions = None
+    return out, recorder, positions
+
+
+def cpu_plan_replay(recorded, plain):
+    """The oracle (the C++ restatement of the reference loop, OpenMP on the
+    host cores) replays the plain plan's batches, config for config, from
+    the same generated specimen: the CPU port's time to the hard gates. Its
+    trajectory is the plain HIP path's bit for bit, so the same batches
+    bring it to the gates; the final status is checked against them."""
+    import oracle
+
+    if len(recorded.created) != 1:
+        return {"error": f"the plan created {len(recorded.created)} worlds; only one is replayed"}
+    packed, config = recorded.created[0]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    with oracle.OracleWorld(packed, config) as w:
+        t0 = time.perf_counter()
+        status = None
+        for cfg, iterations in recorded.calls:
+            status = w.run_batch(cfg, iterations)
+        seconds = time.perf_counter() - t0
+    return {"seconds": round(seconds, 3), "iterations": int(status.total_iterations), "batches": len(recorded.calls),
+            "converged": bool(status.converged), "max_penetration": status.max_penetration,
+            "max_curvature_ratio": status.max_curvature_ratio,
+            "same_iterations_as_plain": int(status.total_iterations) == int(plain["iterations"]),
+            "kind": "port", "threads": threads, "host": host_cpu()}
+
+
+def learned_tta(args, fx, tg, _native, d_min, plain0):
     """The learned warm start: a model trained on the GPU from plain-relaxer
     trajectories of two other specimens of the same distribution (seed
-    offsets 7 and 13; training time reported, not counted in the run), then
-    Recipe.run on the benchmark specimen with proposals every
-    `--learned-interval` iterations, a shorter default-settings stage and the
-    same final hard-gate stage as the plain plan."""
+    offsets 7 and 13), then Recipe.run on the benchmark specimen with
+    proposals every `--learned-interval` iterations, a shorter default-
+    settings stage and the same final hard-gate stage as the plain plan.
+    The same model and plans then run on further specimens (seed offsets
+    101, 202, ...); speed-ups are reported per specimen, as minimum and
+    geometric mean, and with the training time counted."""
     from tangle_amd import learned
 
     t0 = time.perf_counter()
@@ -323,25 +500,58 @@ def learned_tta(args, fx, tg, d_min):
                         "unit": "TFLOP/s",
                         "frac": round(tile_flops * S / (fwd_ms * 1e-3) / 1e12 / 2500.0, 4),
                         "input_GBps": round(128.0 * S / (fwd_ms * 1e-3) / 1e9, 1)}}
-    ws = learned.WarmStart(model, interval=args.learned_interval, scale=args.learned_scale, settle=4,
-                           active_iterations=10 ** 9)
-    r = fx.staged_hard_gate_recipe(fx.bimodal_recipe(scale=args.scale), warm_iterations=args.learned_warm,
-                                   cleanup_iterations=args.tta_max_iterations)
-    settings = fx.hard_gate_settings(d_min, max_iterations=args.learned_warm + args.tta_max_iterations + 1)
+
+    def run_learned(seed_offset):
+        ws = learned.WarmStart(model, interval=args.learned_interval, scale=args.learned_scale, settle=4,
+                               active_iterations=10 ** 9)
+        r = fx.staged_hard_gate_recipe(fx.bimodal_recipe(scale=args.scale, seed_offset=seed_offset),
+                                       warm_iterations=args.learned_warm, cleanup_iterations=args.tta_max_iterations)
+        settings = fx.hard_gate_settings(d_min, max_iterations=args.learned_warm + args.tta_max_iterations + 1)
+        t1 = time.perf_counter()
+        try:
+            res = r.run(settings, warm_start=ws)
+            return {"seconds": round(time.perf_counter() - t1, 3), "iterations": res.iterations,
+                    "converged": res.converged, "max_penetration": res.max_penetration,
+                    "max_curvature_ratio": res.max_curvature_ratio, "plan": r.operations()[2:],
+                    "events": res.events[2:]}
+        except tg.RecipeError as err:
+            return {"seconds": None, "iterations": err.iteration, "converged": False, "error": str(err),
+                    "elapsed_s": round(time.perf_counter() - t1, 3)}
+
     out = {"training_seconds": round(train_s, 3), "training_rows": int(rows),
            "training_loss": [round(float(losses[0]), 5), round(float(losses[-1]), 5)],
            "interval": args.learned_interval, "scale": args.learned_scale, "mlp": mlp}
-    t0 = time.perf_counter()
-    try:
-        res = r.run(settings, warm_start=ws)
-        wall = time.perf_counter() - t0
-        out.update({"seconds": round(wall, 3), "iterations": res.iterations, "converged": res.converged,
-                    "max_penetration": res.max_penetration, "max_curvature_ratio": res.max_curvature_ratio,
-                    "plan": r.operations()[2:], "events": res.events[2:]})
-    except tg.RecipeError as err:
-        out.update({"seconds": None, "iterations": err.iteration, "converged": False, "error": str(err),
-                    "elapsed_s": round(time.perf_counter() - t0, 3)})
+    out.update(run_learned(0))
+    specimens_out = []
+    offsets = [0] + [101 * i for i in range(1, max(1, args.learned_specimens))]
+    for o in offsets:
+        progress(f"learned vs plain, specimen {o}")
+        plain = plain0 if o == 0 else plain_tta(args, fx, tg, _native, d_min, o)[0]
+        learned_run = out if o == 0 else run_learned(o)
+        row = {"seed_offset": o, "plain_seconds": plain.get("seconds"), "plain_iterations": plain.get("iterations"),
+               "learned_seconds": learned_run.get("seconds"), "learned_iterations": learned_run.get("iterations"),
+               "both_converged": bool(plain.get("converged") and learned_run.get("converged"))}
+        if row["both_converged"]:
+            row["speedup"] = round(row["plain_seconds"] / row["learned_seconds"], 3)
+        specimens_out.append(row)
     model.close()
+    speedups = [r["speedup"] for r in specimens_out if "speedup" in r]
+    summary = {"specimens": specimens_out, "converged_pairs": len(speedups)}
+    if speedups:
+        summary["min_speedup"] = min(speedups)
+        summary["geomean_speedup"] = round(float(np.exp(np.mean(np.log(speedups)))), 3)
+    if len(speedups) == len(specimens_out):
+        plain_total = sum(r["plain_seconds"] for r in specimens_out)
+        learned_total = sum(r["learned_seconds"] for r in specimens_out)
+        n = len(specimens_out)
+        # the model is trained once and reused: its cost is counted against
+        # one specimen and against all of them
+        summary["training_inclusive"] = {
+            "one_specimen_speedup": round(specimens_out[0]["plain_seconds"] /
+                                          (train_s + specimens_out[0]["learned_seconds"]), 3),
+            f"{n}_specimens_speedup": round(plain_total / (train_s + learned_total), 3),
+            "break_even_specimens": round(train_s / max(1e-9, (plain_total - learned_total) / n), 2)}
+    out["five_specimens" if len(specimens_out) == 5 else "specimens"] = summary
     return out
 
 
@@ -408,6 +618,7 @@ def batched_throughput(args, fx, tg, _native):
                      "trial_ms": {k: {str(kk): round(vv, 3) for kk, vv in v.items()}
                                   for k, v in tuning.trials.items()}},
         "group_ms_per_iteration": per_iter,
+        "whole_iteration": whole_iteration_roofline(S, V, mean_list, iterations / seconds),
         "contact_roofline": {"bound": "hbm", "kernel": f"tgh::k_contact<{tuning.contact_lanes}>", "bytes_per_launch": int(contact_bytes),
                              "avg_launch_us": round(1000.0 * contact_avg, 2),
                              "achieved": round(contact_bytes / (contact_avg * 1e-3) / 1e9, 1),
@@ -459,6 +670,8 @@ def batched_warm_start(fx, tg):
     model.close()
     if out["plain"]["converged"] == out["learned"]["converged"] == 64:
         out["learned_speedup"] = round(out["plain"]["seconds"] / out["learned"]["seconds"], 3)
+        out["learned_speedup_training_included"] = round(
+            out["plain"]["seconds"] / (out["learned"]["seconds"] + out["training_seconds"]), 3)
     return out
 
 
@@ -594,7 +807,7 @@ def formation_workload(args, fx, tg):
             t1 = time.perf_counter()
             status = w.run_batch(cfg, 4)
             t2 = time.perf_counter()
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
         out["cpu_baseline"] = {"value": round(S * int(status.batch_iterations) / (t2 - t1), 1),
                                "unit": "segment-iters/s", "cores": threads, "kind": "port",
                                "sample": f"{int(status.batch_iterations)} iterations of the fully stacked felt "
@@ -607,7 +820,7 @@ def cpu_baseline(packed, cfg, S, iterations):
     `iterations` iterations of the same specimen, on the host cores."""
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
     with oracle.OracleWorld(packed, cfg) as w:
         t1 = time.perf_counter()
@@ -628,6 +841,8 @@ def cpu_baseline(packed, cfg, S, iterations):
         "value": round(S * done / (t2 - t1), 1),
         "unit": "segment-iters/s",
         "cores": threads,
+        "host": host_cpu(),
+        "build": "g++ -O3 -march=x86-64-v3 -fopenmp (oracle/Makefile)",
         "kind": "port",
         "sample": f"first {done} iterations of the same specimen from the generated state "
                   f"({t2 - t1:.1f} s; upload and sizing {t1 - t0:.2f} s excluded)",
