@@ -65,6 +65,8 @@ def library() -> C.CDLL:
             "orc_world_capture_contacts": (C.c_int, [C.c_void_p, C.c_float, C.c_size_t, C.POINTER(C.c_float),
                                                      C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)]),
             "orc_world_set_fiber_maximum_curvatures": (None, [C.c_void_p, C.POINTER(C.c_float)]),
+            "orc_world_gs_fibers": (None, [C.c_void_p, u32p, u32p]),
+            "orc_gs_counters": (None, [C.POINTER(C.c_uint64)]),
             "orc_world_destroy": (None, [C.c_void_p]),
             "orc_last_error": (C.c_char_p, []),
         }
@@ -179,6 +181,15 @@ class OracleWorld:
         lists = np.zeros(max(1, info.segment_count * info.neighbor_block), dtype=np.uint32)
         _check(library().orc_world_neighbor_lists(self._h, _p(counts, C.c_uint32), _p(lists, C.c_uint32)))
         return counts, lists[: info.segment_count * info.neighbor_block].reshape(info.segment_count, -1)
+
+    def curvature_fiber_counts(self):
+        """Per fiber, at its last curvature cleanup: (projections applied,
+        triplets the GPU sweep visits)."""
+        F = self.info().fiber_count
+        projections = np.zeros(F, dtype=np.uint32)
+        visits = np.zeros(F, dtype=np.uint32)
+        library().orc_world_gs_fibers(self._h, _p(projections, C.c_uint32), _p(visits, C.c_uint32))
+        return projections, visits
 
     def contact_state(self):
         info = self.info()

@@ -128,6 +128,10 @@ struct World {
     std::vector<float> icorr;               // 3V
     std::vector<float> vstep;               // V
     std::vector<float> wall;                // 3V
+    // per fiber, at its last curvature cleanup: projections applied, and
+    // the triplets the GPU sweep visits (its sweeps stop after a forward
+    // pass that projects nothing); observation only
+    std::vector<uint32_t> gs_projections, gs_visits;
 };
 
 // ---- packing: packed.rs:150-320 ------------------------------------------
@@ -926,21 +930,29 @@ bool project_triplet(World &w, uint32_t prev, uint32_t center, uint32_t next, fl
 void curvature_cleanup(World &w, const tgh_config *c) {
     ++g_gs_calls;
     const float st = c->curvature_limit_stiffness, mg = c->curvature_limit_safety_margin;
+    w.gs_projections.assign(w.F, 0u);
+    w.gs_visits.assign(w.F, 0u);
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t fi = 0; fi < (int64_t)w.F; ++fi) {
         uint32_t start = w.fiber_span[2 * fi], count = w.fiber_span[2 * fi + 1];
         if (count < 3 || !w.fiber_active[fi]) continue;
-        uint64_t projections = 0, sweeps_run = 0;
+        uint64_t projections = 0, sweeps_run = 0, visits = 0;
+        bool visiting = true;
         for (uint32_t sweep = 0; sweep < c->curvature_cleanup_sweeps; ++sweep) {
             uint64_t forward = 0;
             for (uint32_t v = start + 2; v < start + count; ++v) forward += project_triplet(w, v - 2, v - 1, v, st, mg);
+            if (visiting) visits += count - 2;
+            if (visiting && forward != 0) visits += count - 2;  // the backward pass
             for (uint32_t l = 2; l < count; ++l) {
                 uint32_t v = start + count - 1 - l;
                 projections += project_triplet(w, v, v + 1, v + 2, st, mg);
             }
             projections += forward;
             if (forward != 0) ++sweeps_run;  // sweeps the GPU kernel's early exit still runs
+            if (forward == 0) visiting = false;
         }
+        w.gs_projections[fi] = (uint32_t)projections;
+        w.gs_visits[fi] = (uint32_t)visits;
 #pragma omp atomic
         g_gs_projections += projections;
 #pragma omp atomic
@@ -1408,6 +1420,16 @@ void orc_gs_counters(uint64_t *out) {
     out[2] = g_gs_fiber_sweeps;
     out[3] = g_gs_max_fiber_projections;
     g_gs_calls = g_gs_projections = g_gs_fiber_sweeps = g_gs_max_fiber_projections = 0;
+}
+
+// Per fiber, at its last curvature cleanup: projections applied and
+// triplets visited (fibers never cleaned up read 0).
+void orc_world_gs_fibers(void *h, uint32_t *projections, uint32_t *visits) {
+    const World &w = *(const World *)h;
+    for (uint32_t f = 0; f < w.F; ++f) {
+        projections[f] = f < w.gs_projections.size() ? w.gs_projections[f] : 0u;
+        visits[f] = f < w.gs_visits.size() ? w.gs_visits[f] : 0u;
+    }
 }
 
 void orc_world_activate(void *h, uint32_t maximum_step, uint64_t *segments, uint64_t *vertices) {

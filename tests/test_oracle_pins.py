@@ -249,3 +249,45 @@ def test_projection_removes_a_large_kink_at_once(kink):
     before, after = projected_excess(kink, 0.2)
     assert before > 0.45
     assert -0.15 * before <= after <= 0.0, (before, after)
+
+
+# ---------------------------------------------------------------------------
+# Curvature cleanup counts per fiber (the work of the GPU's sequential
+# sweep): the per-fiber projections sum to the oracle's own counter of the
+# same call, every applied projection is a visited triplet, and a fiber is
+# visited at least once forward. On the benchmark felt after 200 iterations
+# the test prints the histogram of projections per fiber (run with -s).
+
+
+def curvature_counts(scale, iterations):
+    import ctypes as C
+    from tangle_amd import RelaxationSettings
+
+    packed = fx.bimodal_recipe(scale=scale)._packed(False)
+    cfg = RelaxationSettings().to_config(7e-6)
+    cfg.force_full_iterations = 1
+    counters = (C.c_uint64 * 4)()
+    with oracle.OracleWorld(packed, cfg) as w:
+        w.run_batch(cfg, iterations)
+        oracle.library().orc_gs_counters(counters)  # clears them
+        w.run_batch(cfg, 1)  # one iteration: one cleanup (the last pass only measures)
+        oracle.library().orc_gs_counters(counters)
+        projections, visits = w.curvature_fiber_counts()
+    return packed, list(counters), projections.astype(np.int64), visits.astype(np.int64)
+
+
+@pytest.mark.parametrize("scale,iterations", [(0.02, 20), (1.0, 200)])
+def test_curvature_fiber_counts(scale, iterations):
+    packed, counters, projections, visits = curvature_counts(scale, iterations)
+    calls, total, _, most = counters
+    assert calls == 1
+    assert int(projections.sum()) == total and int(projections.max()) == most
+    inner = np.maximum(packed.counts.astype(np.int64) - 2, 0)
+    assert (projections <= visits).all() and (visits >= inner).all()
+    if scale == 1.0:
+        edges = [0, 1, 2, 5, 10, 20, 50, 100, 200, 500, 1000]
+        hist, _ = np.histogram(projections, bins=edges + [1 << 30])
+        print(f"\n[curvature] fibers {projections.size}, projections {total}, visits {int(visits.sum())}, "
+              f"largest per fiber {most} projections / {int(visits.max())} visits")
+        for lo, hi, n in zip(edges, edges[1:] + [None], hist):
+            print(f"[curvature] projections {lo}..{'' if hi is None else hi - 1}: {n} fibers")
