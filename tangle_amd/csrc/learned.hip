@@ -135,6 +135,7 @@ __device__ __forceinline__ void closest_offset(const GridParams &g, float4 P1, f
 // from `pos`.
 __global__ __launch_bounds__(256) void k_features(const GridParams *grids, const float4 *cP, const float4 *cQ,
                                                   const uint4 *slotC, const uint32_t *ncount, const uint32_t *nbr,
+                                                  const uint32_t *nbr_off,
                                                   const uint32_t *ovf, const float4 *history, uint16_t *X,
                                                   float *scale, uint32_t block, uint32_t ovf_cap, uint32_t S) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(256) void k_features(const GridParams *grids, const
     const float4 P1 = cP[slot], Q1 = cQ[slot];
     const float r1 = __uint_as_float(me.z);
     const uint32_t count = ncount[slot];
-    const uint32_t *list = count <= block ? nbr + (size_t)slot * block
+    const uint32_t *list = count <= block ? nbr + nbr_off[slot]
                                           : (count <= ovf_cap ? ovf + (size_t)slot * ovf_cap : nullptr);
     const uint32_t n = list ? count : 0u;
     float gap[kFeatureNeighbors];
@@ -650,7 +651,7 @@ hipError_t launch_features(const WorldDev &w, const float4 *history, uint16_t *X
     e = launch_refresh_slots(w, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_features, dim3(nblocks(w.S, 256)), dim3(256), 0, s, w.grids, w.cP, w.cQ, w.slotC,
-                       w.ncount, w.nbr, w.ovf, history, X, scale, w.block, w.ovf_cap, w.S);
+                       w.ncount, w.nbr, w.nbr_off, w.ovf, history, X, scale, w.block, w.ovf_cap, w.S);
     return hipGetLastError();
 }
 

@@ -125,7 +125,10 @@ struct WorldDev {
     // cP = (first vertex x, y, z, radius), cQ = (second vertex x, y, z,
     // segment bits)
     float4 *cP, *cQ;
-    uint32_t *ncount, *nbr, *ovf;
+    // lists: per slot its count and, for a listed one, the first entry of
+    // its row in `nbr` (the rows of each 8 consecutive slots are packed in
+    // their 8 x block entries); an overflowed list's row in `ovf`
+    uint32_t *ncount, *nbr, *nbr_off, *ovf;
     float4 *corr0, *corr1;
     float *ratio;  // per vertex: curvature ratio at the last measure (0 until measured)
 };

@@ -12,9 +12,11 @@
 //     arc lengths from the same load.
 //   * Contact and list-build work is laid out in cell-list slot order, so the
 //     lanes of a wave hold spatially adjacent segments and share their
-//     candidate gathers in L2; each slot's list is one contiguous row
-//     (nbr[slot * block + k]) read by a group of lanes, one candidate each.
-//   * Each segment's list is kept sorted while it is built, in LDS, which
+//     candidate gathers in L2. The lists of each 8 consecutive slots are
+//     packed one after another in those slots' 8 x block entries
+//     (nbr[nbr_off[slot] + k]), so that short lists share lines; a group of
+//     lanes reads a list, one candidate each.
+//   * Each segment's list is sorted as it is stored, from LDS, which
 //     replaces the separate sort pass (neighbor.rs:366-408).
 //   * The Jacobi stretch/bend passes and the contact application read one
 //     position buffer and write the other (kernels.rs:1027-1163, 1417-1537):
@@ -451,14 +453,15 @@ __global__ __launch_bounds__(64) void k_build_lists(const DevState *st, const Sp
                                                     const float4 *slotB, const uint4 *slotC, const float4 *cP,
                                                     const float4 *cQ, const uint32_t *seg_cell,
                                                     const uint32_t *cell_counts, const uint32_t *cell_offsets,
-                                                    uint32_t *ncount, uint32_t *nbr, uint32_t *ovf, uint32_t block,
-                                                    uint32_t ovf_cap, uint32_t S) {
+                                                    uint32_t *ncount, uint32_t *nbr, uint32_t *nbr_off, uint32_t *ovf,
+                                                    uint32_t block, uint32_t ovf_cap, uint32_t S) {
     constexpr uint32_t G = kBuildLanes, W = kBuildGroups;
     constexpr uint32_t per_lane = (27 + G - 1) / G;  // stencil cells per lane
     __shared__ BuildSlot slots[W];
     __shared__ uint32_t cell_start[W][28];  // a slot's candidate start of each stencil cell
     __shared__ uint32_t cell_slot[W][27];   // first slot of each stencil cell
     __shared__ uint32_t counts[W];
+    __shared__ uint32_t specs[W];  // the slots' specimens (a slot's is its segment's)
     __shared__ uint2 queue[kBuildQueue];    // (group, candidate slot)
     __shared__ uint32_t keys[W][kBuildKeys];
     __shared__ uint32_t listed[W][kBuildKeys];
@@ -542,7 +545,10 @@ __global__ __launch_bounds__(64) void k_build_lists(const DevState *st, const Sp
                 slots[group] = me;
             }
         }
-        if (lane == 0) counts[group] = 0u;
+        if (lane == 0) {
+            counts[group] = 0u;
+            specs[group] = slot < S ? sstat[slot].w : 0xffffffffu;
+        }
         wave_sync();
         const float arc_p = A.w, arc_q = B.w;
         const float p1x = A.x, p1y = A.y, p1z = A.z, radius = __uint_as_float(M.z);
@@ -629,10 +635,22 @@ __global__ __launch_bounds__(64) void k_build_lists(const DevState *st, const Sp
             }
             wave_sync();
         }
-        // counts, then the listed rows in ascending segment order
+        // counts, then the listed rows in ascending segment order. The 8
+        // slots' rows are packed in their 8 x block entries: a slot's row
+        // follows the listed rows of the slots before it of the same
+        // specimen, from the entries of the first of them (a specimen's
+        // slots rebuild together, so its part of the entries is rewritten
+        // whole).
         const uint32_t c = counts[group];
-        if (live && lane == 0) ncount[slot] = c;
-        uint32_t *out = nbr + (size_t)slot * block;
+        uint32_t part = group, at = 0;
+        while (part > 0 && specs[part - 1] == specs[group]) --part;
+        for (uint32_t k = part; k < group; ++k) at += counts[k] <= block ? counts[k] : 0u;
+        const uint32_t off = (first + part) * block + at;
+        if (live && lane == 0) {
+            ncount[slot] = c;
+            nbr_off[slot] = off;
+        }
+        uint32_t *out = nbr + off;
         if (live && c <= block && c <= kBuildKeys) {
             // rank sort: segments are distinct, so every entry has its own rank
             for (uint32_t i = lane; i < c; i += G) {
@@ -903,7 +921,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                                                  const float4 *slotA, const float4 *slotB,
                                                  const uint4 *slotC, const uint32_t *seg_cell,
                                                  const uint32_t *cell_counts, const uint32_t *cell_offsets,
-                                                 const uint32_t *ncount, const uint32_t *nbr, const uint32_t *ovf,
+                                                 const uint32_t *ncount, const uint32_t *nbr,
+                                                 const uint32_t *nbr_off, const uint32_t *ovf,
                                                  float4 *corr0, float4 *corr1, uint32_t *red, float cf,
                                                  uint32_t aggregation, uint32_t block, uint32_t ovf_cap,
                                                  uint32_t S) {
@@ -924,18 +943,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     const uint32_t slot = xcd_block(blockIdx.x, gridDim.x) * groups + group;
     // The kernel is bound by chains of dependent gathers, so every load that
     // depends only on the slot is issued at once: the slot's records, its
-    // list count and the first entries of its listed row (entries past the
-    // count are never used as addresses).
+    // list count and offset, then the first entries of its listed row (read
+    // only as far as the packed rows of the slot's 8-slot group reach;
+    // entries past the count are never used as addresses).
     const uint32_t sl = slot < S ? slot : 0u;
     const uint4 sc = slotC[sl];
     const float4 P1 = cP[sl], Q1 = cQ[sl];
     const uint32_t listed_count = ncount[sl];
-    const uint32_t *row = nbr + (size_t)sl * block;
+    const uint32_t row_off = nbr_off[sl];
+    const uint32_t *row = nbr + row_off;
+    // entries left in the slot's 8-slot group (0 for a slot never built)
+    const uint32_t region_end = min((sl | 7u) + 1u, S) * block;
+    const uint32_t row_room = row_off < region_end ? region_end - row_off : 0u;
     uint32_t first[kContactRounds];
 #pragma unroll
     for (uint32_t r = 0; r < kContactRounds; ++r) {
         const uint32_t k = r * G + lane;
-        first[r] = k < block ? row[k] : 0u;
+        first[r] = k < block && k < row_room ? row[k] : 0u;
     }
     const bool in_list = slot < S && sc.x != kNoSegment;
     const uint32_t spec = in_list ? sc.w : 0u;
@@ -2585,7 +2609,8 @@ hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
         const uint32_t blocks = blocks_for(w.S, kBuildGroups);
         hipLaunchKernelGGL(k_build_lists, dim3(blocks < kBuildBlocks ? blocks : kBuildBlocks), dim3(64), 0, stream,
                            w.st, w.ss, w.grids, w.sstat, w.fspan, w.slotA, w.slotB, w.slotC, w.cP, w.cQ, w.seg_cell,
-                           w.cell_counts, w.cell_offsets, w.ncount, w.nbr, w.ovf, w.block, w.ovf_cap, w.S);
+                           w.cell_counts, w.cell_offsets, w.ncount, w.nbr, w.nbr_off, w.ovf, w.block, w.ovf_cap,
+                           w.S);
     }
     return hipGetLastError();
 }
@@ -2627,7 +2652,8 @@ hipError_t launch_contact(const WorldDev &w, const IterParams &p, hipStream_t st
     if (e != hipSuccess) return e;
 #define TGH_CONTACT_ARGS                                                                                     \
     w.st, w.ss, w.nspec, w.grids, w.red_slots, w.cP, w.cQ, w.slotA, w.slotB, w.slotC, w.seg_cell, w.cell_counts, w.cell_offsets, \
-        w.ncount, w.nbr, w.ovf, w.corr0, w.corr1, w.red, p.correction_fraction, p.aggregation, w.block, w.ovf_cap, w.S
+        w.ncount, w.nbr, w.nbr_off, w.ovf, w.corr0, w.corr1, w.red, p.correction_fraction, p.aggregation, w.block,  \
+        w.ovf_cap, w.S
     const dim3 grid(blocks_for(w.S, 256u / w.contact_lanes));
     switch (w.contact_lanes) {
         case 4: hipLaunchKernelGGL(k_contact<4>, grid, dim3(256), 0, stream, TGH_CONTACT_ARGS); break;

@@ -425,6 +425,7 @@ int upload_static(tgh_world *w) {
     HIP_TRY(hipMemset(d.corr1, 0, S * sizeof(float4)));
     HIP_TRY(hipMemset(d.ratio, 0, V * sizeof(float)));
     HIP_TRY(hipMemset(d.ncount, 0, S * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(d.nbr_off, 0, S * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.red, 0, (size_t)d.nspec * 3 * d.red_slots * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_counts, 0, d.C * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_offsets, 0, d.C * sizeof(uint32_t)));
@@ -577,6 +578,7 @@ int create_world_packed(std::vector<Packed> &&packed, const tgh_config *c, tgh_w
     if (e == hipSuccess) e = w->alloc(&d.cQ, S);
     if (e == hipSuccess) e = w->alloc(&d.ncount, S);
     if (e == hipSuccess) e = w->alloc(&d.nbr, std::max<size_t>(1, (size_t)d.block * S));
+    if (e == hipSuccess) e = w->alloc(&d.nbr_off, S);
     if (e == hipSuccess) e = w->alloc(&d.corr0, S);
     if (e == hipSuccess) e = w->alloc(&d.corr1, S);
     if (e == hipSuccess) e = w->alloc(&d.ratio, V);
@@ -879,8 +881,9 @@ int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t 
     if (!world || !counts || !lists) return set_error(TGH_ERR_INVALID, "null argument");
     const size_t S = world->d.S, B = world->d.block;
     if (list_capacity < S * B) return set_error(TGH_ERR_SIZE, "list buffer smaller than segment_count*neighbor_block");
-    std::vector<uint32_t> slot_seg(S), slot_count(S), cols(std::max<size_t>(1, S * B));  // rows by slot
+    std::vector<uint32_t> slot_seg(S), slot_count(S), slot_off(S), cols(std::max<size_t>(1, S * B));
     HIP_TRY(hipStreamSynchronize(world->stream));
+    HIP_TRY(hipMemcpy(slot_off.data(), world->d.nbr_off, S * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(slot_seg.data(), world->d.cell_segs, S * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(slot_count.data(), world->d.ncount, S * 4, hipMemcpyDeviceToHost));
     if (S * B) HIP_TRY(hipMemcpy(cols.data(), world->d.nbr, S * B * 4, hipMemcpyDeviceToHost));
@@ -894,8 +897,9 @@ int tgh_world_neighbor_lists(const tgh_world *world, uint32_t *counts, uint32_t 
         // overflowed list's row is never written (unspecified), so only
         // listed rows are mapped.
         if (slot_count[slot] > B) continue;
+        if ((size_t)slot_off[slot] + slot_count[slot] > S * B) continue;  // not built
         for (size_t k = 0; k < slot_count[slot]; ++k) {
-            const uint32_t other = cols[slot * B + k];
+            const uint32_t other = cols[slot_off[slot] + k];
             lists[(size_t)seg * B + k] = other < S ? slot_seg[other] : 0xffffffffu;
         }
     }
