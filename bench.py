@@ -130,7 +130,7 @@ def whole_iteration_roofline(S, V, mean_list, iterations_per_second):
 
 
 def gs_kernel_name(lanes: int) -> str:
-    return {4: "tgh::k_curvature_gs_group<4>", 8: "tgh::k_curvature_gs_lean<8>", 16: "tgh::k_curvature_gs_lean<16>",
+    return {4: "tgh::k_curvature_gs_group<4>", 8: "tgh::k_curvature_gs_seq", 16: "tgh::k_curvature_gs_lean<16>",
             32: "tgh::k_curvature_gs_lean<32>", 64: "tgh::k_curvature_gs_wave64"}.get(lanes,
                                                                                       "tgh::k_curvature_gs_lane")
 

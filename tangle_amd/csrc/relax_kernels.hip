@@ -1778,6 +1778,296 @@ __global__ __launch_bounds__(64) void k_curvature_gs_lean(DevState *st, SpecStat
     if (__ballot(flag) != 0 && lane == 0) st->nstate[0] = 1u;
 }
 
+// Curvature sweep with 8 lanes per fiber walking every triplet in sweep
+// order (no screen). On the felt's critical fibers about nine visits in ten
+// project, so a screen saves few exact evaluations and costs a loop
+// iteration of its own; the sequential walk instead keeps the triplet in
+// registers: a step's three vertices pass to the next step between the two
+// quads, and the one vertex new to the next triplet is read from LDS while
+// the step runs (no step before it changes that vertex).
+//
+// Lane roles as in k_curvature_gs_lean. The exact step is the sequential
+// projection's operations in its order, with the scalar part spread over
+// the quad's lanes: lane 3 takes the square root of 1 - cos^2 while lanes
+// 0..2 take acos's sqrt(z); lane 1 evaluates acos's Q(z) while the others
+// evaluate P(z) (one Horner chain with per-lane coefficients, where a zero
+// start and the constant term 1 reproduce Q's own evaluation exactly); and
+// one division gives P/Q on lane 0 and acos's low-order correction c on
+// lane 1. Each value is the one the sequential walk computes, bit for bit.
+//
+// LDS per fiber: positions (x, y, z, kmax (1 - margin)), and per vertex and
+// axis the clamp bounds (min(lo + e, hi - e), hi - e) of a walled axis or
+// (-inf, inf), with (weight, unused) in the fourth slot; wall reactions
+// when tracked.
+__host__ __device__ inline size_t curvature_seq_stride_floats(uint32_t max_vertices, uint32_t track_walls) {
+    const size_t n = (max_vertices + 1u) & ~1u;
+    return n * (4u + 8u + (track_walls ? 4u : 0u)) + 4u;  // + a sink slot
+}
+
+// lanes 0..3 of every 8 read lanes 4..7 and lanes 4..7 read 0..3 (bank
+// masks pick the direction, so no select)
+__device__ __forceinline__ float swap_quads(float x) {
+    int v = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x104, 0xF, 0x5, false);
+    v = __builtin_amdgcn_update_dpp(v, __float_as_int(x), 0x114, 0xF, 0xA, false);
+    return __int_as_float(v);
+}
+// the IEEE square root for x = +-0, +inf, NaN and x >= 2^-96 (where the
+// scaling of tiny inputs is not taken): the hardware root s, then s - 1 ulp
+// or s + 1 ulp where an exact fused residual shows the root is past it.
+// For x = 0 the residual at s - 1 ulp is NaN and at s + 1 ulp zero, so s
+// stays; for +inf both residuals are NaN.
+__device__ __forceinline__ float sqrt_rn_plain(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float down = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float r = __builtin_fmaf(-down, s, x) <= 0.f ? down : s;
+    return __builtin_fmaf(-up, s, x) > 0.f ? up : r;
+}
+
+// The IEEE division's own instruction sequence (reciprocal, one refinement,
+// two fused corrections, then the special-case fixup) without the operand
+// scaling of v_div_scale / v_div_fmas. That scaling is taken only when the
+// numerator is below 2^-103, the denominator or the quotient is subnormal,
+// or their exponents differ by 96 or more; otherwise the two sequences are
+// the same operations on the same values. The sweep's divisions stay clear of
+// those ranges whenever their result is used: position differences and
+// their numerators are 0 or above 2^-60, segment lengths above 1e-7, acos's
+// Q(z) near 1, its correction's denominator above 2^-13, and the projection
+// denominator between 1e-12 and 2^60. A zero numerator goes through the
+// fixup, which returns the correctly signed zero.
+__device__ __forceinline__ float div_rn_unscaled(float a, float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, r, 1.0f);
+    r = __builtin_fmaf(e, r, r);
+    float q = a * r;
+    float rem = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(rem, r, q);
+    rem = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(rem, r, q);
+    return __builtin_amdgcn_div_fixupf(q, b, a);
+}
+
+// Per-lane constants and LDS views of one fiber's group in the sequential
+// sweep.
+struct SeqLanes {
+    float4 *P;
+    float2 *BND;
+    float *Pf, *W, *sink;
+    bool second, qlane, walled;
+    uint32_t ql, k, o;
+    float hc4, hc3, hc2, hc1, hc0;
+};
+
+// One sweep direction of every group of the wave (DIR 0: forward, 1:
+// backward), steps in lockstep up to the wave's longest chain; a group whose
+// chain is shorter, or that is done, takes no step. Returns whether the
+// group's direction projected anything.
+template <int DIR>
+__device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32_t trip, uint32_t maxtrip,
+                                              float stiffness, uint32_t track_walls) {
+    const float pio2_hi = f32_from_bits(0x3fc90fdau), pio2_lo = f32_from_bits(0x33a22168u);
+    const float pi = f32_from_bits(0x40490fdau);
+    const float hc5 = f32_from_bits(0x3811ef08u);
+    const bool second = q.second, qlane = q.qlane;
+    const uint32_t k = q.k, o = q.o, ql = q.ql;
+    float4 *P = q.P;
+    float2 *BND = q.BND;
+    float *Pf = q.Pf;
+    uint32_t t = (on && DIR == 1) ? trip - 1u : 0u;
+    float a0 = Pf[4u * (t + o) + k], a1 = Pf[4u * (t + o + 1u) + k];
+    bool any = false;
+    for (uint32_t c = 0; c < maxtrip; ++c) {
+        const bool act = on && c < trip;
+        // static records of this triplet, and the vertex the next one adds
+        const float kmax = P[t + 1u].w;
+        const float fwp = BND[4u * t + 3u].x, fwc = BND[4u * (t + 1u) + 3u].x, fwn = BND[4u * (t + 2u) + 3u].x;
+        const float2 b0 = BND[4u * (t + o) + k], b1 = BND[4u * (t + o + 1u) + k];
+        const bool more = act && c + 1u < trip;
+        const uint32_t tn = more ? (DIR == 0 ? t + 1u : t - 1u) : t;
+        const float nextv = Pf[4u * (DIR == 0 ? tn + 2u : tn) + k];
+        // the sequential projection's operations, in its order
+        const float d = a1 - a0;
+        const float len = sqrt_rn_plain(quad_axis_sum(d * d));
+        const float dn = div_rn_unscaled(d, len);
+        const float dn_o = swap_quads(dn), len_o = swap_quads(len);
+        const float cosine = __builtin_amdgcn_fmed3f(quad_axis_sum(dn * dn_o), -1.f, 1.f);  // finite when used
+        const float sine_half = 0.25f * kmax * (len + len_o);
+        const float min_cos = 1.0f - 2.0f * sine_half * sine_half;
+        float violation = min_cos - cosine;
+        bool ok = act && kmax > 0.f && len > 1.0e-7f && len_o > 1.0e-7f && sine_half < 1.0f && violation > 0.f;
+        float w0 = a0, w1 = a1;  // this lane's two positions after the step
+        if (__ballot(ok) != 0) {
+            // (vn - cos un) / l1 in the first quad (gu), (un - cos vn) / l2 in
+            // the second (gv); gu - gv is exact on the first quad, which alone
+            // moves the center, and only its square is used on the second.
+            // None of it waits for the arc cosines below.
+            const float gq = div_rn_unscaled(dn_o - cosine * dn, len);
+            const float gq_o = swap_quads(gq);
+            const float gc = gq - gq_o;
+            const float sg = quad_axis_sum(gq * gq), sg_o = swap_quads(sg);
+            const float suu = second ? sg_o : sg, svv = second ? sg : sg_o;
+            const float den = fwp * suu + fwc * quad_axis_sum(gc * gc) + fwn * svv;
+            // acos of the turning cosine (first quad) and of the limit (second)
+            const float x = second ? min_cos : cosine;
+            const uint32_t ix = __float_as_uint(x) & 0x7fffffffu;
+            const bool negative = (__float_as_uint(x) >> 31) != 0u, small = ix < 0x3f000000u;
+            const float z = small ? x * x : (1.0f - fabsf(x)) * 0.5f;
+            const float sq = sqrt_rn_plain(ql == 3u ? 1.0f - cosine * cosine : z);
+            const float sine = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(sq), 0xFF, 0xF, 0xF, true));
+            const float s = quad_lane(sq, 0);
+            float h = (qlane ? 0.f : z) * hc5;
+            h = h + q.hc4;
+            h = z * h;
+            h = h + q.hc3;
+            h = z * h;
+            h = h + q.hc2;
+            h = z * h;
+            h = h + q.hc1;
+            h = z * h;
+            h = h + q.hc0;
+            const float p = z * h;
+            const float s_hi = __uint_as_float(__float_as_uint(s) & 0xfffff000u);
+            const float qq = quad_lane(h, 1);  // read by every lane (a cross-lane read outside any branch)
+            const float quo = div_rn_unscaled(qlane ? z - s_hi * s_hi : p, qlane ? s + s_hi : qq);
+            const float r = quad_lane(quo, 0), cc = quad_lane(quo, 1);
+            float near_zero = pio2_hi - (x - (pio2_lo - x * r));
+            float below = pi - 2.0f * (s + (r * s - pio2_lo));
+            float above = 2.0f * (s_hi + (r * s + cc));
+            // the three branches are selected, not branched to
+            __asm__ volatile("" : "+v"(near_zero), "+v"(below), "+v"(above));
+            float angle = small ? near_zero : (negative ? below : above);
+            angle = ix <= 0x32800000u ? pio2_hi + pio2_lo : angle;                        // |x| <= 2^-26
+            angle = ix == 0x3f800000u ? (negative ? pi + 2.0f * pio2_lo : 0.0f) : angle;  // |x| = 1
+            // turn - limit on the first quad; the second holds its negation
+            const float diff = angle - swap_quads(angle);
+            violation = sine > 1.0e-3f ? (second ? -diff : diff) * sine : violation;
+            ok = ok && violation > 0.f && den > 1.0e-12f;
+            const float scale = div_rn_unscaled(stiffness * violation, den);
+            // first quad: previous (- gu) and center (+ gc); second quad: next (+ gv)
+            const float f1 = second ? fwn : fwc;
+            const float m0 = a0 - fwp * scale * gq;
+            const float m1 = a1 + f1 * scale * (second ? gq : gc);
+            const float v0 = __builtin_amdgcn_fmed3f(m0, b0.x, b0.y);
+            const float v1 = __builtin_amdgcn_fmed3f(m1, b1.x, b1.y);
+            const bool moved0 = ok && !second && fwp > 0.f, moved1 = ok && f1 > 0.f;
+            if (track_walls) {
+                float *wr0 = (moved0 && q.walled && ql < 3u) ? &q.W[4u * (t + o) + k] : &q.sink[k];
+                float *wr1 = (moved1 && q.walled && ql < 3u) ? &q.W[4u * (t + o + 1u) + k] : &q.sink[k];
+                *wr0 = *wr0 + (m0 - v0);
+                *wr1 = *wr1 + (m1 - v1);
+            }
+            w0 = moved0 ? v0 : a0;
+            w1 = moved1 ? v1 : a1;
+        }
+        // the first quad holds the triplet's previous and center vertices,
+        // the second quad's w1 its next; each lane stores what it holds (the
+        // second quad's w1 twice)
+        if (act) {
+            Pf[4u * (second ? t + 2u : t) + k] = second ? w1 : w0;
+            Pf[4u * (t + o + 1u) + k] = w1;
+        }
+        // the next triplet's operands: forward (center, next | next, new),
+        // backward (new, previous | previous, center)
+        const float x0 = swap_quads(w0), x1 = swap_quads(w1);
+        if (DIR == 0) {
+            a0 = w1;
+            a1 = second ? nextv : x1;
+        } else {
+            a0 = second ? x0 : nextv;
+            a1 = second ? x1 : w0;
+        }
+        any = any || ok;
+        t = tn;
+        // one wave: its LDS accesses complete in order; keep the compiler's
+        lds_order();
+    }
+    return any;
+}
+
+__global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState *ss, const GridParams *grids,
+                                                         const float4 *in, float4 *pos, const float4 *refpos,
+                                                         const float4 *vA, const float4 *vB, const uint4 *fspan,
+                                                         float4 *wall, uint32_t F, uint32_t sweeps, float stiffness,
+                                                         float margin, uint32_t track_walls,
+                                                         uint32_t max_vertices) {
+    constexpr uint32_t L = 8, G = 8;
+    extern __shared__ float4 lds4[];
+    const uint32_t lane = threadIdx.x, group = lane / L, gl = lane % L;
+    const uint32_t nv = (max_vertices + 1u) & ~1u;
+    SeqLanes q;
+    q.P = (float4 *)((float *)lds4 + group * curvature_seq_stride_floats(max_vertices, track_walls));
+    q.BND = (float2 *)(q.P + nv);                          // [vertex][4]: axis bounds, (weight, -)
+    float4 *WR = (float4 *)(q.BND + 4u * nv);              // wall reactions (when tracked)
+    q.sink = (float *)(track_walls ? WR + nv : WR);        // absorbs stores that must not land
+    q.Pf = (float *)q.P;
+    q.W = (float *)WR;
+    float4 *P = q.P;
+    const uint32_t fiber = blockIdx.x * G + group;
+    uint4 span = make_uint4(0u, 0u, 0u, 0u);
+    if (fiber < F) span = fspan[fiber];
+    // a specimen that stopped, or an inactive fiber, keeps its positions
+    if (!running(ss, span.z) || (span.y && (__float_as_uint(vB[span.x].z) & kInactive))) span.y = 0u;
+    const uint32_t start = span.x, n = span.y;
+    const GridParams &g = grids[span.z];
+    const float half_skin = 0.5f * g.skin;
+    const float inf = __builtin_inff();
+    for (uint32_t i = gl; i < n; i += L) {
+        const float4 p = in[start + i];
+        const float4 a = vA[start + i];
+        P[i] = make_float4(p.x, p.y, p.z, a.x * (1.0f - margin));
+        const float e = a.w;
+#pragma unroll
+        for (uint32_t ax = 0; ax < 3; ++ax) {
+            const float lo = g.lower[ax] + e, hi = g.upper[ax] - e;
+            q.BND[4u * i + ax] = g.periodic[ax] ? make_float2(-inf, inf) : make_float2(fminf(lo, hi), hi);
+        }
+        q.BND[4u * i + 3u] = make_float2((__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f, 0.f);
+        if (track_walls) WR[i] = wall[start + i];
+    }
+    // lanes 0..2 hold the axes of the triplet's first segment (previous ->
+    // center), lanes 4..6 those of its second (center -> next); lanes 3 and 7
+    // repeat lane 0's and lane 4's vector values
+    q.second = (gl & 4u) != 0u;
+    q.ql = gl & 3u;
+    q.k = q.ql < 3u ? q.ql : 0u;
+    q.o = q.second ? 1u : 0u;
+    q.walled = !g.periodic[q.k];
+    // acos (fdlibm e_acosf.c): R(z) = P(z) / Q(z), P on lanes 0, 2, 3 and Q on
+    // lane 1 of each quad
+    q.qlane = q.ql == 1u;
+    q.hc4 = q.qlane ? f32_from_bits(0x3d9dc62eu) : f32_from_bits(0x3a4f7f04u);
+    q.hc3 = q.qlane ? f32_from_bits(0xbf303361u) : f32_from_bits(0xbd241146u);
+    q.hc2 = q.qlane ? f32_from_bits(0x4001572du) : f32_from_bits(0x3e4e0aa8u);
+    q.hc1 = q.qlane ? f32_from_bits(0xc019d139u) : f32_from_bits(0xbea6b090u);
+    q.hc0 = q.qlane ? 1.0f : f32_from_bits(0x3e2aaaabu);
+    __syncthreads();
+    const uint32_t trip = n >= 3 ? n - 2 : 0;  // triplet t has center t + 1
+    const uint32_t maxtrip = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_max_u32(trip));  // wave-uniform
+    // a sweep direction that projects nothing ends the group's cleanup: the
+    // directions after it would visit the same unchanged triplets
+    bool done = trip == 0;
+    for (uint32_t sweep = 0; sweep < sweeps; ++sweep) {
+        if (__ballot(!done) == 0) break;
+        done = done || !seq_direction<0>(q, !done, trip, maxtrip, stiffness, track_walls);
+        if (__ballot(!done) == 0) break;
+        done = done || !seq_direction<1>(q, !done, trip, maxtrip, stiffness, track_walls);
+    }
+    __syncthreads();
+    const float limit = half_skin * half_skin;
+    bool flag = false;
+    for (uint32_t i = gl; i < n; i += L) {
+        const uint32_t v = start + i;
+        const float4 old = in[v], qv = P[i];
+        pos[v] = make_float4(qv.x, qv.y, qv.z, old.w);
+        if (track_walls) wall[v] = make_float4(WR[i].x, WR[i].y, WR[i].z, 0.f);
+        const float4 r = refpos[v];
+        const float dx = qv.x - r.x, dy = qv.y - r.y, dz = qv.z - r.z;
+        if (dx * dx + dy * dy + dz * dz > limit) flag = true;
+    }
+    if (flag) ss[span.z].rebuild = 1u;
+    if (__ballot(flag) != 0 && lane == 0) st->nstate[0] = 1u;
+}
+
 // One fiber per wave: the same screening and exact evaluation as
 // k_curvature_gs_group, with every decision wave-uniform. After a screen
 // the wave's three axis lanes run the exact chain in a tight loop: the
@@ -2705,7 +2995,15 @@ hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t 
         p.curvature_margin, w.track_walls, w.max_fiber_vertices
         switch (w.gs_lanes) {
             case 4: hipLaunchKernelGGL(k_curvature_gs_group<4>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
-            case 8: hipLaunchKernelGGL(k_curvature_gs_lean<8>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
+            case 8: {
+                // the sequential walk when its LDS fits, else the screened one
+                const size_t seq = 8u * 4u * curvature_seq_stride_floats(w.max_fiber_vertices, w.track_walls);
+                if (seq <= 64u * 1024u)
+                    hipLaunchKernelGGL(k_curvature_gs_seq, grid, dim3(64), seq, stream, TGH_GS_ARGS);
+                else
+                    hipLaunchKernelGGL(k_curvature_gs_lean<8>, grid, dim3(64), lds, stream, TGH_GS_ARGS);
+                break;
+            }
             case 16: hipLaunchKernelGGL(k_curvature_gs_lean<16>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
             case 32: hipLaunchKernelGGL(k_curvature_gs_lean<32>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
             case 64: hipLaunchKernelGGL(k_curvature_gs_wave64, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;

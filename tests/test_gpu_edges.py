@@ -1,6 +1,7 @@
 """Shape edge cases of the HIP path against the oracle, through the C ABI:
 a fiber too long for the curvature sweep's LDS staging (the one-lane walk
-in global memory takes it), worlds of one-segment fibers (no internal
+in global memory takes it), one too long for the sequential 8-lane sweep
+(the screened 8-lane sweep takes it), worlds of one-segment fibers (no internal
 vertices: no curvature work at all), and a world whose fibers have every
 length from one to sixty-four segments."""
 
@@ -55,6 +56,20 @@ def test_fiber_longer_than_lds_takes_the_global_walk(hip_device):
     cfg = fx.config(force_full_iterations=1, max_step=0.002)
     lanes, pg, po = _run_both(packed, cfg, 2, 15)
     assert lanes == 0
+    assert np.array_equal(pg.view(np.uint32), po.view(np.uint32))
+
+
+def test_fiber_too_long_for_the_sequential_sweep_takes_the_screened_one(hip_device):
+    """A 200-segment coil (201 vertices: too many for the 8-lane sequential
+    sweep's 64 KiB of LDS, few enough for the screened 8-lane sweep) with
+    two short coils: positions and status match the oracle bit for bit."""
+    fibers = [_helix(200, 0.1, 0.01, 40, (0.5, 0.5, 0.3)),
+              _helix(30, 0.05, 0.02, 12, (0.2, 0.8, 0.5), 0.3),
+              _helix(12, 0.05, 0.02, 12, (0.8, 0.2, 0.5), 1.1)]
+    packed = fx.packed(fibers, 0.005, [1.0] * 3, [True] * 3, bend_radius=0.2)
+    cfg = fx.config(force_full_iterations=1, max_step=0.002)
+    lanes, pg, po = _run_both(packed, cfg, 2, 15)
+    assert lanes == 8
     assert np.array_equal(pg.view(np.uint32), po.view(np.uint32))
 
 

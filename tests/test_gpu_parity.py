@@ -122,9 +122,10 @@ def _walled_bimodal():
 
 @pytest.mark.parametrize("case", ["bimodal", "walled", "zigzag"])
 def test_curvature_launch_shapes_agree_bitwise(hip_device, case):
-    """The grouped curvature sweep (screen, then exact projection on three
-    axis lanes) at 4 to 64 lanes per fiber, and the sequential walk
-    with one lane per fiber, give the same positions bit for bit."""
+    """The curvature sweeps give the same positions bit for bit: the
+    sequential 8-lane walk (triplets carried in registers), the screened
+    sweeps (screen, then exact projection on three axis lanes) at 4, 16, 32
+    and 64 lanes per fiber, and the one-lane walk in global memory."""
     if case == "zigzag":
         packed = fx.zigzag()
         cfg = fx.config(penetration_tolerance=0.0, stretch_stiffness=0.0, bend_stiffness=0.0, force_full_iterations=1,
