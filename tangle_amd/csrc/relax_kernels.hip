@@ -1805,9 +1805,11 @@ __host__ __device__ inline size_t curvature_seq_stride_floats(uint32_t max_verti
 }
 
 // lanes 0..3 of every 8 read lanes 4..7 and lanes 4..7 read 0..3 (bank
-// masks pick the direction, so no select)
+// masks pick the direction, so no select): the first move writes lanes
+// 0..3 of each 8 and leaves the others undefined, the second writes those
+// others and keeps the first move's lanes
 __device__ __forceinline__ float swap_quads(float x) {
-    int v = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x104, 0xF, 0x5, false);
+    int v = __builtin_amdgcn_mov_dpp(__float_as_int(x), 0x104, 0xF, 0x5, false);
     v = __builtin_amdgcn_update_dpp(v, __float_as_int(x), 0x114, 0xF, 0xA, false);
     return __int_as_float(v);
 }
