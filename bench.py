@@ -572,7 +572,7 @@ def batched_throughput(args, fx, tg, _native):
     lib = _native.library()
     from tangle_amd.autotune import autotune
     t0 = time.perf_counter()
-    tuning = autotune(packed, cfg, iterations=32)
+    tuning = autotune(packed, cfg, iterations=32, cache=False)  # timed: the trials always run here
     tune_s = time.perf_counter() - t0
     default_tuning = {"cell_size_scale": cfg.cell_size_scale, "neighbor_skin_scale": cfg.neighbor_skin_scale}
     cfg = tuning.configure(cfg)
@@ -614,7 +614,8 @@ def batched_throughput(args, fx, tg, _native):
         "autotune": {"seconds": round(tune_s, 2), "default": default_tuning,
                      "chosen": {"cell_size_scale": tuning.cell_size_scale,
                                 "neighbor_skin_scale": tuning.neighbor_skin_scale,
-                                "contact_lanes": tuning.contact_lanes, "curvature_lanes": tuning.curvature_lanes},
+                                "contact_lanes": tuning.contact_lanes, "curvature_lanes": tuning.curvature_lanes,
+                                "iterations_per_rebuild": round(tuning.rebuild_interval, 2)},
                      "trial_ms": {k: {str(kk): round(vv, 3) for kk, vv in v.items()}
                                   for k, v in tuning.trials.items()}},
         "group_ms_per_iteration": per_iter,

@@ -329,8 +329,10 @@ int tgh_world_set_fiber_maximum_curvatures(tgh_world *world, const float *maximu
  * for bit; tgh_world_set_tuning validates the settings and drops captured
  * graphs so the next batch is captured with the new shape.
  *   curvature_lanes: lanes that sweep one fiber in the curvature Gauss-Seidel
- *     pass: 4, 8, 16, 32 or 64 (fibers staged in LDS), or 0 for one lane per
- *     fiber walking it in global memory.
+ *     pass: 4, 8, 16, 32 or 64 (fibers staged in LDS; at 8 the sweep walks
+ *     every triplet with the triplet carried in registers when the longest
+ *     fiber's records fit in 64 KiB of LDS, and screens ahead otherwise), or
+ *     0 for one lane per fiber walking it in global memory.
  *   contact_lanes: lanes that evaluate one segment's neighbour list in the
  *     contact pass, one candidate each: 4, 8, 16 or 32. */
 typedef struct tgh_tuning {
@@ -340,6 +342,17 @@ typedef struct tgh_tuning {
 } tgh_tuning;
 int tgh_world_get_tuning(const tgh_world *world, tgh_tuning *out);
 int tgh_world_set_tuning(tgh_world *world, const tgh_tuning *tuning);
+
+/* The current HIP device (no counterpart in the reference): its name, its
+ * architecture (gcnArchName) and compute-unit count, so that launch-shape
+ * choices measured on one device are not reused on another. */
+typedef struct tgh_device_info {
+    char name[256];
+    char arch[64];
+    uint32_t compute_units;
+    uint32_t reserved[7];
+} tgh_device_info;
+int tgh_device_info_get(tgh_device_info *out);
 void tgh_world_destroy(tgh_world *world);
 const char *tgh_last_error(void);
 

@@ -143,6 +143,15 @@ class Tuning(C.Structure):
     ]
 
 
+class DeviceInfo(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 256),
+        ("arch", C.c_char * 64),
+        ("compute_units", C.c_uint32),
+        ("reserved", C.c_uint32 * 7),
+    ]
+
+
 class CheckpointStruct(C.Structure):
     _fields_ = [
         ("total_iterations", C.c_uint64),
@@ -369,6 +378,7 @@ SIGNATURES = {
     "tgh_world_formation_target_error": (C.c_int, [C.c_void_p, _f32p, _f32p, C.POINTER(C.c_uint32),
                                                    C.POINTER(C.c_uint32)]),
     "tgh_world_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "tgh_device_info_get": (C.c_int, [C.POINTER(DeviceInfo)]),
     "tgh_world_checkpoint_sizes": (C.c_int, [C.c_void_p, C.POINTER(CheckpointStruct)]),
     "tgh_world_cell_bounds": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "tgh_world_compact_cell": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32]),
@@ -434,6 +444,14 @@ class NativeError(Exception):
     def __init__(self, code: int, message: str):
         super().__init__(message)
         self.code = code
+
+
+def device_info() -> dict:
+    """The current HIP device: name, architecture and compute units."""
+    info = DeviceInfo()
+    check(library().tgh_device_info_get(C.byref(info)))
+    return {"name": info.name.decode(errors="replace"), "arch": info.arch.decode(errors="replace"),
+            "compute_units": int(info.compute_units)}
 
 
 def check(code: int) -> None:

@@ -8,17 +8,43 @@ candidates a list holds and how often lists are rebuilt (lists hold every
 pair within the skin and are summed in ascending order). It times the
 candidates on a scratch world built from the same specimens, from their
 generated state, and returns the fastest combination.
+
+What it picks:
+- contact lanes and curvature lanes, from HIP-event group times;
+- the cell size scale and the Verlet skin scale, from the wall time of a
+  fixed number of iterations with rebuilds included. The skin sets the
+  rebuild cadence (a list is rebuilt once a vertex has moved half the skin),
+  so each grid candidate's measured iterations per rebuild are recorded and
+  the chosen one is returned as `rebuild_interval`.
+
+Results are kept in a JSON cache keyed by the workload class (specimen
+count, fiber and vertex counts, radii, bend limits, cell extents and
+periodicity, mean segment length to three digits), the relaxation settings
+that shape the loop, the candidate sets, the library build and the device,
+so a second run on a workload of the same class (a new seed of the same
+recipe, for instance) skips the trials. The cache lives at
+$TANGLE_AMD_AUTOTUNE_CACHE, else ~/.cache/tangle_amd/autotune.json.
 """
 
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
+import json
+import os
 import time
 from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
 
 from . import _native
 
 GROUPS = ["rebuild", "contact", "measure_assess", "vertex_passes", "curvature_gs"]
+CACHE_VERSION = 1
+# config fields that shape the per-iteration work the trials time
+_CONFIG_KEYS = ("neighbor_capacity", "max_step", "constraint_iterations", "curvature_cleanup_sweeps",
+                "contact_aggregation", "cell_size_scale", "neighbor_skin_scale", "maximum_neighbor_list_bytes")
 
 
 @dataclass
@@ -27,7 +53,9 @@ class Tuning:
     neighbor_skin_scale: float
     contact_lanes: int
     curvature_lanes: int
+    rebuild_interval: float = 0.0  # measured iterations per list rebuild at the chosen grid
     trials: dict = field(default_factory=dict)
+    cached: bool = False
 
     def apply(self, world: _native.World) -> None:
         world.set_tuning(contact_lanes=self.contact_lanes, curvature_lanes=self.curvature_lanes)
@@ -37,6 +65,105 @@ class Tuning:
         c.cell_size_scale = self.cell_size_scale
         c.neighbor_skin_scale = self.neighbor_skin_scale
         return c
+
+    def to_json(self) -> dict:
+        return {"cell_size_scale": self.cell_size_scale, "neighbor_skin_scale": self.neighbor_skin_scale,
+                "contact_lanes": self.contact_lanes, "curvature_lanes": self.curvature_lanes,
+                "rebuild_interval": self.rebuild_interval,
+                "trials": {k: {str(kk): vv for kk, vv in v.items()} for k, v in self.trials.items()}}
+
+    @classmethod
+    def from_json(cls, d: dict, cached: bool = False) -> "Tuning":
+        return cls(cell_size_scale=float(d["cell_size_scale"]), neighbor_skin_scale=float(d["neighbor_skin_scale"]),
+                   contact_lanes=int(d["contact_lanes"]), curvature_lanes=int(d["curvature_lanes"]),
+                   rebuild_interval=float(d.get("rebuild_interval", 0.0)), trials=dict(d.get("trials", {})),
+                   cached=cached)
+
+
+def default_cache_path() -> Path:
+    env = os.environ.get("TANGLE_AMD_AUTOTUNE_CACHE")
+    if env:
+        return Path(env)
+    return Path.home() / ".cache" / "tangle_amd" / "autotune.json"
+
+
+def _specimen_signature(p: _native.PackedAssembly) -> dict:
+    counts = np.asarray(p.counts, dtype=np.int64)
+    placed = np.asarray(p.placed, dtype=np.float64).reshape(-1, 3)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    seg = []
+    for s, n in zip(starts, counts):
+        if n > 1:
+            seg.append(np.linalg.norm(np.diff(placed[s:s + n], axis=0), axis=1))
+    mean_seg = float(np.mean(np.concatenate(seg))) if seg else 0.0
+    bends = [] if p.min_bend_radius is None else sorted(set(float(b) for b in p.min_bend_radius))
+    return {
+        "fibers": int(counts.size), "vertices": int(counts.sum()),
+        "radii": sorted(set(float(r) for r in p.radius)), "bend_radii": bends,
+        "cell": [float(p.struct.cell_lengths[k]) for k in range(3)],
+        "periodic": [int(p.struct.cell_periodic[k]) for k in range(3)],
+        "mean_segment": float(f"{mean_seg:.3g}"),
+    }
+
+
+def _library_identity() -> str:
+    path = Path(_native.LIBRARY_PATH)
+    if not path.exists():
+        return "unknown"
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()[:16]
+
+
+def workload_key(specimens, config: _native.Config, candidates: dict, device: dict, library: str) -> str:
+    """The cache key: a hash of the workload class, the settings that shape
+    the loop, the candidate sets, the library build and the device."""
+    payload = {
+        "version": CACHE_VERSION,
+        "specimens": [_specimen_signature(p) for p in specimens],
+        "config": {k: float(getattr(config, k)) for k in _CONFIG_KEYS},
+        "candidates": {k: [list(map(float, c)) if isinstance(c, tuple) else c for c in v]
+                       for k, v in sorted(candidates.items())},
+        "device": device, "library": library,
+    }
+    text = json.dumps(payload, sort_keys=True, separators=(",", ":"))
+    return hashlib.sha256(text.encode()).hexdigest()
+
+
+def load_cached(path: Path, key: str) -> Tuning | None:
+    try:
+        data = json.loads(Path(path).read_text())
+    except (OSError, ValueError):
+        return None
+    if not isinstance(data, dict) or data.get("version") != CACHE_VERSION:
+        return None
+    entry = data.get("entries", {}).get(key)
+    if not isinstance(entry, dict):
+        return None
+    try:
+        return Tuning.from_json(entry, cached=True)
+    except (KeyError, TypeError, ValueError):
+        return None
+
+
+def store_cached(path: Path, key: str, tuning: Tuning) -> None:
+    """Adds or replaces one entry, writing the file atomically (a reader
+    sees the old file or the new one, never a partial one)."""
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    try:
+        data = json.loads(path.read_text())
+        if not isinstance(data, dict) or data.get("version") != CACHE_VERSION:
+            data = {}
+    except (OSError, ValueError):
+        data = {}
+    data["version"] = CACHE_VERSION
+    data.setdefault("entries", {})[key] = tuning.to_json()
+    tmp = path.with_name(path.name + f".tmp{os.getpid()}")
+    tmp.write_text(json.dumps(data, indent=1, sort_keys=True))
+    os.replace(tmp, path)
 
 
 def _world(specimens, config):
@@ -54,15 +181,31 @@ def _group_ms(world, config, iterations):
 
 
 def autotune(specimens, config: _native.Config, *, iterations: int = 32, contact_lanes=(4, 8, 16, 32),
-             curvature_lanes=(4, 8, 16, 32, 64), skin_scales=(1.0, 2.0, 3.0), cell_scales=(1.0,)) -> Tuning:
+             curvature_lanes=(4, 8, 16, 32, 64), skin_scales=(1.0, 2.0, 3.0), cell_scales=(1.0, 1.25, 1.5),
+             cache: str | os.PathLike | bool | None = None) -> Tuning:
     """Picks contact and curvature lanes from HIP-event group times, then
-    cell size scale and skin scale from the wall time of `iterations`
+    the cell size scale and skin scale from the wall time of `iterations`
     iterations (rebuilds included), each candidate starting from the
-    generated state of `specimens` (PackedAssembly objects)."""
+    generated state of `specimens` (PackedAssembly objects).
+
+    `cache`: None uses the default cache file, a path uses that file, False
+    disables the cache. A hit returns the stored choice with cached=True."""
     specimens = list(specimens)
+    if not specimens:
+        raise ValueError("autotune needs at least one specimen")
     cfg = _native.Config.from_buffer_copy(config)
     cfg.force_full_iterations = 1
-    trials = {"contact": {}, "curvature": {}, "grid": {}}
+    candidates = {"iterations": [int(iterations)], "contact_lanes": list(contact_lanes),
+                  "curvature_lanes": list(curvature_lanes), "skin_scales": list(map(float, skin_scales)),
+                  "cell_scales": list(map(float, cell_scales))}
+    cache_path = None if cache is False else (default_cache_path() if cache in (None, True) else Path(cache))
+    key = None
+    if cache_path is not None:
+        key = workload_key(specimens, cfg, candidates, _native.device_info(), _library_identity())
+        hit = load_cached(cache_path, key)
+        if hit is not None:
+            return hit
+    trials = {"contact": {}, "curvature": {}, "grid": {}, "rebuilds": {}}
     lib = _native.library()
     with _world(specimens, cfg) as world:
         initial = world.download_positions()
@@ -90,9 +233,15 @@ def autotune(specimens, config: _native.Config, *, iterations: int = 32, contact
                 world.set_tuning(contact_lanes=best_contact, curvature_lanes=best_curvature)
                 world.run_batch(c, iterations)  # captures the batch graph outside the timing
                 world.upload_positions(initial)
+                before = int(world.info().neighbor_list_rebuilds)
                 t0 = time.perf_counter()
                 world.run_batch(c, iterations)
                 trials["grid"][(skin, cell)] = 1000.0 * (time.perf_counter() - t0)
+                trials["rebuilds"][(skin, cell)] = int(world.info().neighbor_list_rebuilds) - before
     skin, cell = min(trials["grid"], key=trials["grid"].get)
-    return Tuning(cell_size_scale=cell, neighbor_skin_scale=skin, contact_lanes=best_contact,
-                  curvature_lanes=best_curvature, trials=trials)
+    rebuilds = trials["rebuilds"][(skin, cell)]
+    tuning = Tuning(cell_size_scale=cell, neighbor_skin_scale=skin, contact_lanes=best_contact,
+                    curvature_lanes=best_curvature, rebuild_interval=iterations / max(1, rebuilds), trials=trials)
+    if cache_path is not None:
+        store_cached(cache_path, key, tuning)
+    return tuning

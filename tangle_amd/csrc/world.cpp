@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -1448,6 +1449,20 @@ int tgh_world_set_fiber_maximum_curvatures(tgh_world *world, const float *maximu
         for (uint32_t lf = 0; lf < p.F; ++lf, ++f)
             for (uint32_t l = 1; l + 1 < p.fcount[lf]; ++l) p.kmax[p.fstart[lf] + l] = maximum_curvatures[f];
     }
+    return TGH_OK;
+}
+
+int tgh_device_info_get(tgh_device_info *out) {
+    if (!out) return set_error(TGH_ERR_INVALID, "null argument");
+    *out = tgh_device_info{};
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    hipDeviceProp_t prop{};
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return set_error(TGH_ERR_DEVICE, hipGetErrorString(e));
+    std::snprintf(out->name, sizeof out->name, "%s", prop.name);
+    std::snprintf(out->arch, sizeof out->arch, "%s", prop.gcnArchName);
+    out->compute_units = (uint32_t)prop.multiProcessorCount;
     return TGH_OK;
 }
 

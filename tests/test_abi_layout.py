@@ -30,6 +30,7 @@ STRUCTS = {
     "tgh_batch_status": _native.BatchStatus,
     "tgh_world_info": _native.WorldInfo,
     "tgh_tuning": _native.Tuning,
+    "tgh_device_info": _native.DeviceInfo,
     "tgh_checkpoint": _native.CheckpointStruct,
     "tgh_compaction_metrics": _native.CompactionMetricsStruct,
     "tgh_contact_candidate": _native.ContactCandidateStruct,
