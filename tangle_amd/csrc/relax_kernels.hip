@@ -1797,8 +1797,8 @@ __global__ __launch_bounds__(64) void k_curvature_gs_lean(DevState *st, SpecStat
 //
 // LDS per fiber: positions (x, y, z, kmax (1 - margin)), and per vertex and
 // axis the clamp bounds (min(lo + e, hi - e), hi - e) of a walled axis or
-// (-inf, inf), with (weight, unused) in the fourth slot; wall reactions
-// when tracked.
+// (-inf, inf), with (weight, 0.25 kmax (1 - margin)) in the fourth slot;
+// wall reactions when tracked.
 __host__ __device__ inline size_t curvature_seq_stride_floats(uint32_t max_vertices, uint32_t track_walls) {
     const size_t n = (max_vertices + 1u) & ~1u;
     return n * (4u + 8u + (track_walls ? 4u : 0u)) + 4u;  // + a sink slot
@@ -1837,10 +1837,21 @@ __device__ __forceinline__ float sqrt_rn_plain(float x) {
 // Q(z) near 1, its correction's denominator above 2^-13, and the projection
 // denominator between 1e-12 and 2^60. A zero numerator goes through the
 // fixup, which returns the correctly signed zero.
+__device__ __forceinline__ float recip_refined(float b) {  // the sequence's refined reciprocal
+    const float r = __builtin_amdgcn_rcpf(b);
+    return __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+}
+// the sequence after its reciprocal: divisions by one denominator share it
+__device__ __forceinline__ float div_rn_unscaled_with(float a, float b, float r) {
+    float q = a * r;
+    float rem = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(rem, r, q);
+    rem = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(rem, r, q);
+    return __builtin_amdgcn_div_fixupf(q, b, a);
+}
 __device__ __forceinline__ float div_rn_unscaled(float a, float b) {
-    float r = __builtin_amdgcn_rcpf(b);
-    const float e = __builtin_fmaf(-b, r, 1.0f);
-    r = __builtin_fmaf(e, r, r);
+    float r = recip_refined(b);
     float q = a * r;
     float rem = __builtin_fmaf(-b, q, a);
     q = __builtin_fmaf(rem, r, q);
@@ -1882,7 +1893,8 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
         const bool act = on && c < trip;
         // static records of this triplet, and the vertex the next one adds
         const float kmax = P[t + 1u].w;
-        const float fwp = BND[4u * t + 3u].x, fwc = BND[4u * (t + 1u) + 3u].x, fwn = BND[4u * (t + 2u) + 3u].x;
+        const float2 wc = BND[4u * (t + 1u) + 3u];  // (weight, 0.25 kmax) of the center
+        const float fwp = BND[4u * t + 3u].x, fwc = wc.x, fwn = BND[4u * (t + 2u) + 3u].x, qkmax = wc.y;
         const float2 b0 = BND[4u * (t + o) + k], b1 = BND[4u * (t + o + 1u) + k];
         const bool more = act && c + 1u < trip;
         const uint32_t tn = more ? (DIR == 0 ? t + 1u : t - 1u) : t;
@@ -1890,10 +1902,11 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
         // the sequential projection's operations, in its order
         const float d = a1 - a0;
         const float len = sqrt_rn_plain(quad_axis_sum(d * d));
-        const float dn = div_rn_unscaled(d, len);
+        const float rlen = recip_refined(len);  // shared by d / len and the gradient's / len
+        const float dn = div_rn_unscaled_with(d, len, rlen);
         const float dn_o = swap_quads(dn), len_o = swap_quads(len);
         const float cosine = __builtin_amdgcn_fmed3f(quad_axis_sum(dn * dn_o), -1.f, 1.f);  // finite when used
-        const float sine_half = 0.25f * kmax * (len + len_o);
+        const float sine_half = qkmax * (len + len_o);  // (0.25 kmax) (l1 + l2)
         const float min_cos = 1.0f - 2.0f * sine_half * sine_half;
         float violation = min_cos - cosine;
         bool ok = act && kmax > 0.f && len > 1.0e-7f && len_o > 1.0e-7f && sine_half < 1.0f && violation > 0.f;
@@ -1903,7 +1916,7 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
             // the second (gv); gu - gv is exact on the first quad, which alone
             // moves the center, and only its square is used on the second.
             // None of it waits for the arc cosines below.
-            const float gq = div_rn_unscaled(dn_o - cosine * dn, len);
+            const float gq = div_rn_unscaled_with(dn_o - cosine * dn, len, rlen);
             const float gq_o = swap_quads(gq);
             const float gc = gq - gq_o;
             const float sg = quad_axis_sum(gq * gq), sg_o = swap_quads(sg);
@@ -2023,7 +2036,8 @@ __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState
             const float lo = g.lower[ax] + e, hi = g.upper[ax] - e;
             q.BND[4u * i + ax] = g.periodic[ax] ? make_float2(-inf, inf) : make_float2(fminf(lo, hi), hi);
         }
-        q.BND[4u * i + 3u] = make_float2((__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f, 0.f);
+        q.BND[4u * i + 3u] = make_float2((__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f,
+                                         0.25f * (a.x * (1.0f - margin)));
         if (track_walls) WR[i] = wall[start + i];
     }
     // lanes 0..2 hold the axes of the triplet's first segment (previous ->
