@@ -759,6 +759,17 @@ class RunResult:
                 f"max_penetration={self.max_penetration:.3e}, max_curvature_ratio={self.max_curvature_ratio:.6f})")
 
 
+def _status_json(st) -> dict:
+    return {name: (float(v) if isinstance(v, float) else int(v)) for name, v in st.as_dict().items()}
+
+
+def _status_from_json(d: dict) -> "_native.BatchStatus":
+    st = _native.BatchStatus()
+    for name, _ in st._fields_:
+        setattr(st, name, d[name])
+    return st
+
+
 class _RunState:
     """Batch controller of one Recipe.run: sizes each device batch, layers
     stage settings onto it and applies the stage's gate (plugin.rs:1041-1181,
@@ -794,6 +805,10 @@ class _RunState:
         self.checkpoint_interval = 0
         self.checkpoint_hook = None
         self.last_saved = None
+        # a compaction in progress ({"index", "a"}: its controller state) for
+        # checkpoints, and one restored from a checkpoint for its operation
+        self.compaction_inflight = None
+        self.resume_compaction = None
 
     def _propose(self, world, config, room):
         """Learned warm start: every `interval` iterations of the first
@@ -882,6 +897,46 @@ class _RunState:
             return
         self.checkpoint_hook(failure)
         self.last_saved = self.total
+
+    def compaction_state(self):
+        """The in-flight compaction's controller state as JSON-ready data and
+        its accepted device state (None when no compaction is running):
+        (state, accepted checkpoint or None)."""
+        inflight = self.compaction_inflight
+        if inflight is None or inflight["index"] != self.op_index:
+            return None, None
+        a = inflight["a"]
+        m = a["metrics"]
+        st = a["accepted_status"]
+        state = {
+            "index": inflight["index"], "next": float(a["next"]), "started": a["started"],
+            "windows": int(a["windows"]), "steps": int(a["steps"]), "last": float(a["last"]),
+            "previous": [float(v) for v in a["previous"]], "work": [float(w) for w in a["work"]],
+            "baseline": bool(a["baseline"]),
+            "metrics": None if m is None else {
+                "pressure": [float(v) for v in m.pressure], "contact_energy": float(m.contact_energy),
+                "stretch_energy": float(m.stretch_energy), "bending_energy": float(m.bending_energy),
+                "total_energy": float(m.total_energy),
+                "face_pressures": [[float(v) for v in pair] for pair in m.face_pressures]},
+            "accepted_status": None if st is None else _status_json(st),
+        }
+        return state, a["accepted"]
+
+    @staticmethod
+    def compaction_from_state(state: dict, accepted) -> dict:
+        """The controller state `compaction_state` saved, rebuilt."""
+        f32 = np.float32
+        m = state["metrics"]
+        metrics = None if m is None else _native.CompactionMetrics(
+            tuple(m["pressure"]), m["contact_energy"], m["stretch_energy"], m["bending_energy"], m["total_energy"],
+            tuple(tuple(pair) for pair in m["face_pressures"]))
+        a = dict(next=f32(state["next"]), started=state["started"], windows=int(state["windows"]),
+                 steps=int(state["steps"]), last=f32(state["last"]), previous=list(state["previous"]),
+                 work=[f32(w) for w in state["work"]], metrics=metrics, baseline=bool(state["baseline"]),
+                 accepted=accepted,
+                 accepted_status=None if state["accepted_status"] is None else _status_from_json(
+                     state["accepted_status"]))
+        return {"index": int(state["index"]), "a": a}
 
     def _proposal_window(self, world, config, batch, room):
         """Proposes when one is due and fits in `room` iterations, and
@@ -1057,8 +1112,10 @@ class _RunState:
         tangle_generate/src/recipe/compaction_control.rs:5-432): an admissible
         baseline, then compact-relax-measure steps with adaptive strain,
         rollback to the last accepted state when a window does not relax, and
-        a stop at the target, a guard or a jam."""
-        from . import compaction as cp
+        a stop at the target, a guard or a jam. A checkpoint taken inside the
+        operation holds the controller's state (`compaction_state`), and a
+        resume continues the operation from it, as the reference's
+        checkpoints carry the recipe's compaction state."""
         f32 = np.float32
         world = ctx["world"]
         config = _native.Config.from_buffer_copy(self.config)
@@ -1069,6 +1126,28 @@ class _RunState:
                                               spec.bending_energy_stiffness)
         cf = float(config.correction_fraction)
         state = ctx["state"]
+        resumed = self.resume_compaction if (self.resume_compaction or {}).get("index") == index else None
+        self.resume_compaction = None
+        if resumed is not None:
+            # the controller state a checkpoint took inside this operation
+            a = resumed["a"]
+        else:
+            metrics = world.compaction_metrics(cf, model)
+            self.downloaded += 13 * 4
+            a = dict(next=f32(spec.initial_log_strain), started=self.total, windows=0, steps=0, last=f32(0.0),
+                     previous=list(state["cell"][1]), work=[f32(0.0)] * 3, metrics=metrics, baseline=True,
+                     accepted=None, accepted_status=None)
+        self.compaction_inflight = {"index": index, "a": a}
+        try:
+            self._compact_loop(ctx, index, description, spec, config, model, cf, a, resumed is not None)
+        finally:
+            self.compaction_inflight = None
+
+    def _compact_loop(self, ctx, index, description, spec, config, model, cf, a, resumed):
+        from . import compaction as cp
+        f32 = np.float32
+        world = ctx["world"]
+        state = ctx["state"]
         active_step = state["active_step"]
         fibers = ctx["fibers"]
         formed = [f for f in fibers if active_step is None or f.step <= active_step]
@@ -1078,12 +1157,6 @@ class _RunState:
 
         def lengths_now():
             return list(state["cell"][1])
-
-        metrics = world.compaction_metrics(cf, model)
-        self.downloaded += 13 * 4
-        a = dict(next=f32(spec.initial_log_strain), started=self.total, windows=0, steps=0, last=f32(0.0),
-                 previous=lengths_now(), work=[f32(0.0)] * 3, metrics=metrics, baseline=True,
-                 accepted=None, accepted_status=None)
 
         def finish(reason):
             lengths = lengths_now()
@@ -1145,9 +1218,11 @@ class _RunState:
             self.status_current = True
             return new_world
 
-        # an unconstrained admissible baseline first
-        self.converged = False
-        self._hold(world, config, index, description, R)
+        # an unconstrained admissible baseline first (a resumed operation
+        # continues where its checkpoint left it)
+        if not resumed:
+            self.converged = False
+            self._hold(world, config, index, description, R)
         while True:
             if a["started"] is not None:
                 elapsed = self.total - a["started"]
@@ -1617,7 +1692,7 @@ class Recipe:
         start_index, first_begin, restored = 0, None, None
         if checkpoint is not None and checkpoint.resume:
             path, case_id = checkpoint.source
-            restored, progress = ckpt.load(path, case_id)
+            restored, progress, extra_devices = ckpt.load_with_extras(path, case_id)
             if progress.get("fingerprint") != fingerprint:
                 raise ckpt.CheckpointError("the checkpoint was written by another recipe (its fibers differ)")
             run.total = restored.total_iterations
@@ -1652,6 +1727,15 @@ class Recipe:
                     run.junction_captures = list(progress.get("junction_captures", []))
                 if progress["layer_targets"] is not None:
                     state["targets"] = [np.float32(t) for t in progress["layer_targets"]]
+                run.compactions = list(progress.get("compactions", []))
+                run.compaction_steps = list(progress.get("compaction_steps", []))
+                if progress.get("compaction_state") is not None:
+                    # the compaction's loop continues from the batch status it last saw
+                    run.resume_compaction = run.compaction_from_state(progress["compaction_state"],
+                                                                      extra_devices.get("accepted"))
+                    run.status = _status_from_json(progress["status"])
+                    run.status_current = bool(progress.get("status_current", False))
+                    run.converged = bool(progress.get("converged", False))
         world = (_native.World(packed, config) if restored is None
                  else _native.World.restore(packed, config, restored))
         holder = {"world": world}
@@ -1678,8 +1762,18 @@ class Recipe:
                         "junctions": [[j.id, j.law, j.parameter_set, [list(a) for a in j.anchors]]
                                       for j in run.junctions],
                         "junction_captures": list(run.junction_captures),
+                        "compactions": list(run.compactions), "compaction_steps": list(run.compaction_steps),
+                        "status": None if run.status is None else _status_json(run.status),
+                        "status_current": bool(run.status_current), "converged": bool(run.converged),
                         "failure": failure}
-            nbytes = ckpt.save(checkpoint.path, checkpoint.case_id, holder["world"].checkpoint(), progress)
+            compaction, accepted = run.compaction_state()
+            extras = {}
+            if compaction is not None:
+                progress["compaction_state"] = compaction
+                if accepted is not None:
+                    extras["accepted"] = accepted
+            nbytes = ckpt.save(checkpoint.path, checkpoint.case_id, holder["world"].checkpoint(), progress,
+                               extras=extras)
             report.update(saves=report["saves"] + 1, last_iteration=run.total, last_bytes=nbytes)
 
         if checkpoint is not None:

@@ -7,7 +7,9 @@ reactions, the lifetime iteration count, the held formation targets and the
 cell, which compaction may have changed since creation) and
 the recipe's progress (the operation in progress and where it started, the
 held layer targets, the events so far). Scratch broad-phase state is not
-kept; the first batch after a restore rebuilds it.
+kept; the first batch after a restore rebuilds it. Further named device
+states may ride along (a compaction in progress keeps its last accepted
+state this way); their arrays carry the name as a prefix.
 
 File layout (little-endian), written to a temporary file and renamed over
 the target so a reader never sees a partial checkpoint:
@@ -90,10 +92,7 @@ def _arrays(device: DeviceCheckpoint):
     return out
 
 
-def save(path, case_id: str, device: DeviceCheckpoint, recipe: dict) -> int:
-    """Writes one checkpoint atomically; returns its size in bytes."""
-    path = Path(path)
-    arrays = _arrays(device)
+def _scalars(device: DeviceCheckpoint) -> dict:
     scalars = {"total_iterations": int(device.total_iterations)}
     if device.cell_lower is not None:
         scalars["cell"] = [[float(v) for v in device.cell_lower], [float(v) for v in device.cell_upper]]
@@ -102,13 +101,44 @@ def save(path, case_id: str, device: DeviceCheckpoint, recipe: dict) -> int:
     if device.vertex_target is not None:
         scalars["vertex_target"] = {k: v for k, v in device.vertex_target.items()
                                     if k not in ("indices", "targets", "commands")}
+    return scalars
+
+
+def _device(dev: dict, arrays: dict, prefix: str = "") -> DeviceCheckpoint:
+    ck = DeviceCheckpoint(int(dev["total_iterations"]), arrays[prefix + "positions"],
+                          arrays[prefix + "fiber_active"], arrays[prefix + "wall_reactions"])
+    if "cell" in dev:
+        ck.cell_lower, ck.cell_upper = tuple(dev["cell"][0]), tuple(dev["cell"][1])
+    if "layer_target" in dev:
+        ck.layer_target = dict(dev["layer_target"], targets=arrays[prefix + "layer_targets"])
+    if "vertex_target" in dev:
+        ck.vertex_target = dict(dev["vertex_target"], indices=arrays[prefix + "vertex_indices"],
+                                targets=arrays[prefix + "vertex_targets"], commands=arrays[prefix + "vertex_commands"])
+    for name in ("positions", "wall_reactions"):
+        if arrays[prefix + name].size % 3:
+            raise CheckpointError(f"checkpoint codec failed: {prefix}{name} is not 3 per vertex")
+    return ck
+
+
+def save(path, case_id: str, device: DeviceCheckpoint, recipe: dict, extras: dict | None = None) -> int:
+    """Writes one checkpoint atomically; returns its size in bytes. `extras`
+    maps names to further device states kept in the same file."""
+    path = Path(path)
+    arrays = _arrays(device)
+    scalars = _scalars(device)
+    extra_scalars = {}
+    for name, extra in (extras or {}).items():
+        extra_scalars[name] = _scalars(extra)
+        arrays.update({f"{name}.{k}": v for k, v in _arrays(extra).items()})
     table, offset = [], 0
     for name, arr in arrays.items():
         dtype = "f32" if arr.dtype == np.float32 else "u32"
         table.append({"name": name, "dtype": dtype, "count": int(arr.size), "offset": offset})
         offset += -(-arr.nbytes // 8) * 8
-    header = json.dumps({"case_id": case_id, "device": scalars, "recipe": recipe, "arrays": table},
-                        separators=(",", ":")).encode()
+    head = {"case_id": case_id, "device": scalars, "recipe": recipe, "arrays": table}
+    if extra_scalars:
+        head["extra_devices"] = extra_scalars
+    header = json.dumps(head, separators=(",", ":")).encode()
     tmp = path.with_name(f".{path.name}.tmp")
     path.parent.mkdir(parents=True, exist_ok=True)
     with open(tmp, "wb") as f:
@@ -131,6 +161,12 @@ def save(path, case_id: str, device: DeviceCheckpoint, recipe: dict) -> int:
 def load(path, case_id: str) -> tuple[DeviceCheckpoint, dict]:
     """Reads a checkpoint; raises CheckpointError on a foreign or
     mismatched file (load_checkpoint, tangle_checkpoint/src/lib.rs:374-404)."""
+    ck, recipe, _ = load_with_extras(path, case_id)
+    return ck, recipe
+
+
+def load_with_extras(path, case_id: str) -> tuple[DeviceCheckpoint, dict, dict]:
+    """As `load`, and the further named device states the file holds."""
     path = Path(path)
     try:
         data = path.read_bytes()
@@ -155,17 +191,9 @@ def load(path, case_id: str) -> tuple[DeviceCheckpoint, dict]:
         if stop > len(data):
             raise CheckpointError("checkpoint codec failed: truncated array data")
         arrays[entry["name"]] = np.frombuffer(data[start:stop], dtype=dt).astype(dt.newbyteorder("="))
-    dev = header["device"]
-    ck = DeviceCheckpoint(int(dev["total_iterations"]), arrays["positions"], arrays["fiber_active"],
-                          arrays["wall_reactions"])
-    if "cell" in dev:
-        ck.cell_lower, ck.cell_upper = tuple(dev["cell"][0]), tuple(dev["cell"][1])
-    if "layer_target" in dev:
-        ck.layer_target = dict(dev["layer_target"], targets=arrays["layer_targets"])
-    if "vertex_target" in dev:
-        ck.vertex_target = dict(dev["vertex_target"], indices=arrays["vertex_indices"],
-                                targets=arrays["vertex_targets"], commands=arrays["vertex_commands"])
-    for name in ("positions", "wall_reactions"):
-        if arrays[name].size % 3:
-            raise CheckpointError(f"checkpoint codec failed: {name} is not 3 per vertex")
-    return ck, header["recipe"]
+    try:
+        ck = _device(header["device"], arrays)
+        extras = {name: _device(dev, arrays, name + ".") for name, dev in header.get("extra_devices", {}).items()}
+    except KeyError as err:
+        raise CheckpointError(f"checkpoint codec failed: missing array {err}") from err
+    return ck, header["recipe"], extras

@@ -86,3 +86,24 @@ def test_settings_validation(tmp_path):
         tg.CheckpointSettings("felt", tmp_path / "a.ckpt", interval_iterations=0)
     with pytest.raises(ValueError):
         tg.CheckpointSettings("felt", tmp_path / "a.ckpt", resume_case_id="")
+
+
+def test_named_device_states_ride_along(tmp_path):
+    """Further device states (a compaction's accepted state) travel in the
+    same file under their names, and a file without them loads as before."""
+    path = tmp_path / "x.ckpt"
+    main, extra = sample(), sample()
+    extra.positions = extra.positions + np.float32(1.0)
+    extra.cell_lower, extra.cell_upper = (0.0, 0.0, 0.1), (1.0, 1.0, 0.9)
+    ck.save(path, "c", main, {"operation": 3}, extras={"accepted": extra})
+    got, progress, extras = ck.load_with_extras(path, "c")
+    assert progress == {"operation": 3}
+    assert np.array_equal(got.positions, main.positions)
+    assert set(extras) == {"accepted"}
+    acc = extras["accepted"]
+    assert np.array_equal(acc.positions, extra.positions)
+    assert acc.cell_lower == (0.0, 0.0, 0.1) and acc.cell_upper == (1.0, 1.0, 0.9)
+    assert acc.total_iterations == extra.total_iterations
+    ck.save(path, "c", main, {"operation": 4})
+    assert ck.load_with_extras(path, "c")[2] == {}
+    assert ck.load(path, "c")[1] == {"operation": 4}

@@ -106,8 +106,8 @@ def test_recipe_resume_matches_the_uninterrupted_run(hip_device, tmp_path, monke
     mid = tmp_path / "mid.ckpt"
     original_save = ck.save
 
-    def save_and_keep(path, case_id, device, progress):
-        size = original_save(path, case_id, device, progress)
+    def save_and_keep(path, case_id, device, progress, extras=None):
+        size = original_save(path, case_id, device, progress, extras=extras)
         if device.total_iterations >= 400 and not mid.exists():
             shutil.copyfile(path, mid)
         return size
@@ -158,8 +158,8 @@ def test_resume_keeps_cell_bend_radii_and_junctions(hip_device, tmp_path, monkey
     mid = tmp_path / "mid.ckpt"
     original_save = ck.save
 
-    def save_and_keep(path, case_id, device, progress):
-        size = original_save(path, case_id, device, progress)
+    def save_and_keep(path, case_id, device, progress, extras=None):
+        size = original_save(path, case_id, device, progress, extras=extras)
         if device.total_iterations >= 200 and not mid.exists():
             shutil.copyfile(path, mid)
         return size
@@ -178,5 +178,59 @@ def test_resume_keeps_cell_bend_radii_and_junctions(hip_device, tmp_path, monkey
     assert resumed.iterations == full.iterations
     assert resumed.events == full.events
     assert resumed.junctions == full.junctions
+    assert resumed._cell.origin == full._cell.origin and resumed._cell.lengths == full._cell.lengths
+    assert np.array_equal(bits(resumed.positions), bits(full.positions))
+
+
+def test_resume_inside_a_compaction_matches_the_uninterrupted_run(hip_device, tmp_path, monkeypatch):
+    """A checkpoint taken while a closed-loop compaction is in progress holds
+    the controller's state (strain schedule, windows, steps, work, last
+    metrics, the accepted state it would roll back to): the resumed run ends
+    as the uninterrupted one does, with the same compaction and step
+    reports, events, cell and positions bit for bit."""
+    from test_gpu_compaction import bounded_bimodal_recipe
+    from tangle_amd import compaction as cp
+
+    def recipe():
+        r = bounded_bimodal_recipe()
+        r.relax_for(128)
+        solid = cp.nominal_fiber_volume(r._fibers, None)
+        side = r._cell.lengths
+        target = round(solid / (side[0] * side[1] * side[2]) * 1.12, 4)
+        r.compact(tg.CompactionSettings.volume_fraction(target, kinematics="moving_walls", max_curvature_ratio=1.05))
+        r.relax_for(64)
+        return r
+
+    settings = tg.RelaxationSettings(penetration_tolerance=0.05 * 7e-6, curvature_ratio_tolerance=0.01,
+                                     max_iterations=6000, neighbor_capacity=256)
+    saved = []
+    original_save = ck.save
+
+    def save_and_keep(path, case_id, device, progress, extras=None):
+        size = original_save(path, case_id, device, progress, extras=extras)
+        state = progress.get("compaction_state")
+        # keep the first save taken after the compaction accepted a step
+        if state is not None and state["steps"] >= 1 and not saved:
+            keep = tmp_path / "mid.ckpt"
+            shutil.copyfile(path, keep)
+            saved.append((keep, device.total_iterations, state["steps"]))
+        return size
+
+    monkeypatch.setattr(ck, "save", save_and_keep)
+    full = recipe().run(settings, checkpoint=tg.CheckpointSettings("compact", tmp_path / "a.ckpt",
+                                                                   interval_iterations=50))
+    monkeypatch.setattr(ck, "save", original_save)
+    assert saved, "no checkpoint fell inside the compaction"
+    mid, at, steps = saved[0]
+    assert len(full.compactions) == 1 and full.compactions[0]["steps"] > steps
+    print(f"[resume] inside the compaction at iteration {at} after {steps} steps", flush=True)
+    resumed = recipe().run(settings, checkpoint=tg.CheckpointSettings("compact", tmp_path / "b.ckpt",
+                                                                      interval_iterations=50, resume=True,
+                                                                      resume_path=mid))
+    assert resumed.resumed and resumed.resumed_iteration == at
+    assert resumed.iterations == full.iterations
+    assert resumed.compactions == full.compactions
+    assert resumed.compaction_steps == full.compaction_steps
+    assert resumed.events == full.events
     assert resumed._cell.origin == full._cell.origin and resumed._cell.lengths == full._cell.lengths
     assert np.array_equal(bits(resumed.positions), bits(full.positions))

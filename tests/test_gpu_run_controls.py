@@ -84,8 +84,8 @@ def test_resume_with_proposals_matches_the_uninterrupted_run(untrained_model, tm
     original_save = ck.save
     saved_at = []
 
-    def save_and_keep(path, case_id, device, progress):
-        size = original_save(path, case_id, device, progress)
+    def save_and_keep(path, case_id, device, progress, extras=None):
+        size = original_save(path, case_id, device, progress, extras=extras)
         saved_at.append(int(device.total_iterations))
         if device.total_iterations >= 200 and not mid.exists():
             shutil.copyfile(path, mid)
@@ -121,8 +121,8 @@ def test_fresh_formation_resume_keeps_the_compacted_cell(hip_device, tmp_path, m
     mid = tmp_path / "mid.ckpt"
     original_save = ck.save
 
-    def save_and_keep(path, case_id, device, progress):
-        size = original_save(path, case_id, device, progress)
+    def save_and_keep(path, case_id, device, progress, extras=None):
+        size = original_save(path, case_id, device, progress, extras=extras)
         if device.total_iterations >= 200 and not mid.exists():
             shutil.copyfile(path, mid)
         return size
