@@ -329,10 +329,10 @@ int tgh_world_set_fiber_maximum_curvatures(tgh_world *world, const float *maximu
  * for bit; tgh_world_set_tuning validates the settings and drops captured
  * graphs so the next batch is captured with the new shape.
  *   curvature_lanes: lanes that sweep one fiber in the curvature Gauss-Seidel
- *     pass: 4, 8, 16, 32 or 64 (fibers staged in LDS; at 8 the sweep walks
- *     every triplet with the triplet carried in registers when the longest
- *     fiber's records fit in 64 KiB of LDS, and screens ahead otherwise), or
- *     0 for one lane per fiber walking it in global memory.
+ *     pass: 4, 8, 16, 32 or 64 (fibers staged in LDS, at most 64 KiB per
+ *     workgroup; at 8 the sweep walks every triplet with the triplet
+ *     carried in registers, the other widths screen ahead), or 0 for one
+ *     lane per fiber walking it in global memory.
  *   contact_lanes: lanes that evaluate one segment's neighbour list in the
  *     contact pass, one candidate each: 4, 8, 16 or 32. */
 typedef struct tgh_tuning {

@@ -22,11 +22,12 @@
 //     position buffer and write the other (kernels.rs:1027-1163, 1417-1537):
 //     a find/apply pair becomes one launch with identical per-vertex
 //     arithmetic, and the last pass also applies the contact corrections.
-//   * The curvature Gauss–Seidel stages each fiber in LDS, screens the next
-//     triplets of the sweep in parallel with a conservative bound and
-//     projects the first candidate exactly, one lane per axis
-//     (kernels.rs:1165-1415); it stops after a sweep direction that
-//     projected nothing, which leaves the result unchanged.
+//   * The curvature Gauss–Seidel stages each fiber in LDS and, at 8 lanes
+//     per fiber, walks every triplet in order with the triplet carried in
+//     registers, one lane per axis; the wider groups screen the next
+//     triplets in parallel with a conservative bound and project the first
+//     candidate exactly (kernels.rs:1165-1415). Each stops after a sweep
+//     direction that projected nothing, which leaves the result unchanged.
 //   * Max penetration / curvature ratio / vertex step reduce within each
 //     workgroup and then with one u32 atomicMax per workgroup into one of
 //     the specimen's slots (order-free, so exact); one small workgroup then
@@ -1387,7 +1388,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, SpecSta
                                                            float4 *wall, uint32_t F, uint32_t sweeps, float stiffness,
                                                            float margin, uint32_t track_walls,
                                                            uint32_t max_vertices) {
-    static_assert(L == 4, "group width (8, 16 and 32 lanes: k_curvature_gs_lean)");
+    static_assert(L == 4, "group width (16 and 32 lanes: k_curvature_gs_lean; 8: k_curvature_gs_seq)");
     constexpr uint32_t G = 64 / L;
     extern __shared__ float4 lds4[];
     const uint32_t lane = threadIdx.x, group = lane / L, gl = lane % L, gbase = group * L;
@@ -1536,7 +1537,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_group(DevState *st, SpecSta
     if (__ballot(flag) != 0 && lane == 0) st->nstate[0] = 1u;
 }
 
-// Curvature sweep, straight-line form (L = 8, 16 or 32 lanes per fiber).
+// Curvature sweep, straight-line form (L = 16 or 32 lanes per fiber).
 //
 // The same screening and exact two-quad projection as k_curvature_gs_group,
 // in the same order, so the result is again the sequential sweep's, bit for
@@ -1603,7 +1604,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_lean(DevState *st, SpecStat
                                                           float4 *wall, uint32_t F, uint32_t sweeps, float stiffness,
                                                           float margin, uint32_t track_walls,
                                                           uint32_t max_vertices) {
-    static_assert(L == 8 || L == 16 || L == 32, "group width");
+    static_assert(L == 16 || L == 32, "group width");
     constexpr uint32_t G = 64 / L;
     extern __shared__ float4 lds4[];
     const uint32_t lane = threadIdx.x, group = lane / L, gl = lane % L, gbase = group * L;
@@ -1795,13 +1796,15 @@ __global__ __launch_bounds__(64) void k_curvature_gs_lean(DevState *st, SpecStat
 // one division gives P/Q on lane 0 and acos's low-order correction c on
 // lane 1. Each value is the one the sequential walk computes, bit for bit.
 //
-// LDS per fiber: positions (x, y, z, kmax (1 - margin)), and per vertex and
-// axis the clamp bounds (min(lo + e, hi - e), hi - e) of a walled axis or
-// (-inf, inf), with (weight, 0.25 kmax (1 - margin)) in the fourth slot;
-// wall reactions when tracked.
+// LDS per fiber: positions (x, y, z, kmax (1 - margin)), wall reactions
+// when tracked, and one weight per vertex (0 for a pinned vertex). The
+// clamp bounds (min(lo + e, hi - e), hi - e) of a walled axis, or (-inf,
+// inf), depend on the vertex only through its wall extent e, the fiber's
+// radius, so each lane holds its axis's bounds in registers.
 __host__ __device__ inline size_t curvature_seq_stride_floats(uint32_t max_vertices, uint32_t track_walls) {
     const size_t n = (max_vertices + 1u) & ~1u;
-    return n * (4u + 8u + (track_walls ? 4u : 0u)) + 4u;  // + a sink slot
+    const size_t weights = (n + 3u) & ~(size_t)3u;
+    return n * (4u + (track_walls ? 4u : 0u)) + weights + 4u;  // + a sink slot
 }
 
 // lanes 0..3 of every 8 read lanes 4..7 and lanes 4..7 read 0..3 (bank
@@ -1864,8 +1867,8 @@ __device__ __forceinline__ float div_rn_unscaled(float a, float b) {
 // sweep.
 struct SeqLanes {
     float4 *P;
-    float2 *BND;
-    float *Pf, *W, *sink;
+    float *Pf, *W, *WT, *sink;
+    float2 bound;  // this lane's axis: (lower, upper) clamp of a vertex
     bool second, qlane, walled;
     uint32_t ql, k, o;
     float hc4, hc3, hc2, hc1, hc0;
@@ -1884,8 +1887,8 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
     const bool second = q.second, qlane = q.qlane;
     const uint32_t k = q.k, o = q.o, ql = q.ql;
     float4 *P = q.P;
-    float2 *BND = q.BND;
-    float *Pf = q.Pf;
+    float *Pf = q.Pf, *WT = q.WT;
+    const float2 b0 = q.bound, b1 = q.bound;
     uint32_t t = (on && DIR == 1) ? trip - 1u : 0u;
     float a0 = Pf[4u * (t + o) + k], a1 = Pf[4u * (t + o + 1u) + k];
     bool any = false;
@@ -1893,9 +1896,7 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
         const bool act = on && c < trip;
         // static records of this triplet, and the vertex the next one adds
         const float kmax = P[t + 1u].w;
-        const float2 wc = BND[4u * (t + 1u) + 3u];  // (weight, 0.25 kmax) of the center
-        const float fwp = BND[4u * t + 3u].x, fwc = wc.x, fwn = BND[4u * (t + 2u) + 3u].x, qkmax = wc.y;
-        const float2 b0 = BND[4u * (t + o) + k], b1 = BND[4u * (t + o + 1u) + k];
+        const float fwp = WT[t], fwc = WT[t + 1u], fwn = WT[t + 2u], qkmax = 0.25f * kmax;
         const bool more = act && c + 1u < trip;
         const uint32_t tn = more ? (DIR == 0 ? t + 1u : t - 1u) : t;
         const float nextv = Pf[4u * (DIR == 0 ? tn + 2u : tn) + k];
@@ -2011,9 +2012,9 @@ __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState
     const uint32_t nv = (max_vertices + 1u) & ~1u;
     SeqLanes q;
     q.P = (float4 *)((float *)lds4 + group * curvature_seq_stride_floats(max_vertices, track_walls));
-    q.BND = (float2 *)(q.P + nv);                          // [vertex][4]: axis bounds, (weight, -)
-    float4 *WR = (float4 *)(q.BND + 4u * nv);              // wall reactions (when tracked)
-    q.sink = (float *)(track_walls ? WR + nv : WR);        // absorbs stores that must not land
+    float4 *WR = q.P + nv;                                 // wall reactions (when tracked)
+    q.WT = (float *)(track_walls ? WR + nv : WR);          // weights
+    q.sink = q.WT + ((nv + 3u) & ~3u);                     // absorbs stores that must not land
     q.Pf = (float *)q.P;
     q.W = (float *)WR;
     float4 *P = q.P;
@@ -2030,14 +2031,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState
         const float4 p = in[start + i];
         const float4 a = vA[start + i];
         P[i] = make_float4(p.x, p.y, p.z, a.x * (1.0f - margin));
-        const float e = a.w;
-#pragma unroll
-        for (uint32_t ax = 0; ax < 3; ++ax) {
-            const float lo = g.lower[ax] + e, hi = g.upper[ax] - e;
-            q.BND[4u * i + ax] = g.periodic[ax] ? make_float2(-inf, inf) : make_float2(fminf(lo, hi), hi);
-        }
-        q.BND[4u * i + 3u] = make_float2((__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f,
-                                         0.25f * (a.x * (1.0f - margin)));
+        q.WT[i] = (__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f;
         if (track_walls) WR[i] = wall[start + i];
     }
     // lanes 0..2 hold the axes of the triplet's first segment (previous ->
@@ -2048,6 +2042,12 @@ __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState
     q.k = q.ql < 3u ? q.ql : 0u;
     q.o = q.second ? 1u : 0u;
     q.walled = !g.periodic[q.k];
+    {
+        // the wall extent of every vertex is the fiber's radius
+        const float e = n > 0u ? vA[start].w : 0.f;
+        const float lo = g.lower[q.k] + e, hi = g.upper[q.k] - e;
+        q.bound = q.walled ? make_float2(fminf(lo, hi), hi) : make_float2(-inf, inf);
+    }
     // acos (fdlibm e_acosf.c): R(z) = P(z) / Q(z), P on lanes 0, 2, 3 and Q on
     // lane 1 of each quad
     q.qlane = q.ql == 1u;
@@ -2889,6 +2889,8 @@ inline uint32_t rebuild_blocks(uint64_t n, uint32_t per_block) {
 }  // namespace
 
 size_t curvature_lds_bytes(uint32_t max_vertices, uint32_t lanes, uint32_t track_walls) {
+    if (lanes == 8u)  // the sequential walk's layout
+        return 8u * curvature_seq_stride_floats(max_vertices, track_walls) * sizeof(float);
     return (64u / lanes) * curvature_group_stride_floats(max_vertices, track_walls) * sizeof(float);
 }
 
@@ -3011,15 +3013,7 @@ hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t 
         p.curvature_margin, w.track_walls, w.max_fiber_vertices
         switch (w.gs_lanes) {
             case 4: hipLaunchKernelGGL(k_curvature_gs_group<4>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
-            case 8: {
-                // the sequential walk when its LDS fits, else the screened one
-                const size_t seq = 8u * 4u * curvature_seq_stride_floats(w.max_fiber_vertices, w.track_walls);
-                if (seq <= 64u * 1024u)
-                    hipLaunchKernelGGL(k_curvature_gs_seq, grid, dim3(64), seq, stream, TGH_GS_ARGS);
-                else
-                    hipLaunchKernelGGL(k_curvature_gs_lean<8>, grid, dim3(64), lds, stream, TGH_GS_ARGS);
-                break;
-            }
+            case 8: hipLaunchKernelGGL(k_curvature_gs_seq, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
             case 16: hipLaunchKernelGGL(k_curvature_gs_lean<16>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
             case 32: hipLaunchKernelGGL(k_curvature_gs_lean<32>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
             case 64: hipLaunchKernelGGL(k_curvature_gs_wave64, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;

@@ -542,10 +542,11 @@ int create_world_packed(std::vector<Packed> &&packed, const tgh_config *c, tgh_w
             d.gs_lanes = lanes;
             break;
         }
-    // lanes per segment in the contact pass, one candidate each: 16 suits the
-    // ~20-candidate lists of the bimodal felt; the autotuner picks narrower
-    // groups for shorter lists
-    d.contact_lanes = 16u;
+    // lanes per segment in the contact pass, one candidate each: 8 lanes
+    // issuing three rounds of gathers at once cover the ~20-candidate lists
+    // of the bimodal felt and the slab (faster there than 16, which leaves
+    // more lanes idle); the autotuner picks 4 for shorter lists
+    d.contact_lanes = 8u;
     e = hipSuccess;
     GridParams *grids = nullptr;
     if (e == hipSuccess) e = w->alloc(&d.st, 1);

@@ -59,10 +59,10 @@ def test_fiber_longer_than_lds_takes_the_global_walk(hip_device):
     assert np.array_equal(pg.view(np.uint32), po.view(np.uint32))
 
 
-def test_fiber_too_long_for_the_sequential_sweep_takes_the_screened_one(hip_device):
-    """A 200-segment coil (201 vertices: too many for the 8-lane sequential
-    sweep's 64 KiB of LDS, few enough for the screened 8-lane sweep) with
-    two short coils: positions and status match the oracle bit for bit."""
+def test_long_coil_in_the_sequential_sweep_matches_bitwise(hip_device):
+    """A 200-segment coil (201 vertices, three times the felt's longest
+    fiber) with two short coils, in the 8-lane sequential sweep: positions
+    and status match the oracle bit for bit."""
     fibers = [_helix(200, 0.1, 0.01, 40, (0.5, 0.5, 0.3)),
               _helix(30, 0.05, 0.02, 12, (0.2, 0.8, 0.5), 0.3),
               _helix(12, 0.05, 0.02, 12, (0.8, 0.2, 0.5), 1.1)]
@@ -70,6 +70,20 @@ def test_fiber_too_long_for_the_sequential_sweep_takes_the_screened_one(hip_devi
     cfg = fx.config(force_full_iterations=1, max_step=0.002)
     lanes, pg, po = _run_both(packed, cfg, 2, 15)
     assert lanes == 8
+    assert np.array_equal(pg.view(np.uint32), po.view(np.uint32))
+
+
+def test_fiber_too_long_for_the_sequential_sweep_takes_a_wider_group(hip_device):
+    """A 420-segment coil (421 vertices: more than eight fibers' records of
+    the sequential sweep fit in 64 KiB of LDS) in a periodic box: the world
+    picks 16 lanes per fiber (the screened sweep), and positions and status
+    match the oracle bit for bit."""
+    fibers = [_helix(420, 0.1, 0.005, 60, (0.5, 0.5, 0.25)),
+              _helix(30, 0.05, 0.02, 12, (0.2, 0.8, 0.5), 0.3)]
+    packed = fx.packed(fibers, 0.004, [1.0] * 3, [True] * 3, bend_radius=0.2)
+    cfg = fx.config(force_full_iterations=1, max_step=0.002)
+    lanes, pg, po = _run_both(packed, cfg, 2, 15)
+    assert lanes == 16
     assert np.array_equal(pg.view(np.uint32), po.view(np.uint32))
 
 
