@@ -1796,15 +1796,16 @@ __global__ __launch_bounds__(64) void k_curvature_gs_lean(DevState *st, SpecStat
 // one division gives P/Q on lane 0 and acos's low-order correction c on
 // lane 1. Each value is the one the sequential walk computes, bit for bit.
 //
-// LDS per fiber: positions (x, y, z, kmax (1 - margin)), wall reactions
-// when tracked, and one weight per vertex (0 for a pinned vertex). The
-// clamp bounds (min(lo + e, hi - e), hi - e) of a walled axis, or (-inf,
-// inf), depend on the vertex only through its wall extent e, the fiber's
-// radius, so each lane holds its axis's bounds in registers.
+// LDS per fiber: positions (x, y, z, kmax (1 - margin)) and one weight per
+// vertex (0 for a pinned vertex); with walls tracked, the weight is the
+// fourth float of the vertex's wall reactions. The clamp bounds
+// (min(lo + e, hi - e), hi - e) of a walled axis, or (-inf, inf), depend on
+// the vertex only through its wall extent e, the fiber's radius, so each
+// lane holds its axis's bounds in registers.
 __host__ __device__ inline size_t curvature_seq_stride_floats(uint32_t max_vertices, uint32_t track_walls) {
     const size_t n = (max_vertices + 1u) & ~1u;
-    const size_t weights = (n + 3u) & ~(size_t)3u;
-    return n * (4u + (track_walls ? 4u : 0u)) + weights + 4u;  // + a sink slot
+    if (track_walls) return n * 8u + 4u;                     // + a sink slot
+    return n * 4u + ((n + 3u) & ~(size_t)3u) + 4u;
 }
 
 // lanes 0..3 of every 8 read lanes 4..7 and lanes 4..7 read 0..3 (bank
@@ -1878,9 +1879,10 @@ struct SeqLanes {
 // backward), steps in lockstep up to the wave's longest chain; a group whose
 // chain is shorter, or that is done, takes no step. Returns whether the
 // group's direction projected anything.
-template <int DIR>
+template <int DIR, bool WALLS>
 __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32_t trip, uint32_t maxtrip,
-                                              float stiffness, uint32_t track_walls) {
+                                              float stiffness) {
+    constexpr uint32_t WS = WALLS ? 4u : 1u;  // floats from one vertex's weight to the next
     const float pio2_hi = f32_from_bits(0x3fc90fdau), pio2_lo = f32_from_bits(0x33a22168u);
     const float pi = f32_from_bits(0x40490fdau);
     const float hc5 = f32_from_bits(0x3811ef08u);
@@ -1896,7 +1898,7 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
         const bool act = on && c < trip;
         // static records of this triplet, and the vertex the next one adds
         const float kmax = P[t + 1u].w;
-        const float fwp = WT[t], fwc = WT[t + 1u], fwn = WT[t + 2u], qkmax = 0.25f * kmax;
+        const float fwp = WT[WS * t], fwc = WT[WS * (t + 1u)], fwn = WT[WS * (t + 2u)], qkmax = 0.25f * kmax;
         const bool more = act && c + 1u < trip;
         const uint32_t tn = more ? (DIR == 0 ? t + 1u : t - 1u) : t;
         const float nextv = Pf[4u * (DIR == 0 ? tn + 2u : tn) + k];
@@ -1966,7 +1968,7 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
             const float v0 = __builtin_amdgcn_fmed3f(m0, b0.x, b0.y);
             const float v1 = __builtin_amdgcn_fmed3f(m1, b1.x, b1.y);
             const bool moved0 = ok && !second && fwp > 0.f, moved1 = ok && f1 > 0.f;
-            if (track_walls) {
+            if (WALLS) {
                 float *wr0 = (moved0 && q.walled && ql < 3u) ? &q.W[4u * (t + o) + k] : &q.sink[k];
                 float *wr1 = (moved1 && q.walled && ql < 3u) ? &q.W[4u * (t + o + 1u) + k] : &q.sink[k];
                 *wr0 = *wr0 + (m0 - v0);
@@ -2000,6 +2002,7 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
     return any;
 }
 
+template <bool WALLS>
 __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState *ss, const GridParams *grids,
                                                          const float4 *in, float4 *pos, const float4 *refpos,
                                                          const float4 *vA, const float4 *vB, const uint4 *fspan,
@@ -2011,10 +2014,10 @@ __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState
     const uint32_t lane = threadIdx.x, group = lane / L, gl = lane % L;
     const uint32_t nv = (max_vertices + 1u) & ~1u;
     SeqLanes q;
-    q.P = (float4 *)((float *)lds4 + group * curvature_seq_stride_floats(max_vertices, track_walls));
-    float4 *WR = q.P + nv;                                 // wall reactions (when tracked)
-    q.WT = (float *)(track_walls ? WR + nv : WR);          // weights
-    q.sink = q.WT + ((nv + 3u) & ~3u);                     // absorbs stores that must not land
+    q.P = (float4 *)((float *)lds4 + group * curvature_seq_stride_floats(max_vertices, WALLS));
+    float4 *WR = q.P + nv;                                 // wall reactions and weights (walls tracked)
+    q.WT = WALLS ? (float *)WR + 3 : (float *)WR;          // weights
+    q.sink = WALLS ? (float *)(WR + nv) : q.WT + ((nv + 3u) & ~3u);  // absorbs stores that must not land
     q.Pf = (float *)q.P;
     q.W = (float *)WR;
     float4 *P = q.P;
@@ -2031,8 +2034,13 @@ __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState
         const float4 p = in[start + i];
         const float4 a = vA[start + i];
         P[i] = make_float4(p.x, p.y, p.z, a.x * (1.0f - margin));
-        q.WT[i] = (__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f;
-        if (track_walls) WR[i] = wall[start + i];
+        const float weight = (__float_as_uint(vB[start + i].z) & kPinned) ? 0.f : 1.f;
+        if (WALLS) {
+            const float4 r = wall[start + i];
+            WR[i] = make_float4(r.x, r.y, r.z, weight);
+        } else {
+            q.WT[i] = weight;
+        }
     }
     // lanes 0..2 hold the axes of the triplet's first segment (previous ->
     // center), lanes 4..6 those of its second (center -> next); lanes 3 and 7
@@ -2064,9 +2072,9 @@ __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState
     bool done = trip == 0;
     for (uint32_t sweep = 0; sweep < sweeps; ++sweep) {
         if (__ballot(!done) == 0) break;
-        done = done || !seq_direction<0>(q, !done, trip, maxtrip, stiffness, track_walls);
+        done = done || !seq_direction<0, WALLS>(q, !done, trip, maxtrip, stiffness);
         if (__ballot(!done) == 0) break;
-        done = done || !seq_direction<1>(q, !done, trip, maxtrip, stiffness, track_walls);
+        done = done || !seq_direction<1, WALLS>(q, !done, trip, maxtrip, stiffness);
     }
     __syncthreads();
     const float limit = half_skin * half_skin;
@@ -2075,7 +2083,7 @@ __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState
         const uint32_t v = start + i;
         const float4 old = in[v], qv = P[i];
         pos[v] = make_float4(qv.x, qv.y, qv.z, old.w);
-        if (track_walls) wall[v] = make_float4(WR[i].x, WR[i].y, WR[i].z, 0.f);
+        if (WALLS) wall[v] = make_float4(WR[i].x, WR[i].y, WR[i].z, 0.f);
         const float4 r = refpos[v];
         const float dx = qv.x - r.x, dy = qv.y - r.y, dz = qv.z - r.z;
         if (dx * dx + dy * dy + dz * dz > limit) flag = true;
@@ -3013,7 +3021,12 @@ hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t 
         p.curvature_margin, w.track_walls, w.max_fiber_vertices
         switch (w.gs_lanes) {
             case 4: hipLaunchKernelGGL(k_curvature_gs_group<4>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
-            case 8: hipLaunchKernelGGL(k_curvature_gs_seq, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
+            case 8:
+                if (w.track_walls)
+                    hipLaunchKernelGGL(k_curvature_gs_seq<true>, grid, dim3(64), lds, stream, TGH_GS_ARGS);
+                else
+                    hipLaunchKernelGGL(k_curvature_gs_seq<false>, grid, dim3(64), lds, stream, TGH_GS_ARGS);
+                break;
             case 16: hipLaunchKernelGGL(k_curvature_gs_lean<16>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
             case 32: hipLaunchKernelGGL(k_curvature_gs_lean<32>, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
             case 64: hipLaunchKernelGGL(k_curvature_gs_wave64, grid, dim3(64), lds, stream, TGH_GS_ARGS); break;
