@@ -1893,14 +1893,32 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
     const float2 b0 = q.bound, b1 = q.bound;
     uint32_t t = (on && DIR == 1) ? trip - 1u : 0u;
     float a0 = Pf[4u * (t + o) + k], a1 = Pf[4u * (t + o + 1u) + k];
+    // the triplet's static records (center kmax, the three weights) are read
+    // one step ahead, so no step waits on LDS for them
+    float kmax_n = P[t + 1u].w, fwp_n = WT[WS * t], fwc_n = WT[WS * (t + 1u)], fwn_n = WT[WS * (t + 2u)];
+    // these reads complete before the loop (s_waitcnt lgkmcnt(0)): with
+    // nothing read pending on either way into the loop, its first step does
+    // not wait for the stores of the step before
+    __builtin_amdgcn_s_waitcnt(0xc07f);
     bool any = false;
     for (uint32_t c = 0; c < maxtrip; ++c) {
         const bool act = on && c < trip;
-        // static records of this triplet, and the vertex the next one adds
-        const float kmax = P[t + 1u].w;
-        const float fwp = WT[WS * t], fwc = WT[WS * (t + 1u)], fwn = WT[WS * (t + 2u)], qkmax = 0.25f * kmax;
+        const float kmax = kmax_n, fwp = fwp_n, fwc = fwc_n, fwn = fwn_n, qkmax = 0.25f * kmax;
         const bool more = act && c + 1u < trip;
         const uint32_t tn = more ? (DIR == 0 ? t + 1u : t - 1u) : t;
+        // the next triplet's static records, then the vertex it adds (read
+        // last, so that the wait for it at the end of the step covers all
+        // three reads and none is pending when the next step begins)
+        kmax_n = P[tn + 1u].w;
+        if (DIR == 0) {
+            fwp_n = more ? fwc : fwp;
+            fwc_n = more ? fwn : fwc;
+            fwn_n = WT[WS * (tn + 2u)];
+        } else {
+            fwn_n = more ? fwc : fwn;
+            fwc_n = more ? fwp : fwc;
+            fwp_n = WT[WS * tn];
+        }
         const float nextv = Pf[4u * (DIR == 0 ? tn + 2u : tn) + k];
         // the sequential projection's operations, in its order
         const float d = a1 - a0;
@@ -1979,10 +1997,15 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
         }
         // the first quad holds the triplet's previous and center vertices,
         // the second quad's w1 its next; each lane stores what it holds (the
-        // second quad's w1 twice)
-        if (act) {
-            Pf[4u * (second ? t + 2u : t) + k] = second ? w1 : w0;
-            Pf[4u * (t + o + 1u) + k] = w1;
+        // second quad's w1 twice). A group that takes no step stores to its
+        // sink instead: with no branch, every step issues the same LDS
+        // operations, and the wait for the next vertex's read need not wait
+        // for these stores.
+        {
+            float *s0 = act ? &Pf[4u * (second ? t + 2u : t) + k] : &q.sink[k];
+            float *s1 = act ? &Pf[4u * (t + o + 1u) + k] : &q.sink[k];
+            *s0 = second ? w1 : w0;
+            *s1 = w1;
         }
         // the next triplet's operands: forward (center, next | next, new),
         // backward (new, previous | previous, center)
