@@ -1808,14 +1808,11 @@ __host__ __device__ inline size_t curvature_seq_stride_floats(uint32_t max_verti
     return n * 4u + ((n + 3u) & ~(size_t)3u) + 4u;
 }
 
-// lanes 0..3 of every 8 read lanes 4..7 and lanes 4..7 read 0..3 (bank
-// masks pick the direction, so no select): the first move writes lanes
-// 0..3 of each 8 and leaves the others undefined, the second writes those
-// others and keeps the first move's lanes
+// The other quad of a lane's fiber in the sequential sweep, whose two quads
+// sit 8 lanes apart in a row of 16 lanes: one rotation of each row by 8
+// lanes (DPP row_ror:8) exchanges them.
 __device__ __forceinline__ float swap_quads(float x) {
-    int v = __builtin_amdgcn_mov_dpp(__float_as_int(x), 0x104, 0xF, 0x5, false);
-    v = __builtin_amdgcn_update_dpp(v, __float_as_int(x), 0x114, 0xF, 0xA, false);
-    return __int_as_float(v);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x128, 0xF, 0xF, false));
 }
 // the IEEE square root for x = +-0, +inf, NaN and x >= 2^-96 (where the
 // scaling of tiny inputs is not taken): the hardware root s, then s - 1 ulp
@@ -2034,7 +2031,13 @@ __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState
                                                          uint32_t max_vertices) {
     constexpr uint32_t L = 8, G = 8;
     extern __shared__ float4 lds4[];
-    const uint32_t lane = threadIdx.x, group = lane / L, gl = lane % L;
+    // a row of 16 lanes serves two fibers: fiber h (0 or 1) of the row has
+    // its first quad at lanes 4h..4h+3 and its second 8 lanes on, so that
+    // swap_quads is one rotation; gl numbers a fiber's lanes 0..7 (quad lane,
+    // plus 4 in the second quad)
+    const uint32_t lane = threadIdx.x, within = lane & 15u;
+    const uint32_t group = 2u * (lane >> 4) + ((within >> 2) & 1u);
+    const uint32_t gl = (within & 3u) | ((within >> 1) & 4u);
     const uint32_t nv = (max_vertices + 1u) & ~1u;
     SeqLanes q;
     q.P = (float4 *)((float *)lds4 + group * curvature_seq_stride_floats(max_vertices, WALLS));
@@ -2065,9 +2068,9 @@ __global__ __launch_bounds__(64) void k_curvature_gs_seq(DevState *st, SpecState
             q.WT[i] = weight;
         }
     }
-    // lanes 0..2 hold the axes of the triplet's first segment (previous ->
-    // center), lanes 4..6 those of its second (center -> next); lanes 3 and 7
-    // repeat lane 0's and lane 4's vector values
+    // fiber lanes 0..2 hold the axes of the triplet's first segment
+    // (previous -> center), lanes 4..6 those of its second (center -> next);
+    // lanes 3 and 7 repeat lane 0's and lane 4's vector values
     q.second = (gl & 4u) != 0u;
     q.ql = gl & 3u;
     q.k = q.ql < 3u ? q.ql : 0u;
