@@ -1876,6 +1876,13 @@ struct SeqLanes {
 // backward), steps in lockstep up to the wave's longest chain; a group whose
 // chain is shorter, or that is done, takes no step. Returns whether the
 // group's direction projected anything.
+//
+// The triplet index t is the same for every group of the wave (forward 0 up,
+// backward from the wave's last triplet down): a group whose chain is
+// shorter idles while t lies past its last triplet and walks its own
+// vertices' LDS records along (its region holds the longest fiber's count),
+// so when t reaches its chain the registers hold its triplet. Indices stay
+// inside the group's region: t + 3 <= the longest fiber's vertex count.
 template <int DIR, bool WALLS>
 __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32_t trip, uint32_t maxtrip,
                                               float stiffness) {
@@ -1888,7 +1895,7 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
     float4 *P = q.P;
     float *Pf = q.Pf, *WT = q.WT;
     const float2 b0 = q.bound, b1 = q.bound;
-    uint32_t t = (on && DIR == 1) ? trip - 1u : 0u;
+    uint32_t t = DIR == 1 ? maxtrip - 1u : 0u;  // maxtrip >= 1
     float a0 = Pf[4u * (t + o) + k], a1 = Pf[4u * (t + o + 1u) + k];
     // the triplet's static records (center kmax, the three weights) are read
     // one step ahead, so no step waits on LDS for them
@@ -1899,21 +1906,20 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
     __builtin_amdgcn_s_waitcnt(0xc07f);
     bool any = false;
     for (uint32_t c = 0; c < maxtrip; ++c) {
-        const bool act = on && c < trip;
+        const bool act = on && t < trip;
         const float kmax = kmax_n, fwp = fwp_n, fwc = fwc_n, fwn = fwn_n, qkmax = 0.25f * kmax;
-        const bool more = act && c + 1u < trip;
-        const uint32_t tn = more ? (DIR == 0 ? t + 1u : t - 1u) : t;
+        const uint32_t tn = DIR == 0 ? t + 1u : (t > 0u ? t - 1u : 0u);
         // the next triplet's static records, then the vertex it adds (read
         // last, so that the wait for it at the end of the step covers all
         // three reads and none is pending when the next step begins)
         kmax_n = P[tn + 1u].w;
         if (DIR == 0) {
-            fwp_n = more ? fwc : fwp;
-            fwc_n = more ? fwn : fwc;
+            fwp_n = fwc;
+            fwc_n = fwn;
             fwn_n = WT[WS * (tn + 2u)];
         } else {
-            fwn_n = more ? fwc : fwn;
-            fwc_n = more ? fwp : fwc;
+            fwn_n = fwc;
+            fwc_n = fwp;
             fwp_n = WT[WS * tn];
         }
         const float nextv = Pf[4u * (DIR == 0 ? tn + 2u : tn) + k];
