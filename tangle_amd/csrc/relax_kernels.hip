@@ -1997,6 +1997,7 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
             }
             w0 = moved0 ? v0 : a0;
             w1 = moved1 ? v1 : a1;
+            any = any || ok;  // outside this branch no lane's ok holds
         }
         // the first quad holds the triplet's previous and center vertices,
         // the second quad's w1 its next; each lane stores what it holds (the
@@ -2020,7 +2021,6 @@ __device__ __forceinline__ bool seq_direction(const SeqLanes &q, bool on, uint32
             a0 = second ? x0 : nextv;
             a1 = second ? x1 : w0;
         }
-        any = any || ok;
         t = tn;
         // one wave: its LDS accesses complete in order; keep the compiler's
         lds_order();
