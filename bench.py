@@ -129,8 +129,11 @@ def whole_iteration_roofline(S, V, mean_list, iterations_per_second):
             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5)}
 
 
-def gs_kernel_name(lanes: int) -> str:
-    return {4: "tgh::k_curvature_gs_group<4>", 8: "tgh::k_curvature_gs_seq", 16: "tgh::k_curvature_gs_lean<16>",
+def gs_kernel_name(lanes: int, walls: bool) -> str:
+    """The curvature sweep's kernel as the profiler names it (8 lanes: the
+    sequential walk, specialised on whether walls are tracked)."""
+    seq = "tgh::k_curvature_gs_seq<true>" if walls else "tgh::k_curvature_gs_seq<false>"
+    return {4: "tgh::k_curvature_gs_group<4>", 8: seq, 16: "tgh::k_curvature_gs_lean<16>",
             32: "tgh::k_curvature_gs_lean<32>", 64: "tgh::k_curvature_gs_wave64"}.get(lanes,
                                                                                       "tgh::k_curvature_gs_lane")
 
@@ -205,7 +208,8 @@ def main():
     # dominant kernel group by event time
     dominant = max(GROUPS, key=lambda g: group_ms[g])
     kernels = {
-        "curvature_gs": (gs_kernel_name(lanes), curvature_bytes_per_launch(V)),
+        "curvature_gs": (gs_kernel_name(lanes, not all(packed.struct.cell_periodic[k] for k in range(3))),
+                         curvature_bytes_per_launch(V)),
         "contact": (f"tgh::k_contact<{contact_lanes}>", contact_bytes_per_launch(S, V, mean_list)),
     }
 
