@@ -72,6 +72,9 @@ def parse():
     ap.add_argument("--formation-plies", type=int, default=20, help="plies of the layered felt workload")
     ap.add_argument("--batched-specimens", type=int, default=256, help="specimens in the batched run")
     ap.add_argument("--batched-steps", type=int, default=2, help="timed 128-iteration batches of the batched run")
+    ap.add_argument("--batched-tuning", type=str, default="",
+                    help="skin,cell,contact_lanes,curvature_lanes for the batched run instead of the autotuner's "
+                         "trials (for counter runs, whose serialised dispatches skew the trials' timings)")
     ap.add_argument("--curvature-lanes", type=int, default=-1,
                     help="lanes per fiber in the curvature sweep (0, 16, 32, 64; default: the world's choice)")
     ap.add_argument("--learned-specimens", type=int, default=5,
@@ -576,7 +579,13 @@ def batched_throughput(args, fx, tg, _native):
     lib = _native.library()
     from tangle_amd.autotune import autotune
     t0 = time.perf_counter()
-    tuning = autotune(packed, cfg, iterations=32, cache=False)  # timed: the trials always run here
+    if args.batched_tuning:
+        from tangle_amd.autotune import Tuning
+        skin, cell, cl, gl = args.batched_tuning.split(",")
+        tuning = Tuning(cell_size_scale=float(cell), neighbor_skin_scale=float(skin), contact_lanes=int(cl),
+                        curvature_lanes=int(gl))
+    else:
+        tuning = autotune(packed, cfg, iterations=32, cache=False)  # timed: the trials always run here
     tune_s = time.perf_counter() - t0
     default_tuning = {"cell_size_scale": cfg.cell_size_scale, "neighbor_skin_scale": cfg.neighbor_skin_scale}
     cfg = tuning.configure(cfg)
