@@ -1273,6 +1273,157 @@ __global__ __launch_bounds__(256) void k_vertex_pass(const SpecState *ss, const 
     if (do_contact) spec_block_max(__float_as_uint(fmaxf(step, 0.f)), spec, red, kMetricStep, red_slots, scratch);
 }
 
+// Stretch and rest-chord bend of one vertex in one Jacobi pass, with its
+// wall clamp (the internal half of k_vertex_pass): x, y, z in and out, the
+// reactions added to w.
+__device__ __forceinline__ void internal_move(const GridParams &g, uint32_t flags, float4 sa, float4 sb,
+                                              float4 pm2, float4 pm1, float4 pp1, float4 pp2, float stretch,
+                                              float bend, float &x, float &y, float &z, float4 &w) {
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    if (flags & kHasPrev) accumulate_pull(pm1, x, y, z, sa.y, stretch, cx, cy, cz);
+    if (flags & kHasNext) accumulate_pull(pp1, x, y, z, sa.z, stretch, cx, cy, cz);
+    if (flags & kHasPrev2) accumulate_pull(pm2, x, y, z, sb.x, bend, cx, cy, cz);
+    if (flags & kHasNext2) accumulate_pull(pp2, x, y, z, sb.y, bend, cx, cy, cz);
+    float mx = x + cx, my = y + cy, mz = z + cz;
+    x = wall_move(mx, g.periodic[0], g.lower[0], g.upper[0], sa.w);
+    y = wall_move(my, g.periodic[1], g.lower[1], g.upper[1], sa.w);
+    z = wall_move(mz, g.periodic[2], g.lower[2], g.upper[2], sa.w);
+    if (!g.periodic[0]) w.x += mx - x;
+    if (!g.periodic[1]) w.y += my - y;
+    if (!g.periodic[2]) w.z += mz - z;
+}
+
+// All Jacobi passes of an iteration in one launch. A workgroup owns 256
+// vertices and stages them with 2 x passes more on each side (positions and
+// static records in LDS); pass p (0-based) computes its own vertices and
+// 2 (passes - 1 - p) more on each side, which is what the passes after it
+// read (a vertex reads up to two neighbours on each side), so the last pass
+// has every owned vertex exact without any exchange between workgroups. Each
+// vertex takes k_vertex_pass's operations in its order (the last pass adds
+// the contact corrections and the step metric; a vertex's wall reactions
+// accumulate in a register across the passes, as k_vertex_pass accumulates
+// them in memory). Reads `in`, writes `out` (never in place: neighbouring
+// workgroups read `in`'s halo).
+constexpr uint32_t kPassOwned = 256;
+constexpr uint32_t kMaxFusedPasses = 32;
+__host__ __device__ inline size_t vertex_passes_lds_bytes(uint32_t passes) {
+    return (size_t)(kPassOwned + 4u * passes) * 4u * sizeof(float4);  // two position buffers, vA, vB
+}
+__global__ __launch_bounds__(256) void k_vertex_passes(const SpecState *ss, const GridParams *grids,
+                                                       uint32_t red_slots, const float4 *in, float4 *out,
+                                                       const float4 *vA, const float4 *vB, const float4 *corr0,
+                                                       const float4 *corr1, float4 *wall, uint32_t *red,
+                                                       uint32_t V, float stretch, float bend, float max_step,
+                                                       uint32_t passes, uint32_t track_walls) {
+    extern __shared__ float4 lds4[];
+    __shared__ uint32_t scratch[5];
+    const uint32_t H = 2u * passes, n = kPassOwned + 2u * H;
+    float4 *X0 = lds4, *X1 = lds4 + n, *A = lds4 + 2u * n, *B = lds4 + 3u * n;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t b0 = blockIdx.x * kPassOwned;
+    // local index i holds vertex b0 - H + i; a vertex outside [0, V) is never
+    // read (a vertex reads only neighbours its flags name)
+    for (uint32_t i = tid; i < n; i += 256u) {
+        const int64_t v = (int64_t)b0 - (int64_t)H + (int64_t)i;
+        if (v >= 0 && v < (int64_t)V) {
+            X0[i] = in[v];
+            A[i] = vA[v];
+            B[i] = vB[v];
+        } else {
+            B[i] = make_float4(0.f, 0.f, __uint_as_float(kInactive), 0.f);
+        }
+    }
+    __syncthreads();
+    const uint32_t own = H + tid;
+    const uint32_t vo = b0 + tid;
+    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);  // cleared once per iteration (kernels.rs:14-24)
+    float step = 0.f;
+    uint32_t spec_o = 0;
+    float4 *src = X0, *dst = X1;
+    for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t r = 2u * (passes - 1u - p);
+        const bool last = p + 1u == passes;
+        // this thread's vertices of the pass: its own, and one of the 2r
+        // halo vertices when tid < 2r
+        for (uint32_t k = 0; k < 2u; ++k) {
+            uint32_t i;
+            if (k == 0) {
+                i = own;
+            } else {
+                if (tid >= 2u * r) break;
+                i = tid < r ? H - r + tid : H + kPassOwned + (tid - r);
+            }
+            const float4 sb = B[i];
+            const uint32_t flags = __float_as_uint(sb.z);
+            const uint32_t spec = flags >> kSpecShift;
+            const float4 me = src[i];
+            if (k == 0) spec_o = spec;
+            if ((flags & kInactive) || !running(ss, spec)) {
+                dst[i] = me;
+                continue;
+            }
+            const GridParams &g = grids[spec];
+            const float4 sa = A[i];
+            float4 result = me;
+            float4 wr = k == 0 ? w : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!(flags & kPinned)) {
+                float x = me.x, y = me.y, z = me.z;
+                internal_move(g, flags, sa, sb, (flags & kHasPrev2) ? src[i - 2u] : me,
+                              (flags & kHasPrev) ? src[i - 1u] : me, (flags & kHasNext) ? src[i + 1u] : me,
+                              (flags & kHasNext2) ? src[i + 2u] : me, stretch, bend, x, y, z, wr);
+                if (last) {  // only owned vertices run the last pass
+                    const uint32_t next_seg = vo - __float_as_uint(sb.w);
+                    float cx = 0.f, cy = 0.f, cz = 0.f;
+                    if (flags & kHasPrev) {
+                        float4 c = corr1[next_seg - 1];
+                        cx += c.x;
+                        cy += c.y;
+                        cz += c.z;
+                    }
+                    if (flags & kHasNext) {
+                        float4 c = corr0[next_seg];
+                        cx += c.x;
+                        cy += c.y;
+                        cz += c.z;
+                    }
+                    float len = sqrtf(cx * cx + cy * cy + cz * cz);
+                    if (len > max_step) {
+                        float sc = max_step / len;
+                        cx *= sc;
+                        cy *= sc;
+                        cz *= sc;
+                    }
+                    float mx = x + cx, my = y + cy, mz = z + cz;
+                    x = wall_move(mx, g.periodic[0], g.lower[0], g.upper[0], sa.w);
+                    y = wall_move(my, g.periodic[1], g.lower[1], g.upper[1], sa.w);
+                    z = wall_move(mz, g.periodic[2], g.lower[2], g.upper[2], sa.w);
+                    if (!g.periodic[0]) wr.x += mx - x;
+                    if (!g.periodic[1]) wr.y += my - y;
+                    if (!g.periodic[2]) wr.z += mz - z;
+                    step = sqrtf(cx * cx + cy * cy + cz * cz);
+                }
+                result.x = x;
+                result.y = y;
+                result.z = z;
+            }
+            dst[i] = result;
+            if (k == 0) w = wr;
+        }
+        __syncthreads();
+        float4 *t = src;
+        src = dst;
+        dst = t;
+    }
+    if (vo < V) {
+        const uint32_t flags = __float_as_uint(B[own].z);
+        if (!(flags & kInactive) && running(ss, spec_o)) {
+            out[vo] = src[own];
+            if (track_walls) wall[vo] = w;
+        }
+    }
+    spec_block_max(__float_as_uint(fmaxf(step, 0.f)), vo < V ? spec_o : 0u, red, kMetricStep, red_slots, scratch);
+}
+
 // ---------------------------------------------------------------------------
 // Per-fiber curvature Gauss–Seidel (project_fiber_curvature_in_place and
 // project_curvature_triplet_in_place, kernels.rs:1165-1415), then the
@@ -3026,9 +3177,20 @@ hipError_t launch_measure(const WorldDev &w, const IterParams &p, hipStream_t st
 // also applies contact. The curvature kernel reads the buffer the last pass
 // wrote and writes back to `pos`, so every iteration ends in `pos`.
 static uint32_t vertex_passes(const IterParams &p) { return p.constraint_iterations > 0 ? p.constraint_iterations : 1u; }
+// two or more passes run as one launch (k_vertex_passes), pos -> alt
+static bool fused_vertex_passes(const IterParams &p) {
+    return p.constraint_iterations >= 2u && p.constraint_iterations <= kMaxFusedPasses;
+}
 
 hipError_t launch_vertex_passes(const WorldDev &w, const IterParams &p, hipStream_t stream) {
     const uint32_t passes = vertex_passes(p);
+    if (fused_vertex_passes(p)) {
+        hipLaunchKernelGGL(k_vertex_passes, dim3(blocks_for(w.V, kPassOwned)), dim3(256),
+                           vertex_passes_lds_bytes(passes), stream, w.ss, w.grids, w.red_slots, w.pos, w.alt, w.vA,
+                           w.vB, w.corr0, w.corr1, w.wall, w.red, w.V, p.stretch, p.bend, p.max_step, passes,
+                           w.track_walls);
+        return hipGetLastError();
+    }
     float4 *src = w.pos, *dst = w.alt;
     for (uint32_t i = 0; i < passes; ++i) {
         hipLaunchKernelGGL(k_vertex_pass, dim3(blocks_for(w.V, 256)), dim3(256), 0, stream, w.ss, w.grids,
@@ -3043,7 +3205,7 @@ hipError_t launch_vertex_passes(const WorldDev &w, const IterParams &p, hipStrea
 }
 
 hipError_t launch_curvature(const WorldDev &w, const IterParams &p, hipStream_t stream) {
-    const float4 *src = (vertex_passes(p) % 2 == 1) ? w.alt : w.pos;
+    const float4 *src = (fused_vertex_passes(p) || vertex_passes(p) % 2 == 1) ? w.alt : w.pos;
     if (w.gs_lanes != 0 && w.max_fiber_vertices <= kMaxLdsFiberVertices) {
         const size_t lds = curvature_lds_bytes(w.max_fiber_vertices, w.gs_lanes, w.track_walls);
         const uint32_t per_block = 64u / w.gs_lanes;

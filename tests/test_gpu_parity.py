@@ -294,3 +294,31 @@ def test_create_packed_equals_create(hip_device):
         sa, sb = a.run_batch(cfg, 30), b.run_batch(cfg, 30)
         assert sa.as_dict() == sb.as_dict()
         assert np.array_equal(a.download_positions().view(np.uint32), b.download_positions().view(np.uint32))
+
+
+@pytest.mark.parametrize("walled", [False, True])
+@pytest.mark.parametrize("passes", [1, 2, 3, 8, 33])
+def test_jacobi_pass_counts_match_bitwise(hip_device, passes, walled):
+    """Every count of Jacobi passes per iteration matches the oracle bit for
+    bit: one pass alone, two to 32 passes in one launch (the workgroup
+    recomputes a halo of two vertices per remaining pass), more than that
+    one launch per pass; in a periodic cell and in one walled in z, where
+    the passes' wall reactions feed the clamps."""
+    if walled:
+        from test_gpu_compaction import bounded_bimodal_recipe
+
+        recipe = bounded_bimodal_recipe(scale=0.02)
+    else:
+        recipe = fx.bimodal_recipe(scale=0.02)
+    packed = recipe._packed(False)
+    cfg = fx.hard_gate_settings(7e-6, force_full_iterations=True).to_config(7e-6)
+    cfg.constraint_iterations = passes
+    print(f"[case] passes={passes} walled={walled}", flush=True)
+    gpu, ref = both(packed, cfg)
+    with gpu, ref:
+        sg = gpu.run_batch(cfg, 20)
+        so = ref.run_batch(cfg, 20)
+        assert sg.total_iterations == so.total_iterations == 20
+        assert np.float32(sg.max_displacement) == np.float32(so.max_displacement)
+        pg, po = gpu.download_positions(), ref.download_positions()
+        assert np.array_equal(pg.view(np.uint32), po.view(np.uint32))
