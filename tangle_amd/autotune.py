@@ -12,7 +12,10 @@ generated state, and returns the fastest combination.
 What it picks:
 - contact lanes and curvature lanes, from HIP-event group times;
 - the cell size scale and the Verlet skin scale, from the wall time of a
-  fixed number of iterations with rebuilds included. The skin sets the
+  fixed number of iterations with rebuilds included (four times the lane
+  trials' count by default: from the generated state most passes rebuild,
+  and a short trial overrates a wide skin, whose longer lists cost every
+  pass once rebuilds thin out). The skin sets the
   rebuild cadence (a list is rebuilt once a vertex has moved half the skin),
   so each grid candidate's measured iterations per rebuild are recorded and
   the chosen one is returned as `rebuild_interval`.
@@ -182,11 +185,12 @@ def _group_ms(world, config, iterations):
 
 def autotune(specimens, config: _native.Config, *, iterations: int = 32, contact_lanes=(4, 8, 16, 32),
              curvature_lanes=(4, 8, 16, 32, 64), skin_scales=(1.0, 2.0, 3.0), cell_scales=(1.0, 1.25, 1.5),
-             cache: str | os.PathLike | bool | None = None) -> Tuning:
-    """Picks contact and curvature lanes from HIP-event group times, then
-    the cell size scale and skin scale from the wall time of `iterations`
-    iterations (rebuilds included), each candidate starting from the
-    generated state of `specimens` (PackedAssembly objects).
+             grid_iterations: int | None = None, cache: str | os.PathLike | bool | None = None) -> Tuning:
+    """Picks contact and curvature lanes from HIP-event group times over
+    `iterations` iterations, then the cell size scale and skin scale from
+    the wall time of `grid_iterations` iterations (rebuilds included;
+    default 4 x `iterations`), each candidate starting from the generated
+    state of `specimens` (PackedAssembly objects).
 
     `cache`: None uses the default cache file, a path uses that file, False
     disables the cache. A hit returns the stored choice with cached=True."""
@@ -195,7 +199,9 @@ def autotune(specimens, config: _native.Config, *, iterations: int = 32, contact
         raise ValueError("autotune needs at least one specimen")
     cfg = _native.Config.from_buffer_copy(config)
     cfg.force_full_iterations = 1
-    candidates = {"iterations": [int(iterations)], "contact_lanes": list(contact_lanes),
+    grid_iterations = int(grid_iterations) if grid_iterations is not None else 4 * int(iterations)
+    candidates = {"iterations": [int(iterations)], "grid_iterations": [grid_iterations],
+                  "contact_lanes": list(contact_lanes),
                   "curvature_lanes": list(curvature_lanes), "skin_scales": list(map(float, skin_scales)),
                   "cell_scales": list(map(float, cell_scales))}
     cache_path = None if cache is False else (default_cache_path() if cache in (None, True) else Path(cache))
@@ -231,17 +237,18 @@ def autotune(specimens, config: _native.Config, *, iterations: int = 32, contact
             c.cell_size_scale = cell
             with _world(specimens, c) as world:
                 world.set_tuning(contact_lanes=best_contact, curvature_lanes=best_curvature)
-                world.run_batch(c, iterations)  # captures the batch graph outside the timing
+                world.run_batch(c, grid_iterations)  # captures the batch graph outside the timing
                 world.upload_positions(initial)
                 before = int(world.info().neighbor_list_rebuilds)
                 t0 = time.perf_counter()
-                world.run_batch(c, iterations)
+                world.run_batch(c, grid_iterations)
                 trials["grid"][(skin, cell)] = 1000.0 * (time.perf_counter() - t0)
                 trials["rebuilds"][(skin, cell)] = int(world.info().neighbor_list_rebuilds) - before
     skin, cell = min(trials["grid"], key=trials["grid"].get)
     rebuilds = trials["rebuilds"][(skin, cell)]
     tuning = Tuning(cell_size_scale=cell, neighbor_skin_scale=skin, contact_lanes=best_contact,
-                    curvature_lanes=best_curvature, rebuild_interval=iterations / max(1, rebuilds), trials=trials)
+                    curvature_lanes=best_curvature, rebuild_interval=grid_iterations / max(1, rebuilds),
+                    trials=trials)
     if cache_path is not None:
         store_cached(cache_path, key, tuning)
     return tuning

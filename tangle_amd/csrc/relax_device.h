@@ -106,6 +106,7 @@ struct WorldDev {
     uint32_t max_fiber_vertices;
     uint32_t gs_lanes;       // lanes per fiber in the curvature sweep (0, 4, 8, 16, 32 or 64)
     uint32_t contact_lanes;  // lanes per segment in the contact pass (4, 8, 16 or 32)
+    uint32_t prune_lists;    // the list build bounds stencil cells by distance (k_build_lists<true>)
     DevState *st;
     uint32_t *red;  // [nspec][3][red_slots] u32 bit patterns of nonnegative floats
     float4 *pos, *alt, *refpos, *wall, *vA, *vB;
@@ -116,6 +117,9 @@ struct WorldDev {
     uint32_t *factive;  // per fiber: 1 active, 0 not
     uint32_t *cell_counts, *cell_offsets, *cell_cursor, *cell_segs, *seg_cell, *scattered;
     uint32_t *cell_spec;  // per cell: its specimen
+    // per cell: the largest half length plus radius of its segments at the
+    // last rebuild's binning (f32 bits), for the list build's cell bound
+    uint32_t *cell_reach;
     // slot records in cell-list order, refreshed from `pos` every iteration:
     // slotA/slotB = the segment's two vertex records (x, y, z, rest arc
     // length), slotC = (segment, fiber, radius bits, specimen)

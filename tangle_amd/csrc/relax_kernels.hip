@@ -231,7 +231,8 @@ __device__ __forceinline__ uint32_t vertex_spec(const float4 *vB, uint32_t v) {
 __global__ __launch_bounds__(256) void k_rebuild_clear(const DevState *st, const SpecState *ss,
                                                        const uint32_t *cell_spec, uint32_t *cell_counts,
                                                        uint32_t *cell_cursor, uint32_t C, const float4 *pos,
-                                                       float4 *refpos, const float4 *vB, uint32_t V) {
+                                                       float4 *refpos, const float4 *vB, uint32_t V,
+                                                       uint32_t *cell_reach) {
     if (st->nstate[0] == 0) return;
     for (uint32_t first = blockIdx.x * 256u; first < C; first += gridDim.x * 256u) {
         if (!any_rebuilding(ss, cell_spec[first], cell_spec[min(first + 255u, C - 1u)])) continue;
@@ -239,6 +240,7 @@ __global__ __launch_bounds__(256) void k_rebuild_clear(const DevState *st, const
         if (i < C && rebuilding(ss, cell_spec[i])) {
             cell_counts[i] = 0;
             cell_cursor[i] = 0;
+            cell_reach[i] = 0;
         }
     }
     // The list snapshot (neighbor.rs:410-422) is taken from the positions the
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(256) void k_rebuild_clear(const DevState *st, const
 __global__ __launch_bounds__(256) void k_bin_segments(const DevState *st, const SpecState *ss,
                                                       const GridParams *grids, const float4 *pos, const uint4 *sstat,
                                                       const uint32_t *factive, uint32_t *seg_cell,
-                                                      uint32_t *cell_counts, uint32_t S) {
+                                                      uint32_t *cell_counts, uint32_t *cell_reach, uint32_t S) {
     if (st->nstate[0] == 0) return;
     for (uint32_t first = blockIdx.x * 256u; first < S; first += gridDim.x * 256u) {
         if (!any_rebuilding(ss, sstat[first].w, sstat[min(first + 255u, S - 1u)].w)) continue;
@@ -267,9 +269,16 @@ __global__ __launch_bounds__(256) void k_bin_segments(const DevState *st, const 
         }
         const GridParams &g = grids[me.w];
         const uint32_t v0 = me.x;
-        const uint32_t cell = g.cell_base + cell_index(g, pos[v0], pos[v0 + 1]);
+        const float4 p = pos[v0], q = pos[v0 + 1];
+        const uint32_t cell = g.cell_base + cell_index(g, p, q);
         seg_cell[s] = cell;
         atomicAdd(&cell_counts[cell], 1u);
+        // half length plus radius, for the list build's bound (a max of
+        // nonnegative floats is a max of their bits)
+        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        const float ext = 0.5f * sqrtf(dx * dx + dy * dy + dz * dz) + __uint_as_float(me.z);
+        // a NaN bounds nothing: its cell is never left out
+        atomicMax(&cell_reach[cell], ext == ext ? __float_as_uint(ext) : 0x7f800000u);
     }
 }
 
@@ -434,6 +443,60 @@ __device__ __forceinline__ void list_image(const float *length, const float *inv
     }
 }
 
+// A slot's view of the stencil boxes, for the list build's cell bound: per
+// axis the lower edge of the box of offset 0 (the cell holding the slot's
+// midpoint s, unwrapped on a periodic axis, as the binning computes it
+// before wrapping), the cell width and a rounding margin (the cell's binning
+// and the reach test's midpoint difference each round the coordinates). On
+// a periodic axis every candidate's nearest image lies in the box next to
+// s's cell when the axis has five cells or more (any other image is at least
+// 3 cells away); with fewer the axis bounds nothing. On a walled axis the
+// first and last cells also hold the midpoints beyond the walls.
+struct StencilBoxes {
+    float s[3], lo0[3], w[3], margin[3];
+    int32_t c0[3], cells[3];
+    bool use[3], walled[3];
+};
+__device__ __forceinline__ StencilBoxes stencil_boxes(const GridParams &g, uint32_t hx, uint32_t hy, uint32_t hz,
+                                                      float sx, float sy, float sz) {
+    StencilBoxes b;
+    const uint32_t h[3] = {hx, hy, hz};
+    const float sv[3] = {sx, sy, sz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        b.s[k] = sv[k];
+        b.w[k] = g.width[k];
+        b.cells[k] = (int32_t)g.cells[k];
+        b.walled[k] = !g.periodic[k];
+        b.use[k] = !g.periodic[k] || g.cells[k] >= 5u;
+        const float c = g.periodic[k] ? floorf((sv[k] - g.lower[k]) / g.width[k]) : (float)h[k];
+        b.c0[k] = (int32_t)h[k];
+        b.lo0[k] = g.lower[k] + c * g.width[k];
+        b.margin[k] = 1.0e-3f * g.width[k] + 1.0e-5f * (fabsf(g.lower[k]) + fabsf(g.upper[k]) + fabsf(sv[k]));
+    }
+    return b;
+}
+// Squared distance from the slot's midpoint to the box of stencil cell n,
+// less the margins.
+__device__ __forceinline__ float stencil_gap2(const StencilBoxes &b, uint32_t n) {
+    const int32_t o[3] = {(int32_t)(n % 3u) - 1, (int32_t)((n / 3u) % 3u) - 1, (int32_t)(n / 9u) - 1};
+    const float inf = __builtin_inff();
+    float d2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (!b.use[k]) continue;
+        float lo = b.lo0[k] + (float)o[k] * b.w[k], hi = lo + b.w[k];
+        if (b.walled[k]) {
+            const int32_t c = b.c0[k] + o[k];
+            if (c <= 0) lo = -inf;
+            if (c >= b.cells[k] - 1) hi = inf;
+        }
+        const float d = fmaxf(fmaxf(lo - b.s[k], b.s[k] - hi) - b.margin[k], 0.f);
+        d2 += d * d;
+    }
+    return d2;
+}
+
 // The list build (build_segment_neighbor_lists, neighbor.rs:155-364), one
 // wave per kBuildGroups consecutive cell-list slots, kBuildLanes lanes each.
 // A slot's candidates are its 27 stencil cells' segments, cell by cell in
@@ -449,13 +512,15 @@ __device__ __forceinline__ void list_image(const float *length, const float *inv
 // written to `nbr` as slots in ascending segment order, which replaces the
 // separate sort (neighbor.rs:366-408) and keeps the reference's summation
 // order; an overflowed segment keeps its `ovf` row.
+template <bool PRUNE>
 __global__ __launch_bounds__(64) void k_build_lists(const DevState *st, const SpecState *ss, const GridParams *grids,
                                                     const uint4 *sstat, const uint4 *fspan, const float4 *slotA,
                                                     const float4 *slotB, const uint4 *slotC, const float4 *cP,
                                                     const float4 *cQ, const uint32_t *seg_cell,
                                                     const uint32_t *cell_counts, const uint32_t *cell_offsets,
                                                     uint32_t *ncount, uint32_t *nbr, uint32_t *nbr_off, uint32_t *ovf,
-                                                    uint32_t block, uint32_t ovf_cap, uint32_t S) {
+                                                    const uint32_t *cell_reach, uint32_t block, uint32_t ovf_cap,
+                                                    uint32_t S) {
     constexpr uint32_t G = kBuildLanes, W = kBuildGroups;
     constexpr uint32_t per_lane = (27 + G - 1) / G;  // stencil cells per lane
     __shared__ BuildSlot slots[W];
@@ -505,6 +570,19 @@ __global__ __launch_bounds__(64) void k_build_lists(const DevState *st, const Sp
             const uint32_t home = seg_cell[seg] - g.cell_base;
             const uint32_t hx = home % g.cells[0], hy = (home / g.cells[0]) % g.cells[1],
                            hz = home / (g.cells[0] * g.cells[1]);
+            // A stencil cell none of whose candidates can pass the reach test
+            // below is left out (its candidates would all be rejected, so the
+            // accepted sequence is the same): the test's reach is at most
+            // this segment's half length, radius and skin plus the largest
+            // half length and radius of the cell's segments, widened by its
+            // (1 + 1e-4) and more.
+            float rself = 0.f;
+            StencilBoxes boxes;
+            if (PRUNE) {
+                const float bdx = B.x - A.x, bdy = B.y - A.y, bdz = B.z - A.z;
+                rself = 0.5f * sqrtf(bdx * bdx + bdy * bdy + bdz * bdz) + __uint_as_float(M.z) + skin;
+                boxes = stencil_boxes(g, hx, hy, hz, 0.5f * (A.x + B.x), 0.5f * (A.y + B.y), 0.5f * (A.z + B.z));
+            }
             // the lane's stencil cells, then their starts in the candidate sequence
             uint32_t local = 0;
             for (uint32_t j = 0; j < per_lane; ++j) {
@@ -515,6 +593,10 @@ __global__ __launch_bounds__(64) void k_build_lists(const DevState *st, const Sp
                     cell += g.cell_base;
                     cn = cell_counts[cell];
                     cs = cell_offsets[cell];
+                    if (PRUNE) {
+                        const float rmax = (rself + __uint_as_float(cell_reach[cell])) * 1.001f;
+                        if (stencil_gap2(boxes, n) > rmax * rmax) cn = 0;
+                    }
                 }
                 cell_start[group][n] = local;  // lane-relative for now
                 cell_slot[group][n] = cs;
@@ -3094,9 +3176,9 @@ hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
     if (w.S == 0 || w.C == 0 || w.V == 0) return hipErrorInvalidValue;
     const uint32_t clear_n = w.C > w.V ? w.C : w.V;
     hipLaunchKernelGGL(k_rebuild_clear, dim3(rebuild_blocks(clear_n, 256)), dim3(256), 0, stream, w.st, w.ss, w.cell_spec,
-                       w.cell_counts, w.cell_cursor, w.C, w.pos, w.refpos, w.vB, w.V);
+                       w.cell_counts, w.cell_cursor, w.C, w.pos, w.refpos, w.vB, w.V, w.cell_reach);
     hipLaunchKernelGGL(k_bin_segments, dim3(rebuild_blocks(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.grids, w.pos,
-                       w.sstat, w.factive, w.seg_cell, w.cell_counts, w.S);
+                       w.sstat, w.factive, w.seg_cell, w.cell_counts, w.cell_reach, w.S);
     hipLaunchKernelGGL(k_scan_cells, dim3(w.nspec), dim3(1024), 0, stream, w.st, w.ss, w.grids, w.cell_counts,
                        w.cell_offsets, w.cell_segs, w.scattered, w.slotC);
     hipLaunchKernelGGL(k_scatter_segments, dim3(rebuild_blocks(w.S, 256)), dim3(256), 0, stream, w.st, w.ss, w.sstat,
@@ -3106,10 +3188,15 @@ hipError_t launch_rebuild(const WorldDev &w, hipStream_t stream) {
                        w.slotC, w.cP, w.cQ, w.S);
     {
         const uint32_t blocks = blocks_for(w.S, kBuildGroups);
-        hipLaunchKernelGGL(k_build_lists, dim3(blocks < kBuildBlocks ? blocks : kBuildBlocks), dim3(64), 0, stream,
-                           w.st, w.ss, w.grids, w.sstat, w.fspan, w.slotA, w.slotB, w.slotC, w.cP, w.cQ, w.seg_cell,
-                           w.cell_counts, w.cell_offsets, w.ncount, w.nbr, w.nbr_off, w.ovf, w.block, w.ovf_cap,
-                           w.S);
+        const dim3 grid(blocks < kBuildBlocks ? blocks : kBuildBlocks);
+#define TGH_BUILD_ARGS                                                                                        \
+    w.st, w.ss, w.grids, w.sstat, w.fspan, w.slotA, w.slotB, w.slotC, w.cP, w.cQ, w.seg_cell, w.cell_counts, \
+        w.cell_offsets, w.ncount, w.nbr, w.nbr_off, w.ovf, w.cell_reach, w.block, w.ovf_cap, w.S
+        if (w.prune_lists)
+            hipLaunchKernelGGL(k_build_lists<true>, grid, dim3(64), 0, stream, TGH_BUILD_ARGS);
+        else
+            hipLaunchKernelGGL(k_build_lists<false>, grid, dim3(64), 0, stream, TGH_BUILD_ARGS);
+#undef TGH_BUILD_ARGS
     }
     return hipGetLastError();
 }
@@ -3288,9 +3375,9 @@ hipError_t launch_capture(const WorldDev &w, const GridParams *capture_grid, uin
     hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(256), 0, stream, c.st, c.ss, nullptr, c.nspec);
     const uint32_t clear_n = c.C > c.V ? c.C : c.V;
     hipLaunchKernelGGL(k_rebuild_clear, dim3(rebuild_blocks(clear_n, 256)), dim3(256), 0, stream, c.st, c.ss, c.cell_spec,
-                       c.cell_counts, c.cell_cursor, c.C, c.pos, c.refpos, c.vB, c.V);
+                       c.cell_counts, c.cell_cursor, c.C, c.pos, c.refpos, c.vB, c.V, c.cell_reach);
     hipLaunchKernelGGL(k_bin_segments, dim3(rebuild_blocks(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.grids, c.pos,
-                       c.sstat, c.factive, c.seg_cell, c.cell_counts, c.S);
+                       c.sstat, c.factive, c.seg_cell, c.cell_counts, c.cell_reach, c.S);
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, c.st, c.ss, c.grids, c.cell_counts,
                        c.cell_offsets, c.cell_segs, c.scattered, c.slotC);
     hipLaunchKernelGGL(k_scatter_segments, dim3(rebuild_blocks(c.S, 256)), dim3(256), 0, stream, c.st, c.ss, c.sstat,

@@ -281,6 +281,8 @@ float segment_length_now(const Packed &p, uint32_t f, uint32_t l) {
 
 int size_grid(const Packed &p, const tgh_config *c, Sizing &z) {
     float max_rest = 0.f, max_init = 0.f, max_r = 0.f, max_leaf = 0.f;
+    double ext_sum = 0.0;  // half length plus radius, summed over segments
+    uint64_t segments = 0;
     for (uint32_t f = 0; f < p.F; ++f) {
         max_r = std::fmax(max_r, p.radius[f]);
         for (uint32_t l = 0; l + 1 < p.fcount[f]; ++l) {
@@ -289,6 +291,8 @@ int size_grid(const Packed &p, const tgh_config *c, Sizing &z) {
             max_rest = std::fmax(max_rest, rest);
             max_init = std::fmax(max_init, now);
             max_leaf = std::fmax(max_leaf, std::fmax(rest, now));
+            ext_sum += 0.5 * now + p.radius[f];
+            ++segments;
         }
     }
     const float max_len = std::fmax(max_rest, max_init);
@@ -308,6 +312,7 @@ int size_grid(const Packed &p, const tgh_config *c, Sizing &z) {
     z.cell_size = (piece + margin) * c->cell_size_scale;
     z.segment_cell_size = (max_len + margin) * c->cell_size_scale;
     z.max_radius = max_r;
+    z.list_reach = (float)(2.0 * ext_sum / (double)std::max<uint64_t>(1, segments)) + z.skin;
     uint64_t C = 1;
     for (int k = 0; k < 3; ++k) {
         float extent = p.upper[k] - p.lower[k];
@@ -428,6 +433,7 @@ int upload_static(tgh_world *w) {
     HIP_TRY(hipMemset(d.ncount, 0, S * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.nbr_off, 0, S * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.red, 0, (size_t)d.nspec * 3 * d.red_slots * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(d.cell_reach, 0, d.C * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_counts, 0, d.C * sizeof(uint32_t)));
     HIP_TRY(hipMemset(d.cell_offsets, 0, d.C * sizeof(uint32_t)));
     {
@@ -547,6 +553,14 @@ int create_world_packed(std::vector<Packed> &&packed, const tgh_config *c, tgh_w
     // of the bimodal felt and the slab (faster there than 16, which leaves
     // more lanes idle); the autotuner picks 4 for shorter lists
     d.contact_lanes = 8u;
+    // the list build leaves out stencil cells beyond reach when cells are
+    // wide against a typical pair's reach (the bimodal felt's are 1.9 times
+    // it, their width set by its longest segments): where they are about as
+    // wide, as in the batched specimens, few cells drop out and the bound's
+    // registers cost more than it saves
+    d.prune_lists = 1u;
+    for (uint32_t sp = 0; sp < count; ++sp)
+        if (!(w->sizes[sp].cell_size >= 1.25f * w->sizes[sp].list_reach)) d.prune_lists = 0u;
     e = hipSuccess;
     GridParams *grids = nullptr;
     if (e == hipSuccess) e = w->alloc(&d.st, 1);
@@ -570,6 +584,7 @@ int create_world_packed(std::vector<Packed> &&packed, const tgh_config *c, tgh_w
     if (e == hipSuccess) e = w->alloc(&d.cell_offsets, C);
     if (e == hipSuccess) e = w->alloc(&d.cell_cursor, C);
     if (e == hipSuccess) e = w->alloc(&d.cell_spec, std::max<uint64_t>(1, C));
+    if (e == hipSuccess) e = w->alloc(&d.cell_reach, std::max<uint64_t>(1, C));
     if (e == hipSuccess) e = w->alloc(&d.cell_segs, S);
     if (e == hipSuccess) e = w->alloc(&d.seg_cell, S);
     if (e == hipSuccess) e = w->alloc(&d.scattered, S);

@@ -36,6 +36,7 @@ struct Sizing {
     float skin = 0.f, cell_size = 0.f;
     float segment_cell_size = 0.f;  // whole-segment grid of the contact capture (world.rs:405)
     float max_radius = 0.f;
+    float list_reach = 0.f;  // the list test's reach for a pair of mean half length plus radius, at creation
     uint32_t cells[3] = {1, 1, 1};
     uint32_t C = 1;
     size_t slots = 1;

@@ -322,3 +322,43 @@ def test_jacobi_pass_counts_match_bitwise(hip_device, passes, walled):
         assert np.float32(sg.max_displacement) == np.float32(so.max_displacement)
         pg, po = gpu.download_positions(), ref.download_positions()
         assert np.array_equal(pg.view(np.uint32), po.view(np.uint32))
+
+
+@pytest.mark.parametrize("walled", [False, True])
+def test_lists_bit_exact_with_fibers_images_away(hip_device, walled):
+    """The list build leaves out stencil cells that no candidate within
+    reach can occupy; the cells and pair lists still equal the oracle's when
+    fibers sit periodic images away from the primary cell (shifted by
+    whole cell lengths, fiber by fiber), on a grid of eight cells a side,
+    periodic in all axes or walled in z."""
+    if walled:
+        from test_gpu_compaction import bounded_bimodal_recipe
+
+        recipe = bounded_bimodal_recipe(scale=0.3)
+    else:
+        recipe = fx.bimodal_recipe(scale=0.3)
+    packed = recipe._packed(False)
+    cfg = fx.hard_gate_settings(7e-6).to_config(7e-6)
+    lengths = np.array([packed.struct.cell_lengths[k] for k in range(3)])
+    periodic = np.array([bool(packed.struct.cell_periodic[k]) for k in range(3)])
+    counts = np.asarray(packed.counts, dtype=np.int64)
+    gpu, ref = both(packed, cfg)
+    with gpu, ref:
+        print(f"[case] walled={walled} cells={list(gpu.info().cells)}", flush=True)
+        assert min(gpu.info().cells) >= 5
+        p = gpu.download_positions().astype(np.float64).reshape(-1, 3)
+        start = 0
+        for f, n in enumerate(counts):
+            shift = np.array([(f % 5) - 2, (3 * f % 5) - 2, (f % 3) - 1], dtype=np.float64) * lengths * periodic
+            p[start:start + n, :3] += shift
+            start += n
+        p = p.astype(np.float32)
+        for _ in range(2):
+            gpu.upload_positions(p)
+            ref.upload_positions(p)
+            gpu.rebuild_lists()
+            ref.rebuild_lists()
+            listed, _ = _lists_equal(gpu, ref)
+            assert listed > 0
+            gpu.run_batch(cfg, 30)
+            p = gpu.download_positions()
